@@ -1,0 +1,245 @@
+/*
+ * piv_xcorr2d.h -- C ABI of the MI355X-native multi-grid PIV cross-correlation
+ * engine (libpivx.so).
+ *
+ * The reference exposes this path as the C++ class `xcorr2d_multi`
+ * (xcorr2d_V1.0/xcorr2d_multi_grid.cuh:12-208) driven by `main`
+ * (xcorr2d_V1.0/main.cpp:227-376).  Each entry point below replaces one method
+ * of that class (cited per function); the object-with-public-members pattern is
+ * replaced by an opaque handle, plain pointers and sizes, and int status codes.
+ *
+ * Conventions
+ *  - Every call returns PIVX_OK (0) or a positive PIVX_ERR_* code; the text of
+ *    the last failure is available from pivx_last_error(ctx) (or
+ *    pivx_last_error(NULL) for failures of pivx_init and of the host utilities).
+ *  - No call exits the process, prints a message box or waits for input.
+ *  - A handle owns one HIP stream on the device it was created on; all GPU work
+ *    of that handle is ordered on that stream.  Calls on one handle must be
+ *    serialised by the caller; distinct handles are independent.
+ *  - Buffers passed in are caller-owned and only read/written during the call.
+ *  - grid = 1 or 2 selects the coarse or fine grid (the reference's _p1/_p2).
+ */
+#ifndef PIV_XCORR2D_H
+#define PIV_XCORR2D_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PIVX_OK 0
+#define PIVX_ERR_PARAM 1   /* a parameter the reference rejects, or an invalid argument */
+#define PIVX_ERR_HIP 2     /* a HIP runtime failure (allocation, launch, copy) */
+#define PIVX_ERR_STATE 3   /* call out of order (e.g. no frames uploaded yet) */
+#define PIVX_ERR_IO 4      /* file could not be opened, read or written */
+#define PIVX_ERR_FORMAT 5  /* malformed parameter, index or TIFF file */
+
+typedef struct pivx_ctx pivx_ctx;
+
+/* The 30 scalars of xcorr2d_multi::init (xcorr2d_multi_grid.cu:716-718), in
+ * the same order and with the same meaning. */
+typedef struct pivx_init_params {
+  int nx, ny, nsec_cc;
+  int ws_x1, ws_y1, spacing_x1, spacing_y1, npass1;
+  int ws_x2, ws_y2, spacing_x2, spacing_y2, npass2;
+  int wx1_median, wy1_median, wx2_median, wy2_median;
+  int wx1_smooth, wy1_smooth, wx2_smooth, wy2_smooth;
+  int npeak_save;
+  float max_dispx_percent1, max_dispy_percent1, max_dispx_percent2, max_dispy_percent2;
+  float dispx_lim1, dispx_lim2, dispy_lim1, dispy_lim2;
+} pivx_init_params;
+
+/* Derived geometry of one grid (xcorr2d_multi_grid.cu:751-894). */
+typedef struct pivx_geometry {
+  int ws_x, ws_y, spacing_x, spacing_y;
+  int nx_vec, ny_vec, nt_vec;
+  int border_x, border_y;      /* peak-search border, cu:751-754 */
+  int npeak_slots;             /* reference candidate cap ws_x*ws_y/16, cu:768-769 */
+  int left_x, left_y, right_x, right_y;   /* zero padding, cu:831-880 */
+  int nx_aug, ny_aug;          /* padded ("aug") frame size, cu:882-891 */
+  int nsec, neach, nlast;      /* reference memory sections, cu:805-829 */
+} pivx_geometry;
+
+/* Validation-stage parameters of one grid, as passed by main.cpp:299-318. */
+typedef struct pivx_filter_params {
+  int npass_median, min_nneigh;
+  float residual_thr;   /* read but unused by the reference (cu:1451-1545) */
+  float insert_thr;
+  int npass_smooth;
+} pivx_filter_params;
+
+/* Counters of the most recent pass of each kind (no reference counterpart). */
+typedef struct pivx_stats {
+  long long overflow_windows[2];  /* windows whose local-maximum count exceeded the reference's slot cap */
+  long long outliers[2];          /* vectors whose first candidate failed the median test (last median pass) */
+  int fill_sweeps[2];             /* hole-filling sweeps run in the last validation of each grid */
+  long long oob_taps[2];          /* deformation taps that fell outside the padded frame (read as 0) */
+} pivx_stats;
+
+/* ------------------------------------------------------------------ lifecycle */
+
+/* xcorr2d_multi::init (cu:716-1269) + gpusetup::AssignGpu (gpuinfo.cu:62-66).
+ * Applies the reference's acceptance rules exactly (4 <= ws <= 1024, ws % 4 == 0
+ * for both grids, cu:896-920), plus rejections for inputs on which the
+ * reference divides by zero or reads out of bounds (nsec_cc < 1, spacing < 1,
+ * npeak_save < 1, image smaller than one spacing). */
+int pivx_init(const pivx_init_params* p, int device_id, pivx_ctx** out);
+
+/* xcorr2d_multi::freemem (cu:1976-2063).  Accepts NULL. */
+int pivx_freemem(pivx_ctx* ctx);
+
+/* Text of the last error on ctx (or of the last context-free failure when ctx
+ * is NULL).  Never NULL; empty when there was no error. */
+const char* pivx_last_error(const pivx_ctx* ctx);
+
+int pivx_get_geometry(const pivx_ctx* ctx, int grid, pivx_geometry* g);
+
+/* ------------------------------------------------------------------ inputs */
+
+/* xcorr2d_multi::prep_img_data (cu:1271-1282).  The reference takes the frames
+ * from its public pinned members h_img1_org/h_img2_org (written by
+ * main.cpp:264-266); here the caller passes host pointers to two nx*ny 8-bit
+ * frames with row pitch `pitch` bytes (>= nx).  Copies them to the device. */
+int pivx_prep_img_data(pivx_ctx* ctx, const uint8_t* img1, const uint8_t* img2, size_t pitch);
+
+/* Same, from frames already resident in device memory of the handle's GPU
+ * (device-to-device copy, no host staging). */
+int pivx_prep_img_data_device(pivx_ctx* ctx, const uint8_t* d_img1, const uint8_t* d_img2, size_t pitch);
+
+/* ------------------------------------------------------------------ passes */
+
+/* xcorr2d_multi::Calc_cc_map_p1 (cu:1284-1449): one grid-1 correlation pass.
+ * idx_pass == 0 zeroes the grid-1 predictor and correlates the undeformed
+ * frames; idx_pass > 0 deforms frame 2 by the extended filtered field. */
+int pivx_calc_cc_map_p1(pivx_ctx* ctx, int idx_pass);
+
+/* xcorr2d_multi::MedianFilter_Smoothing_p1 (cu:1451-1545): normalised median
+ * test with candidate replacement, hole filling (always), top-hat smoothing. */
+int pivx_median_filter_smoothing_p1(pivx_ctx* ctx, int npass_median, int min_nneigh,
+                                    float residual_thr, float insert_thr, int npass_smooth);
+
+/* xcorr2d_multi::extend_filtered_vecfield_p1 (cu:1547-1576). */
+int pivx_extend_filtered_vecfield_p1(pivx_ctx* ctx);
+
+/* xcorr2d_multi::Upsampling_vector_field (cu:1578-1587): bilinear grid-1 ->
+ * grid-2 predictor. */
+int pivx_upsampling_vector_field(pivx_ctx* ctx);
+
+/* xcorr2d_multi::extend_filtered_vecfield_p2 (cu:1589-1618). */
+int pivx_extend_filtered_vecfield_p2(pivx_ctx* ctx);
+
+/* xcorr2d_multi::Calc_cc_map_p2 (cu:1620-1764): one grid-2 pass (always
+ * deformed). */
+int pivx_calc_cc_map_p2(pivx_ctx* ctx);
+
+/* xcorr2d_multi::MedianFilter_Smoothing_p2 (cu:1766-1863); iffill selects hole
+ * filling as in the reference. */
+int pivx_median_filter_smoothing_p2(pivx_ctx* ctx, int npass_median, int min_nneigh,
+                                    float residual_thr, float insert_thr, int npass_smooth,
+                                    int iffill);
+
+/* The per-pair pass sequence of main.cpp:288-375 without file output: grid-1
+ * passes, upsampling, grid-2 passes, then the grid-2 "median-only" re-run
+ * (written to the separate median-only field, see pivx_get_filtered).  All work
+ * is enqueued on the handle's stream; no host synchronisation inside. */
+int pivx_run_pair(pivx_ctx* ctx, const pivx_filter_params* grid1, const pivx_filter_params* grid2);
+
+/* Wait for all work of the handle. */
+int pivx_synchronize(pivx_ctx* ctx);
+
+/* ------------------------------------------------------------------ outputs */
+
+/* xcorr2d_multi::write_vec_data_general (cu:1865-1973): selection 0 = grid-1
+ * raw candidates, 1 = grid-1 filtered, 2 = grid-2 raw, 3 = grid-2 filtered.
+ * crlf != 0 writes CRLF line ends (the reference's Windows text mode). */
+int pivx_write_vec_data_general(pivx_ctx* ctx, const char* path, int selection, int crlf);
+
+int pivx_get_dims(const pivx_ctx* ctx, int grid, int* nx_vec, int* ny_vec);
+/* Vector centres as written by the reference (cu:1237-1254). */
+int pivx_get_grid_coords(const pivx_ctx* ctx, int grid, float* x, float* y);
+/* Raw candidates [nt_vec][npeak_save] (d_disp_{x,y}_p*, d_valid_p*). */
+int pivx_get_raw(pivx_ctx* ctx, int grid, float* dx, float* dy, uint8_t* valid);
+/* Filtered field [nt_vec] (d_disp_{x,y}_filter_p*, d_valid_filter_p*). */
+int pivx_get_filtered(pivx_ctx* ctx, int grid, float* dx, float* dy, uint8_t* valid);
+/* Integer peak data [nt_vec][npeak_save]: correlation value and row-major pixel
+ * index j*ws_x+i in the fftshifted window plane, and the uncapped local-maximum
+ * count per window [nt_vec]. Any pointer may be NULL. */
+int pivx_get_peaks(pivx_ctx* ctx, int grid, float* value, int* index, int* ncand);
+/* The deformed frame 2 of the last deformed pass of `grid`, [ny_aug][nx_aug]. */
+int pivx_get_deformed(pivx_ctx* ctx, int grid, float* out);
+int pivx_get_stats(pivx_ctx* ctx, pivx_stats* st);
+
+/* Stage-test hooks: overwrite the filtered field (the predictor) or the raw
+ * candidates of a grid with caller data, so each stage can be fed identical
+ * inputs.  valid may be NULL (all valid). */
+int pivx_set_filtered(pivx_ctx* ctx, int grid, const float* dx, const float* dy, const uint8_t* valid);
+int pivx_set_raw(pivx_ctx* ctx, int grid, const float* dx, const float* dy, const uint8_t* valid);
+
+/* Average device time (ms) of the kernels of the last pass of a stage, measured
+ * with HIP events on the handle's stream when timing is enabled.
+ * stage: 0 deformation, 1 correlation, 2 median, 3 fill, 4 smoothing. */
+int pivx_enable_timing(pivx_ctx* ctx, int enable);
+int pivx_get_stage_ms(pivx_ctx* ctx, int grid, int stage, double* ms, int* launches);
+
+/* ------------------------------------------------------------------ host utilities */
+
+/* Parameter.dat (main.cpp:93-158): 50 positional lines, CR/LF tolerant; numeric
+ * lines parsed like _wtoi/_wtof (leading number, 0 on garbage).  Fewer than 50
+ * lines -> PIVX_ERR_FORMAT ("Parameter missing"). */
+typedef struct pivx_parameter_file {
+  char prefix_f1[1024], suffix_f1[256], prefix_f2[1024], suffix_f2[256];
+  char dir_index[1024], folder_result[1024];
+  int device_id, ifmhe, mhe_ws;
+  float mhe_thresh;
+  pivx_init_params init;           /* nx .. dispy_lim2 */
+  int npass1_median, min_nneigh_p1;
+  float residual_threshold_p1, insert_threshold_p1;
+  int npass1_smooth;
+  int npass2_median, min_nneigh_p2;
+  float residual_threshold_p2, insert_threshold_p2;
+  int npass2_smooth;
+} pivx_parameter_file;
+int pivx_parse_parameter_file(const char* path, pivx_parameter_file* out);
+
+/* Pair-index file (main.cpp:170-190): one integer per line until EOF.  Returns
+ * the number of indices (may exceed max_n; only max_n are stored), or -1. */
+int pivx_read_pair_index(const char* path, int* idx, int max_n);
+
+/* The five output names of main.cpp:327-376 for pair `pair`; which = 0 (grid-1
+ * raw), 1 (grid-1 filtered), 2 (grid-2 raw), 3 (grid-2 filtered), 4 (grid-2
+ * median-only).  Writes folder + name into out (size cap). */
+int pivx_output_name(const pivx_parameter_file* pf, int pair, int which, char* out, size_t cap);
+
+/* 8-bit TIFF (ImageIOPkg.cpp:124-429): header fields and strip read. */
+int pivx_tiff_info(const char* path, int* width, int* height, int* bits_per_sample);
+int pivx_tiff_read8(const char* path, uint8_t* buf, size_t cap);
+/* Minimal baseline 8-bit writer (one strip) used to build test inputs. */
+int pivx_tiff_write8(const char* path, const uint8_t* buf, int width, int height);
+
+/* Seeded synthetic particle-image pair (no reference counterpart; the
+ * reference's inputs are not available).  flow: 0 uniform (a0,a1), 1
+ * Lamb-Oseen vortex at the image centre (a0 = core radius, a1 = peak swirl
+ * px) plus linear shear a2 px/px, 2 sum of 8 random Fourier modes with maximum
+ * displacement a0.  Frame 2 renders every particle displaced by the flow at its
+ * frame-1 position. */
+typedef struct pivx_synth_params {
+  int nx, ny;
+  unsigned long long seed;
+  float density;       /* particles per pixel */
+  float sigma;         /* Gaussian particle sigma, px */
+  float imin, imax;    /* peak intensity range */
+  float background, noise_sigma;
+  int flow;
+  float a0, a1, a2;
+} pivx_synth_params;
+int pivx_synth_pair(const pivx_synth_params* sp, uint8_t* img1, uint8_t* img2);
+/* The synthetic flow's displacement at n points. */
+int pivx_synth_flow(const pivx_synth_params* sp, const float* x, const float* y, int n,
+                    float* u, float* v);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PIV_XCORR2D_H */

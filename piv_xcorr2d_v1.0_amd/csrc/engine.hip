@@ -1,0 +1,838 @@
+// Host engine of the MI355X PIV path: geometry, device memory, pass
+// sequencing and the C ABI declared in include/piv_xcorr2d.h.
+//
+// Replaces the host side of class xcorr2d_multi
+// (xcorr2d_V1.0/xcorr2d_multi_grid.cu:706-2063, .cuh:12-208).  Differences in
+// form, not in results:
+//   * frames live once in HBM as plain nx*ny u8 images; the four zero-padded
+//     per-grid copies and their 4*Ny row copies (cu:933-954, 1271-1282) are
+//     replaced by bounds-checked gathers that read 0 outside the frame;
+//   * each correlation pass is one fused kernel (no cuFFT plans, no
+//     sectioning: nsec_cc is validated and reported but every pass covers all
+//     windows at once);
+//   * the hole-filling loop runs on the device (no per-sweep host copy);
+//   * everything is ordered on one HIP stream owned by the handle.
+#include "../../include/piv_xcorr2d.h"
+#include "kernels.h"
+#include "hostio.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using namespace pivx;
+
+namespace {
+
+constexpr int kExBorder = 3;       // vec_ex_border_p1/p2 (cuh:190-191)
+constexpr float kEpsilon = 0.1f;   // epsilon (cuh:192)
+
+struct GridBuf {
+  pivx_geometry g{};
+  bool fast = false;
+  float *dx = nullptr, *dy = nullptr;  // raw candidates [nt*K]
+  uint8_t* dv = nullptr;
+  float* pkv = nullptr;
+  int* pki = nullptr;
+  int* ncand = nullptr;                // [nt]
+  float *fx = nullptr, *fy = nullptr;  // filtered field [nt]
+  uint8_t* fv = nullptr;
+  float *sx = nullptr, *sy = nullptr;  // ping-pong partner of the filtered field
+  uint8_t* sv = nullptr;
+  float *ex = nullptr, *ey = nullptr;  // extended field (nx+6)*(ny+6)
+  float* deform = nullptr;             // deformed frame 2 [ny_aug][nx_aug]
+  float2 *twx = nullptr, *twy = nullptr;   // generic-path twiddles
+  float2* gscratch = nullptr;              // generic-path planes for big windows
+  size_t gscratch_windows = 0;
+  float* med_scratch = nullptr;            // generic median lists
+  bool deformed_once = false;
+};
+
+}  // namespace
+
+struct pivx_ctx {
+  pivx_init_params p{};
+  int device = 0;
+  hipStream_t stream = nullptr;
+  GridBuf grid[2];
+  float *mo_x = nullptr, *mo_y = nullptr;  // grid-2 median-only field (run_pair)
+  uint8_t* mo_v = nullptr;
+  uint8_t *f1 = nullptr, *f2 = nullptr;    // frames [ny][nx]
+  bool have_frames = false;
+  unsigned long long* counters = nullptr;  // overflow[2], outliers[2], oob[2]
+  int* sweeps = nullptr;                   // [2]
+  int *list_a = nullptr, *list_b = nullptr;
+  float *tmp_x = nullptr, *tmp_y = nullptr;
+  uint8_t* tmp_f = nullptr;
+  std::vector<void*> allocs;
+  std::string err;
+  // Stage timing: event pairs recorded on the stream without host waits,
+  // resolved by pivx_get_stage_ms.
+  bool timing = false;
+  std::vector<hipEvent_t> ev_pool;
+  size_t ev_used = 0;
+  int ev_start = -1;
+  struct Rec { int gi, stage, a, b; };
+  std::vector<Rec> recs;
+};
+
+namespace {
+
+int fail(pivx_ctx* c, int code, const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  if (c) c->err = buf;
+  set_global_error(buf);
+  return code;
+}
+
+#define HIPCHK(ctx, expr)                                                                   \
+  do {                                                                                      \
+    hipError_t e_ = (expr);                                                                 \
+    if (e_ != hipSuccess) return fail(ctx, PIVX_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+template <class T>
+int dalloc(pivx_ctx* c, T** p, size_t n) {
+  void* q = nullptr;
+  size_t bytes = std::max<size_t>(n * sizeof(T), 16);
+  hipError_t e = hipMalloc(&q, bytes);
+  if (e != hipSuccess) return fail(c, PIVX_ERR_HIP, "hipMalloc(%zu bytes): %s", bytes, hipGetErrorString(e));
+  c->allocs.push_back(q);
+  *p = static_cast<T*>(q);
+  return PIVX_OK;
+}
+
+// Reference vector count along one axis (cu:771-777).
+int vec_count(int n, int sp) {
+  int t1 = (n - sp / 2 - 1) / sp;
+  int t2 = n - (sp / 2 + 1 + t1 * sp);
+  while (t2 < (sp / 2 - 1)) {
+    t1 -= 1;
+    t2 += sp;
+  }
+  return t1 + 1;
+}
+
+// Geometry of one grid (cu:751-894).
+pivx_geometry make_geometry(int nx, int ny, int nsec, int wsx, int wsy, int spx, int spy, float pctx, float pcty) {
+  pivx_geometry g{};
+  g.ws_x = wsx;
+  g.ws_y = wsy;
+  g.spacing_x = spx;
+  g.spacing_y = spy;
+  g.border_x = std::max(1, wsx / 2 - (int)roundf((float)wsx * pctx));
+  g.border_y = std::max(1, wsy / 2 - (int)roundf((float)wsy * pcty));
+  g.npeak_slots = wsx * wsy / 4 / 4;
+  g.nx_vec = vec_count(nx, spx);
+  g.ny_vec = vec_count(ny, spy);
+  g.nt_vec = g.nx_vec * g.ny_vec;
+  g.nsec = nsec;
+  if (g.nt_vec > 0 && nsec > 0) {
+    if (g.nt_vec % nsec == 0) {
+      g.neach = g.nt_vec / nsec;
+      g.nlast = g.neach;
+    } else {
+      g.neach = g.nt_vec / nsec;
+      g.nlast = g.nt_vec - (nsec - 1) * g.neach;
+      while (g.nlast > g.neach && g.neach > 0) {
+        g.nlast -= g.neach;
+        g.nsec += 1;
+      }
+    }
+  }
+  const int hx = wsx / 2, hy = wsy / 2;
+  g.left_x = (spx / 2 < hx) ? hx - spx / 2 : 0;
+  g.left_y = (spy / 2 < hy) ? hy - spy / 2 : 0;
+  g.right_x = ((nx - spx / 2 - 1 - (g.nx_vec - 1) * spx) < (hx - 1)) ? spx / 2 + 1 + (g.nx_vec - 1) * spx + hx - 1 - nx : 0;
+  g.right_y = ((ny - spy / 2 - 1 - (g.ny_vec - 1) * spy) < (hy - 1)) ? spy / 2 + 1 + (g.ny_vec - 1) * spy + hy - 1 - ny : 0;
+  g.nx_aug = nx + g.left_x + g.right_x;
+  g.ny_aug = ny + g.left_y + g.right_y;
+  return g;
+}
+
+int validate(const pivx_init_params* p) {
+  // Reference checks, same order and messages (cu:896-920).
+  if (p->ws_x1 < 4 || p->ws_y1 < 4) return fail(nullptr, PIVX_ERR_PARAM, "The interrogation window size (1st grid) is too small!");
+  if (p->ws_x1 > 1024 || p->ws_y1 > 1024) return fail(nullptr, PIVX_ERR_PARAM, "The interrogation window size (1st grid) is too big!");
+  if (p->ws_x2 < 4 || p->ws_y2 < 4) return fail(nullptr, PIVX_ERR_PARAM, "The interrogation window size (2nd grid) is too small!");
+  if (p->ws_x2 > 1024 || p->ws_y2 > 1024) return fail(nullptr, PIVX_ERR_PARAM, "The interrogation window size (2nd grid) is too big!");
+  if (p->ws_x1 % 4 != 0 || p->ws_x2 % 4 != 0) return fail(nullptr, PIVX_ERR_PARAM, "The window x-dimension is not divisible by 4!");
+  if (p->ws_y1 % 4 != 0 || p->ws_y2 % 4 != 0) return fail(nullptr, PIVX_ERR_PARAM, "The window y-dimension is not divisible by 4!");
+  // Inputs on which the reference divides by zero or indexes out of bounds.
+  if (p->nx < 1 || p->ny < 1) return fail(nullptr, PIVX_ERR_PARAM, "image size must be positive (nx=%d, ny=%d)", p->nx, p->ny);
+  if (p->nsec_cc < 1) return fail(nullptr, PIVX_ERR_PARAM, "Nsec_cc must be >= 1 (got %d)", p->nsec_cc);
+  if (p->spacing_x1 < 1 || p->spacing_y1 < 1 || p->spacing_x2 < 1 || p->spacing_y2 < 1)
+    return fail(nullptr, PIVX_ERR_PARAM, "vector spacing must be >= 1");
+  if (p->npeak_save < 1) return fail(nullptr, PIVX_ERR_PARAM, "Npeak_save must be >= 1 (got %d)", p->npeak_save);
+  if (p->npass1 < 0 || p->npass2 < 0) return fail(nullptr, PIVX_ERR_PARAM, "pass counts must be >= 0");
+  if (p->wx1_median < 1 || p->wy1_median < 1 || p->wx2_median < 1 || p->wy2_median < 1 || p->wx1_smooth < 1 ||
+      p->wy1_smooth < 1 || p->wx2_smooth < 1 || p->wy2_smooth < 1)
+    return fail(nullptr, PIVX_ERR_PARAM, "median/smoothing window sizes must be >= 1");
+  return PIVX_OK;
+}
+
+void host_twiddles(int n, std::vector<float2>& tw) {
+  tw.resize(n);
+  for (int m = 0; m < n; ++m) {
+    const double a = -2.0 * M_PI * (double)m / (double)n;
+    tw[m] = make_float2((float)std::cos(a), (float)std::sin(a));
+  }
+}
+
+GridBuf* grid_of(pivx_ctx* c, int grid) {
+  if (!c || (grid != 1 && grid != 2)) return nullptr;
+  return &c->grid[grid - 1];
+}
+
+int next_event(pivx_ctx* c) {
+  if (c->ev_used == c->ev_pool.size()) {
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return -1;
+    c->ev_pool.push_back(e);
+  }
+  return (int)c->ev_used++;
+}
+void tic(pivx_ctx* c) {
+  if (!c->timing) return;
+  c->ev_start = next_event(c);
+  if (c->ev_start >= 0) (void)hipEventRecord(c->ev_pool[c->ev_start], c->stream);
+}
+void toc(pivx_ctx* c, int gi, int stage) {
+  if (!c->timing || c->ev_start < 0) return;
+  const int e = next_event(c);
+  if (e < 0) return;
+  (void)hipEventRecord(c->ev_pool[e], c->stream);
+  c->recs.push_back({gi, stage, c->ev_start, e});
+  c->ev_start = -1;
+}
+
+int check_launch(pivx_ctx* c, hipError_t e, const char* what) {
+  if (e != hipSuccess) return fail(c, PIVX_ERR_HIP, "GPU kernel error at %s: %s", what, hipGetErrorString(e));
+  return PIVX_OK;
+}
+
+// Deformation of frame 2 by the extended field of grid gi (cu:1300-1301, 1624-1625).
+int run_deform(pivx_ctx* c, int gi) {
+  GridBuf& G = c->grid[gi];
+  const pivx_geometry& g = G.g;
+  DefArgs d{};
+  d.out = G.deform;
+  d.nx_aug = g.nx_aug;
+  d.ny_aug = g.ny_aug;
+  d.f2 = c->f2;
+  d.pitch = c->p.nx;
+  d.nx = c->p.nx;
+  d.ny = c->p.ny;
+  d.left_x = g.left_x;
+  d.left_y = g.left_y;
+  d.hfws_x = g.ws_x / 2;
+  d.hfws_y = g.ws_y / 2;
+  d.ex_x = G.ex;
+  d.ex_y = G.ey;
+  d.nxe = g.nx_vec + 2 * kExBorder;
+  d.nye = g.ny_vec + 2 * kExBorder;
+  d.x_st = (float)(g.ws_x / 2 - g.spacing_x * kExBorder);
+  d.y_st = (float)(g.ws_y / 2 - g.spacing_y * kExBorder);
+  d.spx = (float)g.spacing_x;
+  d.spy = (float)g.spacing_y;
+  d.oob = c->counters + 4 + gi;
+  int rc = check_launch(c, launch_zero_u64(d.oob, 1, c->stream), "zero");
+  if (rc) return rc;
+  tic(c);
+  rc = check_launch(c, launch_deform(d, c->stream), "Kernel_Window_Deformation_2d");
+  toc(c, gi, 0);
+  G.deformed_once = true;
+  return rc;
+}
+
+int run_xcorr(pivx_ctx* c, int gi, bool deformed) {
+  GridBuf& G = c->grid[gi];
+  const pivx_geometry& g = G.g;
+  XcArgs a{};
+  a.f1 = c->f1;
+  a.f2u8 = deformed ? nullptr : c->f2;
+  a.f2def = deformed ? G.deform : nullptr;
+  a.pitch1 = c->p.nx;
+  a.pitch2 = c->p.nx;
+  a.def_pitch = g.nx_aug;
+  a.nx = c->p.nx;
+  a.ny = c->p.ny;
+  a.left_x = g.left_x;
+  a.left_y = g.left_y;
+  a.nx_vec = g.nx_vec;
+  a.ny_vec = g.ny_vec;
+  a.sp_x = g.spacing_x;
+  a.sp_y = g.spacing_y;
+  a.bx = g.border_x;
+  a.by = g.border_y;
+  a.K = c->p.npeak_save;
+  // Both grids use the first-grid limit variables (cu:1440, 1755).
+  a.lx1 = c->p.dispx_lim1;
+  a.lx2 = c->p.dispx_lim2;
+  a.ly1 = c->p.dispy_lim1;
+  a.ly2 = c->p.dispy_lim2;
+  a.pred_x = G.fx;
+  a.pred_y = G.fy;
+  a.disp_x = G.dx;
+  a.disp_y = G.dy;
+  a.valid = G.dv;
+  a.pk_val = G.pkv;
+  a.pk_idx = G.pki;
+  a.ncand = G.ncand;
+  a.overflow = c->counters + 0 + gi;
+  a.slot_cap = g.npeak_slots;
+  int rc = check_launch(c, launch_zero_u64(a.overflow, 1, c->stream), "zero");
+  if (rc) return rc;
+  tic(c);
+  hipError_t e = G.fast ? launch_xcorr(a, g.ws_x, g.ws_y, nullptr, 0, nullptr, nullptr, c->stream)
+                        : launch_xcorr(a, g.ws_x, g.ws_y, G.gscratch, G.gscratch_windows, G.twx, G.twy, c->stream);
+  rc = check_launch(c, e, "correlation");
+  toc(c, gi, 1);
+  return rc;
+}
+
+// MedianFilter_Smoothing_p1/p2 (cu:1451-1545, 1766-1863) writing into (ox, oy, ov).
+int run_filter(pivx_ctx* c, int gi, int npass_median, int min_nneigh, float insert_thr, int npass_smooth, int iffill,
+               float* ox, float* oy, uint8_t* ov) {
+  GridBuf& G = c->grid[gi];
+  const pivx_geometry& g = G.g;
+  const int nt = g.nt_vec;
+  const int K = c->p.npeak_save;
+  const int wxm = gi == 0 ? c->p.wx1_median : c->p.wx2_median;
+  const int wym = gi == 0 ? c->p.wy1_median : c->p.wy2_median;
+  const int wxs = gi == 0 ? c->p.wx1_smooth : c->p.wx2_smooth;
+  const int wys = gi == 0 ? c->p.wy1_smooth : c->p.wy2_smooth;
+  int rc;
+  tic(c);
+  if (npass_median <= 0) {
+    rc = check_launch(c, launch_prep(G.dx, G.dy, G.dv, K, ox, oy, ov, nt, c->stream), "median_filter_data_prep");
+    if (rc) return rc;
+  } else {
+    rc = check_launch(c, launch_zero_u64(c->counters + 2 + gi, 1, c->stream), "zero");
+    if (rc) return rc;
+    const float* srcx = G.dx;
+    const float* srcy = G.dy;
+    const uint8_t* srcv = G.dv;
+    int sstride = K;
+    for (int p = 0; p < npass_median; ++p) {
+      const bool to_out = ((npass_median - 1 - p) % 2) == 0;
+      float* tx = to_out ? ox : G.sx;
+      float* ty = to_out ? oy : G.sy;
+      uint8_t* tv = to_out ? ov : G.sv;
+      MedArgs m{};
+      m.sx = srcx;
+      m.sy = srcy;
+      m.sv = srcv;
+      m.sstride = sstride;
+      m.cx = G.dx;
+      m.cy = G.dy;
+      m.cv = G.dv;
+      m.K = K;
+      m.dx = tx;
+      m.dy = ty;
+      m.dv = tv;
+      m.nx_vec = g.nx_vec;
+      m.ny_vec = g.ny_vec;
+      m.wx = wxm;
+      m.wy = wym;
+      m.hfwx = (wxm - 1) / 2;
+      m.hfwy = (wym - 1) / 2;
+      m.min_nneigh = min_nneigh;
+      m.thr = insert_thr;
+      m.eps = kEpsilon;
+      m.outliers = (p == npass_median - 1) ? c->counters + 2 + gi : nullptr;
+      m.scratch = G.med_scratch;
+      rc = check_launch(c, launch_median(m, c->stream), "median_filtering");
+      if (rc) return rc;
+      srcx = tx;
+      srcy = ty;
+      srcv = tv;
+      sstride = 1;
+    }
+  }
+  toc(c, gi, 2);
+  if (iffill == 1) {
+    FillArgs f{};
+    f.dx = ox;
+    f.dy = oy;
+    f.dv = ov;
+    f.nx_vec = g.nx_vec;
+    f.ny_vec = g.ny_vec;
+    f.list_a = c->list_a;
+    f.list_b = c->list_b;
+    f.tmp_x = c->tmp_x;
+    f.tmp_y = c->tmp_y;
+    f.tmp_f = c->tmp_f;
+    f.sweeps = c->sweeps + gi;
+    tic(c);
+    rc = check_launch(c, launch_fill(f, c->stream), "fill_blank_vecfield");
+    toc(c, gi, 3);
+    if (rc) return rc;
+  }
+  if (npass_smooth > 0) {
+    tic(c);
+    // ping-pong so that the last pass lands in (ox, oy)
+    float *cx = ox, *cy = oy;
+    if (npass_smooth % 2 == 1) {
+      rc = check_launch(c, launch_copy_field(ox, oy, nullptr, G.sx, G.sy, nullptr, nt, c->stream), "copy");
+      if (rc) return rc;
+      cx = G.sx;
+      cy = G.sy;
+    }
+    for (int s = 0; s < npass_smooth; ++s) {
+      float* nx_ = (cx == ox) ? G.sx : ox;
+      float* ny_ = (cy == oy) ? G.sy : oy;
+      SmoothArgs sa{};
+      sa.sx = cx;
+      sa.sy = cy;
+      sa.dx = nx_;
+      sa.dy = ny_;
+      sa.nx_vec = g.nx_vec;
+      sa.ny_vec = g.ny_vec;
+      sa.wx = wxs;
+      sa.wy = wys;
+      sa.hfwx = (wxs - 1) / 2;
+      sa.hfwy = (wys - 1) / 2;
+      rc = check_launch(c, launch_smooth(sa, c->stream), "smoothing_tophat");
+      if (rc) return rc;
+      cx = nx_;
+      cy = ny_;
+    }
+    toc(c, gi, 4);
+  }
+  return PIVX_OK;
+}
+
+int extend(pivx_ctx* c, int gi) {
+  GridBuf& G = c->grid[gi];
+  return check_launch(c, launch_extend(G.fx, G.fy, G.ex, G.ey, G.g.nx_vec, G.g.ny_vec, kExBorder, c->stream),
+                      "extend_filtered_vecfield");
+}
+
+int alloc_grid(pivx_ctx* c, int gi) {
+  GridBuf& G = c->grid[gi];
+  const pivx_geometry& g = G.g;
+  const size_t nt = (size_t)g.nt_vec, K = (size_t)c->p.npeak_save;
+  int rc = 0;
+  rc |= dalloc(c, &G.dx, nt * K);
+  rc |= dalloc(c, &G.dy, nt * K);
+  rc |= dalloc(c, &G.dv, nt * K);
+  rc |= dalloc(c, &G.pkv, nt * K);
+  rc |= dalloc(c, &G.pki, nt * K);
+  rc |= dalloc(c, &G.ncand, nt);
+  rc |= dalloc(c, &G.fx, nt);
+  rc |= dalloc(c, &G.fy, nt);
+  rc |= dalloc(c, &G.fv, nt);
+  rc |= dalloc(c, &G.sx, nt);
+  rc |= dalloc(c, &G.sy, nt);
+  rc |= dalloc(c, &G.sv, nt);
+  const size_t ne = (size_t)(g.nx_vec + 2 * kExBorder) * (g.ny_vec + 2 * kExBorder);
+  rc |= dalloc(c, &G.ex, ne);
+  rc |= dalloc(c, &G.ey, ne);
+  rc |= dalloc(c, &G.deform, (size_t)g.nx_aug * g.ny_aug);
+  if (rc) return PIVX_ERR_HIP;
+  HIPCHK(c, hipMemsetAsync(G.ex, 0, ne * sizeof(float), c->stream));
+  HIPCHK(c, hipMemsetAsync(G.ey, 0, ne * sizeof(float), c->stream));
+  HIPCHK(c, hipMemsetAsync(G.fx, 0, nt * sizeof(float), c->stream));
+  HIPCHK(c, hipMemsetAsync(G.fy, 0, nt * sizeof(float), c->stream));
+  HIPCHK(c, hipMemsetAsync(G.fv, 0, nt, c->stream));
+  HIPCHK(c, hipMemsetAsync(G.dx, 0, nt * K * sizeof(float), c->stream));
+  HIPCHK(c, hipMemsetAsync(G.dy, 0, nt * K * sizeof(float), c->stream));
+  HIPCHK(c, hipMemsetAsync(G.dv, 0, nt * K, c->stream));
+  G.fast = xcorr_fast_supported(g.ws_x, g.ws_y);
+  if (!G.fast) {
+    std::vector<float2> tx, ty;
+    host_twiddles(g.ws_x, tx);
+    host_twiddles(g.ws_y, ty);
+    if (dalloc(c, &G.twx, tx.size()) || dalloc(c, &G.twy, ty.size())) return PIVX_ERR_HIP;
+    HIPCHK(c, hipMemcpy(G.twx, tx.data(), tx.size() * sizeof(float2), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(G.twy, ty.data(), ty.size() * sizeof(float2), hipMemcpyHostToDevice));
+    const size_t plane = (size_t)g.ws_x * g.ws_y * sizeof(float2);
+    if (2 * plane > 64 * 1024) {
+      // about 1 GiB of plane scratch, at most all windows
+      size_t w = std::max<size_t>(1, (size_t(1) << 30) / (2 * plane));
+      w = std::min<size_t>(w, nt);
+      G.gscratch_windows = w;
+      if (dalloc(c, &G.gscratch, w * 2 * (size_t)g.ws_x * g.ws_y)) return PIVX_ERR_HIP;
+    }
+  }
+  const int wxm = gi == 0 ? c->p.wx1_median : c->p.wx2_median;
+  const int wym = gi == 0 ? c->p.wy1_median : c->p.wy2_median;
+  if ((long long)wxm * wym > 64) {
+    if (dalloc(c, &G.med_scratch, nt * 2 * (size_t)wxm * wym)) return PIVX_ERR_HIP;
+  }
+  return PIVX_OK;
+}
+
+}  // namespace
+
+// ====================================================================== C ABI
+
+extern "C" {
+
+const char* pivx_last_error(const pivx_ctx* ctx) { return ctx ? ctx->err.c_str() : global_error(); }
+
+int pivx_init(const pivx_init_params* p, int device_id, pivx_ctx** out) {
+  if (!p || !out) return fail(nullptr, PIVX_ERR_PARAM, "null argument");
+  *out = nullptr;
+  int rc = validate(p);
+  if (rc) return rc;
+  pivx_geometry g1 = make_geometry(p->nx, p->ny, p->nsec_cc, p->ws_x1, p->ws_y1, p->spacing_x1, p->spacing_y1,
+                                   p->max_dispx_percent1, p->max_dispy_percent1);
+  pivx_geometry g2 = make_geometry(p->nx, p->ny, p->nsec_cc, p->ws_x2, p->ws_y2, p->spacing_x2, p->spacing_y2,
+                                   p->max_dispx_percent2, p->max_dispy_percent2);
+  if (g1.nx_vec < 1 || g1.ny_vec < 1 || g2.nx_vec < 1 || g2.ny_vec < 1)
+    return fail(nullptr, PIVX_ERR_PARAM, "image %dx%d holds no vector at the given spacing", p->nx, p->ny);
+  int ndev = 0;
+  hipError_t e = hipGetDeviceCount(&ndev);
+  if (e != hipSuccess || ndev < 1) return fail(nullptr, PIVX_ERR_HIP, "No GPU on this machine!");
+  if (device_id < 0 || device_id >= ndev)
+    return fail(nullptr, PIVX_ERR_PARAM, "device id %d out of range (%d devices)", device_id, ndev);
+  e = hipSetDevice(device_id);
+  if (e != hipSuccess) return fail(nullptr, PIVX_ERR_HIP, "hipSetDevice: %s", hipGetErrorString(e));
+  pivx_ctx* c = new pivx_ctx();
+  c->p = *p;
+  c->device = device_id;
+  c->grid[0].g = g1;
+  c->grid[1].g = g2;
+  e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete c;
+    return fail(nullptr, PIVX_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
+  }
+  const size_t npix = (size_t)p->nx * p->ny;
+  const size_t ntmax = (size_t)std::max(g1.nt_vec, g2.nt_vec);
+  rc = 0;
+  rc |= dalloc(c, &c->f1, npix);
+  rc |= dalloc(c, &c->f2, npix);
+  rc |= dalloc(c, &c->counters, 8);
+  rc |= dalloc(c, &c->sweeps, 2);
+  rc |= dalloc(c, &c->list_a, ntmax);
+  rc |= dalloc(c, &c->list_b, ntmax);
+  rc |= dalloc(c, &c->tmp_x, ntmax);
+  rc |= dalloc(c, &c->tmp_y, ntmax);
+  rc |= dalloc(c, &c->tmp_f, ntmax);
+  rc |= dalloc(c, &c->mo_x, (size_t)g2.nt_vec);
+  rc |= dalloc(c, &c->mo_y, (size_t)g2.nt_vec);
+  rc |= dalloc(c, &c->mo_v, (size_t)g2.nt_vec);
+  if (rc == 0) rc = alloc_grid(c, 0);
+  if (rc == 0) rc = alloc_grid(c, 1);
+  if (rc == 0 && hipMemsetAsync(c->counters, 0, 8 * sizeof(unsigned long long), c->stream) != hipSuccess) rc = PIVX_ERR_HIP;
+  if (rc == 0 && hipMemsetAsync(c->sweeps, 0, 2 * sizeof(int), c->stream) != hipSuccess) rc = PIVX_ERR_HIP;
+  if (rc == 0 && hipStreamSynchronize(c->stream) != hipSuccess) rc = fail(c, PIVX_ERR_HIP, "init synchronisation failed");
+  if (rc) {
+    std::string msg = c->err;
+    pivx_freemem(c);
+    set_global_error(msg.c_str());
+    return rc;
+  }
+  *out = c;
+  return PIVX_OK;
+}
+
+int pivx_freemem(pivx_ctx* c) {
+  if (!c) return PIVX_OK;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (void* q : c->allocs) (void)hipFree(q);
+  for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return PIVX_OK;
+}
+
+int pivx_get_geometry(const pivx_ctx* c, int grid, pivx_geometry* g) {
+  if (!c || !g || (grid != 1 && grid != 2)) return fail(nullptr, PIVX_ERR_PARAM, "bad argument");
+  *g = c->grid[grid - 1].g;
+  return PIVX_OK;
+}
+
+static int upload(pivx_ctx* c, const uint8_t* a, const uint8_t* b, size_t pitch, hipMemcpyKind kind) {
+  if (!c || !a || !b) return fail(c, PIVX_ERR_PARAM, "null frame");
+  if (pitch < (size_t)c->p.nx) return fail(c, PIVX_ERR_PARAM, "pitch %zu < nx %d", pitch, c->p.nx);
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipMemcpy2DAsync(c->f1, c->p.nx, a, pitch, c->p.nx, c->p.ny, kind, c->stream));
+  HIPCHK(c, hipMemcpy2DAsync(c->f2, c->p.nx, b, pitch, c->p.nx, c->p.ny, kind, c->stream));
+  if (kind == hipMemcpyHostToDevice) HIPCHK(c, hipStreamSynchronize(c->stream));  // caller may reuse its buffers
+  c->have_frames = true;
+  return PIVX_OK;
+}
+
+int pivx_prep_img_data(pivx_ctx* c, const uint8_t* img1, const uint8_t* img2, size_t pitch) {
+  return upload(c, img1, img2, pitch, hipMemcpyHostToDevice);
+}
+
+int pivx_prep_img_data_device(pivx_ctx* c, const uint8_t* d1, const uint8_t* d2, size_t pitch) {
+  return upload(c, d1, d2, pitch, hipMemcpyDeviceToDevice);
+}
+
+int pivx_calc_cc_map_p1(pivx_ctx* c, int idx_pass) {
+  if (!c) return fail(nullptr, PIVX_ERR_PARAM, "null handle");
+  if (!c->have_frames) return fail(c, PIVX_ERR_STATE, "no frames: call pivx_prep_img_data first");
+  if (idx_pass < 0) return fail(c, PIVX_ERR_PARAM, "idx_pass must be >= 0");
+  GridBuf& G = c->grid[0];
+  int rc;
+  if (idx_pass == 0) {
+    // set_float_zero_kernel on the grid-1 predictor (cu:1287-1298)
+    rc = check_launch(c, launch_zero_f32(G.fx, G.g.nt_vec, c->stream), "set_float_zero_kernel");
+    if (!rc) rc = check_launch(c, launch_zero_f32(G.fy, G.g.nt_vec, c->stream), "set_float_zero_kernel");
+    if (rc) return rc;
+    return run_xcorr(c, 0, false);
+  }
+  rc = run_deform(c, 0);
+  if (rc) return rc;
+  return run_xcorr(c, 0, true);
+}
+
+int pivx_calc_cc_map_p2(pivx_ctx* c) {
+  if (!c) return fail(nullptr, PIVX_ERR_PARAM, "null handle");
+  if (!c->have_frames) return fail(c, PIVX_ERR_STATE, "no frames: call pivx_prep_img_data first");
+  int rc = run_deform(c, 1);
+  if (rc) return rc;
+  return run_xcorr(c, 1, true);
+}
+
+int pivx_median_filter_smoothing_p1(pivx_ctx* c, int npass_median, int min_nneigh, float residual_thr,
+                                    float insert_thr, int npass_smooth) {
+  (void)residual_thr;  // read but never used by the reference
+  if (!c) return fail(nullptr, PIVX_ERR_PARAM, "null handle");
+  GridBuf& G = c->grid[0];
+  return run_filter(c, 0, npass_median, min_nneigh, insert_thr, npass_smooth, 1, G.fx, G.fy, G.fv);
+}
+
+int pivx_median_filter_smoothing_p2(pivx_ctx* c, int npass_median, int min_nneigh, float residual_thr,
+                                    float insert_thr, int npass_smooth, int iffill) {
+  (void)residual_thr;
+  if (!c) return fail(nullptr, PIVX_ERR_PARAM, "null handle");
+  GridBuf& G = c->grid[1];
+  return run_filter(c, 1, npass_median, min_nneigh, insert_thr, npass_smooth, iffill, G.fx, G.fy, G.fv);
+}
+
+int pivx_extend_filtered_vecfield_p1(pivx_ctx* c) {
+  if (!c) return fail(nullptr, PIVX_ERR_PARAM, "null handle");
+  return extend(c, 0);
+}
+
+int pivx_extend_filtered_vecfield_p2(pivx_ctx* c) {
+  if (!c) return fail(nullptr, PIVX_ERR_PARAM, "null handle");
+  return extend(c, 1);
+}
+
+int pivx_upsampling_vector_field(pivx_ctx* c) {
+  if (!c) return fail(nullptr, PIVX_ERR_PARAM, "null handle");
+  GridBuf& G1 = c->grid[0];
+  GridBuf& G2 = c->grid[1];
+  const pivx_geometry& g1 = G1.g;
+  const pivx_geometry& g2 = G2.g;
+  return check_launch(
+      c,
+      launch_upsample(G2.fx, G2.fy, G1.ex, G1.ey, g1.nx_vec + 2 * kExBorder, g1.ny_vec + 2 * kExBorder,
+                      (float)(g1.spacing_x / 2 - kExBorder * g1.spacing_x),
+                      (float)(g1.spacing_y / 2 - kExBorder * g1.spacing_y), (float)g1.spacing_x, (float)g1.spacing_y,
+                      g2.spacing_x, g2.spacing_y, g2.nx_vec, g2.ny_vec, c->stream),
+      "Upsampling_vecfield_kernel_2d");
+}
+
+int pivx_run_pair(pivx_ctx* c, const pivx_filter_params* f1, const pivx_filter_params* f2) {
+  if (!c || !f1 || !f2) return fail(c, PIVX_ERR_PARAM, "null argument");
+  if (!c->have_frames) return fail(c, PIVX_ERR_STATE, "no frames: call pivx_prep_img_data first");
+  HIPCHK(c, hipSetDevice(c->device));
+  int rc;
+  for (int j = 0; j < c->p.npass1; ++j) {
+    if (j > 0 && (rc = extend(c, 0))) return rc;
+    if ((rc = pivx_calc_cc_map_p1(c, j))) return rc;
+    if ((rc = run_filter(c, 0, f1->npass_median, f1->min_nneigh, f1->insert_thr, f1->npass_smooth, 1, c->grid[0].fx,
+                         c->grid[0].fy, c->grid[0].fv)))
+      return rc;
+  }
+  if (c->p.npass2 > 0) {
+    if ((rc = extend(c, 0))) return rc;
+    if ((rc = pivx_upsampling_vector_field(c))) return rc;
+    for (int j = 0; j < c->p.npass2; ++j) {
+      if ((rc = extend(c, 1))) return rc;
+      if ((rc = pivx_calc_cc_map_p2(c))) return rc;
+      if ((rc = run_filter(c, 1, f2->npass_median, f2->min_nneigh, f2->insert_thr, f2->npass_smooth, 1, c->grid[1].fx,
+                           c->grid[1].fy, c->grid[1].fv)))
+        return rc;
+    }
+    // median-only re-run on the last raw candidates (main.cpp:363-375)
+    if ((rc = run_filter(c, 1, f2->npass_median, f2->min_nneigh, f2->insert_thr, 0, 0, c->mo_x, c->mo_y, c->mo_v)))
+      return rc;
+  }
+  return PIVX_OK;
+}
+
+int pivx_synchronize(pivx_ctx* c) {
+  if (!c) return fail(nullptr, PIVX_ERR_PARAM, "null handle");
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return PIVX_OK;
+}
+
+int pivx_get_dims(const pivx_ctx* c, int grid, int* nx_vec, int* ny_vec) {
+  if (!c || (grid != 1 && grid != 2)) return fail(nullptr, PIVX_ERR_PARAM, "bad argument");
+  if (nx_vec) *nx_vec = c->grid[grid - 1].g.nx_vec;
+  if (ny_vec) *ny_vec = c->grid[grid - 1].g.ny_vec;
+  return PIVX_OK;
+}
+
+int pivx_get_grid_coords(const pivx_ctx* c, int grid, float* x, float* y) {
+  if (!c || (grid != 1 && grid != 2)) return fail(nullptr, PIVX_ERR_PARAM, "bad argument");
+  const pivx_geometry& g = c->grid[grid - 1].g;
+  for (int j = 0; j < g.ny_vec; ++j)
+    for (int k = 0; k < g.nx_vec; ++k) {
+      if (x) x[j * g.nx_vec + k] = (float)(g.spacing_x / 2 + k * g.spacing_x);
+      if (y) y[j * g.nx_vec + k] = (float)(g.spacing_y / 2 + j * g.spacing_y);
+    }
+  return PIVX_OK;
+}
+
+static int d2h(pivx_ctx* c, void* dst, const void* src, size_t bytes) {
+  if (!dst) return PIVX_OK;
+  HIPCHK(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+  return PIVX_OK;
+}
+static int h2d(pivx_ctx* c, void* dst, const void* src, size_t bytes) {
+  HIPCHK(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+  return PIVX_OK;
+}
+
+int pivx_get_raw(pivx_ctx* c, int grid, float* dx, float* dy, uint8_t* valid) {
+  GridBuf* G = grid_of(c, grid);
+  if (!G) return fail(c, PIVX_ERR_PARAM, "bad argument");
+  const size_t n = (size_t)G->g.nt_vec * c->p.npeak_save;
+  int rc = d2h(c, dx, G->dx, n * 4) | d2h(c, dy, G->dy, n * 4) | d2h(c, valid, G->dv, n);
+  if (rc) return rc;
+  return pivx_synchronize(c);
+}
+
+int pivx_get_filtered(pivx_ctx* c, int grid, float* dx, float* dy, uint8_t* valid) {
+  if (c && grid == 3) {  // grid-2 median-only field of pivx_run_pair
+    const size_t n = (size_t)c->grid[1].g.nt_vec;
+    int rc = d2h(c, dx, c->mo_x, n * 4) | d2h(c, dy, c->mo_y, n * 4) | d2h(c, valid, c->mo_v, n);
+    if (rc) return rc;
+    return pivx_synchronize(c);
+  }
+  GridBuf* G = grid_of(c, grid);
+  if (!G) return fail(c, PIVX_ERR_PARAM, "bad argument");
+  const size_t n = (size_t)G->g.nt_vec;
+  int rc = d2h(c, dx, G->fx, n * 4) | d2h(c, dy, G->fy, n * 4) | d2h(c, valid, G->fv, n);
+  if (rc) return rc;
+  return pivx_synchronize(c);
+}
+
+int pivx_get_peaks(pivx_ctx* c, int grid, float* value, int* index, int* ncand) {
+  GridBuf* G = grid_of(c, grid);
+  if (!G) return fail(c, PIVX_ERR_PARAM, "bad argument");
+  const size_t n = (size_t)G->g.nt_vec * c->p.npeak_save;
+  int rc = d2h(c, value, G->pkv, n * 4) | d2h(c, index, G->pki, n * 4) | d2h(c, ncand, G->ncand, G->g.nt_vec * 4);
+  if (rc) return rc;
+  return pivx_synchronize(c);
+}
+
+int pivx_get_deformed(pivx_ctx* c, int grid, float* out) {
+  GridBuf* G = grid_of(c, grid);
+  if (!G || !out) return fail(c, PIVX_ERR_PARAM, "bad argument");
+  int rc = d2h(c, out, G->deform, (size_t)G->g.nx_aug * G->g.ny_aug * 4);
+  if (rc) return rc;
+  return pivx_synchronize(c);
+}
+
+int pivx_get_stats(pivx_ctx* c, pivx_stats* st) {
+  if (!c || !st) return fail(c, PIVX_ERR_PARAM, "bad argument");
+  unsigned long long h[8];
+  int sw[2];
+  int rc = d2h(c, h, c->counters, sizeof(h)) | d2h(c, sw, c->sweeps, sizeof(sw));
+  if (rc) return rc;
+  rc = pivx_synchronize(c);
+  if (rc) return rc;
+  for (int g = 0; g < 2; ++g) {
+    st->overflow_windows[g] = (long long)h[0 + g];
+    st->outliers[g] = (long long)h[2 + g];
+    st->oob_taps[g] = (long long)h[4 + g];
+    st->fill_sweeps[g] = sw[g];
+  }
+  return PIVX_OK;
+}
+
+int pivx_set_filtered(pivx_ctx* c, int grid, const float* dx, const float* dy, const uint8_t* valid) {
+  GridBuf* G = grid_of(c, grid);
+  if (!G || !dx || !dy) return fail(c, PIVX_ERR_PARAM, "bad argument");
+  const size_t n = (size_t)G->g.nt_vec;
+  int rc = h2d(c, G->fx, dx, n * 4) | h2d(c, G->fy, dy, n * 4);
+  if (rc) return rc;
+  if (valid) rc = h2d(c, G->fv, valid, n);
+  else HIPCHK(c, hipMemsetAsync(G->fv, 1, n, c->stream));
+  if (rc) return rc;
+  return pivx_synchronize(c);
+}
+
+int pivx_set_raw(pivx_ctx* c, int grid, const float* dx, const float* dy, const uint8_t* valid) {
+  GridBuf* G = grid_of(c, grid);
+  if (!G || !dx || !dy) return fail(c, PIVX_ERR_PARAM, "bad argument");
+  const size_t n = (size_t)G->g.nt_vec * c->p.npeak_save;
+  int rc = h2d(c, G->dx, dx, n * 4) | h2d(c, G->dy, dy, n * 4);
+  if (rc) return rc;
+  if (valid) rc = h2d(c, G->dv, valid, n);
+  else HIPCHK(c, hipMemsetAsync(G->dv, 1, n, c->stream));
+  if (rc) return rc;
+  return pivx_synchronize(c);
+}
+
+int pivx_enable_timing(pivx_ctx* c, int enable) {
+  if (!c) return fail(nullptr, PIVX_ERR_PARAM, "null handle");
+  c->timing = enable != 0;
+  c->ev_used = 0;
+  c->ev_start = -1;
+  c->recs.clear();
+  return PIVX_OK;
+}
+
+int pivx_get_stage_ms(pivx_ctx* c, int grid, int stage, double* ms, int* launches) {
+  if (!c || (grid != 1 && grid != 2) || stage < 0 || stage > 4) return fail(c, PIVX_ERR_PARAM, "bad argument");
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  double sum = 0.0;
+  int n = 0;
+  for (const auto& r : c->recs) {
+    if (r.gi != grid - 1 || r.stage != stage) continue;
+    float t = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&t, c->ev_pool[r.a], c->ev_pool[r.b]));
+    sum += t;
+    ++n;
+  }
+  if (ms) *ms = n ? sum / n : 0.0;
+  if (launches) *launches = n;
+  return PIVX_OK;
+}
+
+int pivx_write_vec_data_general(pivx_ctx* c, const char* path, int selection, int crlf) {
+  if (!c || !path) return fail(c, PIVX_ERR_PARAM, "bad argument");
+  const int grid = (selection == 0 || selection == 1) ? 1 : 2;
+  const pivx_geometry& g = c->grid[grid - 1].g;
+  const int nt = g.nt_vec, K = c->p.npeak_save;
+  std::vector<float> gx(nt), gy(nt);
+  pivx_get_grid_coords(c, grid, gx.data(), gy.data());
+  int rc;
+  if (selection == 0 || selection == 2) {
+    std::vector<float> dx((size_t)nt * K), dy((size_t)nt * K);
+    if ((rc = pivx_get_raw(c, grid, dx.data(), dy.data(), nullptr))) return rc;
+    rc = write_tecplot(path, g.nx_vec, g.ny_vec, gx.data(), gy.data(), dx.data(), dy.data(), K, true, crlf != 0);
+  } else if (selection == 1 || selection == 3 || selection == 4) {
+    std::vector<float> dx(nt), dy(nt);
+    if ((rc = pivx_get_filtered(c, selection == 4 ? 3 : grid, dx.data(), dy.data(), nullptr))) return rc;
+    rc = write_tecplot(path, g.nx_vec, g.ny_vec, gx.data(), gy.data(), dx.data(), dy.data(), 1, false, crlf != 0);
+  } else {
+    return fail(c, PIVX_ERR_PARAM, "selection must be 0..4");
+  }
+  if (rc) return fail(c, PIVX_ERR_IO, "Error occurs while the vector file is being written: %s", path);
+  return PIVX_OK;
+}
+
+}  // extern "C"

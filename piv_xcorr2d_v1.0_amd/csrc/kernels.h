@@ -1,0 +1,82 @@
+// Device-kernel argument blocks and launchers of the PIV engine (gfx950).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pivx {
+
+// Fused correlate-and-peak pass over all windows of one grid.
+struct XcArgs {
+  const uint8_t* f1;      // frame 1, image coordinates, row pitch pitch1
+  const uint8_t* f2u8;    // frame 2 (undeformed pass) or nullptr
+  const float* f2def;     // deformed frame 2 in padded ("aug") coordinates or nullptr
+  long long pitch1, pitch2;
+  int def_pitch;          // row length of f2def (nx_aug)
+  int nx, ny;             // image size
+  int left_x, left_y;     // aug coordinate = image coordinate + left
+  int nx_vec, ny_vec, sp_x, sp_y;
+  int bx, by, K;          // search border, Npeak_save
+  float lx1, lx2, ly1, ly2;
+  const float* pred_x;    // predictor (filtered field), nullptr = 0
+  const float* pred_y;
+  float* disp_x; float* disp_y; uint8_t* valid;   // [nt_vec][K]
+  float* pk_val; int* pk_idx;                     // [nt_vec][K]
+  int* ncand;                                     // [nt_vec]
+  unsigned long long* overflow;                   // windows with ncand > slot_cap
+  int slot_cap;
+  int chunks_x;           // blocks per vector row (fast path)
+};
+
+struct DefArgs {
+  float* out; int nx_aug, ny_aug;
+  const uint8_t* f2; long long pitch; int nx, ny; int left_x, left_y;
+  int hfws_x, hfws_y;     // only for counting taps outside the reference's padded buffer
+  const float* ex_x; const float* ex_y; int nxe, nye;
+  float x_st, y_st, spx, spy;
+  unsigned long long* oob;
+};
+
+struct MedArgs {
+  const float* sx; const float* sy; const uint8_t* sv; int sstride;   // neighbourhood field
+  const float* cx; const float* cy; const uint8_t* cv; int K;         // raw candidates
+  float* dx; float* dy; uint8_t* dv;                                  // output filtered field
+  int nx_vec, ny_vec, wx, wy, hfwx, hfwy, min_nneigh;
+  float thr, eps;
+  unsigned long long* outliers;
+  float* scratch;          // generic path only: 4*wx*wy floats per vector
+};
+
+struct FillArgs {
+  float* dx; float* dy; uint8_t* dv;
+  int nx_vec, ny_vec;
+  int* list_a; int* list_b; float* tmp_x; float* tmp_y; uint8_t* tmp_f;
+  int* sweeps;
+};
+
+struct SmoothArgs {
+  const float* sx; const float* sy; float* dx; float* dy;
+  int nx_vec, ny_vec, wx, wy, hfwx, hfwy;
+};
+
+// Launchers (return hipGetLastError() of the launch).
+hipError_t launch_xcorr(const XcArgs& a, int ws_x, int ws_y, float2* gscratch, size_t gscratch_windows,
+                        const float2* twx, const float2* twy, hipStream_t s);
+bool xcorr_fast_supported(int ws_x, int ws_y);
+hipError_t launch_deform(const DefArgs& a, hipStream_t s);
+hipError_t launch_extend(const float* fx, const float* fy, float* ex, float* ey, int nx, int ny, int border,
+                         hipStream_t s);
+hipError_t launch_upsample(float* dx2, float* dy2, const float* ex1x, const float* ex1y, int nxe1, int nye1,
+                           float x_st1, float y_st1, float sp1x, float sp1y, int sp2x, int sp2y, int nx2, int ny2,
+                           hipStream_t s);
+hipError_t launch_median(const MedArgs& a, hipStream_t s);
+hipError_t launch_fill(const FillArgs& a, hipStream_t s);
+hipError_t launch_smooth(const SmoothArgs& a, hipStream_t s);
+hipError_t launch_zero_f32(float* p, size_t n, hipStream_t s);
+// filter := candidate 0 (median_filter_data_prep, cu:484-495); copy of a field.
+hipError_t launch_prep(const float* cx, const float* cy, const uint8_t* cv, int K, float* dx, float* dy, uint8_t* dv,
+                       int nt, hipStream_t s);
+hipError_t launch_copy_field(const float* sx, const float* sy, const uint8_t* sv, float* dx, float* dy, uint8_t* dv,
+                             int nt, hipStream_t s);
+hipError_t launch_zero_u64(unsigned long long* p, size_t n, hipStream_t s);
+
+}  // namespace pivx

@@ -1,0 +1,986 @@
+// Hand-written gfx950 kernels of the multi-grid PIV pipeline.
+//
+// Reference stages and their replacements (xcorr2d_V1.0/xcorr2d_multi_grid.cu):
+//   organize_data_fft2d_multi(_deformed) cu:497-517/654-673, the three batched
+//   cuFFT C2C transforms cu:1334-1350, multiply_conjugate cu:519-534,
+//   complex2amp_xcorr2d cu:536-545, fftshift2d_xcorr2d_row/col cu:547-583,
+//   find_local_max_2d cu:585-606, reduce_max_index_general/final cu:4-172,
+//   replace_local_max_2d cu:298-307, peak_subpixel_4peak_2d cu:309-350 and
+//   kernel_remove_nan cu:457-471
+//     -> k_xcorr_fast<WX,WY,WPB> (one fused launch per pass, windows resident in
+//        LDS, register FFTs) and k_xcorr_generic (any legal size, LDS or global
+//        planes).
+//   Kernel_Window_Deformation_2d cu:608-652           -> k_deform
+//   extend_filtered_vecfield_p1/p2 cu:1547-1618       -> k_extend
+//   Upsampling_vecfield_kernel_2d cu:675-704          -> k_upsample
+//   median_filter_data_prep, organize_median_filter_data, median_validity,
+//   Calc_median, Calc_devi, median_filtering cu:174-296,484-495 -> k_median
+//   fill_blank_vecfield(_update) + host loop cu:352-397,1514-1528 -> k_fill
+//   organize_smooth_filter_data + smoothing_tophat cu:399-443 -> k_smooth
+#include "kernels.h"
+#include "fft_reg.h"
+#include <math.h>
+
+namespace pivx {
+
+// ============================================================== shared helpers
+
+// 4 * conj(F1[k]) * F2[k] from the packed spectrum Z = FFT(f1 + i*f2):
+// F1 = (Z[k] + conj(Z[-k]))/2, F2 = (Z[k] - conj(Z[-k]))/(2i).
+// The reference's product order is conj(F1)*F2 (cu:530-531).
+__device__ __forceinline__ float2 xprod(float2 A, float2 Am) {
+  float f1r = A.x + Am.x, f1i = A.y - Am.y;   // 2*F1
+  float f2r = A.y + Am.y, f2i = Am.x - A.x;   // 2*F2
+  return make_float2(f1r * f2r + f1i * f2i, f1r * f2i - f1i * f2r);
+}
+
+// Order used by the top-N rounds: larger value first, ties to the lower
+// row-major pixel index (the reference keeps the lower slot on ties, cu:7,51;
+// its slots are filled in atomic order, here fixed to raster order).
+__device__ __forceinline__ bool peak_before(float v, int i, float bv, int bi) {
+  return v > bv || (v == bv && i < bi);
+}
+
+__device__ __forceinline__ void wave_argmax(float& bv, int& bi) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    float ov = __shfl_xor(bv, off, 64);
+    int oi = __shfl_xor(bi, off, 64);
+    if (peak_before(ov, oi, bv, bi)) { bv = ov; bi = oi; }
+  }
+}
+
+__device__ __forceinline__ int wave_sum(int v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// 3-point Gaussian fit of one peak (cu:319-347) plus the NaN/Inf removal of
+// kernel_remove_nan (cu:457-471).  cc is the fftshifted plane with row stride
+// sx; (i, j) the peak pixel.  Returns validity; writes the displacement.
+__device__ __forceinline__ bool subpixel_fit(const float* cc, int sx, int wsx, int wsy, int i, int j, float v,
+                                             float px, float py, float lx1, float lx2, float ly1, float ly2,
+                                             float& dx, float& dy) {
+#pragma clang fp contract(off)
+  dx = 0.f;
+  dy = 0.f;
+  if (i == 0 || i == wsx - 1 || j == 0 || j == wsy - 1 || !(v > 0.f)) return false;
+  float f1 = logf(v);
+  float f0 = logf(cc[j * sx + i - 1]);
+  float f2 = logf(cc[j * sx + i + 1]);
+  float x = (float)i + (f0 - f2) / (f0 + f2 - 2 * f1) / 2 - (float)wsx / 2 + px;
+  f0 = logf(cc[(j - 1) * sx + i]);
+  f2 = logf(cc[(j + 1) * sx + i]);
+  float y = (float)j + (f0 - f2) / (f0 + f2 - 2 * f1) / 2 - (float)wsy / 2 + py;
+  if (x < lx1 || x > lx2 || y < ly1 || y > ly2) return false;
+  if (isnan(x) || isnan(y) || isinf(x) || isinf(y)) return false;
+  dx = x;
+  dy = y;
+  return true;
+}
+
+// Gather of one window pixel: frame-1 value and frame-2 value at aug
+// coordinates (ax, ay).  Image pixels outside the frame read as 0 (the
+// reference's zero-padded aug_ex buffers, cu:951-954, 1271-1282).
+__device__ __forceinline__ float2 gather_px(const XcArgs& a, int ax, int ay) {
+  int xi = ax - a.left_x, yi = ay - a.left_y;
+  bool in = (unsigned)xi < (unsigned)a.nx && (unsigned)yi < (unsigned)a.ny;
+  float v1 = 0.f, v2 = 0.f;
+  if (in) v1 = (float)a.f1[(long long)yi * a.pitch1 + xi];
+  if (a.f2def) v2 = a.f2def[(long long)ay * a.def_pitch + ax];
+  else if (in) v2 = (float)a.f2u8[(long long)yi * a.pitch2 + xi];
+  return make_float2(v1, v2);
+}
+
+// ============================================================== fast fused kernel
+//
+// One workgroup = WPB horizontally consecutive windows of one vector row.
+// LDS holds one padded complex plane per window (row stride WX+1 so that both
+// row-per-lane and column-per-lane accesses are bank-conflict free).
+//   1 gather  z = f1 + i*f2                       (coalesced along window rows)
+//   2 row FFTs, one row per thread, in registers
+//   3 column FFTs, one column per thread
+//   4 Hermitian split -> conj(F1)*F2 for windows a=2p, b=2p+1, packed as
+//     Q = Pa + i*Pb, inverse column FFTs
+//   5 inverse row FFTs: Re -> cc_a, Im -> cc_b, scaled by 1/N and written
+//     fftshifted as float planes (into window b's LDS plane)
+//   6 per window, one wave: 4-neighbour local maxima inside the search border,
+//     Npeak_save rounds of wave64 arg-max, Gaussian fit of each peak.
+template <int WX, int WY, int WPB>
+struct FastCfg {
+  static constexpr int LPW = WX > WY ? WX : WY;
+  static constexpr int NT = ((WPB * LPW + 63) / 64) * 64;
+  static constexpr int SX = WX + 1;
+};
+
+template <int WX, int WY, int WPB>
+__global__ void __launch_bounds__((FastCfg<WX, WY, WPB>::NT)) k_xcorr_fast(XcArgs a) {
+  static_assert(WPB % 2 == 0, "windows are processed in pairs");
+  constexpr int NT = FastCfg<WX, WY, WPB>::NT;
+  constexpr int SX = FastCfg<WX, WY, WPB>::SX;
+  constexpr int NPIX = WX * WY;
+  __shared__ float2 pl[WPB][WY][SX];
+
+  const int tid = threadIdx.x;
+  const int iy = blockIdx.x / a.chunks_x;
+  const int ix0 = (blockIdx.x - iy * a.chunks_x) * WPB;
+
+  // 1. gather
+  for (int e = tid; e < WPB * NPIX; e += NT) {
+    const int wl = e / NPIX;
+    const int rem = e - wl * NPIX;
+    const int r = rem / WX, c = rem - (rem / WX) * WX;
+    const int ix = ix0 + wl;
+    float2 z = make_float2(0.f, 0.f);
+    if (ix < a.nx_vec) z = gather_px(a, ix * a.sp_x + c, iy * a.sp_y + r);
+    pl[wl][r][c] = z;
+  }
+  __syncthreads();
+
+  // 2. forward row FFTs
+  if (tid < WPB * WY) {
+    const int wl = tid / WY, r = tid - (tid / WY) * WY;
+    float2 x[WX];
+    static_for<WX>([&](auto c) { x[c] = pl[wl][r][c]; });
+    fft_reg<WX, false>(x);
+    static_for<WX>([&](auto c) { pl[wl][r][c] = x[c]; });
+  }
+  __syncthreads();
+
+  // 3. forward column FFTs
+  if (tid < WPB * WX) {
+    const int wl = tid / WX, c = tid - (tid / WX) * WX;
+    float2 x[WY];
+    static_for<WY>([&](auto r) { x[r] = pl[wl][r][c]; });
+    fft_reg<WY, false>(x);
+    static_for<WY>([&](auto r) { pl[wl][r][c] = x[r]; });
+  }
+  __syncthreads();
+
+  // 4. split + conj-multiply + pair packing, inverse column FFTs
+  {
+    const bool act = tid < (WPB / 2) * WX;
+    const int p = tid / WX, c = tid - (tid / WX) * WX;
+    const int cm = (WX - c) % WX;
+    float2 q[WY];
+    if (act) {
+      static_for<WY>([&](auto rc) {
+        constexpr int r = rc;
+        constexpr int rm = (WY - r) % WY;
+        float2 pa = xprod(pl[2 * p][r][c], pl[2 * p][rm][cm]);
+        float2 pb = xprod(pl[2 * p + 1][r][c], pl[2 * p + 1][rm][cm]);
+        q[r] = make_float2(pa.x - pb.y, pa.y + pb.x);
+      });
+    }
+    __syncthreads();
+    if (act) {
+      fft_reg<WY, true>(q);
+      static_for<WY>([&](auto r) { pl[2 * p][r][c] = q[r]; });
+    }
+  }
+  __syncthreads();
+
+  // 5. inverse row FFTs -> shifted correlation planes
+  if (tid < (WPB / 2) * WY) {
+    const int p = tid / WY, r = tid - (tid / WY) * WY;
+    float2 x[WX];
+    static_for<WX>([&](auto c) { x[c] = pl[2 * p][r][c]; });
+    fft_reg<WX, true>(x);
+    float* cca = reinterpret_cast<float*>(&pl[2 * p + 1][0][0]);
+    float* ccb = cca + WY * SX;
+    const int rs = (r + WY / 2) % WY;
+    static_for<WX>([&](auto cc) {
+      constexpr int c = cc;
+      constexpr int cs = (c + WX / 2) % WX;
+      // |IFFT(4 conj(F1) F2)| * 0.25 / N == |IFFT(conj(F1) F2)| / N  (cu:542)
+      cca[rs * SX + cs] = (fabsf(x[c].x) * 0.25f) / (float)NPIX;
+      ccb[rs * SX + cs] = (fabsf(x[c].y) * 0.25f) / (float)NPIX;
+    });
+  }
+  __syncthreads();
+
+  // 6. peaks, one wave per window
+  constexpr int NW = NT / 64;
+  constexpr int PPL = (NPIX + 63) / 64;
+  const int lane = tid & 63, wave = tid >> 6;
+  for (int wl = wave; wl < WPB; wl += NW) {
+    const int ix = ix0 + wl;
+    if (ix >= a.nx_vec) break;
+    const float* cc = reinterpret_cast<const float*>(&pl[wl | 1][0][0]) + (wl & 1) * WY * SX;
+    const int w = iy * a.nx_vec + ix;
+    float v[PPL];
+    int cnt = 0;
+    static_for<PPL>([&](auto mc) {
+      constexpr int m = mc;
+      const int pidx = lane + 64 * m;
+      const int i = pidx % WX, j = pidx / WX;
+      float val = 0.f;
+      if (pidx < NPIX && i >= a.bx && i <= WX - 1 - a.bx && j >= a.by && j <= WY - 1 - a.by) {
+        const float c0 = cc[j * SX + i];
+        if (c0 > 0.f && c0 >= cc[j * SX + i - 1] && c0 >= cc[j * SX + i + 1] && c0 >= cc[(j - 1) * SX + i] &&
+            c0 >= cc[(j + 1) * SX + i]) {
+          val = c0;
+          ++cnt;
+        }
+      }
+      v[m] = val;
+    });
+    cnt = wave_sum(cnt);
+    const float px = a.pred_x ? a.pred_x[w] : 0.f;
+    const float py = a.pred_y ? a.pred_y[w] : 0.f;
+    if (lane == 0) {
+      a.ncand[w] = cnt;
+      if (cnt > a.slot_cap) atomicAdd(a.overflow, 1ull);
+    }
+    float pv = INFINITY;
+    int pi = -1;
+    for (int k = 0; k < a.K; ++k) {
+      float bv = 0.f;
+      int bi = 0x7fffffff;
+      static_for<PPL>([&](auto mc) {
+        constexpr int m = mc;
+        const int pidx = lane + 64 * m;
+        const float val = v[m];
+        const bool after = val < pv || (val == pv && pidx > pi);
+        if (val > 0.f && after && peak_before(val, pidx, bv, bi)) {
+          bv = val;
+          bi = pidx;
+        }
+      });
+      wave_argmax(bv, bi);
+      pv = bv;
+      pi = bi;
+      if (lane == 0) {
+        const long long o = (long long)w * a.K + k;
+        float dx = 0.f, dy = 0.f;
+        bool ok = false;
+        if (bv > 0.f) {
+          const int i = bi % WX, j = bi / WX;
+          ok = subpixel_fit(cc, SX, WX, WY, i, j, bv, px, py, a.lx1, a.lx2, a.ly1, a.ly2, dx, dy);
+        }
+        a.disp_x[o] = dx;
+        a.disp_y[o] = dy;
+        a.valid[o] = ok ? 1 : 0;
+        a.pk_val[o] = bv;
+        a.pk_idx[o] = bv > 0.f ? bi : -1;
+      }
+    }
+  }
+}
+
+// ============================================================== generic fused kernel
+//
+// Any legal window (4..1024, multiple of 4, radices from the host plan).  One
+// workgroup per window; two complex planes either in dynamic LDS or in a
+// global scratch slice.  Batched Stockham stages with runtime radix over all
+// lines of a plane, then the same product / inverse / peak logic as above,
+// with the exact magnitude sqrt(re^2 + im^2) of cu:542.
+struct GenPlan {
+  int nrx, nry;
+  int rx[16], ry[16];
+};
+
+__device__ float2* fft_lines(float2* src, float2* dst, int N, int lines, int es, int ls, const int* rad, int nrad,
+                             const float2* tw, bool inv) {
+  int Ns = 1;
+  for (int s = 0; s < nrad; ++s) {
+    const int R = rad[s];
+    const int NR = N / R;
+    const int tstep = N / (Ns * R);
+    for (int item = threadIdx.x; item < lines * NR; item += blockDim.x) {
+      const int line = item / NR, j = item - (item / NR) * NR;
+      const int k = j % Ns;
+      const float2* in = src + (long long)line * ls;
+      float2* out = dst + (long long)line * ls;
+      const int idxD = (j / Ns) * Ns * R + k;
+      if (R == 2 || R == 4) {
+        float2 v[4];
+        for (int r = 0; r < R; ++r) {
+          float2 x = in[(long long)(j + r * NR) * es];
+          float2 w = tw[(r * k * tstep) % N];
+          if (inv) w.y = -w.y;
+          v[r] = make_float2(x.x * w.x - x.y * w.y, x.x * w.y + x.y * w.x);
+        }
+        if (R == 2) {
+          out[(long long)idxD * es] = cadd(v[0], v[1]);
+          out[(long long)(idxD + Ns) * es] = csub(v[0], v[1]);
+        } else {
+          float2 t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
+          float2 t2 = cadd(v[1], v[3]), d = csub(v[1], v[3]);
+          float2 t3 = inv ? make_float2(-d.y, d.x) : make_float2(d.y, -d.x);
+          out[(long long)idxD * es] = cadd(t0, t2);
+          out[(long long)(idxD + Ns) * es] = cadd(t1, t3);
+          out[(long long)(idxD + 2 * Ns) * es] = csub(t0, t2);
+          out[(long long)(idxD + 3 * Ns) * es] = csub(t1, t3);
+        }
+      } else {
+        // direct length-R DFT, one output at a time (odd prime radices)
+        for (int qo = 0; qo < R; ++qo) {
+          float2 acc = make_float2(0.f, 0.f);
+          for (int r = 0; r < R; ++r) {
+            float2 x = in[(long long)(j + r * NR) * es];
+            float2 w = tw[((long long)r * k * tstep + (long long)qo * r * NR) % N];
+            if (inv) w.y = -w.y;
+            acc.x += x.x * w.x - x.y * w.y;
+            acc.y += x.x * w.y + x.y * w.x;
+          }
+          out[(long long)(idxD + qo * Ns) * es] = acc;
+        }
+      }
+    }
+    __syncthreads();
+    float2* t = src;
+    src = dst;
+    dst = t;
+    Ns *= R;
+  }
+  return src;
+}
+
+__device__ void block_argmax(float& bv, int& bi, float* sv, int* si) {
+  wave_argmax(bv, bi);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int nw = (blockDim.x + 63) >> 6;
+  __syncthreads();
+  if (lane == 0) {
+    sv[wave] = bv;
+    si[wave] = bi;
+  }
+  __syncthreads();
+  bv = sv[0];
+  bi = si[0];
+  for (int q = 1; q < nw; ++q)
+    if (peak_before(sv[q], si[q], bv, bi)) {
+      bv = sv[q];
+      bi = si[q];
+    }
+}
+
+__global__ void __launch_bounds__(256) k_xcorr_generic(XcArgs a, int WX, int WY, GenPlan plan, const float2* twx,
+                                                       const float2* twy, float2* gscratch, int win0) {
+  extern __shared__ float2 dyn[];
+  __shared__ float red_v[8];
+  __shared__ int red_i[8];
+  const int N = WX * WY;
+  const int w = win0 + blockIdx.x;
+  if (w >= a.nx_vec * a.ny_vec) return;
+  const int iy = w / a.nx_vec, ix = w - (w / a.nx_vec) * a.nx_vec;
+  float2* A = gscratch ? gscratch + (long long)blockIdx.x * 2 * N : dyn;
+  float2* B = A + N;
+
+  for (int e = threadIdx.x; e < N; e += blockDim.x) {
+    const int r = e / WX, c = e - (e / WX) * WX;
+    A[e] = gather_px(a, ix * a.sp_x + c, iy * a.sp_y + r);
+  }
+  __syncthreads();
+  float2* Z = fft_lines(A, B, WX, WY, 1, WX, plan.rx, plan.nrx, twx, false);
+  float2* O = (Z == A) ? B : A;
+  Z = fft_lines(Z, O, WY, WX, WX, 1, plan.ry, plan.nry, twy, false);
+  O = (Z == A) ? B : A;
+  for (int e = threadIdx.x; e < N; e += blockDim.x) {
+    const int r = e / WX, c = e - (e / WX) * WX;
+    const int rm = (WY - r) % WY, cm = (WX - c) % WX;
+    O[e] = xprod(Z[e], Z[rm * WX + cm]);
+  }
+  __syncthreads();
+  Z = fft_lines(O, Z, WX, WY, 1, WX, plan.rx, plan.nrx, twx, true);
+  O = (Z == A) ? B : A;
+  Z = fft_lines(Z, O, WY, WX, WX, 1, plan.ry, plan.nry, twy, true);
+  O = (Z == A) ? B : A;
+  float* cc = reinterpret_cast<float*>(O);
+  float* cand = cc + N;
+  for (int e = threadIdx.x; e < N; e += blockDim.x) {
+    const int r = e / WX, c = e - (e / WX) * WX;
+    const int rs = (r + WY / 2) % WY, cs = (c + WX / 2) % WX;
+    const float2 q = Z[e];
+    cc[rs * WX + cs] = (sqrtf(q.x * q.x + q.y * q.y) * 0.25f) / (float)N;
+  }
+  __syncthreads();
+  int cnt = 0;
+  for (int e = threadIdx.x; e < N; e += blockDim.x) {
+    const int j = e / WX, i = e - (e / WX) * WX;
+    float val = 0.f;
+    if (i >= a.bx && i <= WX - 1 - a.bx && j >= a.by && j <= WY - 1 - a.by) {
+      const float c0 = cc[e];
+      if (c0 > 0.f && c0 >= cc[e - 1] && c0 >= cc[e + 1] && c0 >= cc[e - WX] && c0 >= cc[e + WX]) {
+        val = c0;
+        ++cnt;
+      }
+    }
+    cand[e] = val;
+  }
+  cnt = wave_sum(cnt);
+  __shared__ int cnt_s[8];
+  if ((threadIdx.x & 63) == 0) cnt_s[threadIdx.x >> 6] = cnt;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int t = 0;
+    for (int q = 0; q < (int)((blockDim.x + 63) >> 6); ++q) t += cnt_s[q];
+    a.ncand[w] = t;
+    if (t > a.slot_cap) atomicAdd(a.overflow, 1ull);
+  }
+  const float px = a.pred_x ? a.pred_x[w] : 0.f;
+  const float py = a.pred_y ? a.pred_y[w] : 0.f;
+  float pv = INFINITY;
+  int pi = -1;
+  for (int k = 0; k < a.K; ++k) {
+    float bv = 0.f;
+    int bi = 0x7fffffff;
+    for (int e = threadIdx.x; e < N; e += blockDim.x) {
+      const float val = cand[e];
+      const bool after = val < pv || (val == pv && e > pi);
+      if (val > 0.f && after && peak_before(val, e, bv, bi)) {
+        bv = val;
+        bi = e;
+      }
+    }
+    block_argmax(bv, bi, red_v, red_i);
+    pv = bv;
+    pi = bi;
+    if (threadIdx.x == 0) {
+      const long long o = (long long)w * a.K + k;
+      float dx = 0.f, dy = 0.f;
+      bool ok = false;
+      if (bv > 0.f) ok = subpixel_fit(cc, WX, WX, WY, bi % WX, bi / WX, bv, px, py, a.lx1, a.lx2, a.ly1, a.ly2, dx, dy);
+      a.disp_x[o] = dx;
+      a.disp_y[o] = dy;
+      a.valid[o] = ok ? 1 : 0;
+      a.pk_val[o] = bv;
+      a.pk_idx[o] = bv > 0.f ? bi : -1;
+    }
+  }
+}
+
+// ============================================================== deformation
+// Kernel_Window_Deformation_2d (cu:608-652): for every aug pixel, bilinear
+// displacement from the extended filtered field, then bilinear sample of frame
+// 2 at the displaced position.  Out-of-frame taps read 0 (the reference's zero
+// padding); taps beyond the reference's padded buffer are counted.
+__global__ void __launch_bounds__(256) k_deform(DefArgs a) {
+#pragma clang fp contract(off)
+  const long long n = (long long)a.nx_aug * a.ny_aug;
+  unsigned oob = 0;
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x) {
+    const int idy = (int)(t / a.nx_aug);
+    const int idx = (int)(t - (long long)idy * a.nx_aug);
+    float ax = ((float)idx - a.x_st) / a.spx;
+    float ay = ((float)idy - a.y_st) / a.spy;
+    int ixf = (int)floorf(ax), iyf = (int)floorf(ay);
+    ax = ax - (float)ixf;
+    ay = ay - (float)iyf;
+    float ce1 = (1 - ax) * (1 - ay), ce2 = ax * (1 - ay), ce3 = (1 - ax) * ay, ce4 = ax * ay;
+    ixf = min(max(ixf, 0), a.nxe - 2);
+    iyf = min(max(iyf, 0), a.nye - 2);
+    const int i1 = iyf * a.nxe + ixf, i2 = i1 + 1, i3 = i1 + a.nxe, i4 = i3 + 1;
+    float sx = (float)idx + ce1 * a.ex_x[i1] + ce2 * a.ex_x[i2] + ce3 * a.ex_x[i3] + ce4 * a.ex_x[i4];
+    float sy = (float)idy + ce1 * a.ex_y[i1] + ce2 * a.ex_y[i2] + ce3 * a.ex_y[i3] + ce4 * a.ex_y[i4];
+    const int jx = (int)floorf(sx), jy = (int)floorf(sy);
+    sx = sx - (float)jx;
+    sy = sy - (float)jy;
+    ce1 = (1 - sx) * (1 - sy);
+    ce2 = sx * (1 - sy);
+    ce3 = (1 - sx) * sy;
+    ce4 = sx * sy;
+    const int xi = jx - a.left_x, yi = jy - a.left_y;
+    float t1 = 0.f, t2 = 0.f, t3 = 0.f, t4 = 0.f;
+    const bool x0 = (unsigned)xi < (unsigned)a.nx, x1 = (unsigned)(xi + 1) < (unsigned)a.nx;
+    const bool y0 = (unsigned)yi < (unsigned)a.ny, y1 = (unsigned)(yi + 1) < (unsigned)a.ny;
+    const uint8_t* row0 = a.f2 + (long long)yi * a.pitch;
+    if (y0 && x0) t1 = (float)row0[xi];
+    if (y0 && x1) t2 = (float)row0[xi + 1];
+    if (y1 && x0) t3 = (float)row0[a.pitch + xi];
+    if (y1 && x1) t4 = (float)row0[a.pitch + xi + 1];
+    // taps outside the reference's aug_ex buffer ([-hfws, n_aug + hfws) in aug coordinates)
+    const int ex0 = jx + a.hfws_x, ey0 = jy + a.hfws_y;
+    if (ex0 < 0 || ex0 + 1 >= a.nx_aug + 2 * a.hfws_x || ey0 < 0 || ey0 + 1 >= a.ny_aug + 2 * a.hfws_y) ++oob;
+    a.out[t] = ce1 * t1 + ce2 * t2 + ce3 * t3 + ce4 * t4;
+  }
+  if (oob) atomicAdd(a.oob, (unsigned long long)oob);
+}
+
+// ============================================================== field extension
+// extend_filtered_vecfield_p1/p2 (cu:1547-1618): copy into an (nx+2b)x(ny+2b)
+// field with the border replicated b times (b = 3, cuh:190-191).
+__global__ void k_extend(const float* fx, const float* fy, float* ex, float* ey, int nx, int ny, int b) {
+  const int nxe = nx + 2 * b, nye = ny + 2 * b;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nxe * nye; t += gridDim.x * blockDim.x) {
+    const int j = t / nxe, i = t - (t / nxe) * nxe;
+    const int si = min(max(i - b, 0), nx - 1), sj = min(max(j - b, 0), ny - 1);
+    ex[t] = fx[sj * nx + si];
+    ey[t] = fy[sj * nx + si];
+  }
+}
+
+// ============================================================== upsampling
+// Upsampling_vecfield_kernel_2d (cu:675-704): grid-2 predictor at image
+// position (k*sp2 + sp2/2) from the extended grid-1 field (origin x_st1).
+__global__ void k_upsample(float* dx2, float* dy2, const float* ex, const float* ey, int nxe, int nye, float x_st1,
+                           float y_st1, float sp1x, float sp1y, int sp2x, int sp2y, int nx2, int ny2) {
+#pragma clang fp contract(off)
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nx2 * ny2; t += gridDim.x * blockDim.x) {
+    const int vy = t / nx2, vx = t - (t / nx2) * nx2;
+    const int idx = vx * sp2x + sp2x / 2, idy = vy * sp2y + sp2y / 2;
+    float ax = ((float)idx - x_st1) / sp1x;
+    float ay = ((float)idy - y_st1) / sp1y;
+    int ix = (int)floorf(ax), iy = (int)floorf(ay);
+    ax = ax - (float)ix;
+    ay = ay - (float)iy;
+    const float ce1 = (1 - ax) * (1 - ay), ce2 = ax * (1 - ay), ce3 = (1 - ax) * ay, ce4 = ax * ay;
+    ix = min(max(ix, 0), nxe - 2);
+    iy = min(max(iy, 0), nye - 2);
+    const int i1 = iy * nxe + ix, i2 = i1 + 1, i3 = i1 + nxe, i4 = i3 + 1;
+    dx2[t] = ce1 * ex[i1] + ce2 * ex[i2] + ce3 * ex[i3] + ce4 * ex[i4];
+    dy2[t] = ce1 * ey[i1] + ce2 * ey[i2] + ce3 * ey[i3] + ce4 * ey[i4];
+  }
+}
+
+// ============================================================== median test
+// One thread per vector.  Neighbours (valid entries of the current field in
+// the wx*wy window, skipping slot (wx*wy-1)/2, cu:255-283) are held in a
+// fixed slot array padded with +inf, sorted by a bitonic network; the exact
+// median is the same order statistic the reference's partial bubble sort
+// produces (cu:174-207).  Deviations |v - median| (cu:209-223), their median,
+// then the candidate test of median_filtering (cu:225-253).
+template <int NS>
+__device__ __forceinline__ void bitonic_sort(float (&v)[NS]) {
+#pragma unroll
+  for (int k = 2; k <= NS; k <<= 1) {
+#pragma unroll
+    for (int j = k >> 1; j > 0; j >>= 1) {
+#pragma unroll
+      for (int i = 0; i < NS; ++i) {
+        const int l = i ^ j;
+        if (l > i) {
+          const float lo = fminf(v[i], v[l]), hi = fmaxf(v[i], v[l]);
+          if ((i & k) == 0) {
+            v[i] = lo;
+            v[l] = hi;
+          } else {
+            v[i] = hi;
+            v[l] = lo;
+          }
+        }
+      }
+    }
+  }
+}
+
+template <int NS>
+__device__ __forceinline__ float select_sorted(const float (&s)[NS], int idx) {
+  float r = 0.f;
+#pragma unroll
+  for (int i = 0; i < NS; ++i)
+    if (i == idx) r = s[i];
+  return r;
+}
+
+template <int NS>
+__device__ __forceinline__ float median_of(float (&s)[NS], int n) {
+#pragma clang fp contract(off)
+  bitonic_sort<NS>(s);
+  if (n <= 0) return 0.f;  // reference reads outside the list here (cu:199); defined as 0
+  if (n % 2 == 0) return (select_sorted<NS>(s, n / 2) + select_sorted<NS>(s, n / 2 - 1)) / 2;
+  return select_sorted<NS>(s, (n - 1) / 2);
+}
+
+__device__ __forceinline__ void median_decide(const MedArgs& a, int t, bool vmed, float mx, float my, float rx,
+                                              float ry) {
+#pragma clang fp contract(off)
+  float ox = 0.f, oy = 0.f;
+  bool ok = false;
+  int acc = -1;
+  if (vmed) {
+    for (int k = 0; k < a.K; ++k) {
+      const long long o = (long long)t * a.K + k;
+      if (!a.cv[o]) continue;
+      const float devix = fabsf(a.cx[o] - mx) / (rx + a.eps);
+      const float deviy = fabsf(a.cy[o] - my) / (ry + a.eps);
+      if (devix < a.thr && deviy < a.thr) {
+        ok = true;
+        ox = a.cx[o];
+        oy = a.cy[o];
+        acc = k;
+        break;
+      }
+    }
+  }
+  a.dx[t] = ox;
+  a.dy[t] = oy;
+  a.dv[t] = ok ? 1 : 0;
+  if (a.outliers && acc != 0) atomicAdd(a.outliers, 1ull);
+}
+
+template <int NS>
+__global__ void __launch_bounds__(128) k_median(MedArgs a) {
+#pragma clang fp contract(off)
+  const int nt = a.nx_vec * a.ny_vec;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nt) return;
+  const int vy = t / a.nx_vec, vx = t - (t / a.nx_vec) * a.nx_vec;
+  const int ntm = a.wx * a.wy;
+  const int skip = (ntm - 1) / 2;
+  float X[NS], Y[NS];
+  int n = 0;
+  int col = 0, row = 0;
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    float xv = INFINITY, yv = INFINITY;
+    if (s < ntm && s != skip) {
+      const int qx = vx - a.hfwx + col, qy = vy - a.hfwy + row;
+      if (qx >= 0 && qx < a.nx_vec && qy >= 0 && qy < a.ny_vec) {
+        const long long q = (long long)(qy * a.nx_vec + qx) * a.sstride;
+        if (a.sv[q]) {
+          xv = a.sx[q];
+          yv = a.sy[q];
+          ++n;
+        }
+      }
+    }
+    X[s] = xv;
+    Y[s] = yv;
+    if (++col == a.wx) {
+      col = 0;
+      ++row;
+    }
+  }
+  const bool vmed = n >= a.min_nneigh;
+  float mx = 0.f, my = 0.f, rx = 0.f, ry = 0.f;
+  if (vmed) {
+    mx = median_of<NS>(X, n);
+    my = median_of<NS>(Y, n);
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      X[s] = s < n ? fabsf(X[s] - mx) : INFINITY;
+      Y[s] = s < n ? fabsf(Y[s] - my) : INFINITY;
+    }
+    rx = median_of<NS>(X, n);
+    ry = median_of<NS>(Y, n);
+  }
+  median_decide(a, t, vmed, mx, my, rx, ry);
+}
+
+// Windows larger than the register path: per-vector lists in global scratch,
+// insertion sort (any wx*wy the reference accepts).
+__device__ float median_scratch(float* s, int n) {
+#pragma clang fp contract(off)
+  for (int i = 1; i < n; ++i) {
+    const float v = s[i];
+    int j = i - 1;
+    while (j >= 0 && s[j] > v) {
+      s[j + 1] = s[j];
+      --j;
+    }
+    s[j + 1] = v;
+  }
+  if (n <= 0) return 0.f;
+  if (n % 2 == 0) return (s[n / 2] + s[n / 2 - 1]) / 2;
+  return s[(n - 1) / 2];
+}
+
+__global__ void __launch_bounds__(128) k_median_generic(MedArgs a) {
+#pragma clang fp contract(off)
+  const int nt = a.nx_vec * a.ny_vec;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nt) return;
+  const int vy = t / a.nx_vec, vx = t - (t / a.nx_vec) * a.nx_vec;
+  const int ntm = a.wx * a.wy;
+  float* X = a.scratch + (long long)t * 2 * ntm;
+  float* Y = X + ntm;
+  int n = 0;
+  for (int s = 0; s < ntm; ++s) {
+    if (s == (ntm - 1) / 2) continue;
+    const int qx = vx - a.hfwx + s % a.wx, qy = vy - a.hfwy + s / a.wx;
+    if (qx >= 0 && qx < a.nx_vec && qy >= 0 && qy < a.ny_vec) {
+      const long long q = (long long)(qy * a.nx_vec + qx) * a.sstride;
+      if (a.sv[q]) {
+        X[n] = a.sx[q];
+        Y[n] = a.sy[q];
+        ++n;
+      }
+    }
+  }
+  const bool vmed = n >= a.min_nneigh;
+  float mx = 0.f, my = 0.f, rx = 0.f, ry = 0.f;
+  if (vmed) {
+    mx = median_scratch(X, n);
+    my = median_scratch(Y, n);
+    for (int s = 0; s < n; ++s) {
+      X[s] = fabsf(X[s] - mx);
+      Y[s] = fabsf(Y[s] - my);
+    }
+    rx = median_scratch(X, n);
+    ry = median_scratch(Y, n);
+  }
+  median_decide(a, t, vmed, mx, my, rx, ry);
+}
+
+// ============================================================== hole filling
+// Jacobi sweeps of fill_blank_vecfield (cu:352-397) until a sweep fills
+// nothing, run by one workgroup over a compacted hole list so that no host
+// round trip is needed per sweep (the reference copies a counter back per
+// sweep, cu:1527).  Each sweep first computes every hole's 3x3 mean of valid
+// neighbours (raster sum order), then applies all fills at once.
+__global__ void __launch_bounds__(1024) k_fill(FillArgs a) {
+#pragma clang fp contract(off)
+  __shared__ int n_holes, n_next, n_filled;
+  const int nt = a.nx_vec * a.ny_vec;
+  if (threadIdx.x == 0) {
+    n_holes = 0;
+    n_filled = 0;
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < nt; t += blockDim.x)
+    if (!a.dv[t]) a.list_a[atomicAdd(&n_holes, 1)] = t;
+  __syncthreads();
+  int* cur = a.list_a;
+  int* nxt = a.list_b;
+  int sweeps = 0;
+  while (true) {
+    const int nh = n_holes;
+    if (nh == 0) break;
+    for (int h = threadIdx.x; h < nh; h += blockDim.x) {
+      const int t = cur[h];
+      const int vy = t / a.nx_vec, vx = t - (t / a.nx_vec) * a.nx_vec;
+      int cnt = 0;
+      float sx = 0.f, sy = 0.f;
+      for (int j = max(0, vy - 1); j <= min(a.ny_vec - 1, vy + 1); ++j)
+        for (int k = max(0, vx - 1); k <= min(a.nx_vec - 1, vx + 1); ++k) {
+          const int q = j * a.nx_vec + k;
+          if (a.dv[q]) {
+            ++cnt;
+            sx += a.dx[q];
+            sy += a.dy[q];
+          }
+        }
+      a.tmp_f[h] = cnt > 0;
+      if (cnt > 0) {
+        a.tmp_x[h] = sx / (float)cnt;
+        a.tmp_y[h] = sy / (float)cnt;
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) n_next = 0;
+    __syncthreads();
+    int filled = 0;
+    for (int h = threadIdx.x; h < nh; h += blockDim.x) {
+      const int t = cur[h];
+      if (a.tmp_f[h]) {
+        a.dx[t] = a.tmp_x[h];
+        a.dy[t] = a.tmp_y[h];
+        a.dv[t] = 1;
+        ++filled;
+      } else {
+        nxt[atomicAdd(&n_next, 1)] = t;
+      }
+    }
+    if (filled) atomicAdd(&n_filled, filled);
+    __syncthreads();
+    ++sweeps;
+    const bool progress = n_filled > 0;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      n_holes = n_next;
+      n_filled = 0;
+    }
+    __syncthreads();
+    if (!progress) break;
+    int* tt = cur;
+    cur = nxt;
+    nxt = tt;
+  }
+  if (threadIdx.x == 0 && a.sweeps) *a.sweeps = sweeps;
+}
+
+// ============================================================== smoothing
+// organize_smooth_filter_data + smoothing_tophat (cu:399-443): mean over the
+// in-bounds wx*wy window including the centre, ignoring validity; raster sum.
+__global__ void k_smooth(SmoothArgs a) {
+#pragma clang fp contract(off)
+  const int nt = a.nx_vec * a.ny_vec;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nt; t += gridDim.x * blockDim.x) {
+    const int vy = t / a.nx_vec, vx = t - (t / a.nx_vec) * a.nx_vec;
+    float sx = 0.f, sy = 0.f;
+    int n = 0;
+    for (int r = 0; r < a.wy; ++r) {
+      const int qy = vy - a.hfwy + r;
+      if (qy < 0 || qy >= a.ny_vec) continue;
+      for (int c = 0; c < a.wx; ++c) {
+        const int qx = vx - a.hfwx + c;
+        if (qx < 0 || qx >= a.nx_vec) continue;
+        sx += a.sx[qy * a.nx_vec + qx];
+        sy += a.sy[qy * a.nx_vec + qx];
+        ++n;
+      }
+    }
+    a.dx[t] = sx / (float)n;
+    a.dy[t] = sy / (float)n;
+  }
+}
+
+__global__ void k_zero_f32(float* p, size_t n) {
+  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) p[t] = 0.f;
+}
+__global__ void k_prep(const float* cx, const float* cy, const uint8_t* cv, int K, float* dx, float* dy, uint8_t* dv,
+                       int nt) {
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nt; t += gridDim.x * blockDim.x) {
+    const long long o = (long long)t * K;
+    dv[t] = cv[o];
+    dx[t] = cx[o];
+    dy[t] = cy[o];
+  }
+}
+__global__ void k_copy_field(const float* sx, const float* sy, const uint8_t* sv, float* dx, float* dy, uint8_t* dv,
+                             int nt) {
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nt; t += gridDim.x * blockDim.x) {
+    dx[t] = sx[t];
+    dy[t] = sy[t];
+    if (sv) dv[t] = sv[t];
+  }
+}
+__global__ void k_zero_u64(unsigned long long* p, size_t n) {
+  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) p[t] = 0;
+}
+
+// ============================================================== launchers
+
+static inline int grid_for(long long n, int block, int cap = 65536) {
+  long long g = (n + block - 1) / block;
+  if (g < 1) g = 1;
+  if (g > cap) g = cap;
+  return (int)g;
+}
+
+template <int WX, int WY, int WPB>
+static hipError_t launch_fast(const XcArgs& a0, hipStream_t s) {
+  XcArgs a = a0;
+  a.chunks_x = (a.nx_vec + WPB - 1) / WPB;
+  const int blocks = a.chunks_x * a.ny_vec;
+  hipLaunchKernelGGL((k_xcorr_fast<WX, WY, WPB>), dim3(blocks), dim3(FastCfg<WX, WY, WPB>::NT), 0, s, a);
+  return hipGetLastError();
+}
+
+#define PIVX_FAST_SIZES(X) \
+  X(32, 32, 8)             \
+  X(64, 64, 2)             \
+  X(16, 16, 16)            \
+  X(24, 24, 8)             \
+  X(48, 48, 4)             \
+  X(64, 32, 4)             \
+  X(32, 64, 4)             \
+  X(32, 16, 8)             \
+  X(16, 32, 8)             \
+  X(48, 24, 8)
+
+bool xcorr_fast_supported(int ws_x, int ws_y) {
+#define PIVX_CASE(WX, WY, WPB) \
+  if (ws_x == WX && ws_y == WY) return true;
+  PIVX_FAST_SIZES(PIVX_CASE)
+#undef PIVX_CASE
+  return false;
+}
+
+static void factor_radices(int n, int* rad, int& nr) {
+  nr = 0;
+  while (n % 4 == 0) { rad[nr++] = 4; n /= 4; }
+  while (n % 2 == 0) { rad[nr++] = 2; n /= 2; }
+  for (int p = 3; n > 1; p += 2)
+    while (n % p == 0) { rad[nr++] = p; n /= p; }
+}
+
+hipError_t launch_xcorr(const XcArgs& a, int ws_x, int ws_y, float2* gscratch, size_t gscratch_windows,
+                        const float2* twx, const float2* twy, hipStream_t s) {
+  if (twx == nullptr) {
+#define PIVX_CASE(WX, WY, WPB) \
+  if (ws_x == WX && ws_y == WY) return launch_fast<WX, WY, WPB>(a, s);
+    PIVX_FAST_SIZES(PIVX_CASE)
+#undef PIVX_CASE
+    return hipErrorInvalidValue;
+  }
+  GenPlan plan;
+  factor_radices(ws_x, plan.rx, plan.nrx);
+  factor_radices(ws_y, plan.ry, plan.nry);
+  const long long nwin = (long long)a.nx_vec * a.ny_vec;
+  const size_t plane_bytes = (size_t)ws_x * ws_y * sizeof(float2);
+  if (2 * plane_bytes <= 64 * 1024) {
+    hipLaunchKernelGGL(k_xcorr_generic, dim3((unsigned)nwin), dim3(256), 2 * plane_bytes, s, a, ws_x, ws_y, plan, twx,
+                       twy, (float2*)nullptr, 0);
+    return hipGetLastError();
+  }
+  if (gscratch == nullptr || gscratch_windows == 0) return hipErrorInvalidValue;
+  for (long long w0 = 0; w0 < nwin; w0 += (long long)gscratch_windows) {
+    const long long nb = (nwin - w0) < (long long)gscratch_windows ? (nwin - w0) : (long long)gscratch_windows;
+    hipLaunchKernelGGL(k_xcorr_generic, dim3((unsigned)nb), dim3(256), 0, s, a, ws_x, ws_y, plan, twx, twy, gscratch,
+                       (int)w0);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+hipError_t launch_deform(const DefArgs& a, hipStream_t s) {
+  const long long n = (long long)a.nx_aug * a.ny_aug;
+  hipLaunchKernelGGL(k_deform, dim3(grid_for(n, 256, 1 << 20)), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_extend(const float* fx, const float* fy, float* ex, float* ey, int nx, int ny, int border,
+                         hipStream_t s) {
+  const long long n = (long long)(nx + 2 * border) * (ny + 2 * border);
+  hipLaunchKernelGGL(k_extend, dim3(grid_for(n, 256)), dim3(256), 0, s, fx, fy, ex, ey, nx, ny, border);
+  return hipGetLastError();
+}
+
+hipError_t launch_upsample(float* dx2, float* dy2, const float* ex1x, const float* ex1y, int nxe1, int nye1,
+                           float x_st1, float y_st1, float sp1x, float sp1y, int sp2x, int sp2y, int nx2, int ny2,
+                           hipStream_t s) {
+  hipLaunchKernelGGL(k_upsample, dim3(grid_for((long long)nx2 * ny2, 256)), dim3(256), 0, s, dx2, dy2, ex1x, ex1y,
+                     nxe1, nye1, x_st1, y_st1, sp1x, sp1y, sp2x, sp2y, nx2, ny2);
+  return hipGetLastError();
+}
+
+hipError_t launch_median(const MedArgs& a, hipStream_t s) {
+  const int nt = a.nx_vec * a.ny_vec;
+  const int ntm = a.wx * a.wy;
+  dim3 g(grid_for(nt, 128, 1 << 30)), b(128);
+  if (ntm <= 8) hipLaunchKernelGGL(k_median<8>, g, b, 0, s, a);
+  else if (ntm <= 16) hipLaunchKernelGGL(k_median<16>, g, b, 0, s, a);
+  else if (ntm <= 32) hipLaunchKernelGGL(k_median<32>, g, b, 0, s, a);
+  else if (ntm <= 64) hipLaunchKernelGGL(k_median<64>, g, b, 0, s, a);
+  else {
+    if (a.scratch == nullptr) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_median_generic, g, b, 0, s, a);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_fill(const FillArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(k_fill, dim3(1), dim3(1024), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_smooth(const SmoothArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(k_smooth, dim3(grid_for((long long)a.nx_vec * a.ny_vec, 256)), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_zero_f32(float* p, size_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_zero_f32, dim3(grid_for((long long)n, 256)), dim3(256), 0, s, p, n);
+  return hipGetLastError();
+}
+hipError_t launch_prep(const float* cx, const float* cy, const uint8_t* cv, int K, float* dx, float* dy, uint8_t* dv,
+                       int nt, hipStream_t s) {
+  hipLaunchKernelGGL(k_prep, dim3(grid_for(nt, 256)), dim3(256), 0, s, cx, cy, cv, K, dx, dy, dv, nt);
+  return hipGetLastError();
+}
+hipError_t launch_copy_field(const float* sx, const float* sy, const uint8_t* sv, float* dx, float* dy, uint8_t* dv,
+                             int nt, hipStream_t s) {
+  hipLaunchKernelGGL(k_copy_field, dim3(grid_for(nt, 256)), dim3(256), 0, s, sx, sy, sv, dx, dy, dv, nt);
+  return hipGetLastError();
+}
+hipError_t launch_zero_u64(unsigned long long* p, size_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_zero_u64, dim3(grid_for((long long)n, 256)), dim3(256), 0, s, p, n);
+  return hipGetLastError();
+}
+
+}  // namespace pivx

@@ -1,0 +1,311 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle on the
+same seeded inputs.
+
+Bar (written per assertion below):
+  * integer peak index, candidate order and validity flags bit-exact, except
+    in windows the FP64 oracle flags as ties (two candidate values, or a
+    candidate and its neighbour, closer than tie_margin * peak);
+  * sub-pixel displacements within 1e-3 px;
+  * FP32 element-wise stages (deformation, upsampling, median test, fill,
+    smoothing) bit-exact against the oracle's FP32 transcription on identical
+    inputs, and within 1e-4 px of the FP64 oracle.
+"""
+import numpy as np
+import pytest
+
+from conftest import config1_pair, config1_params, geom_dict, tie_margin
+
+pytestmark = pytest.mark.gpu
+
+SUBPIX_TOL = 1e-3
+
+
+def _report(name, **kw):
+    print(f"[parity] {name}: " + ", ".join(f"{k}={v}" for k, v in kw.items()), flush=True)
+
+
+def check_pass(name, gpu, ref, ws, K, min_ok=0.9):
+    """gpu: dict(val, idx, dx, dy, v); ref: oracle xcorr_pass output."""
+    ok = ref["tie"] == 0
+    frac_ok = float(ok.mean())
+    bad_idx = int((gpu["idx"][ok] != ref["pk_idx"][ok]).any(axis=1).sum())
+    bad_valid = int((gpu["v"][ok] != ref["valid"][ok]).any(axis=1).sum())
+    both = ok[:, None] & (gpu["v"] == 1) & (ref["valid"] == 1)
+    ex = float(np.abs(gpu["dx"] - ref["dx"])[both].max()) if both.any() else 0.0
+    ey = float(np.abs(gpu["dy"] - ref["dy"])[both].max()) if both.any() else 0.0
+    peak = max(float(ref["pk_val"].max()), 1.0)
+    relv = float((np.abs(gpu["val"] - ref["pk_val"]) / peak)[ok].max()) if ok.any() else 0.0
+    _report(name, windows=len(ok), tie_excluded=int((~ok).sum()), idx_mismatch=bad_idx, valid_mismatch=bad_valid,
+            max_subpix_err=f"{max(ex, ey):.2e}", max_rel_peak_err=f"{relv:.2e}")
+    assert frac_ok >= min_ok, f"too many tie-excluded windows ({1 - frac_ok:.1%})"
+    assert bad_idx == 0, "peak indices differ outside ties"
+    assert bad_valid == 0, "validity flags differ outside ties"
+    assert max(ex, ey) <= SUBPIX_TOL
+    assert relv <= tie_margin(*ws)
+
+
+def gpu_pass(x, grid):
+    val, idx, nc = x.peaks(grid)
+    dx, dy, v = x.raw(grid)
+    return dict(val=val, idx=idx, dx=dx, dy=dy, v=v, nc=nc)
+
+
+def test_config1_pass0_matches_oracle(pivx, orc):
+    """512x512, 32x32 @16, top-4, undeformed pass."""
+    a, b = config1_pair(pivx)
+    p = config1_params(pivx)
+    with pivx.Xcorr2dMulti(p) as x:
+        x.prep_img_data(a, b)
+        x.Calc_cc_map_p1(0)
+        gpu = gpu_pass(x, 1)
+        g = geom_dict(x.geometry(1))
+    assert g["nx_vec"] == 32 and g["ny_vec"] == 32
+    ref = orc.xcorr_pass(a, f2u8=b, geom=g, ws=(32, 32), sp=(16, 16), K=4, tie_rel=tie_margin(32, 32))
+    check_pass("config1 pass0", gpu, ref, (32, 32), 4)
+    # uncapped candidate counts agree wherever the window is not tie-flagged
+    ok = ref["tie"] == 0
+    assert (gpu["nc"][ok] == ref["ncand"][ok]).mean() > 0.98
+    # interior vectors recover the imposed (3.3, -1.7) displacement
+    gx, gy = np.meshgrid(np.arange(32) * 16 + 8, np.arange(32) * 16 + 8)
+    interior = ((gx > 24) & (gx < 488) & (gy > 24) & (gy < 488)).ravel()
+    good = interior & (gpu["v"][:, 0] == 1)
+    assert np.median(np.abs(gpu["dx"][good, 0] - 3.3)) < 0.1
+    assert np.median(np.abs(gpu["dy"][good, 0] + 1.7)) < 0.1
+
+
+def test_deformation_bitexact_and_deformed_pass(pivx, orc):
+    """Kernel_Window_Deformation_2d against the FP32 transcription (bit-exact)
+    and the FP64 oracle; then the deformed correlation pass on identical
+    inputs."""
+    a, b = config1_pair(pivx, seed=3)
+    p = config1_params(pivx, npass1=2)
+    rng = np.random.default_rng(7)
+    with pivx.Xcorr2dMulti(p) as x:
+        g = geom_dict(x.geometry(1))
+        nt = g["nx_vec"] * g["ny_vec"]
+        fx = (3.3 + 0.3 * rng.standard_normal(nt)).astype(np.float32)
+        fy = (-1.7 + 0.3 * rng.standard_normal(nt)).astype(np.float32)
+        x.prep_img_data(a, b)
+        x.set_filtered(1, fx, fy)
+        x.extend_filtered_vecfield_p1()
+        x.Calc_cc_map_p1(1)
+        d = x.deformed(1)
+        gpu = gpu_pass(x, 1)
+    ex, ey = orc.extend(fx.reshape(32, 32), fy.reshape(32, 32))
+    d32 = orc.deform(b, g, (32, 32), (16, 16), ex, ey, prec=32)
+    d64 = orc.deform(b, g, (32, 32), (16, 16), ex, ey, prec=64)
+    _report("deformation", shape=d.shape, max_diff_f32=float(np.abs(d - d32).max()),
+            max_diff_f64=f"{float(np.abs(d - d64).max()):.2e}")
+    np.testing.assert_array_equal(d, d32.astype(np.float32))
+    assert np.abs(d - d64).max() < 1e-3
+    ref = orc.xcorr_pass(a, f2def=d.astype(np.float64), geom=g, ws=(32, 32), sp=(16, 16), K=4,
+                         pred=(fx.astype(np.float64), fy.astype(np.float64)), tie_rel=tie_margin(32, 32))
+    check_pass("config1 deformed pass", gpu, ref, (32, 32), 4)
+
+
+@pytest.mark.parametrize("ws", [(16, 16), (24, 24), (48, 48), (64, 64), (64, 32), (32, 64), (32, 16), (16, 32),
+                                (48, 24)])
+def test_fast_window_shapes(pivx, orc, ws):
+    """Every compiled register-FFT window shape, undeformed pass, 50% overlap."""
+    a, b = config1_pair(pivx, seed=11, nx=384, ny=320, u=2.6, v=1.3)
+    sp = (ws[0] // 2, ws[1] // 2)
+    p = config1_params(pivx, nx=384, ny=320, ws_x1=ws[0], ws_y1=ws[1], spacing_x1=sp[0], spacing_y1=sp[1],
+                       max_dispx_percent1=0.4, max_dispy_percent1=0.4)
+    with pivx.Xcorr2dMulti(p) as x:
+        x.prep_img_data(a, b)
+        x.Calc_cc_map_p1(0)
+        gpu = gpu_pass(x, 1)
+        g = geom_dict(x.geometry(1))
+    ref = orc.xcorr_pass(a, f2u8=b, geom=g, ws=ws, sp=sp, K=4, tie_rel=tie_margin(*ws))
+    check_pass(f"fast {ws[0]}x{ws[1]}", gpu, ref, ws, 4, min_ok=0.8)
+
+
+@pytest.mark.parametrize("ws,sp", [((20, 12), (10, 6)), ((36, 28), (18, 14)), ((44, 44), (22, 22)),
+                                   ((128, 128), (64, 64)), ((8, 8), (4, 4))])
+def test_generic_window_shapes(pivx, orc, ws, sp):
+    """Shapes outside the compiled set take the generic kernel (runtime radix
+    plan; 128x128 uses global-memory planes)."""
+    a, b = config1_pair(pivx, seed=5, nx=320, ny=256, u=1.4, v=-2.2)
+    p = config1_params(pivx, nx=320, ny=256, ws_x1=ws[0], ws_y1=ws[1], spacing_x1=sp[0], spacing_y1=sp[1],
+                       max_dispx_percent1=0.4, max_dispy_percent1=0.4)
+    with pivx.Xcorr2dMulti(p) as x:
+        x.prep_img_data(a, b)
+        x.Calc_cc_map_p1(0)
+        gpu = gpu_pass(x, 1)
+        g = geom_dict(x.geometry(1))
+    ref = orc.xcorr_pass(a, f2u8=b, geom=g, ws=ws, sp=sp, K=4, tie_rel=tie_margin(*ws))
+    check_pass(f"generic {ws[0]}x{ws[1]}", gpu, ref, ws, 4, min_ok=0.7)
+
+
+def _random_candidates(rng, nt, K, outlier_frac=0.1):
+    dx = (2.0 + 0.2 * rng.standard_normal((nt, K))).astype(np.float32)
+    dy = (-1.0 + 0.2 * rng.standard_normal((nt, K))).astype(np.float32)
+    out = rng.random(nt) < outlier_frac
+    dx[out, 0] += rng.uniform(-8, 8, out.sum()).astype(np.float32)
+    dy[out, 0] += rng.uniform(-8, 8, out.sum()).astype(np.float32)
+    v = (rng.random((nt, K)) > 0.15).astype(np.uint8)
+    dx[v == 0] = 0
+    dy[v == 0] = 0
+    return dx, dy, v
+
+
+@pytest.mark.parametrize("win,npm,nps,fill", [((5, 5), 2, 1, 1), ((3, 3), 1, 2, 1), ((7, 5), 3, 0, 0),
+                                              ((9, 9), 1, 1, 1), ((4, 4), 2, 1, 1)])
+def test_validation_bitexact(pivx, orc, win, npm, nps, fill):
+    """MedianFilter_Smoothing_p2 on injected candidates: median passes, fill,
+    smoothing, bit-exact against the FP32 transcription."""
+    rng = np.random.default_rng(win[0] * 10 + npm)
+    p = config1_params(pivx, nx=640, ny=480, ws_x2=32, ws_y2=32, spacing_x2=16, spacing_y2=16, npass2=1,
+                       wx2_median=win[0], wy2_median=win[1], wx2_smooth=3, wy2_smooth=5)
+    with pivx.Xcorr2dMulti(p) as x:
+        nxv, nyv = x.dims(2)
+        nt = nxv * nyv
+        dx, dy, v = _random_candidates(rng, nt, 4)
+        v[rng.random(nt) < 0.02, :] = 0  # some vectors without any valid candidate
+        x.set_raw(2, dx, dy, v)
+        x.MedianFilter_Smoothing_p2(npm, 5, 2.0, 2.0, nps, fill)
+        gx, gy, gv = x.filtered(2)
+        st = x.stats()
+    K = 4
+    cx, cy, cv = dx.astype(np.float64).ravel(), dy.astype(np.float64).ravel(), v.ravel()
+    sx, sy, sv = dx[:, 0].astype(np.float64), dy[:, 0].astype(np.float64), v[:, 0].copy()
+    for _ in range(npm):
+        sx, sy, sv, _m = orc.median_pass((sx, sy, sv), (cx, cy, cv), K, (nxv, nyv), win, 5, 2.0, prec=32)
+    sweeps = None
+    if fill:
+        sx, sy, sv, sweeps = orc.fill(sx, sy, sv, (nxv, nyv), prec=32)
+    for _ in range(nps):
+        sx, sy = orc.smooth(sx, sy, (nxv, nyv), (3, 5), prec=32)
+    _report(f"validation {win} npm={npm} nps={nps} fill={fill}", vectors=nt, invalid=int((sv == 0).sum()),
+            max_diff=float(max(np.abs(gx - sx).max(), np.abs(gy - sy).max())), sweeps_gpu=st["fill_sweeps"][1],
+            sweeps_oracle=sweeps)
+    np.testing.assert_array_equal(gv, sv)
+    np.testing.assert_array_equal(gx, sx.astype(np.float32))
+    np.testing.assert_array_equal(gy, sy.astype(np.float32))
+    if fill:
+        assert st["fill_sweeps"][1] == sweeps
+
+
+def test_upsampling_bitexact(pivx, orc):
+    p = config1_params(pivx, nx=1000, ny=700, ws_x1=64, ws_y1=64, spacing_x1=32, spacing_y1=32, npass2=1,
+                       ws_x2=32, ws_y2=32, spacing_x2=16, spacing_y2=16)
+    rng = np.random.default_rng(3)
+    with pivx.Xcorr2dMulti(p) as x:
+        n1 = x.dims(1)
+        n2 = x.dims(2)
+        fx = rng.standard_normal(n1[0] * n1[1]).astype(np.float32)
+        fy = rng.standard_normal(n1[0] * n1[1]).astype(np.float32)
+        x.set_filtered(1, fx, fy)
+        x.extend_filtered_vecfield_p1()
+        x.Upsampling_vector_field()
+        ux, uy, _ = x.filtered(2)
+    ex, ey = orc.extend(fx.reshape(n1[1], n1[0]), fy.reshape(n1[1], n1[0]))
+    rx, ry = orc.upsample(ex, ey, (32, 32), (16, 16), n2, prec=32)
+    np.testing.assert_array_equal(ux, rx.astype(np.float32))
+    np.testing.assert_array_equal(uy, ry.astype(np.float32))
+
+
+def test_two_grid_stagewise(pivx, orc):
+    """Two-grid, multi-pass pipeline on a Lamb-Oseen flow; at every pass the
+    oracle is fed the GPU's own inputs of that stage (predictor, deformed
+    frame, raw candidates) so stage errors do not compound."""
+    nx, ny = 768, 512
+    sp_ = pivx.synth_params(nx, ny, seed=21, flow=1, a0=120.0, a1=6.0, a2=0.002)
+    a, b = pivx.synth_pair(sp_)
+    p = config1_params(pivx, nx=nx, ny=ny, ws_x1=64, ws_y1=64, spacing_x1=32, spacing_y1=32, npass1=2,
+                       ws_x2=32, ws_y2=32, spacing_x2=16, spacing_y2=16, npass2=2,
+                       max_dispx_percent1=0.4, max_dispy_percent1=0.4, max_dispx_percent2=0.4,
+                       max_dispy_percent2=0.4)
+    fp = pivx.FilterParams(2, 5, 2.0, 2.0, 1)
+    with pivx.Xcorr2dMulti(p) as x:
+        x.prep_img_data(a, b)
+        geo = {1: geom_dict(x.geometry(1)), 2: geom_dict(x.geometry(2))}
+        wss = {1: (64, 64), 2: (32, 32)}
+        sps = {1: (32, 32), 2: (16, 16)}
+        for grid, npass in ((1, 2), (2, 2)):
+            g = geo[grid]
+            nvec = (g["nx_vec"], g["ny_vec"])
+            if grid == 2:
+                x.extend_filtered_vecfield_p1()
+                x.Upsampling_vector_field()
+            for j in range(npass):
+                if grid == 1 and j > 0:
+                    x.extend_filtered_vecfield_p1()
+                if grid == 2:
+                    x.extend_filtered_vecfield_p2()
+                pred = x.filtered(grid)[:2] if (grid == 2 or j > 0) else (np.zeros(nvec[0] * nvec[1]),) * 2
+                if grid == 1:
+                    x.Calc_cc_map_p1(j)
+                else:
+                    x.Calc_cc_map_p2()
+                gpu = gpu_pass(x, grid)
+                deformed = grid == 2 or j > 0
+                kw = dict(geom=g, ws=wss[grid], sp=sps[grid], K=4, tie_rel=tie_margin(*wss[grid]),
+                          pred=(pred[0].astype(np.float64), pred[1].astype(np.float64)))
+                if deformed:
+                    d = x.deformed(grid)
+                    ex, ey = orc.extend(pred[0].reshape(nvec[1], nvec[0]), pred[1].reshape(nvec[1], nvec[0]))
+                    d32 = orc.deform(b, g, wss[grid], sps[grid], ex, ey, prec=32)
+                    np.testing.assert_array_equal(d, d32.astype(np.float32))
+                    ref = orc.xcorr_pass(a, f2def=d.astype(np.float64), **kw)
+                else:
+                    ref = orc.xcorr_pass(a, f2u8=b, **kw)
+                check_pass(f"two-grid g{grid} pass{j}", gpu, ref, wss[grid], 4, min_ok=0.85)
+                # validation on the GPU's raw candidates
+                if grid == 1:
+                    x.MedianFilter_Smoothing_p1(2, 5, 2.0, 2.0, 1)
+                else:
+                    x.MedianFilter_Smoothing_p2(2, 5, 2.0, 2.0, 1, 1)
+                fx, fy, fv = x.filtered(grid)
+                cx, cy, cv = (gpu["dx"].astype(np.float64).ravel(), gpu["dy"].astype(np.float64).ravel(),
+                              gpu["v"].ravel())
+                sx, sy, sv = gpu["dx"][:, 0].astype(np.float64), gpu["dy"][:, 0].astype(np.float64), gpu["v"][:, 0]
+                for _ in range(2):
+                    sx, sy, sv, _m = orc.median_pass((sx, sy, sv), (cx, cy, cv), 4, nvec, (5, 5), 5, 2.0, prec=32)
+                sx, sy, sv, _s = orc.fill(sx, sy, sv, nvec, prec=32)
+                sx, sy = orc.smooth(sx, sy, nvec, (3, 3), prec=32)
+                np.testing.assert_array_equal(fv, sv)
+                np.testing.assert_array_equal(fx, sx.astype(np.float32))
+                np.testing.assert_array_equal(fy, sy.astype(np.float32))
+        del fp
+
+
+def test_run_pair_matches_stepwise_and_is_deterministic(pivx):
+    """pivx_run_pair reproduces the reference's main-loop sequence exactly
+    (same result as calling the per-pass entry points) and is bit-identical
+    across runs."""
+    nx, ny = 640, 480
+    sp_ = pivx.synth_params(nx, ny, seed=4, flow=1, a0=100.0, a1=5.0, a2=0.002)
+    a, b = pivx.synth_pair(sp_)
+    p = config1_params(pivx, nx=nx, ny=ny, ws_x1=64, ws_y1=64, spacing_x1=32, spacing_y1=32, npass1=2,
+                       ws_x2=32, ws_y2=32, spacing_x2=16, spacing_y2=16, npass2=2,
+                       max_dispx_percent1=0.4, max_dispy_percent1=0.4, max_dispx_percent2=0.4,
+                       max_dispy_percent2=0.4)
+    f1 = pivx.FilterParams(2, 5, 2.0, 3.0, 1)
+    f2 = pivx.FilterParams(2, 5, 2.0, 3.0, 1)
+    with pivx.Xcorr2dMulti(p) as x:
+        x.prep_img_data(a, b)
+        x.run_pair(f1, f2)
+        r1 = [x.raw(1), x.filtered(1), x.raw(2), x.filtered(2), x.filtered(3)]
+        x.run_pair(f1, f2)
+        r2 = [x.raw(1), x.filtered(1), x.raw(2), x.filtered(2), x.filtered(3)]
+    with pivx.Xcorr2dMulti(p) as y:
+        y.prep_img_data(a, b)
+        for j in range(2):
+            if j > 0:
+                y.extend_filtered_vecfield_p1()
+            y.Calc_cc_map_p1(j)
+            y.MedianFilter_Smoothing_p1(2, 5, 2.0, 3.0, 1)
+        y.extend_filtered_vecfield_p1()
+        y.Upsampling_vector_field()
+        for j in range(2):
+            y.extend_filtered_vecfield_p2()
+            y.Calc_cc_map_p2()
+            y.MedianFilter_Smoothing_p2(2, 5, 2.0, 3.0, 1, 1)
+        s = [y.raw(1), y.filtered(1), y.raw(2), y.filtered(2)]
+        y.MedianFilter_Smoothing_p2(2, 5, 2.0, 3.0, 0, 0)
+        s.append(y.filtered(2))
+    for A, B, S in zip(r1, r2, s):
+        for u, w, z in zip(A, B, S):
+            np.testing.assert_array_equal(u, w)
+            np.testing.assert_array_equal(u, z)
