@@ -183,6 +183,11 @@ int pivx_set_raw(pivx_ctx* ctx, int grid, const float* dx, const float* dy, cons
 int pivx_enable_timing(pivx_ctx* ctx, int enable);
 int pivx_get_stage_ms(pivx_ctx* ctx, int grid, int stage, double* ms, int* launches);
 
+/* Measurement helper (no reference counterpart): `reps` streaming copies of
+ * `bytes` between two scratch buffers on the handle's stream; returns the
+ * achieved read+write bandwidth in GB/s.  Allocates and frees its buffers. */
+int pivx_bandwidth_probe(pivx_ctx* ctx, size_t bytes, int reps, double* gbps);
+
 /* ------------------------------------------------------------------ host utilities */
 
 /* Parameter.dat (main.cpp:93-158): 50 positional lines, CR/LF tolerant; numeric
@@ -211,6 +216,13 @@ int pivx_read_pair_index(const char* path, int* idx, int max_n);
  * raw), 1 (grid-1 filtered), 2 (grid-2 raw), 3 (grid-2 filtered), 4 (grid-2
  * median-only).  Writes folder + name into out (size cap). */
 int pivx_output_name(const pivx_parameter_file* pf, int pair, int which, char* out, size_t cap);
+
+/* The Tecplot text format of write_vec_data_general (cu:1875-1886) for
+ * caller arrays (host only): raw != 0 writes x y dx1 dy1 .. dx4 dy4 from
+ * [nt][K] candidate arrays (columns for k >= K are written as 0.00), else
+ * x y dx dy from [nt] arrays. */
+int pivx_write_tecplot(const char* path, int nx_vec, int ny_vec, const float* gx, const float* gy, const float* dx,
+                       const float* dy, int K, int raw, int crlf);
 
 /* 8-bit TIFF (ImageIOPkg.cpp:124-429): header fields and strip read. */
 int pivx_tiff_info(const char* path, int* width, int* height, int* bits_per_sample);

@@ -113,9 +113,11 @@ _SIGS = {
     "pivx_set_raw": [_P, C.c_int, _P, _P, _P],
     "pivx_enable_timing": [_P, C.c_int],
     "pivx_get_stage_ms": [_P, C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int)],
+    "pivx_bandwidth_probe": [_P, C.c_size_t, C.c_int, C.POINTER(C.c_double)],
     "pivx_parse_parameter_file": [C.c_char_p, C.POINTER(ParameterFile)],
     "pivx_read_pair_index": [C.c_char_p, _P, C.c_int],
     "pivx_output_name": [C.POINTER(ParameterFile), C.c_int, C.c_int, C.c_char_p, C.c_size_t],
+    "pivx_write_tecplot": [C.c_char_p, C.c_int, C.c_int, _P, _P, _P, _P, C.c_int, C.c_int, C.c_int],
     "pivx_tiff_info": [C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)],
     "pivx_tiff_read8": [C.c_char_p, _P, C.c_size_t],
     "pivx_tiff_write8": [C.c_char_p, _P, C.c_int, C.c_int],
@@ -315,6 +317,12 @@ class Xcorr2dMulti:
         v = None if valid is None else np.ascontiguousarray(valid, np.uint8)
         self._c(lib().pivx_set_raw(self.h, grid, _ptr(dx), _ptr(dy), _ptr(v)))
 
+    def bandwidth_probe(self, nbytes=1 << 30, reps=20):
+        """Streaming-copy GB/s (read + write) on this handle's GPU."""
+        g = C.c_double()
+        self._c(lib().pivx_bandwidth_probe(self.h, nbytes, reps, C.byref(g)))
+        return g.value
+
     def enable_timing(self, on=True):
         self._c(lib().pivx_enable_timing(self.h, 1 if on else 0))
 
@@ -345,6 +353,16 @@ def output_name(pf: ParameterFile, pair: int, which: int) -> str:
     buf = C.create_string_buffer(8192)
     _check(lib().pivx_output_name(C.byref(pf), pair, which, buf, len(buf)))
     return buf.value.decode()
+
+
+def write_tecplot(path: str, nx_vec: int, ny_vec: int, gx, gy, dx, dy, raw: bool, crlf: bool = False):
+    gx = np.ascontiguousarray(gx, np.float32)
+    gy = np.ascontiguousarray(gy, np.float32)
+    dx = np.ascontiguousarray(dx, np.float32)
+    dy = np.ascontiguousarray(dy, np.float32)
+    K = dx.shape[1] if (raw and dx.ndim == 2) else 1
+    _check(lib().pivx_write_tecplot(path.encode(), nx_vec, ny_vec, _ptr(gx), _ptr(gy), _ptr(dx), _ptr(dy), K,
+                                    1 if raw else 0, 1 if crlf else 0))
 
 
 def tiff_info(path: str):

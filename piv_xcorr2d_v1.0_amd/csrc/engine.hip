@@ -812,6 +812,37 @@ int pivx_get_stage_ms(pivx_ctx* c, int grid, int stage, double* ms, int* launche
   return PIVX_OK;
 }
 
+int pivx_bandwidth_probe(pivx_ctx* c, size_t bytes, int reps, double* gbps) {
+  if (!c || bytes < 16 || reps < 1 || !gbps) return fail(c, PIVX_ERR_PARAM, "bad argument");
+  bytes &= ~(size_t)15;
+  HIPCHK(c, hipSetDevice(c->device));
+  void *a = nullptr, *b = nullptr;
+  HIPCHK(c, hipMalloc(&a, bytes));
+  hipError_t e = hipMalloc(&b, bytes);
+  if (e != hipSuccess) {
+    (void)hipFree(a);
+    return fail(c, PIVX_ERR_HIP, "hipMalloc: %s", hipGetErrorString(e));
+  }
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  float ms = 0.f;
+  e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  if (e == hipSuccess) e = hipMemsetAsync(a, 1, bytes, c->stream);
+  if (e == hipSuccess) e = launch_stream_copy(a, b, bytes, c->stream);  // warm-up
+  if (e == hipSuccess) e = hipEventRecord(e0, c->stream);
+  for (int r = 0; r < reps && e == hipSuccess; ++r) e = launch_stream_copy(a, b, bytes, c->stream);
+  if (e == hipSuccess) e = hipEventRecord(e1, c->stream);
+  if (e == hipSuccess) e = hipEventSynchronize(e1);
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipFree(a);
+  (void)hipFree(b);
+  if (e != hipSuccess) return fail(c, PIVX_ERR_HIP, "bandwidth probe: %s", hipGetErrorString(e));
+  *gbps = 2.0 * (double)bytes * reps / (ms * 1e-3) / 1e9;
+  return PIVX_OK;
+}
+
 int pivx_write_vec_data_general(pivx_ctx* c, const char* path, int selection, int crlf) {
   if (!c || !path) return fail(c, PIVX_ERR_PARAM, "bad argument");
   const int grid = (selection == 0 || selection == 1) ? 1 : 2;

@@ -397,6 +397,13 @@ int pivx_output_name(const pivx_parameter_file* pf, int pair, int which, char* o
   return PIVX_OK;
 }
 
+int pivx_write_tecplot(const char* path, int nx_vec, int ny_vec, const float* gx, const float* gy, const float* dx,
+                       const float* dy, int K, int raw, int crlf) {
+  if (!path || !gx || !gy || !dx || !dy || nx_vec < 0 || ny_vec < 0 || (raw && K < 1))
+    return io_fail(PIVX_ERR_PARAM, "bad arguments");
+  return pivx::write_tecplot(path, nx_vec, ny_vec, gx, gy, dx, dy, raw ? K : 1, raw != 0, crlf != 0);
+}
+
 int pivx_tiff_info(const char* path, int* w, int* h, int* bps) {
   TiffInfo ti;
   int rc = tiff_parse(path, ti, nullptr);

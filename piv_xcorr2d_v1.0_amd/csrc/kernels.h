@@ -78,5 +78,6 @@ hipError_t launch_prep(const float* cx, const float* cy, const uint8_t* cv, int 
 hipError_t launch_copy_field(const float* sx, const float* sy, const uint8_t* sv, float* dx, float* dy, uint8_t* dv,
                              int nt, hipStream_t s);
 hipError_t launch_zero_u64(unsigned long long* p, size_t n, hipStream_t s);
+hipError_t launch_stream_copy(const void* src, void* dst, size_t bytes, hipStream_t s);
 
 }  // namespace pivx

@@ -838,6 +838,13 @@ __global__ void k_copy_field(const float* sx, const float* sy, const uint8_t* sv
     if (sv) dv[t] = sv[t];
   }
 }
+// Streaming copy used to measure the achievable HBM bandwidth and to calibrate
+// the FETCH_SIZE / WRITE_SIZE counters against a known byte count.
+__global__ void __launch_bounds__(256) k_stream_copy(const float4* __restrict__ src, float4* __restrict__ dst,
+                                                     size_t n4) {
+  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n4; t += (size_t)gridDim.x * blockDim.x)
+    dst[t] = src[t];
+}
 __global__ void k_zero_u64(unsigned long long* p, size_t n) {
   for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) p[t] = 0;
 }
@@ -976,6 +983,12 @@ hipError_t launch_prep(const float* cx, const float* cy, const uint8_t* cv, int 
 hipError_t launch_copy_field(const float* sx, const float* sy, const uint8_t* sv, float* dx, float* dy, uint8_t* dv,
                              int nt, hipStream_t s) {
   hipLaunchKernelGGL(k_copy_field, dim3(grid_for(nt, 256)), dim3(256), 0, s, sx, sy, sv, dx, dy, dv, nt);
+  return hipGetLastError();
+}
+hipError_t launch_stream_copy(const void* src, void* dst, size_t bytes, hipStream_t s) {
+  const size_t n4 = bytes / 16;
+  hipLaunchKernelGGL(k_stream_copy, dim3(grid_for((long long)n4, 256, 256 * 64)), dim3(256), 0, s,
+                     (const float4*)src, (float4*)dst, n4);
   return hipGetLastError();
 }
 hipError_t launch_zero_u64(unsigned long long* p, size_t n, hipStream_t s) {
