@@ -146,6 +146,12 @@ int pivx_median_filter_smoothing_p2(pivx_ctx* ctx, int npass_median, int min_nne
  * is enqueued on the handle's stream; no host synchronisation inside. */
 int pivx_run_pair(pivx_ctx* ctx, const pivx_filter_params* grid1, const pivx_filter_params* grid2);
 
+/* Graph mode (no reference counterpart): with on != 0, pivx_run_pair captures
+ * the whole per-pair sequence once into a hipGraph on the handle's stream and
+ * replays it (re-captured when the filter parameters change).  Results are
+ * identical to the eager sequence. */
+int pivx_set_graph_mode(pivx_ctx* ctx, int on);
+
 /* Wait for all work of the handle. */
 int pivx_synchronize(pivx_ctx* ctx);
 
@@ -179,7 +185,8 @@ int pivx_set_raw(pivx_ctx* ctx, int grid, const float* dx, const float* dy, cons
 
 /* Average device time (ms) of the kernels of the last pass of a stage, measured
  * with HIP events on the handle's stream when timing is enabled.
- * stage: 0 deformation, 1 correlation, 2 median, 3 fill, 4 smoothing. */
+ * stage: 0 deformation, 1 correlation, 2 median, 3 fill, 4 smoothing,
+ * 5 one whole pair replayed as a graph (grid argument ignored: use 1). */
 int pivx_enable_timing(pivx_ctx* ctx, int enable);
 int pivx_get_stage_ms(pivx_ctx* ctx, int grid, int stage, double* ms, int* launches);
 

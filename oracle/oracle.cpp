@@ -96,6 +96,30 @@ struct FFT {
     const int p = small_factor(n);
     const int m = n / p;
     for (int q = 0; q < p; ++q) run(in + (long)q * stride, m, stride * p, scratch + (long)q * m, sign, out, tw, f * p);
+    if (p == 2 || p == 4) {
+      // t_q = s_q[k] * w_n^{q k}; then a length-p DFT of the t_q
+      for (int k = 0; k < m; ++k) {
+        C t[4];
+        t[0] = scratch[k];
+        for (int q = 1; q < p; ++q) {
+          C w = tw[(long)q * k * f];
+          if (sign > 0) w = std::conj(w);
+          t[q] = scratch[(long)q * m + k] * w;
+        }
+        if (p == 2) {
+          out[k] = t[0] + t[1];
+          out[k + m] = t[0] - t[1];
+        } else {
+          const C a = t[0] + t[2], b = t[0] - t[2], c = t[1] + t[3], d0 = t[1] - t[3];
+          const C d = sign < 0 ? C(d0.imag(), -d0.real()) : C(-d0.imag(), d0.real());  // d0 * (-+i)
+          out[k] = a + c;
+          out[k + m] = b + d;
+          out[k + 2 * m] = a - c;
+          out[k + 3 * m] = b - d;
+        }
+      }
+      return;
+    }
     for (int k = 0; k < m; ++k)
       for (int s = 0; s < p; ++s) {
         C acc(0, 0);

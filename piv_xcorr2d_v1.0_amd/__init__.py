@@ -101,6 +101,7 @@ _SIGS = {
     "pivx_median_filter_smoothing_p2": [_P, C.c_int, C.c_int, C.c_float, C.c_float, C.c_int, C.c_int],
     "pivx_run_pair": [_P, C.POINTER(FilterParams), C.POINTER(FilterParams)],
     "pivx_synchronize": [_P],
+    "pivx_set_graph_mode": [_P, C.c_int],
     "pivx_write_vec_data_general": [_P, C.c_char_p, C.c_int, C.c_int],
     "pivx_get_dims": [_P, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)],
     "pivx_get_grid_coords": [_P, C.c_int, _P, _P],
@@ -249,6 +250,9 @@ class Xcorr2dMulti:
 
     def run_pair(self, f1: FilterParams, f2: FilterParams):
         self._c(lib().pivx_run_pair(self.h, C.byref(f1), C.byref(f2)))
+
+    def set_graph_mode(self, on=True):
+        self._c(lib().pivx_set_graph_mode(self.h, 1 if on else 0))
 
     def synchronize(self):
         self._c(lib().pivx_synchronize(self.h))

@@ -78,6 +78,12 @@ struct pivx_ctx {
   int ev_start = -1;
   struct Rec { int gi, stage, a, b; };
   std::vector<Rec> recs;
+  // Graph mode: pivx_run_pair replays one captured hipGraph of the whole
+  // per-pair sequence (re-captured when the filter parameters change).
+  bool use_graph = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  pivx_filter_params gf[2] = {};
 };
 
 namespace {
@@ -538,11 +544,14 @@ int pivx_init(const pivx_init_params* p, int device_id, pivx_ctx** out) {
   return PIVX_OK;
 }
 
+static void drop_graph(pivx_ctx* c);
+
 int pivx_freemem(pivx_ctx* c) {
   if (!c) return PIVX_OK;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (void* q : c->allocs) (void)hipFree(q);
+  drop_graph(c);
   for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -641,10 +650,7 @@ int pivx_upsampling_vector_field(pivx_ctx* c) {
       "Upsampling_vecfield_kernel_2d");
 }
 
-int pivx_run_pair(pivx_ctx* c, const pivx_filter_params* f1, const pivx_filter_params* f2) {
-  if (!c || !f1 || !f2) return fail(c, PIVX_ERR_PARAM, "null argument");
-  if (!c->have_frames) return fail(c, PIVX_ERR_STATE, "no frames: call pivx_prep_img_data first");
-  HIPCHK(c, hipSetDevice(c->device));
+static int run_pair_enqueue(pivx_ctx* c, const pivx_filter_params* f1, const pivx_filter_params* f2) {
   int rc;
   for (int j = 0; j < c->p.npass1; ++j) {
     if (j > 0 && (rc = extend(c, 0))) return rc;
@@ -667,6 +673,59 @@ int pivx_run_pair(pivx_ctx* c, const pivx_filter_params* f1, const pivx_filter_p
     if ((rc = run_filter(c, 1, f2->npass_median, f2->min_nneigh, f2->insert_thr, 0, 0, c->mo_x, c->mo_y, c->mo_v)))
       return rc;
   }
+  return PIVX_OK;
+}
+
+static void drop_graph(pivx_ctx* c) {
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
+  c->gexec = nullptr;
+  c->graph = nullptr;
+}
+
+static bool same_filter(const pivx_filter_params& a, const pivx_filter_params& b) {
+  return a.npass_median == b.npass_median && a.min_nneigh == b.min_nneigh && a.insert_thr == b.insert_thr &&
+         a.npass_smooth == b.npass_smooth;
+}
+
+int pivx_run_pair(pivx_ctx* c, const pivx_filter_params* f1, const pivx_filter_params* f2) {
+  if (!c || !f1 || !f2) return fail(c, PIVX_ERR_PARAM, "null argument");
+  if (!c->have_frames) return fail(c, PIVX_ERR_STATE, "no frames: call pivx_prep_img_data first");
+  HIPCHK(c, hipSetDevice(c->device));
+  if (!c->use_graph) return run_pair_enqueue(c, f1, f2);
+  if (!c->gexec || !same_filter(c->gf[0], *f1) || !same_filter(c->gf[1], *f2)) {
+    drop_graph(c);
+    const bool timing = c->timing;
+    c->timing = false;  // no event records inside the captured sequence
+    HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int rc = run_pair_enqueue(c, f1, f2);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(c->stream, &g);
+    c->timing = timing;
+    if (rc) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    if (e != hipSuccess) return fail(c, PIVX_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    c->graph = g;
+    e = hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+      drop_graph(c);
+      return fail(c, PIVX_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+    }
+    c->gf[0] = *f1;
+    c->gf[1] = *f2;
+  }
+  tic(c);
+  HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));
+  toc(c, 0, 5);
+  return PIVX_OK;
+}
+
+int pivx_set_graph_mode(pivx_ctx* c, int on) {
+  if (!c) return fail(nullptr, PIVX_ERR_PARAM, "null handle");
+  c->use_graph = on != 0;
+  if (!c->use_graph) drop_graph(c);
   return PIVX_OK;
 }
 
@@ -796,7 +855,7 @@ int pivx_enable_timing(pivx_ctx* c, int enable) {
 }
 
 int pivx_get_stage_ms(pivx_ctx* c, int grid, int stage, double* ms, int* launches) {
-  if (!c || (grid != 1 && grid != 2) || stage < 0 || stage > 4) return fail(c, PIVX_ERR_PARAM, "bad argument");
+  if (!c || (grid != 1 && grid != 2) || stage < 0 || stage > 5) return fail(c, PIVX_ERR_PARAM, "bad argument");
   HIPCHK(c, hipStreamSynchronize(c->stream));
   double sum = 0.0;
   int n = 0;

@@ -8,7 +8,7 @@ Bar (written per assertion below):
   * sub-pixel displacements within 1e-3 px;
   * FP32 element-wise stages (deformation, upsampling, median test, fill,
     smoothing) bit-exact against the oracle's FP32 transcription on identical
-    inputs, and within 1e-4 px of the FP64 oracle.
+    inputs; the deformed frame within 0.05 grey levels of the FP64 oracle.
 """
 import numpy as np
 import pytest
@@ -97,7 +97,9 @@ def test_deformation_bitexact_and_deformed_pass(pivx, orc):
     _report("deformation", shape=d.shape, max_diff_f32=float(np.abs(d - d32).max()),
             max_diff_f64=f"{float(np.abs(d - d64).max()):.2e}")
     np.testing.assert_array_equal(d, d32.astype(np.float32))
-    assert np.abs(d - d64).max() < 1e-3
+    # FP32 sample positions near x ~ 500 carry ~3e-5 px of rounding; times
+    # intensity gradients up to ~255 grey levels/px that is ~0.01-0.02.
+    assert np.abs(d - d64).max() < 0.05
     ref = orc.xcorr_pass(a, f2def=d.astype(np.float64), geom=g, ws=(32, 32), sp=(16, 16), K=4,
                          pred=(fx.astype(np.float64), fy.astype(np.float64)), tie_rel=tie_margin(32, 32))
     check_pass("config1 deformed pass", gpu, ref, (32, 32), 4)
@@ -309,3 +311,48 @@ def test_run_pair_matches_stepwise_and_is_deterministic(pivx):
         for u, w, z in zip(A, B, S):
             np.testing.assert_array_equal(u, w)
             np.testing.assert_array_equal(u, z)
+
+
+def test_graph_replays_match_eager(pivx):
+    """Graph mode: the captured per-pair sequence replayed several times, with
+    eager work and host synchronisation between replays, gives the eager
+    results bit for bit every time."""
+    nx, ny = 640, 480
+    sp_ = pivx.synth_params(nx, ny, seed=8, flow=1, a0=100.0, a1=5.0, a2=0.002)
+    a, b = pivx.synth_pair(sp_)
+    a2, b2 = pivx.synth_pair(pivx.synth_params(nx, ny, seed=9, flow=0, a0=1.0, a1=2.0))
+    p = config1_params(pivx, nx=nx, ny=ny, ws_x1=64, ws_y1=64, spacing_x1=32, spacing_y1=32, npass1=2,
+                       ws_x2=32, ws_y2=32, spacing_x2=16, spacing_y2=16, npass2=2,
+                       max_dispx_percent1=0.4, max_dispy_percent1=0.4, max_dispx_percent2=0.4,
+                       max_dispy_percent2=0.4)
+    fp = pivx.FilterParams(2, 5, 2.0, 2.0, 1)
+
+    def grab(x):
+        return [x.raw(1), x.filtered(1), x.raw(2), x.filtered(2), x.filtered(3)]
+
+    with pivx.Xcorr2dMulti(p) as x:
+        x.prep_img_data(a, b)
+        x.run_pair(fp, fp)
+        ref = grab(x)
+        x.prep_img_data(a2, b2)
+        x.run_pair(fp, fp)
+        ref2 = grab(x)
+        x.set_graph_mode(True)
+        for rep in range(3):
+            x.prep_img_data(a, b)
+            x.run_pair(fp, fp)
+            got = grab(x)
+            for A, B in zip(ref, got):
+                for u, w in zip(A, B):
+                    np.testing.assert_array_equal(u, w)
+            x.set_graph_mode(False)
+            x.prep_img_data(a2, b2)
+            x.run_pair(fp, fp)            # eager work between replays
+            x.set_graph_mode(True)
+            x.prep_img_data(a2, b2)
+            x.run_pair(fp, fp)
+            got2 = grab(x)
+            for A, B in zip(ref2, got2):
+                for u, w in zip(A, B):
+                    np.testing.assert_array_equal(u, w)
+    _report("graph replay", replays=6)

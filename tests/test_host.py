@@ -224,10 +224,9 @@ def _tiff_bytes(img, big=False, strips=1, short_offsets=False):
     if n == 1:
         out += ent(273, typ, 1, offs[0]) + ent(278, 4, 1, rows) + ent(279, typ, 1, len(data[0]))
     elif n * sz <= 4:  # two SHORT values fit in the entry itself (TIFF 6.0, IFD entry rule)
-        inline = lambda vals: struct.pack(e + "HHI", 0, typ, n)[:0] + struct.pack(e + "HH", *vals)  # noqa: E731
-        out += struct.pack(e + "HHI", 273, typ, n) + inline(offs)
+        out += struct.pack(e + "HHI", 273, typ, n) + struct.pack(e + "HH", *offs)
         out += ent(278, 4, 1, rows)
-        out += struct.pack(e + "HHI", 279, typ, n) + inline([len(d) for d in data])
+        out += struct.pack(e + "HHI", 279, typ, n) + struct.pack(e + "HH", *[len(d) for d in data])
     else:
         out += ent(273, typ, n, off_tab) + ent(278, 4, 1, rows) + ent(279, typ, n, cnt_tab)
     out += struct.pack(e + "I", 0)
