@@ -153,6 +153,7 @@ def main():
         fill2_ms, _ = x.stage_ms(2, 3)
         smooth2_ms, _ = x.stage_ms(2, 4)
         x.enable_timing(False)
+        copy_gbps = x.bandwidth_probe(1 << 30, 10)  # measured streaming-copy peak (read + write)
         st = x.stats()
         fdx, fdy, fv = x.filtered(2)
         gx, gy = x.grid_coords(2)
@@ -207,11 +208,13 @@ def main():
             "flops_per_window": f_win,
             "hbm_alg_GBps": round(hbm_alg, 1),
             "hbm_alg_frac": round(hbm_alg / PEAK_HBM_GBS, 4),
+            "measured_copy_GBps": round(copy_gbps, 1),
         },
         "stages_ms": {
             "grid1_corr": round(corr1_ms, 4),
             "grid2_deform": round(def2_ms, 4),
             "grid2_deform_GBps": round(def_bytes / (def2_ms * 1e-3) / 1e9, 1) if def2_ms > 0 else None,
+            "grid2_deform_hbm_frac": round(def_bytes / (def2_ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4) if def2_ms > 0 else None,
             "grid2_corr": round(corr2_ms, 4),
             "grid2_median": round(med2_ms, 4),
             "grid2_fill": round(fill2_ms, 4),

@@ -70,12 +70,12 @@ typedef struct pivx_filter_params {
   int npass_smooth;
 } pivx_filter_params;
 
-/* Counters of the most recent pass of each kind (no reference counterpart). */
+/* Diagnostics (no reference counterpart), per grid. */
 typedef struct pivx_stats {
-  long long overflow_windows[2];  /* windows whose local-maximum count exceeded the reference's slot cap */
-  long long outliers[2];          /* vectors whose first candidate failed the median test (last median pass) */
-  int fill_sweeps[2];             /* hole-filling sweeps run in the last validation of each grid */
-  long long oob_taps[2];          /* deformation taps that fell outside the padded frame (read as 0) */
+  long long overflow_windows[2];  /* last correlation pass: windows whose local-maximum count exceeds the reference's slot cap */
+  long long outliers[2];          /* last median pass: vectors whose first candidate was not the one accepted */
+  int fill_sweeps[2];             /* last hole filling: sweeps run, counted like the reference's loop */
+  long long oob_taps[2];          /* cumulative since init: deformation taps beyond the reference's padded buffer (read as 0) */
 } pivx_stats;
 
 /* ------------------------------------------------------------------ lifecycle */

@@ -131,8 +131,45 @@ struct FFT {
         out[k + (long)s * m] = acc;
       }
   }
+  // Power-of-two lengths: iterative radix-4/2 Stockham autosort passes.
+  static void stockham_pow2(C* x, int n, int sign, Plan& P) {
+    C* src = x;
+    C* dst = P.a.data();
+    for (int ns = 1; ns < n;) {
+      const int R = ((n / ns) % 4 == 0) ? 4 : 2;
+      const int nr = n / R, step = n / (ns * R);
+      for (int j = 0; j < nr; ++j) {
+        const int k = j & (ns - 1);
+        C v[4];
+        for (int r = 0; r < R; ++r) {
+          C w = P.tw[(long)r * k * step];
+          if (sign > 0) w = std::conj(w);
+          v[r] = src[j + r * nr] * w;
+        }
+        const int d = (j - k) * R + k;
+        if (R == 2) {
+          dst[d] = v[0] + v[1];
+          dst[d + ns] = v[0] - v[1];
+        } else {
+          const C a = v[0] + v[2], b = v[0] - v[2], c = v[1] + v[3], e0 = v[1] - v[3];
+          const C e = sign < 0 ? C(e0.imag(), -e0.real()) : C(-e0.imag(), e0.real());
+          dst[d] = a + c;
+          dst[d + ns] = b + e;
+          dst[d + 2 * ns] = a - c;
+          dst[d + 3 * ns] = b - e;
+        }
+      }
+      std::swap(src, dst);
+      ns *= R;
+    }
+    if (src != x) std::copy(src, src + n, x);
+  }
   static void transform(C* x, int n, int sign) {
     Plan& P = plan(n);
+    if ((n & (n - 1)) == 0) {
+      stockham_pow2(x, n, sign, P);
+      return;
+    }
     std::copy(x, x + n, P.a.begin());
     run(P.a.data(), n, 1, P.b.data(), sign, P.c.data(), P.tw.data(), 1);
     std::copy(P.b.begin(), P.b.end(), x);

@@ -49,6 +49,7 @@ struct GridBuf {
   float2* gscratch = nullptr;              // generic-path planes for big windows
   size_t gscratch_windows = 0;
   float* med_scratch = nullptr;            // generic median lists
+  int8_t* accepted = nullptr;              // [nt] accepted candidate of the last median pass
   bool deformed_once = false;
 };
 
@@ -63,7 +64,7 @@ struct pivx_ctx {
   uint8_t* mo_v = nullptr;
   uint8_t *f1 = nullptr, *f2 = nullptr;    // frames [ny][nx]
   bool have_frames = false;
-  unsigned long long* counters = nullptr;  // overflow[2], outliers[2], oob[2]
+  unsigned long long* counters = nullptr;  // [2]: cumulative out-of-buffer deformation taps per grid
   int* sweeps = nullptr;                   // [2]
   int *list_a = nullptr, *list_b = nullptr;
   float *tmp_x = nullptr, *tmp_y = nullptr;
@@ -249,11 +250,9 @@ int run_deform(pivx_ctx* c, int gi) {
   d.y_st = (float)(g.ws_y / 2 - g.spacing_y * kExBorder);
   d.spx = (float)g.spacing_x;
   d.spy = (float)g.spacing_y;
-  d.oob = c->counters + 4 + gi;
-  int rc = check_launch(c, launch_zero_u64(d.oob, 1, c->stream), "zero");
-  if (rc) return rc;
+  d.oob = c->counters + gi;
   tic(c);
-  rc = check_launch(c, launch_deform(d, c->stream), "Kernel_Window_Deformation_2d");
+  const int rc = check_launch(c, launch_deform(d, c->stream), "Kernel_Window_Deformation_2d");
   toc(c, gi, 0);
   G.deformed_once = true;
   return rc;
@@ -293,14 +292,10 @@ int run_xcorr(pivx_ctx* c, int gi, bool deformed) {
   a.pk_val = G.pkv;
   a.pk_idx = G.pki;
   a.ncand = G.ncand;
-  a.overflow = c->counters + 0 + gi;
-  a.slot_cap = g.npeak_slots;
-  int rc = check_launch(c, launch_zero_u64(a.overflow, 1, c->stream), "zero");
-  if (rc) return rc;
   tic(c);
   hipError_t e = G.fast ? launch_xcorr(a, g.ws_x, g.ws_y, nullptr, 0, nullptr, nullptr, c->stream)
                         : launch_xcorr(a, g.ws_x, g.ws_y, G.gscratch, G.gscratch_windows, G.twx, G.twy, c->stream);
-  rc = check_launch(c, e, "correlation");
+  const int rc = check_launch(c, e, "correlation");
   toc(c, gi, 1);
   return rc;
 }
@@ -322,8 +317,6 @@ int run_filter(pivx_ctx* c, int gi, int npass_median, int min_nneigh, float inse
     rc = check_launch(c, launch_prep(G.dx, G.dy, G.dv, K, ox, oy, ov, nt, c->stream), "median_filter_data_prep");
     if (rc) return rc;
   } else {
-    rc = check_launch(c, launch_zero_u64(c->counters + 2 + gi, 1, c->stream), "zero");
-    if (rc) return rc;
     const float* srcx = G.dx;
     const float* srcy = G.dy;
     const uint8_t* srcv = G.dv;
@@ -354,7 +347,7 @@ int run_filter(pivx_ctx* c, int gi, int npass_median, int min_nneigh, float inse
       m.min_nneigh = min_nneigh;
       m.thr = insert_thr;
       m.eps = kEpsilon;
-      m.outliers = (p == npass_median - 1) ? c->counters + 2 + gi : nullptr;
+      m.accepted = (p == npass_median - 1) ? G.accepted : nullptr;
       m.scratch = G.med_scratch;
       rc = check_launch(c, launch_median(m, c->stream), "median_filtering");
       if (rc) return rc;
@@ -434,6 +427,7 @@ int alloc_grid(pivx_ctx* c, int gi) {
   rc |= dalloc(c, &G.pkv, nt * K);
   rc |= dalloc(c, &G.pki, nt * K);
   rc |= dalloc(c, &G.ncand, nt);
+  rc |= dalloc(c, &G.accepted, nt);
   rc |= dalloc(c, &G.fx, nt);
   rc |= dalloc(c, &G.fy, nt);
   rc |= dalloc(c, &G.fv, nt);
@@ -453,6 +447,8 @@ int alloc_grid(pivx_ctx* c, int gi) {
   HIPCHK(c, hipMemsetAsync(G.dx, 0, nt * K * sizeof(float), c->stream));
   HIPCHK(c, hipMemsetAsync(G.dy, 0, nt * K * sizeof(float), c->stream));
   HIPCHK(c, hipMemsetAsync(G.dv, 0, nt * K, c->stream));
+  HIPCHK(c, hipMemsetAsync(G.ncand, 0, nt * sizeof(int), c->stream));
+  HIPCHK(c, hipMemsetAsync(G.accepted, 0, nt, c->stream));
   G.fast = xcorr_fast_supported(g.ws_x, g.ws_y);
   if (!G.fast) {
     std::vector<float2> tx, ty;
@@ -519,7 +515,7 @@ int pivx_init(const pivx_init_params* p, int device_id, pivx_ctx** out) {
   rc = 0;
   rc |= dalloc(c, &c->f1, npix);
   rc |= dalloc(c, &c->f2, npix);
-  rc |= dalloc(c, &c->counters, 8);
+  rc |= dalloc(c, &c->counters, 2);
   rc |= dalloc(c, &c->sweeps, 2);
   rc |= dalloc(c, &c->list_a, ntmax);
   rc |= dalloc(c, &c->list_b, ntmax);
@@ -531,7 +527,7 @@ int pivx_init(const pivx_init_params* p, int device_id, pivx_ctx** out) {
   rc |= dalloc(c, &c->mo_v, (size_t)g2.nt_vec);
   if (rc == 0) rc = alloc_grid(c, 0);
   if (rc == 0) rc = alloc_grid(c, 1);
-  if (rc == 0 && hipMemsetAsync(c->counters, 0, 8 * sizeof(unsigned long long), c->stream) != hipSuccess) rc = PIVX_ERR_HIP;
+  if (rc == 0 && hipMemsetAsync(c->counters, 0, 2 * sizeof(unsigned long long), c->stream) != hipSuccess) rc = PIVX_ERR_HIP;
   if (rc == 0 && hipMemsetAsync(c->sweeps, 0, 2 * sizeof(int), c->stream) != hipSuccess) rc = PIVX_ERR_HIP;
   if (rc == 0 && hipStreamSynchronize(c->stream) != hipSuccess) rc = fail(c, PIVX_ERR_HIP, "init synchronisation failed");
   if (rc) {
@@ -806,16 +802,29 @@ int pivx_get_deformed(pivx_ctx* c, int grid, float* out) {
 
 int pivx_get_stats(pivx_ctx* c, pivx_stats* st) {
   if (!c || !st) return fail(c, PIVX_ERR_PARAM, "bad argument");
-  unsigned long long h[8];
+  unsigned long long h[2];
   int sw[2];
   int rc = d2h(c, h, c->counters, sizeof(h)) | d2h(c, sw, c->sweeps, sizeof(sw));
   if (rc) return rc;
+  std::vector<int> nc[2];
+  std::vector<int8_t> acc[2];
+  for (int g = 0; g < 2; ++g) {
+    const size_t nt = (size_t)c->grid[g].g.nt_vec;
+    nc[g].resize(nt);
+    acc[g].resize(nt);
+    rc = d2h(c, nc[g].data(), c->grid[g].ncand, nt * sizeof(int)) | d2h(c, acc[g].data(), c->grid[g].accepted, nt);
+    if (rc) return rc;
+  }
   rc = pivx_synchronize(c);
   if (rc) return rc;
   for (int g = 0; g < 2; ++g) {
-    st->overflow_windows[g] = (long long)h[0 + g];
-    st->outliers[g] = (long long)h[2 + g];
-    st->oob_taps[g] = (long long)h[4 + g];
+    const int cap = c->grid[g].g.npeak_slots;
+    long long ov = 0, out = 0;
+    for (int v : nc[g]) ov += v > cap;
+    for (int8_t a : acc[g]) out += a != 0;
+    st->overflow_windows[g] = ov;
+    st->outliers[g] = out;
+    st->oob_taps[g] = (long long)h[g];
     st->fill_sweeps[g] = sw[g];
   }
   return PIVX_OK;

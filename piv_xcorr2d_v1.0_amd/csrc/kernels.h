@@ -21,9 +21,7 @@ struct XcArgs {
   const float* pred_y;
   float* disp_x; float* disp_y; uint8_t* valid;   // [nt_vec][K]
   float* pk_val; int* pk_idx;                     // [nt_vec][K]
-  int* ncand;                                     // [nt_vec]
-  unsigned long long* overflow;                   // windows with ncand > slot_cap
-  int slot_cap;
+  int* ncand;                                     // [nt_vec] uncapped local-maximum count
   int chunks_x;           // blocks per vector row (fast path)
 };
 
@@ -33,7 +31,7 @@ struct DefArgs {
   int hfws_x, hfws_y;     // only for counting taps outside the reference's padded buffer
   const float* ex_x; const float* ex_y; int nxe, nye;
   float x_st, y_st, spx, spy;
-  unsigned long long* oob;
+  unsigned long long* oob;   // cumulative count of taps beyond the reference's padded buffer
 };
 
 struct MedArgs {
@@ -42,7 +40,7 @@ struct MedArgs {
   float* dx; float* dy; uint8_t* dv;                                  // output filtered field
   int nx_vec, ny_vec, wx, wy, hfwx, hfwy, min_nneigh;
   float thr, eps;
-  unsigned long long* outliers;
+  int8_t* accepted;        // [nt] index of the accepted candidate, -1 if none (may be null)
   float* scratch;          // generic path only: 4*wx*wy floats per vector
 };
 

@@ -229,10 +229,7 @@ __global__ void __launch_bounds__((FastCfg<WX, WY, WPB>::NT)) k_xcorr_fast(XcArg
     cnt = wave_sum(cnt);
     const float px = a.pred_x ? a.pred_x[w] : 0.f;
     const float py = a.pred_y ? a.pred_y[w] : 0.f;
-    if (lane == 0) {
-      a.ncand[w] = cnt;
-      if (cnt > a.slot_cap) atomicAdd(a.overflow, 1ull);
-    }
+    if (lane == 0) a.ncand[w] = cnt;
     float pv = INFINITY;
     int pi = -1;
     for (int k = 0; k < a.K; ++k) {
@@ -418,7 +415,6 @@ __global__ void __launch_bounds__(256) k_xcorr_generic(XcArgs a, int WX, int WY,
     int t = 0;
     for (int q = 0; q < (int)((blockDim.x + 63) >> 6); ++q) t += cnt_s[q];
     a.ncand[w] = t;
-    if (t > a.slot_cap) atomicAdd(a.overflow, 1ull);
   }
   const float px = a.pred_x ? a.pred_x[w] : 0.f;
   const float py = a.pred_y ? a.pred_y[w] : 0.f;
@@ -608,7 +604,7 @@ __device__ __forceinline__ void median_decide(const MedArgs& a, int t, bool vmed
   a.dx[t] = ox;
   a.dy[t] = oy;
   a.dv[t] = ok ? 1 : 0;
-  if (a.outliers && acc != 0) atomicAdd(a.outliers, 1ull);
+  if (a.accepted) a.accepted[t] = (int8_t)acc;
 }
 
 template <int NS>
@@ -738,6 +734,7 @@ __global__ void __launch_bounds__(1024) k_fill(FillArgs a) {
   int sweeps = 0;
   while (true) {
     const int nh = n_holes;
+    ++sweeps;  // the reference runs one more (empty) sweep before its loop ends
     if (nh == 0) break;
     for (int h = threadIdx.x; h < nh; h += blockDim.x) {
       const int t = cur[h];
@@ -776,7 +773,6 @@ __global__ void __launch_bounds__(1024) k_fill(FillArgs a) {
     }
     if (filled) atomicAdd(&n_filled, filled);
     __syncthreads();
-    ++sweeps;
     const bool progress = n_filled > 0;
     __syncthreads();
     if (threadIdx.x == 0) {
