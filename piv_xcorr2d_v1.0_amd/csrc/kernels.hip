@@ -114,6 +114,21 @@ struct FastCfg {
   static constexpr int SX = WX + 1;
 };
 
+// Synchronisation of the threads that share the LDS planes: a single-wave
+// group only needs the compiler not to move LDS accesses across this point
+// (a wave's LDS operations are performed in issue order); larger groups use
+// the workgroup barrier.
+template <int NT>
+__device__ __forceinline__ void group_sync() {
+  if constexpr (NT <= 64) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  } else {
+    __syncthreads();
+  }
+}
+
 template <int WX, int WY, int WPB>
 __global__ void __launch_bounds__((FastCfg<WX, WY, WPB>::NT)) k_xcorr_fast(XcArgs a) {
   static_assert(WPB % 2 == 0, "windows are processed in pairs");
@@ -126,17 +141,32 @@ __global__ void __launch_bounds__((FastCfg<WX, WY, WPB>::NT)) k_xcorr_fast(XcArg
   const int iy = blockIdx.x / a.chunks_x;
   const int ix0 = (blockIdx.x - iy * a.chunks_x) * WPB;
 
-  // 1. gather
-  for (int e = tid; e < WPB * NPIX; e += NT) {
-    const int wl = e / NPIX;
-    const int rem = e - wl * NPIX;
-    const int r = rem / WX, c = rem - (rem / WX) * WX;
-    const int ix = ix0 + wl;
-    float2 z = make_float2(0.f, 0.f);
-    if (ix < a.nx_vec) z = gather_px(a, ix * a.sp_x + c, iy * a.sp_y + r);
-    pl[wl][r][c] = z;
+  // 1. gather: all of a thread's loads are issued before the LDS stores
+  {
+    constexpr int NG = (WPB * NPIX + NT - 1) / NT;
+    float2 z[NG];
+    static_for<NG>([&](auto gc) {
+      constexpr int g = gc;
+      const int e = tid + g * NT;
+      const int wl = e / NPIX;
+      const int rem = e - wl * NPIX;
+      const int r = rem / WX, c = rem - (rem / WX) * WX;
+      const int ix = ix0 + wl;
+      z[g] = make_float2(0.f, 0.f);
+      if ((WPB * NPIX % NT == 0 || e < WPB * NPIX) && ix < a.nx_vec)
+        z[g] = gather_px(a, ix * a.sp_x + c, iy * a.sp_y + r);
+    });
+    static_for<NG>([&](auto gc) {
+      constexpr int g = gc;
+      const int e = tid + g * NT;
+      if (WPB * NPIX % NT == 0 || e < WPB * NPIX) {
+        const int wl = e / NPIX;
+        const int rem = e - wl * NPIX;
+        pl[wl][rem / WX][rem - (rem / WX) * WX] = z[g];
+      }
+    });
   }
-  __syncthreads();
+  group_sync<NT>();
 
   // 2. forward row FFTs
   if (tid < WPB * WY) {
@@ -146,7 +176,7 @@ __global__ void __launch_bounds__((FastCfg<WX, WY, WPB>::NT)) k_xcorr_fast(XcArg
     fft_reg<WX, false>(x);
     static_for<WX>([&](auto c) { pl[wl][r][c] = x[c]; });
   }
-  __syncthreads();
+  group_sync<NT>();
 
   // 3. forward column FFTs
   if (tid < WPB * WX) {
@@ -156,7 +186,7 @@ __global__ void __launch_bounds__((FastCfg<WX, WY, WPB>::NT)) k_xcorr_fast(XcArg
     fft_reg<WY, false>(x);
     static_for<WY>([&](auto r) { pl[wl][r][c] = x[r]; });
   }
-  __syncthreads();
+  group_sync<NT>();
 
   // 4. split + conj-multiply + pair packing, inverse column FFTs
   {
@@ -173,13 +203,13 @@ __global__ void __launch_bounds__((FastCfg<WX, WY, WPB>::NT)) k_xcorr_fast(XcArg
         q[r] = make_float2(pa.x - pb.y, pa.y + pb.x);
       });
     }
-    __syncthreads();
+    group_sync<NT>();
     if (act) {
       fft_reg<WY, true>(q);
       static_for<WY>([&](auto r) { pl[2 * p][r][c] = q[r]; });
     }
   }
-  __syncthreads();
+  group_sync<NT>();
 
   // 5. inverse row FFTs -> shifted correlation planes
   if (tid < (WPB / 2) * WY) {
@@ -198,7 +228,7 @@ __global__ void __launch_bounds__((FastCfg<WX, WY, WPB>::NT)) k_xcorr_fast(XcArg
       ccb[rs * SX + cs] = (fabsf(x[c].y) * 0.25f) / (float)NPIX;
     });
   }
-  __syncthreads();
+  group_sync<NT>();
 
   // 6. peaks, one wave per window
   constexpr int NW = NT / 64;
@@ -230,37 +260,46 @@ __global__ void __launch_bounds__((FastCfg<WX, WY, WPB>::NT)) k_xcorr_fast(XcArg
     const float px = a.pred_x ? a.pred_x[w] : 0.f;
     const float py = a.pred_y ? a.pred_y[w] : 0.f;
     if (lane == 0) a.ncand[w] = cnt;
+    // Npeak_save rounds of arg-max; round k's winner is kept by lane k % 64
+    // and all fits of a group of 64 rounds run in parallel afterwards.
     float pv = INFINITY;
     int pi = -1;
-    for (int k = 0; k < a.K; ++k) {
-      float bv = 0.f;
-      int bi = 0x7fffffff;
-      static_for<PPL>([&](auto mc) {
-        constexpr int m = mc;
-        const int pidx = lane + 64 * m;
-        const float val = v[m];
-        const bool after = val < pv || (val == pv && pidx > pi);
-        if (val > 0.f && after && peak_before(val, pidx, bv, bi)) {
-          bv = val;
-          bi = pidx;
+    for (int k0 = 0; k0 < a.K; k0 += 64) {
+      const int kn = min(64, a.K - k0);
+      float my_v = 0.f;
+      int my_i = -1;
+      for (int k = 0; k < kn; ++k) {
+        float bv = 0.f;
+        int bi = 0x7fffffff;
+        static_for<PPL>([&](auto mc) {
+          constexpr int m = mc;
+          const int pidx = lane + 64 * m;
+          const float val = v[m];
+          const bool after = val < pv || (val == pv && pidx > pi);
+          if (val > 0.f && after && peak_before(val, pidx, bv, bi)) {
+            bv = val;
+            bi = pidx;
+          }
+        });
+        wave_argmax(bv, bi);
+        pv = bv;
+        pi = bi;
+        if (lane == k) {
+          my_v = bv;
+          my_i = bv > 0.f ? bi : -1;
         }
-      });
-      wave_argmax(bv, bi);
-      pv = bv;
-      pi = bi;
-      if (lane == 0) {
-        const long long o = (long long)w * a.K + k;
+      }
+      if (lane < kn) {
+        const long long o = (long long)w * a.K + k0 + lane;
         float dx = 0.f, dy = 0.f;
         bool ok = false;
-        if (bv > 0.f) {
-          const int i = bi % WX, j = bi / WX;
-          ok = subpixel_fit(cc, SX, WX, WY, i, j, bv, px, py, a.lx1, a.lx2, a.ly1, a.ly2, dx, dy);
-        }
+        if (my_v > 0.f) ok = subpixel_fit(cc, SX, WX, WY, my_i % WX, my_i / WX, my_v, px, py, a.lx1, a.lx2, a.ly1,
+                                          a.ly2, dx, dy);
         a.disp_x[o] = dx;
         a.disp_y[o] = dy;
         a.valid[o] = ok ? 1 : 0;
-        a.pk_val[o] = bv;
-        a.pk_idx[o] = bv > 0.f ? bi : -1;
+        a.pk_val[o] = my_v;
+        a.pk_idx[o] = my_i;
       }
     }
   }
@@ -453,24 +492,44 @@ __global__ void __launch_bounds__(256) k_xcorr_generic(XcArgs a, int WX, int WY,
 // displacement from the extended filtered field, then bilinear sample of frame
 // 2 at the displaced position.  Out-of-frame taps read 0 (the reference's zero
 // padding); taps beyond the reference's padded buffer are counted.
+// One thread per aug column, DEF_ROWS consecutive rows: the column's cell
+// coordinate is computed once and the four field samples of each cell are
+// reloaded only when the row crosses into the next cell.
+constexpr int DEF_ROWS = 16;
+
 __global__ void __launch_bounds__(256) k_deform(DefArgs a) {
 #pragma clang fp contract(off)
-  const long long n = (long long)a.nx_aug * a.ny_aug;
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= a.nx_aug) return;
   unsigned oob = 0;
-  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x) {
-    const int idy = (int)(t / a.nx_aug);
-    const int idx = (int)(t - (long long)idy * a.nx_aug);
-    float ax = ((float)idx - a.x_st) / a.spx;
+  const float ax0 = ((float)idx - a.x_st) / a.spx;
+  int ixf = (int)floorf(ax0);
+  const float ax = ax0 - (float)ixf;
+  ixf = min(max(ixf, 0), a.nxe - 2);
+  int cell = -1;
+  float ex1 = 0.f, ex2 = 0.f, ex3 = 0.f, ex4 = 0.f, ey1 = 0.f, ey2 = 0.f, ey3 = 0.f, ey4 = 0.f;
+  const int y_end = min(a.ny_aug, (int)(blockIdx.y + 1) * DEF_ROWS);
+  for (int idy = blockIdx.y * DEF_ROWS; idy < y_end; ++idy) {
+    const long long t = (long long)idy * a.nx_aug + idx;
     float ay = ((float)idy - a.y_st) / a.spy;
-    int ixf = (int)floorf(ax), iyf = (int)floorf(ay);
-    ax = ax - (float)ixf;
+    int iyf = (int)floorf(ay);
     ay = ay - (float)iyf;
     float ce1 = (1 - ax) * (1 - ay), ce2 = ax * (1 - ay), ce3 = (1 - ax) * ay, ce4 = ax * ay;
-    ixf = min(max(ixf, 0), a.nxe - 2);
     iyf = min(max(iyf, 0), a.nye - 2);
-    const int i1 = iyf * a.nxe + ixf, i2 = i1 + 1, i3 = i1 + a.nxe, i4 = i3 + 1;
-    float sx = (float)idx + ce1 * a.ex_x[i1] + ce2 * a.ex_x[i2] + ce3 * a.ex_x[i3] + ce4 * a.ex_x[i4];
-    float sy = (float)idy + ce1 * a.ex_y[i1] + ce2 * a.ex_y[i2] + ce3 * a.ex_y[i3] + ce4 * a.ex_y[i4];
+    if (iyf != cell) {
+      cell = iyf;
+      const int i1 = iyf * a.nxe + ixf, i3 = i1 + a.nxe;
+      ex1 = a.ex_x[i1];
+      ex2 = a.ex_x[i1 + 1];
+      ex3 = a.ex_x[i3];
+      ex4 = a.ex_x[i3 + 1];
+      ey1 = a.ex_y[i1];
+      ey2 = a.ex_y[i1 + 1];
+      ey3 = a.ex_y[i3];
+      ey4 = a.ex_y[i3 + 1];
+    }
+    float sx = (float)idx + ce1 * ex1 + ce2 * ex2 + ce3 * ex3 + ce4 * ex4;
+    float sy = (float)idy + ce1 * ey1 + ce2 * ey2 + ce3 * ey3 + ce4 * ey4;
     const int jx = (int)floorf(sx), jy = (int)floorf(sy);
     sx = sx - (float)jx;
     sy = sy - (float)jy;
@@ -863,16 +922,18 @@ static hipError_t launch_fast(const XcArgs& a0, hipStream_t s) {
   return hipGetLastError();
 }
 
+// (WX, WY, windows per workgroup).  Shapes up to 32 wide run one 64-lane wave
+// per workgroup (wave-level synchronisation only); larger ones use barriers.
 #define PIVX_FAST_SIZES(X) \
-  X(32, 32, 8)             \
+  X(32, 32, 2)             \
   X(64, 64, 2)             \
-  X(16, 16, 16)            \
-  X(24, 24, 8)             \
+  X(16, 16, 4)             \
+  X(24, 24, 2)             \
   X(48, 48, 4)             \
   X(64, 32, 4)             \
   X(32, 64, 4)             \
-  X(32, 16, 8)             \
-  X(16, 32, 8)             \
+  X(32, 16, 2)             \
+  X(16, 32, 2)             \
   X(48, 24, 8)
 
 bool xcorr_fast_supported(int ws_x, int ws_y) {
@@ -922,8 +983,8 @@ hipError_t launch_xcorr(const XcArgs& a, int ws_x, int ws_y, float2* gscratch, s
 }
 
 hipError_t launch_deform(const DefArgs& a, hipStream_t s) {
-  const long long n = (long long)a.nx_aug * a.ny_aug;
-  hipLaunchKernelGGL(k_deform, dim3(grid_for(n, 256, 1 << 20)), dim3(256), 0, s, a);
+  const dim3 grid((a.nx_aug + 255) / 256, (a.ny_aug + DEF_ROWS - 1) / DEF_ROWS);
+  hipLaunchKernelGGL(k_deform, grid, dim3(256), 0, s, a);
   return hipGetLastError();
 }
 
