@@ -305,6 +305,240 @@ __global__ void __launch_bounds__((FastCfg<WX, WY, WPB>::NT)) k_xcorr_fast(XcArg
   }
 }
 
+// ============================================================== wave-per-pair kernel
+//
+// Windows up to 32x32: one 64-lane wave owns two horizontally adjacent
+// windows a (lanes 0-31) and b (lanes 32-63); no workgroup barriers.
+//   1 each lane loads one window row of frame 1 and frame 2 straight into
+//     registers (wide loads; per-pixel zero fill only at the image border)
+//   2 row FFT in registers, rows to LDS
+//   3 column FFT, one column per lane, both windows at once
+//   4 Hermitian split per window on all 64 lanes (own column in registers,
+//     mirror column from LDS), window b publishes conj(F1)F2 through LDS,
+//     window a packs Q = Pa + i*Pb and inverse-transforms the columns
+//   5 inverse row FFTs give cc_a = Re, cc_b = Im (scaled, fftshifted)
+//   6 peaks: each half-wave walks its window by columns (3 LDS reads per
+//     pixel), Npeak_save rounds of 32-lane arg-max, winners removed in place,
+//     Gaussian fits in parallel.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ void half_argmax(float& bv, int& bi) {
+#pragma unroll
+  for (int off = 16; off > 0; off >>= 1) {
+    float ov = __shfl_xor(bv, off, 32);
+    int oi = __shfl_xor(bi, off, 32);
+    if (peak_before(ov, oi, bv, bi)) { bv = ov; bi = oi; }
+  }
+}
+
+__device__ __forceinline__ int half_sum(int v) {
+#pragma unroll
+  for (int off = 16; off > 0; off >>= 1) v += __shfl_xor(v, off, 32);
+  return v;
+}
+
+// WX consecutive u8 pixels as floats (4-byte chunk loads, any alignment).
+template <int WX>
+__device__ __forceinline__ void load_u8_row(const uint8_t* p, float (&f)[WX]) {
+  static_for<WX / 4>([&](auto qc) {
+    constexpr int q = qc;
+    uint32_t w;
+    __builtin_memcpy(&w, p + 4 * q, 4);
+    f[4 * q + 0] = (float)(w & 255u);
+    f[4 * q + 1] = (float)((w >> 8) & 255u);
+    f[4 * q + 2] = (float)((w >> 16) & 255u);
+    f[4 * q + 3] = (float)(w >> 24);
+  });
+}
+
+// One window row of WX pixels starting at aug coordinates (ax0, ay):
+// x[c] = (frame 1, frame 2).  Rows fully inside the image use wide loads;
+// others read pixel by pixel with 0 outside the frame.
+template <int WX>
+__device__ __forceinline__ void load_row(const XcArgs& a, int ax0, int ay, float2 (&x)[WX]) {
+  const int xi0 = ax0 - a.left_x, yi = ay - a.left_y;
+  const bool row_in = (unsigned)yi < (unsigned)a.ny;
+  const bool inside = row_in && xi0 >= 0 && xi0 + WX <= a.nx;
+  float f1[WX], f2[WX];
+  if (inside) {
+    load_u8_row<WX>(a.f1 + (long long)yi * a.pitch1 + xi0, f1);
+  } else {
+    static_for<WX>([&](auto cc) {
+      constexpr int c = cc;
+      f1[c] = (row_in && (unsigned)(xi0 + c) < (unsigned)a.nx) ? (float)a.f1[(long long)yi * a.pitch1 + xi0 + c] : 0.f;
+    });
+  }
+  if (a.f2def) {
+    const float* q = a.f2def + (long long)ay * a.def_pitch + ax0;
+    static_for<WX / 4>([&](auto qc) {
+      constexpr int k = qc;
+      float4 t;
+      __builtin_memcpy(&t, q + 4 * k, 16);
+      f2[4 * k + 0] = t.x;
+      f2[4 * k + 1] = t.y;
+      f2[4 * k + 2] = t.z;
+      f2[4 * k + 3] = t.w;
+    });
+  } else if (inside) {
+    load_u8_row<WX>(a.f2u8 + (long long)yi * a.pitch2 + xi0, f2);
+  } else {
+    static_for<WX>([&](auto cc) {
+      constexpr int c = cc;
+      f2[c] = (row_in && (unsigned)(xi0 + c) < (unsigned)a.nx) ? (float)a.f2u8[(long long)yi * a.pitch2 + xi0 + c] : 0.f;
+    });
+  }
+  static_for<WX>([&](auto c) { x[c] = make_float2(f1[c], f2[c]); });
+}
+
+template <int WX, int WY>
+__global__ void __launch_bounds__(64) k_xcorr_pair(XcArgs a) {
+  static_assert(WX <= 32 && WY <= 32 && WX % 4 == 0 && WY % 2 == 0, "pair kernel covers windows up to 32x32");
+  constexpr int SX = WX + 1;
+  constexpr int NPIX = WX * WY;
+  __shared__ float2 pl[2][WY][SX];
+
+  const int lane = threadIdx.x;
+  const int h = lane >> 5, l = lane & 31;
+  const int iy = blockIdx.x / a.chunks_x;
+  const int ix = (blockIdx.x - iy * a.chunks_x) * 2 + h;
+  const bool win_ok = ix < a.nx_vec;
+
+  // 1-2. row gather + forward row FFT
+  if (l < WY) {
+    float2 x[WX];
+    if (win_ok) {
+      load_row<WX>(a, ix * a.sp_x, iy * a.sp_y + l, x);
+    } else {
+      static_for<WX>([&](auto c) { x[c] = make_float2(0.f, 0.f); });
+    }
+    fft_reg<WX, false>(x);
+    static_for<WX>([&](auto c) { pl[h][l][c] = x[c]; });
+  }
+  wave_sync();
+
+  // 3. forward column FFTs (both windows)
+  float2 col[WY];
+  if (l < WX) {
+    static_for<WY>([&](auto r) { col[r] = pl[h][r][l]; });
+    fft_reg<WY, false>(col);
+    static_for<WY>([&](auto r) { pl[h][r][l] = col[r]; });
+  }
+  wave_sync();
+
+  // 4. split per window: P[ky] = 4 conj(F1) F2 at (ky, l)
+  if (l < WX) {
+    const int lm = (WX - l) % WX;
+    static_for<WY>([&](auto rc) {
+      constexpr int r = rc;
+      constexpr int rm = (WY - r) % WY;
+      col[r] = xprod(col[r], pl[h][rm][lm]);
+    });
+  }
+  wave_sync();
+  if (h == 1 && l < WX) static_for<WY>([&](auto r) { pl[1][r][l] = col[r]; });
+  wave_sync();
+  if (h == 0 && l < WX) {
+    static_for<WY>([&](auto r) {
+      const float2 pb = pl[1][r][l];
+      col[r] = make_float2(col[r].x - pb.y, col[r].y + pb.x);
+    });
+    fft_reg<WY, true>(col);
+    static_for<WY>([&](auto r) { pl[0][r][l] = col[r]; });
+  }
+  wave_sync();
+
+  // 5. inverse row FFTs -> shifted correlation planes (in plane 1)
+  float* const cc0 = reinterpret_cast<float*>(&pl[1][0][0]);
+  if (h == 0 && l < WY) {
+    float2 x[WX];
+    static_for<WX>([&](auto c) { x[c] = pl[0][l][c]; });
+    fft_reg<WX, true>(x);
+    const int rs = (l + WY / 2) % WY;
+    static_for<WX>([&](auto cc) {
+      constexpr int c = cc;
+      constexpr int cs = (c + WX / 2) % WX;
+      cc0[rs * SX + cs] = (fabsf(x[c].x) * 0.25f) / (float)NPIX;
+      cc0[WY * SX + rs * SX + cs] = (fabsf(x[c].y) * 0.25f) / (float)NPIX;
+    });
+  }
+  wave_sync();
+
+  // 6. peaks: half-wave h walks window h column l
+  const float* cc = cc0 + h * WY * SX;
+  const bool col_ok = l < WX && l >= a.bx && l <= WX - 1 - a.bx;
+  const int cl = l < WX ? l : WX - 1;
+  const int lft = cl > 0 ? cl - 1 : 0, rgt = cl < WX - 1 ? cl + 1 : WX - 1;
+  float v[WY];
+  int cnt = 0;
+  {
+    float up = cc[cl];
+    float cur = cc[SX + cl];
+    v[0] = 0.f;
+    v[WY - 1] = 0.f;
+    static_for<WY - 2>([&](auto jc) {
+      constexpr int j = jc + 1;
+      const float dn = cc[(j + 1) * SX + cl];
+      const float c0 = cur;
+      const float lv = cc[j * SX + lft], rv = cc[j * SX + rgt];
+      // non-short-circuit form keeps v[] in registers
+      const bool is_max = col_ok & (j >= a.by) & (j <= WY - 1 - a.by) & (c0 > 0.f) & (c0 >= up) & (c0 >= dn) &
+                          (c0 >= lv) & (c0 >= rv);
+      v[j] = is_max ? c0 : 0.f;
+      cnt += is_max ? 1 : 0;
+      up = cur;
+      cur = dn;
+    });
+  }
+  cnt = half_sum(cnt);
+  const int w = iy * a.nx_vec + ix;
+  if (win_ok && l == 0) a.ncand[w] = cnt;
+  const float px = (win_ok && a.pred_x) ? a.pred_x[w] : 0.f;
+  const float py = (win_ok && a.pred_y) ? a.pred_y[w] : 0.f;
+  for (int k0 = 0; k0 < a.K; k0 += 32) {
+    const int kn = min(32, a.K - k0);
+    float my_v = 0.f;
+    int my_i = -1;
+    for (int k = 0; k < kn; ++k) {
+      float bv = 0.f;
+      int bi = 0x7fffffff;
+      static_for<WY>([&](auto jc) {
+        constexpr int j = jc;
+        const bool g = v[j] > bv;  // strict: within a column the upper row (lower index) wins ties
+        bv = g ? v[j] : bv;
+        bi = g ? j * WX + cl : bi;
+      });
+      half_argmax(bv, bi);
+      {  // the owner lane removes the winner
+        const int jw = (bv > 0.f && bi % WX == cl) ? bi / WX : -1;
+        static_for<WY>([&](auto jc) {
+          constexpr int j = jc;
+          v[j] = (j == jw) ? 0.f : v[j];
+        });
+      }
+      if (l == k) {
+        my_v = bv;
+        my_i = bv > 0.f ? bi : -1;
+      }
+    }
+    if (win_ok && l < kn) {
+      const long long o = (long long)w * a.K + k0 + l;
+      float dx = 0.f, dy = 0.f;
+      bool ok = false;
+      if (my_v > 0.f)
+        ok = subpixel_fit(cc, SX, WX, WY, my_i % WX, my_i / WX, my_v, px, py, a.lx1, a.lx2, a.ly1, a.ly2, dx, dy);
+      a.disp_x[o] = dx;
+      a.disp_y[o] = dy;
+      a.valid[o] = ok ? 1 : 0;
+      a.pk_val[o] = my_v;
+      a.pk_idx[o] = my_i;
+    }
+  }
+}
+
 // ============================================================== generic fused kernel
 //
 // Any legal window (4..1024, multiple of 4, radices from the host plan).  One
@@ -913,8 +1147,17 @@ static inline int grid_for(long long n, int block, int cap = 65536) {
   return (int)g;
 }
 
+template <int WX, int WY>
+static hipError_t launch_pair(const XcArgs& a0, hipStream_t s) {
+  XcArgs a = a0;
+  a.chunks_x = (a.nx_vec + 1) / 2;
+  hipLaunchKernelGGL((k_xcorr_pair<WX, WY>), dim3(a.chunks_x * a.ny_vec), dim3(64), 0, s, a);
+  return hipGetLastError();
+}
+
 template <int WX, int WY, int WPB>
 static hipError_t launch_fast(const XcArgs& a0, hipStream_t s) {
+  if constexpr (WX <= 32 && WY <= 32 && WX + WY > 32) return launch_pair<WX, WY>(a0, s);
   XcArgs a = a0;
   a.chunks_x = (a.nx_vec + WPB - 1) / WPB;
   const int blocks = a.chunks_x * a.ny_vec;
