@@ -72,13 +72,18 @@ struct pivx_ctx {
   std::vector<void*> allocs;
   std::string err;
   // Stage timing: event pairs recorded on the stream without host waits,
-  // resolved by pivx_get_stage_ms.
-  bool timing = false;
-  std::vector<hipEvent_t> ev_pool;
-  size_t ev_used = 0;
-  int ev_start = -1;
+  // resolved by pivx_get_stage_ms.  The captured graph carries its own event
+  // records, so replays are timed from inside the graph.
   struct Rec { int gi, stage, a, b; };
-  std::vector<Rec> recs;
+  struct Timeline {
+    std::vector<hipEvent_t> pool;
+    size_t used = 0;
+    int start = -1;
+    std::vector<Rec> recs;
+  };
+  bool timing = false;
+  Timeline eager_tl, graph_tl;
+  Timeline* tl = &eager_tl;
   // Graph mode: pivx_run_pair replays one captured hipGraph of the whole
   // per-pair sequence (re-captured when the filter parameters change).
   bool use_graph = false;
@@ -199,26 +204,28 @@ GridBuf* grid_of(pivx_ctx* c, int grid) {
   return &c->grid[grid - 1];
 }
 
-int next_event(pivx_ctx* c) {
-  if (c->ev_used == c->ev_pool.size()) {
+int next_event(pivx_ctx::Timeline& t) {
+  if (t.used == t.pool.size()) {
     hipEvent_t e = nullptr;
     if (hipEventCreate(&e) != hipSuccess) return -1;
-    c->ev_pool.push_back(e);
+    t.pool.push_back(e);
   }
-  return (int)c->ev_used++;
+  return (int)t.used++;
 }
 void tic(pivx_ctx* c) {
   if (!c->timing) return;
-  c->ev_start = next_event(c);
-  if (c->ev_start >= 0) (void)hipEventRecord(c->ev_pool[c->ev_start], c->stream);
+  pivx_ctx::Timeline& t = *c->tl;
+  t.start = next_event(t);
+  if (t.start >= 0) (void)hipEventRecord(t.pool[t.start], c->stream);
 }
 void toc(pivx_ctx* c, int gi, int stage) {
-  if (!c->timing || c->ev_start < 0) return;
-  const int e = next_event(c);
+  pivx_ctx::Timeline& t = *c->tl;
+  if (!c->timing || t.start < 0) return;
+  const int e = next_event(t);
   if (e < 0) return;
-  (void)hipEventRecord(c->ev_pool[e], c->stream);
-  c->recs.push_back({gi, stage, c->ev_start, e});
-  c->ev_start = -1;
+  (void)hipEventRecord(t.pool[e], c->stream);
+  t.recs.push_back({gi, stage, t.start, e});
+  t.start = -1;
 }
 
 int check_launch(pivx_ctx* c, hipError_t e, const char* what) {
@@ -548,7 +555,8 @@ int pivx_freemem(pivx_ctx* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (void* q : c->allocs) (void)hipFree(q);
   drop_graph(c);
-  for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
+  for (hipEvent_t e : c->eager_tl.pool) (void)hipEventDestroy(e);
+  for (hipEvent_t e : c->graph_tl.pool) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return PIVX_OK;
@@ -691,13 +699,21 @@ int pivx_run_pair(pivx_ctx* c, const pivx_filter_params* f1, const pivx_filter_p
   if (!c->use_graph) return run_pair_enqueue(c, f1, f2);
   if (!c->gexec || !same_filter(c->gf[0], *f1) || !same_filter(c->gf[1], *f2)) {
     drop_graph(c);
+    // the captured sequence records its own stage events (replays overwrite them)
     const bool timing = c->timing;
-    c->timing = false;  // no event records inside the captured sequence
+    c->timing = true;
+    c->graph_tl.used = 0;
+    c->graph_tl.start = -1;
+    c->graph_tl.recs.clear();
+    c->tl = &c->graph_tl;
+    for (int k = 0; k < 256; ++k) next_event(c->graph_tl);  // create events before capture begins
+    c->graph_tl.used = 0;
     HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     int rc = run_pair_enqueue(c, f1, f2);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(c->stream, &g);
     c->timing = timing;
+    c->tl = &c->eager_tl;
     if (rc) {
       if (g) (void)hipGraphDestroy(g);
       return rc;
@@ -857,9 +873,9 @@ int pivx_set_raw(pivx_ctx* c, int grid, const float* dx, const float* dy, const 
 int pivx_enable_timing(pivx_ctx* c, int enable) {
   if (!c) return fail(nullptr, PIVX_ERR_PARAM, "null handle");
   c->timing = enable != 0;
-  c->ev_used = 0;
-  c->ev_start = -1;
-  c->recs.clear();
+  c->eager_tl.used = 0;
+  c->eager_tl.start = -1;
+  c->eager_tl.recs.clear();
   return PIVX_OK;
 }
 
@@ -868,10 +884,11 @@ int pivx_get_stage_ms(pivx_ctx* c, int grid, int stage, double* ms, int* launche
   HIPCHK(c, hipStreamSynchronize(c->stream));
   double sum = 0.0;
   int n = 0;
-  for (const auto& r : c->recs) {
+  const pivx_ctx::Timeline& tl = (stage != 5 && c->use_graph && c->gexec) ? c->graph_tl : c->eager_tl;
+  for (const auto& r : tl.recs) {
     if (r.gi != grid - 1 || r.stage != stage) continue;
     float t = 0.f;
-    HIPCHK(c, hipEventElapsedTime(&t, c->ev_pool[r.a], c->ev_pool[r.b]));
+    HIPCHK(c, hipEventElapsedTime(&t, tl.pool[r.a], tl.pool[r.b]));
     sum += t;
     ++n;
   }

@@ -539,6 +539,161 @@ __global__ void __launch_bounds__(64) k_xcorr_pair(XcArgs a) {
   }
 }
 
+// ============================================================== wave-per-window kernel
+//
+// Windows up to 64x64: one 64-lane wave owns one window, no workgroup
+// barriers.  The product P = 4 conj(F1) F2 is Hermitian (the correlation is
+// real), so the inverse runs on half the spectrum: column IFFTs for
+// kx = 0..WX/2, then real-output row IFFTs two rows at a time
+// (H = G[y] + i G[y + WY/2], with G[y][-kx] = conj(G[y][kx])).
+template <int WX, int WY>
+__global__ void __launch_bounds__(64) k_xcorr_single(XcArgs a) {
+  static_assert(WX <= 64 && WY <= 64 && WX % 4 == 0 && WY % 4 == 0, "single-wave kernel covers windows up to 64x64");
+  constexpr int SX = WX + 1;
+  constexpr int NPIX = WX * WY;
+  constexpr int HX = WX / 2, HY = WY / 2;
+  __shared__ float2 pl[WY][SX];
+
+  const int lane = threadIdx.x;
+  const int iy = blockIdx.x / a.nx_vec;
+  const int ix = blockIdx.x - iy * a.nx_vec;
+
+  // 1-2. row gather + forward row FFT
+  if (lane < WY) {
+    float2 x[WX];
+    load_row<WX>(a, ix * a.sp_x, iy * a.sp_y + lane, x);
+    fft_reg<WX, false>(x);
+    static_for<WX>([&](auto c) { pl[lane][c] = x[c]; });
+  }
+  wave_sync();
+
+  // 3. forward column FFTs
+  float2 col[WY];
+  if (lane < WX) {
+    static_for<WY>([&](auto r) { col[r] = pl[r][lane]; });
+    fft_reg<WY, false>(col);
+    static_for<WY>([&](auto r) { pl[r][lane] = col[r]; });
+  }
+  wave_sync();
+
+  // 4. split, then inverse column FFTs of the non-negative half of the spectrum
+  if (lane <= HX) {
+    const int lm = (WX - lane) % WX;
+    static_for<WY>([&](auto rc) {
+      constexpr int r = rc;
+      constexpr int rm = (WY - r) % WY;
+      col[r] = xprod(col[r], pl[rm][lm]);
+    });
+    fft_reg<WY, true>(col);
+  }
+  wave_sync();  // every mirror read is done before the plane is overwritten
+  if (lane <= HX) static_for<WY>([&](auto r) { pl[r][lane] = col[r]; });
+  wave_sync();
+
+  // 5. real-output row IFFTs, rows y and y + WY/2 packed in one transform
+  float* const ccp = reinterpret_cast<float*>(&pl[0][0]);
+  float2 hrow[WX];
+  if (lane < HY) {
+    static_for<WX>([&](auto xc) {
+      constexpr int x = xc;
+      float2 g0, g1;
+      if constexpr (x <= HX) {
+        g0 = pl[lane][x];
+        g1 = pl[lane + HY][x];
+      } else {
+        g0 = pl[lane][WX - x];
+        g1 = pl[lane + HY][WX - x];
+        g0.y = -g0.y;
+        g1.y = -g1.y;
+      }
+      hrow[x] = make_float2(g0.x - g1.y, g0.y + g1.x);
+    });
+    fft_reg<WX, true>(hrow);
+  }
+  wave_sync();  // all of G is read before the correlation plane overwrites it
+  if (lane < HY) {
+    const int r0 = lane, r1 = lane + HY;
+    const int rs0 = (r0 + HY) % WY, rs1 = (r1 + HY) % WY;
+    static_for<WX>([&](auto xc) {
+      constexpr int x = xc;
+      constexpr int xs = (x + HX) % WX;
+      ccp[rs0 * SX + xs] = (fabsf(hrow[x].x) * 0.25f) / (float)NPIX;
+      ccp[rs1 * SX + xs] = (fabsf(hrow[x].y) * 0.25f) / (float)NPIX;
+    });
+  }
+  wave_sync();
+
+  // 6. peaks: lane l walks column l of the window
+  const float* cc = ccp;
+  const bool col_ok = lane < WX && lane >= a.bx && lane <= WX - 1 - a.bx;
+  const int cl = lane < WX ? lane : WX - 1;
+  const int lft = cl > 0 ? cl - 1 : 0, rgt = cl < WX - 1 ? cl + 1 : WX - 1;
+  float v[WY];
+  int cnt = 0;
+  {
+    float up = cc[cl];
+    float cur = cc[SX + cl];
+    v[0] = 0.f;
+    v[WY - 1] = 0.f;
+    static_for<WY - 2>([&](auto jc) {
+      constexpr int j = jc + 1;
+      const float dn = cc[(j + 1) * SX + cl];
+      const float c0 = cur;
+      const float lv = cc[j * SX + lft], rv = cc[j * SX + rgt];
+      const bool is_max = col_ok & (j >= a.by) & (j <= WY - 1 - a.by) & (c0 > 0.f) & (c0 >= up) & (c0 >= dn) &
+                          (c0 >= lv) & (c0 >= rv);
+      v[j] = is_max ? c0 : 0.f;
+      cnt += is_max ? 1 : 0;
+      up = cur;
+      cur = dn;
+    });
+  }
+  cnt = wave_sum(cnt);
+  const int w = blockIdx.x;
+  if (lane == 0) a.ncand[w] = cnt;
+  const float px = a.pred_x ? a.pred_x[w] : 0.f;
+  const float py = a.pred_y ? a.pred_y[w] : 0.f;
+  for (int k0 = 0; k0 < a.K; k0 += 64) {
+    const int kn = min(64, a.K - k0);
+    float my_v = 0.f;
+    int my_i = -1;
+    for (int k = 0; k < kn; ++k) {
+      float bv = 0.f;
+      int bi = 0x7fffffff;
+      static_for<WY>([&](auto jc) {
+        constexpr int j = jc;
+        const bool g = v[j] > bv;
+        bv = g ? v[j] : bv;
+        bi = g ? j * WX + cl : bi;
+      });
+      wave_argmax(bv, bi);
+      {
+        const int jw = (bv > 0.f && lane < WX && bi % WX == lane) ? bi / WX : -1;
+        static_for<WY>([&](auto jc) {
+          constexpr int j = jc;
+          v[j] = (j == jw) ? 0.f : v[j];
+        });
+      }
+      if (lane == k) {
+        my_v = bv;
+        my_i = bv > 0.f ? bi : -1;
+      }
+    }
+    if (lane < kn) {
+      const long long o = (long long)w * a.K + k0 + lane;
+      float dx = 0.f, dy = 0.f;
+      bool ok = false;
+      if (my_v > 0.f)
+        ok = subpixel_fit(cc, SX, WX, WY, my_i % WX, my_i / WX, my_v, px, py, a.lx1, a.lx2, a.ly1, a.ly2, dx, dy);
+      a.disp_x[o] = dx;
+      a.disp_y[o] = dy;
+      a.valid[o] = ok ? 1 : 0;
+      a.pk_val[o] = my_v;
+      a.pk_idx[o] = my_i;
+    }
+  }
+}
+
 // ============================================================== generic fused kernel
 //
 // Any legal window (4..1024, multiple of 4, radices from the host plan).  One
@@ -1155,9 +1310,16 @@ static hipError_t launch_pair(const XcArgs& a0, hipStream_t s) {
   return hipGetLastError();
 }
 
+template <int WX, int WY>
+static hipError_t launch_single(const XcArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL((k_xcorr_single<WX, WY>), dim3(a.nx_vec * a.ny_vec), dim3(64), 0, s, a);
+  return hipGetLastError();
+}
+
 template <int WX, int WY, int WPB>
 static hipError_t launch_fast(const XcArgs& a0, hipStream_t s) {
   if constexpr (WX <= 32 && WY <= 32 && WX + WY > 32) return launch_pair<WX, WY>(a0, s);
+  if constexpr (WX <= 64 && WY <= 64 && (WX > 32 || WY > 32)) return launch_single<WX, WY>(a0, s);
   XcArgs a = a0;
   a.chunks_x = (a.nx_vec + WPB - 1) / WPB;
   const int blocks = a.chunks_x * a.ny_vec;
