@@ -148,7 +148,21 @@ def main():
             x.run_pair(fp, fp)
         x.synchronize()
         dt = time.perf_counter() - t0
-        # per-stage HIP events of the timed steps (in graph mode: of the last replay)
+        pair_gpu_ms, _ = x.stage_ms(1, 5)  # HIP events around each graph replay
+        if not args.eager:
+            # Stage events cannot sit inside a replayed graph: the same K pairs
+            # are run once more kernel by kernel with HIP events around every
+            # stage on the engine stream (kernel durations do not depend on
+            # how the launch was issued; the graph only removes launch gaps).
+            x.set_graph_mode(False)
+            x.enable_timing(True)
+            t1 = time.perf_counter()
+            for _ in range(args.steps):
+                x.run_pair(fp, fp)
+            x.synchronize()
+            dt_eager = time.perf_counter() - t1
+        else:
+            dt_eager = dt
         corr2_ms, n_corr2 = x.stage_ms(2, 1)
         corr1_ms, _ = x.stage_ms(1, 1)
         def2_ms, _ = x.stage_ms(2, 0)
@@ -198,6 +212,8 @@ def main():
             "grid2_corr_windows_per_s": round(g2.nt_vec / (corr2_ms * 1e-3), 1),
             "parallelism": "single GPU",
             "launch": "eager" if args.eager else "hipGraph per pair",
+            "pair_gpu_ms": round(pair_gpu_ms, 4) if pair_gpu_ms else None,
+            "eager_ms_per_pair": round(dt_eager * 1000.0 / args.steps, 4),
         },
         "roofline": {
             "bound": "mfma",

@@ -66,14 +66,14 @@ struct pivx_ctx {
   bool have_frames = false;
   unsigned long long* counters = nullptr;  // [2]: cumulative out-of-buffer deformation taps per grid
   int* sweeps = nullptr;                   // [2]
+  int* nholes = nullptr;                   // hole-list counter of the fill kernels
   int *list_a = nullptr, *list_b = nullptr;
   float *tmp_x = nullptr, *tmp_y = nullptr;
   uint8_t* tmp_f = nullptr;
   std::vector<void*> allocs;
   std::string err;
   // Stage timing: event pairs recorded on the stream without host waits,
-  // resolved by pivx_get_stage_ms.  The captured graph carries its own event
-  // records, so replays are timed from inside the graph.
+  // resolved by pivx_get_stage_ms.  Graph replays are timed whole (stage 5).
   struct Rec { int gi, stage, a, b; };
   struct Timeline {
     std::vector<hipEvent_t> pool;
@@ -82,8 +82,7 @@ struct pivx_ctx {
     std::vector<Rec> recs;
   };
   bool timing = false;
-  Timeline eager_tl, graph_tl;
-  Timeline* tl = &eager_tl;
+  Timeline tl;
   // Graph mode: pivx_run_pair replays one captured hipGraph of the whole
   // per-pair sequence (re-captured when the filter parameters change).
   bool use_graph = false;
@@ -214,12 +213,12 @@ int next_event(pivx_ctx::Timeline& t) {
 }
 void tic(pivx_ctx* c) {
   if (!c->timing) return;
-  pivx_ctx::Timeline& t = *c->tl;
+  pivx_ctx::Timeline& t = c->tl;
   t.start = next_event(t);
   if (t.start >= 0) (void)hipEventRecord(t.pool[t.start], c->stream);
 }
 void toc(pivx_ctx* c, int gi, int stage) {
-  pivx_ctx::Timeline& t = *c->tl;
+  pivx_ctx::Timeline& t = c->tl;
   if (!c->timing || t.start < 0) return;
   const int e = next_event(t);
   if (e < 0) return;
@@ -377,6 +376,7 @@ int run_filter(pivx_ctx* c, int gi, int npass_median, int min_nneigh, float inse
     f.tmp_x = c->tmp_x;
     f.tmp_y = c->tmp_y;
     f.tmp_f = c->tmp_f;
+    f.nholes = c->nholes;
     f.sweeps = c->sweeps + gi;
     tic(c);
     rc = check_launch(c, launch_fill(f, c->stream), "fill_blank_vecfield");
@@ -524,6 +524,7 @@ int pivx_init(const pivx_init_params* p, int device_id, pivx_ctx** out) {
   rc |= dalloc(c, &c->f2, npix);
   rc |= dalloc(c, &c->counters, 2);
   rc |= dalloc(c, &c->sweeps, 2);
+  rc |= dalloc(c, &c->nholes, 1);
   rc |= dalloc(c, &c->list_a, ntmax);
   rc |= dalloc(c, &c->list_b, ntmax);
   rc |= dalloc(c, &c->tmp_x, ntmax);
@@ -536,6 +537,7 @@ int pivx_init(const pivx_init_params* p, int device_id, pivx_ctx** out) {
   if (rc == 0) rc = alloc_grid(c, 1);
   if (rc == 0 && hipMemsetAsync(c->counters, 0, 2 * sizeof(unsigned long long), c->stream) != hipSuccess) rc = PIVX_ERR_HIP;
   if (rc == 0 && hipMemsetAsync(c->sweeps, 0, 2 * sizeof(int), c->stream) != hipSuccess) rc = PIVX_ERR_HIP;
+  if (rc == 0 && hipMemsetAsync(c->nholes, 0, sizeof(int), c->stream) != hipSuccess) rc = PIVX_ERR_HIP;
   if (rc == 0 && hipStreamSynchronize(c->stream) != hipSuccess) rc = fail(c, PIVX_ERR_HIP, "init synchronisation failed");
   if (rc) {
     std::string msg = c->err;
@@ -555,8 +557,7 @@ int pivx_freemem(pivx_ctx* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (void* q : c->allocs) (void)hipFree(q);
   drop_graph(c);
-  for (hipEvent_t e : c->eager_tl.pool) (void)hipEventDestroy(e);
-  for (hipEvent_t e : c->graph_tl.pool) (void)hipEventDestroy(e);
+  for (hipEvent_t e : c->tl.pool) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return PIVX_OK;
@@ -699,21 +700,14 @@ int pivx_run_pair(pivx_ctx* c, const pivx_filter_params* f1, const pivx_filter_p
   if (!c->use_graph) return run_pair_enqueue(c, f1, f2);
   if (!c->gexec || !same_filter(c->gf[0], *f1) || !same_filter(c->gf[1], *f2)) {
     drop_graph(c);
-    // the captured sequence records its own stage events (replays overwrite them)
+    // stage events are not captured: a replay is timed as a whole (stage 5)
     const bool timing = c->timing;
-    c->timing = true;
-    c->graph_tl.used = 0;
-    c->graph_tl.start = -1;
-    c->graph_tl.recs.clear();
-    c->tl = &c->graph_tl;
-    for (int k = 0; k < 256; ++k) next_event(c->graph_tl);  // create events before capture begins
-    c->graph_tl.used = 0;
+    c->timing = false;
     HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     int rc = run_pair_enqueue(c, f1, f2);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(c->stream, &g);
     c->timing = timing;
-    c->tl = &c->eager_tl;
     if (rc) {
       if (g) (void)hipGraphDestroy(g);
       return rc;
@@ -873,9 +867,9 @@ int pivx_set_raw(pivx_ctx* c, int grid, const float* dx, const float* dy, const 
 int pivx_enable_timing(pivx_ctx* c, int enable) {
   if (!c) return fail(nullptr, PIVX_ERR_PARAM, "null handle");
   c->timing = enable != 0;
-  c->eager_tl.used = 0;
-  c->eager_tl.start = -1;
-  c->eager_tl.recs.clear();
+  c->tl.used = 0;
+  c->tl.start = -1;
+  c->tl.recs.clear();
   return PIVX_OK;
 }
 
@@ -884,7 +878,7 @@ int pivx_get_stage_ms(pivx_ctx* c, int grid, int stage, double* ms, int* launche
   HIPCHK(c, hipStreamSynchronize(c->stream));
   double sum = 0.0;
   int n = 0;
-  const pivx_ctx::Timeline& tl = (stage != 5 && c->use_graph && c->gexec) ? c->graph_tl : c->eager_tl;
+  const pivx_ctx::Timeline& tl = c->tl;
   for (const auto& r : tl.recs) {
     if (r.gi != grid - 1 || r.stage != stage) continue;
     float t = 0.f;

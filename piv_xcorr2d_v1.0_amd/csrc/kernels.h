@@ -48,6 +48,7 @@ struct FillArgs {
   float* dx; float* dy; uint8_t* dv;
   int nx_vec, ny_vec;
   int* list_a; int* list_b; float* tmp_x; float* tmp_y; uint8_t* tmp_f;
+  int* nholes;             // hole counter, 0 between fills
   int* sweeps;
 };
 

@@ -1165,17 +1165,21 @@ __global__ void __launch_bounds__(128) k_median_generic(MedArgs a) {
 // round trip is needed per sweep (the reference copies a counter back per
 // sweep, cu:1527).  Each sweep first computes every hole's 3x3 mean of valid
 // neighbours (raster sum order), then applies all fills at once.
+// Hole list of the field (any order: each sweep's result is independent of
+// the order holes are visited in).  *nholes is 0 on entry; k_fill resets it.
+__global__ void __launch_bounds__(256) k_fill_list(FillArgs a) {
+  const int nt = a.nx_vec * a.ny_vec;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nt; t += gridDim.x * blockDim.x)
+    if (!a.dv[t]) a.list_a[atomicAdd(a.nholes, 1)] = t;
+}
+
 __global__ void __launch_bounds__(1024) k_fill(FillArgs a) {
 #pragma clang fp contract(off)
   __shared__ int n_holes, n_next, n_filled;
-  const int nt = a.nx_vec * a.ny_vec;
   if (threadIdx.x == 0) {
-    n_holes = 0;
+    n_holes = *a.nholes;
     n_filled = 0;
   }
-  __syncthreads();
-  for (int t = threadIdx.x; t < nt; t += blockDim.x)
-    if (!a.dv[t]) a.list_a[atomicAdd(&n_holes, 1)] = t;
   __syncthreads();
   int* cur = a.list_a;
   int* nxt = a.list_b;
@@ -1233,7 +1237,10 @@ __global__ void __launch_bounds__(1024) k_fill(FillArgs a) {
     cur = nxt;
     nxt = tt;
   }
-  if (threadIdx.x == 0 && a.sweeps) *a.sweeps = sweeps;
+  if (threadIdx.x == 0) {
+    if (a.sweeps) *a.sweeps = sweeps;
+    *a.nholes = 0;
+  }
 }
 
 // ============================================================== smoothing
@@ -1424,6 +1431,9 @@ hipError_t launch_median(const MedArgs& a, hipStream_t s) {
 }
 
 hipError_t launch_fill(const FillArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(k_fill_list, dim3(grid_for((long long)a.nx_vec * a.ny_vec, 256, 2048)), dim3(256), 0, s, a);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_fill, dim3(1), dim3(1024), 0, s, a);
   return hipGetLastError();
 }
