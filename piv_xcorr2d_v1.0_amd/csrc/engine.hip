@@ -179,6 +179,9 @@ int validate(const pivx_init_params* p) {
   if (p->ws_y1 % 4 != 0 || p->ws_y2 % 4 != 0) return fail(nullptr, PIVX_ERR_PARAM, "The window y-dimension is not divisible by 4!");
   // Inputs on which the reference divides by zero or indexes out of bounds.
   if (p->nx < 1 || p->ny < 1) return fail(nullptr, PIVX_ERR_PARAM, "image size must be positive (nx=%d, ny=%d)", p->nx, p->ny);
+  // the reference indexes frames with int (cu:933-954); frame offsets here are 32-bit too
+  if ((long long)p->nx * p->ny > 0x7fffffffLL || p->nx >= (1 << 24) || p->ny >= (1 << 24))
+    return fail(nullptr, PIVX_ERR_PARAM, "image too large (nx*ny = %lld pixels, at most 2^31-1)", (long long)p->nx * p->ny);
   if (p->nsec_cc < 1) return fail(nullptr, PIVX_ERR_PARAM, "Nsec_cc must be >= 1 (got %d)", p->nsec_cc);
   if (p->spacing_x1 < 1 || p->spacing_y1 < 1 || p->spacing_x2 < 1 || p->spacing_y2 < 1)
     return fail(nullptr, PIVX_ERR_PARAM, "vector spacing must be >= 1");

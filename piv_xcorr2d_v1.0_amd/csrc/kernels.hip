@@ -20,6 +20,7 @@
 #include "kernels.h"
 #include "fft_reg.h"
 #include <math.h>
+#include <type_traits>
 
 namespace pivx {
 
@@ -48,6 +49,16 @@ __device__ __forceinline__ void wave_argmax(float& bv, int& bi) {
     int oi = __shfl_xor(bi, off, 64);
     if (peak_before(ov, oi, bv, bi)) { bv = ov; bi = oi; }
   }
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own
+// L2.  Renumber them so that every XCD walks one contiguous range of windows
+// (neighbouring windows share most of their pixels).
+__device__ __forceinline__ int xcd_remap(int b, int nb) {
+  constexpr int NXCD = 8;
+  const int q = nb / NXCD, r = nb % NXCD;
+  const int x = b % NXCD, k = b / NXCD;
+  return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + k;
 }
 
 __device__ __forceinline__ int wave_sum(int v) {
@@ -403,8 +414,9 @@ __global__ void __launch_bounds__(64) k_xcorr_pair(XcArgs a) {
 
   const int lane = threadIdx.x;
   const int h = lane >> 5, l = lane & 31;
-  const int iy = blockIdx.x / a.chunks_x;
-  const int ix = (blockIdx.x - iy * a.chunks_x) * 2 + h;
+  const int bid = xcd_remap(blockIdx.x, gridDim.x);
+  const int iy = bid / a.chunks_x;
+  const int ix = (bid - iy * a.chunks_x) * 2 + h;
   const bool win_ok = ix < a.nx_vec;
 
   // 1-2. row gather + forward row FFT
@@ -467,57 +479,64 @@ __global__ void __launch_bounds__(64) k_xcorr_pair(XcArgs a) {
   }
   wave_sync();
 
-  // 6. peaks: half-wave h walks window h column l
+  // 6. peaks: half-wave h walks window h column l, keeping a bit mask of the
+  //    rows of its column that are local maxima; Npeak_save rounds of 32-lane
+  //    arg-max over the lanes' best maxima
   const float* cc = cc0 + h * WY * SX;
   const bool col_ok = l < WX && l >= a.bx && l <= WX - 1 - a.bx;
   const int cl = l < WX ? l : WX - 1;
   const int lft = cl > 0 ? cl - 1 : 0, rgt = cl < WX - 1 ? cl + 1 : WX - 1;
-  float v[WY];
-  int cnt = 0;
+  uint32_t bits = 0;
   {
     float up = cc[cl];
     float cur = cc[SX + cl];
-    v[0] = 0.f;
-    v[WY - 1] = 0.f;
     static_for<WY - 2>([&](auto jc) {
       constexpr int j = jc + 1;
       const float dn = cc[(j + 1) * SX + cl];
       const float c0 = cur;
       const float lv = cc[j * SX + lft], rv = cc[j * SX + rgt];
-      // non-short-circuit form keeps v[] in registers
+      // non-short-circuit form keeps the walk branch-free
       const bool is_max = col_ok & (j >= a.by) & (j <= WY - 1 - a.by) & (c0 > 0.f) & (c0 >= up) & (c0 >= dn) &
                           (c0 >= lv) & (c0 >= rv);
-      v[j] = is_max ? c0 : 0.f;
-      cnt += is_max ? 1 : 0;
+      bits |= (uint32_t)is_max << j;
       up = cur;
       cur = dn;
     });
   }
-  cnt = half_sum(cnt);
+  const int n = half_sum(__popc(bits));
   const int w = iy * a.nx_vec + ix;
-  if (win_ok && l == 0) a.ncand[w] = cnt;
+  if (win_ok && l == 0) a.ncand[w] = n;
   const float px = (win_ok && a.pred_x) ? a.pred_x[w] : 0.f;
   const float py = (win_ok && a.pred_y) ? a.pred_y[w] : 0.f;
+  // lane best over its column's maxima (rows ascend, so the strict compare
+  // keeps the lower raster index on ties); the owner of a round's winner
+  // clears that row and refreshes
+  float mv;
+  int mi;
+  auto refresh = [&]() {
+    mv = 0.f;
+    mi = 0x7fffffff;
+    for (uint32_t rb = bits; rb; rb &= rb - 1) {
+      const int j = __builtin_ctz(rb);
+      const float val = cc[j * SX + cl];
+      if (val > mv) {
+        mv = val;
+        mi = j * WX + cl;
+      }
+    }
+  };
+  refresh();
   for (int k0 = 0; k0 < a.K; k0 += 32) {
     const int kn = min(32, a.K - k0);
     float my_v = 0.f;
     int my_i = -1;
     for (int k = 0; k < kn; ++k) {
-      float bv = 0.f;
-      int bi = 0x7fffffff;
-      static_for<WY>([&](auto jc) {
-        constexpr int j = jc;
-        const bool g = v[j] > bv;  // strict: within a column the upper row (lower index) wins ties
-        bv = g ? v[j] : bv;
-        bi = g ? j * WX + cl : bi;
-      });
+      float bv = mv;
+      int bi = mi;
       half_argmax(bv, bi);
-      {  // the owner lane removes the winner
-        const int jw = (bv > 0.f && bi % WX == cl) ? bi / WX : -1;
-        static_for<WY>([&](auto jc) {
-          constexpr int j = jc;
-          v[j] = (j == jw) ? 0.f : v[j];
-        });
+      if (bv > 0.f && bi == mi) {
+        bits &= ~((uint32_t)1 << (mi / WX));
+        refresh();
       }
       if (l == k) {
         my_v = bv;
@@ -555,8 +574,9 @@ __global__ void __launch_bounds__(64) k_xcorr_single(XcArgs a) {
   __shared__ float2 pl[WY][SX];
 
   const int lane = threadIdx.x;
-  const int iy = blockIdx.x / a.nx_vec;
-  const int ix = blockIdx.x - iy * a.nx_vec;
+  const int bid = xcd_remap(blockIdx.x, gridDim.x);
+  const int iy = bid / a.nx_vec;
+  const int ix = bid - iy * a.nx_vec;
 
   // 1-2. row gather + forward row FFT
   if (lane < WY) {
@@ -623,56 +643,64 @@ __global__ void __launch_bounds__(64) k_xcorr_single(XcArgs a) {
   }
   wave_sync();
 
-  // 6. peaks: lane l walks column l of the window
+  // 6. peaks: lane l walks column l of the window, keeping a bit mask of the
+  //    rows that are local maxima; Npeak_save rounds of 64-lane arg-max
   const float* cc = ccp;
+  using Bits = typename std::conditional<(WY <= 32), uint32_t, uint64_t>::type;
   const bool col_ok = lane < WX && lane >= a.bx && lane <= WX - 1 - a.bx;
   const int cl = lane < WX ? lane : WX - 1;
   const int lft = cl > 0 ? cl - 1 : 0, rgt = cl < WX - 1 ? cl + 1 : WX - 1;
-  float v[WY];
-  int cnt = 0;
+  Bits bits = 0;
   {
     float up = cc[cl];
     float cur = cc[SX + cl];
-    v[0] = 0.f;
-    v[WY - 1] = 0.f;
     static_for<WY - 2>([&](auto jc) {
       constexpr int j = jc + 1;
       const float dn = cc[(j + 1) * SX + cl];
       const float c0 = cur;
       const float lv = cc[j * SX + lft], rv = cc[j * SX + rgt];
+      // non-short-circuit form keeps the walk branch-free
       const bool is_max = col_ok & (j >= a.by) & (j <= WY - 1 - a.by) & (c0 > 0.f) & (c0 >= up) & (c0 >= dn) &
                           (c0 >= lv) & (c0 >= rv);
-      v[j] = is_max ? c0 : 0.f;
-      cnt += is_max ? 1 : 0;
+      bits |= (Bits)is_max << j;
       up = cur;
       cur = dn;
     });
   }
-  cnt = wave_sum(cnt);
-  const int w = blockIdx.x;
-  if (lane == 0) a.ncand[w] = cnt;
+  const int n = wave_sum(__popcll((unsigned long long)bits));
+  const int w = bid;
+  if (lane == 0) a.ncand[w] = n;
   const float px = a.pred_x ? a.pred_x[w] : 0.f;
   const float py = a.pred_y ? a.pred_y[w] : 0.f;
+  // lane best over its column's maxima (rows ascend, so the strict compare
+  // keeps the lower raster index on ties); the owner of a round's winner
+  // clears that row and refreshes
+  float mv;
+  int mi;
+  auto refresh = [&]() {
+    mv = 0.f;
+    mi = 0x7fffffff;
+    for (Bits rb = bits; rb; rb &= rb - 1) {
+      const int j = __builtin_ctzll(rb);
+      const float val = cc[j * SX + cl];
+      if (val > mv) {
+        mv = val;
+        mi = j * WX + cl;
+      }
+    }
+  };
+  refresh();
   for (int k0 = 0; k0 < a.K; k0 += 64) {
     const int kn = min(64, a.K - k0);
     float my_v = 0.f;
     int my_i = -1;
     for (int k = 0; k < kn; ++k) {
-      float bv = 0.f;
-      int bi = 0x7fffffff;
-      static_for<WY>([&](auto jc) {
-        constexpr int j = jc;
-        const bool g = v[j] > bv;
-        bv = g ? v[j] : bv;
-        bi = g ? j * WX + cl : bi;
-      });
+      float bv = mv;
+      int bi = mi;
       wave_argmax(bv, bi);
-      {
-        const int jw = (bv > 0.f && lane < WX && bi % WX == lane) ? bi / WX : -1;
-        static_for<WY>([&](auto jc) {
-          constexpr int j = jc;
-          v[j] = (j == jw) ? 0.f : v[j];
-        });
+      if (bv > 0.f && bi == mi) {
+        bits &= ~((Bits)1 << (mi / WX));
+        refresh();
       }
       if (lane == k) {
         my_v = bv;
@@ -881,30 +909,50 @@ __global__ void __launch_bounds__(256) k_xcorr_generic(XcArgs a, int WX, int WY,
 // displacement from the extended filtered field, then bilinear sample of frame
 // 2 at the displaced position.  Out-of-frame taps read 0 (the reference's zero
 // padding); taps beyond the reference's padded buffer are counted.
-// One thread per aug column, DEF_ROWS consecutive rows: the column's cell
-// coordinate is computed once and the four field samples of each cell are
-// reloaded only when the row crosses into the next cell.
+// One thread per aug column, DEF_ROWS consecutive rows.  The row terms
+// (cell row, fraction, 1 - fraction) are computed once per workgroup into
+// LDS and the column terms once per thread, with the same operations as the
+// per-pixel formula; the four field samples of a cell are reloaded only when
+// the row crosses into the next cell.  The four frame taps are loaded
+// unconditionally from clamped in-frame addresses (32-bit offsets from the
+// frame base) and zeroed when outside, so a pixel waits on one round trip.
 constexpr int DEF_ROWS = 16;
 
 __global__ void __launch_bounds__(256) k_deform(DefArgs a) {
 #pragma clang fp contract(off)
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= a.nx_aug) return;
-  unsigned oob = 0;
-  const float ax0 = ((float)idx - a.x_st) / a.spx;
-  int ixf = (int)floorf(ax0);
-  const float ax = ax0 - (float)ixf;
-  ixf = min(max(ixf, 0), a.nxe - 2);
-  int cell = -1;
-  float ex1 = 0.f, ex2 = 0.f, ex3 = 0.f, ex4 = 0.f, ey1 = 0.f, ey2 = 0.f, ey3 = 0.f, ey4 = 0.f;
-  const int y_end = min(a.ny_aug, (int)(blockIdx.y + 1) * DEF_ROWS);
-  for (int idy = blockIdx.y * DEF_ROWS; idy < y_end; ++idy) {
-    const long long t = (long long)idy * a.nx_aug + idx;
+  __shared__ float4 rowt[DEF_ROWS];
+  const int y0 = blockIdx.y * DEF_ROWS;
+  const int nrow = min(DEF_ROWS, a.ny_aug - y0);
+  if (threadIdx.x < nrow) {
+    const int idy = y0 + threadIdx.x;
     float ay = ((float)idy - a.y_st) / a.spy;
     int iyf = (int)floorf(ay);
     ay = ay - (float)iyf;
-    float ce1 = (1 - ax) * (1 - ay), ce2 = ax * (1 - ay), ce3 = (1 - ax) * ay, ce4 = ax * ay;
     iyf = min(max(iyf, 0), a.nye - 2);
+    rowt[threadIdx.x] = make_float4(ay, 1 - ay, __int_as_float(iyf), 0.f);
+  }
+  __syncthreads();
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= a.nx_aug) return;
+  unsigned oob = 0;
+  const float fidx = (float)idx;
+  const float ax0 = (fidx - a.x_st) / a.spx;
+  int ixf = (int)floorf(ax0);
+  const float ax = ax0 - (float)ixf;
+  const float ax1 = 1 - ax;
+  ixf = min(max(ixf, 0), a.nxe - 2);
+  int cell = -1;
+  float ex1 = 0.f, ex2 = 0.f, ex3 = 0.f, ex4 = 0.f, ey1 = 0.f, ey2 = 0.f, ey3 = 0.f, ey4 = 0.f;
+  const int nxm = a.nx - 1, nym = a.ny - 1;
+  const unsigned pitch = (unsigned)a.pitch;
+  const unsigned span_x = (unsigned)(a.nx_aug + 2 * a.hfws_x - 1), span_y = (unsigned)(a.ny_aug + 2 * a.hfws_y - 1);
+  float* out = a.out + (long long)y0 * a.nx_aug + idx;
+  for (int r = 0; r < nrow; ++r, out += a.nx_aug) {
+    const float4 rt = rowt[r];
+    const float ay = rt.x, ay1 = rt.y;
+    const int iyf = __builtin_amdgcn_readfirstlane(__float_as_int(rt.z));
+    const float fidy = (float)(y0 + r);
+    float ce1 = ax1 * ay1, ce2 = ax * ay1, ce3 = ax1 * ay, ce4 = ax * ay;
     if (iyf != cell) {
       cell = iyf;
       const int i1 = iyf * a.nxe + ixf, i3 = i1 + a.nxe;
@@ -917,8 +965,8 @@ __global__ void __launch_bounds__(256) k_deform(DefArgs a) {
       ey3 = a.ex_y[i3];
       ey4 = a.ex_y[i3 + 1];
     }
-    float sx = (float)idx + ce1 * ex1 + ce2 * ex2 + ce3 * ex3 + ce4 * ex4;
-    float sy = (float)idy + ce1 * ey1 + ce2 * ey2 + ce3 * ey3 + ce4 * ey4;
+    float sx = fidx + ce1 * ex1 + ce2 * ex2 + ce3 * ex3 + ce4 * ex4;
+    float sy = fidy + ce1 * ey1 + ce2 * ey2 + ce3 * ey3 + ce4 * ey4;
     const int jx = (int)floorf(sx), jy = (int)floorf(sy);
     sx = sx - (float)jx;
     sy = sy - (float)jy;
@@ -927,18 +975,19 @@ __global__ void __launch_bounds__(256) k_deform(DefArgs a) {
     ce3 = (1 - sx) * sy;
     ce4 = sx * sy;
     const int xi = jx - a.left_x, yi = jy - a.left_y;
-    float t1 = 0.f, t2 = 0.f, t3 = 0.f, t4 = 0.f;
-    const bool x0 = (unsigned)xi < (unsigned)a.nx, x1 = (unsigned)(xi + 1) < (unsigned)a.nx;
-    const bool y0 = (unsigned)yi < (unsigned)a.ny, y1 = (unsigned)(yi + 1) < (unsigned)a.ny;
-    const uint8_t* row0 = a.f2 + (long long)yi * a.pitch;
-    if (y0 && x0) t1 = (float)row0[xi];
-    if (y0 && x1) t2 = (float)row0[xi + 1];
-    if (y1 && x0) t3 = (float)row0[a.pitch + xi];
-    if (y1 && x1) t4 = (float)row0[a.pitch + xi + 1];
+    const bool x0 = (unsigned)xi <= (unsigned)nxm, x1 = (unsigned)(xi + 1) <= (unsigned)nxm;
+    const bool yok0 = (unsigned)yi <= (unsigned)nym, yok1 = (unsigned)(yi + 1) <= (unsigned)nym;
+    const unsigned cx0 = (unsigned)min(max(xi, 0), nxm), cx1 = (unsigned)min(max(xi + 1, 0), nxm);
+    // 24-bit products: frames hold fewer than 2^31 pixels and rows fewer than 2^24
+    const unsigned r0 = __umul24((unsigned)min(max(yi, 0), nym), pitch);
+    const unsigned r1 = __umul24((unsigned)min(max(yi + 1, 0), nym), pitch);
+    const float v1 = (float)a.f2[r0 + cx0], v2 = (float)a.f2[r0 + cx1];
+    const float v3 = (float)a.f2[r1 + cx0], v4 = (float)a.f2[r1 + cx1];
+    const float t1 = (yok0 && x0) ? v1 : 0.f, t2 = (yok0 && x1) ? v2 : 0.f;
+    const float t3 = (yok1 && x0) ? v3 : 0.f, t4 = (yok1 && x1) ? v4 : 0.f;
     // taps outside the reference's aug_ex buffer ([-hfws, n_aug + hfws) in aug coordinates)
-    const int ex0 = jx + a.hfws_x, ey0 = jy + a.hfws_y;
-    if (ex0 < 0 || ex0 + 1 >= a.nx_aug + 2 * a.hfws_x || ey0 < 0 || ey0 + 1 >= a.ny_aug + 2 * a.hfws_y) ++oob;
-    a.out[t] = ce1 * t1 + ce2 * t2 + ce3 * t3 + ce4 * t4;
+    oob += ((unsigned)(jx + a.hfws_x) >= span_x || (unsigned)(jy + a.hfws_y) >= span_y) ? 1u : 0u;
+    *out = ce1 * t1 + ce2 * t2 + ce3 * t3 + ce4 * t4;
   }
   if (oob) atomicAdd(a.oob, (unsigned long long)oob);
 }
