@@ -324,10 +324,11 @@ __global__ void __launch_bounds__((FastCfg<WX, WY, WPB>::NT)) k_xcorr_fast(XcArg
 //     registers (wide loads; per-pixel zero fill only at the image border)
 //   2 row FFT in registers, rows to LDS
 //   3 column FFT, one column per lane, both windows at once
-//   4 Hermitian split per window on all 64 lanes (own column in registers,
-//     mirror column from LDS), window b publishes conj(F1)F2 through LDS,
-//     window a packs Q = Pa + i*Pb and inverse-transforms the columns
-//   5 inverse row FFTs give cc_a = Re, cc_b = Im (scaled, fftshifted)
+//   4 Hermitian split of both windows, packed as Q = Pa + i*Pb; each column
+//     IFFT is split over the lane pair (l, l+32) by one radix-2 step (each
+//     half transforms the even or the odd rows)
+//   5 row IFFTs split the same way, halves exchanged through LDS; Re is
+//     cc_a, Im is cc_b (scaled, fftshifted)
 //   6 peaks: each half-wave walks its window by columns (3 LDS reads per
 //     pixel), Npeak_save rounds of 32-lane arg-max, winners removed in place,
 //     Gaussian fits in parallel.
@@ -441,40 +442,64 @@ __global__ void __launch_bounds__(64) k_xcorr_pair(XcArgs a) {
   }
   wave_sync();
 
-  // 4. split per window: P[ky] = 4 conj(F1) F2 at (ky, l)
+  // 4. split + column IFFTs.  Q = Pa + i*Pb with P = 4 conj(F1) F2; every
+  //    column IFFT of length WY is shared by the lane pair (l, l + 32):
+  //    half h transforms the rows ky = 2t + h (length WY/2), then
+  //    X[y] = E[y] + w^y O[y] and X[y + WY/2] = E[y] - w^y O[y] (w = e^{2 pi i/WY}).
+  constexpr int HX = WX / 2, HY = WY / 2;
+  float2 q[HY];
   if (l < WX) {
     const int lm = (WX - l) % WX;
-    static_for<WY>([&](auto rc) {
-      constexpr int r = rc;
-      constexpr int rm = (WY - r) % WY;
-      col[r] = xprod(col[r], pl[h][rm][lm]);
+    static_for<HY>([&](auto tc) {
+      constexpr int t = tc;
+      const int ky = 2 * t + h;
+      const int km = t == 0 ? h * (WY - 1) : WY - ky;
+      const float2 pa = xprod(pl[0][ky][l], pl[0][km][lm]);
+      const float2 pb = xprod(pl[1][ky][l], pl[1][km][lm]);
+      q[t] = make_float2(pa.x - pb.y, pa.y + pb.x);
     });
+    fft_reg<HY, true>(q);
+    if (h) static_for<HY>([&](auto kc) { q[kc] = twiddle<kc, WY, true>(q[kc]); });
   }
-  wave_sync();
-  if (h == 1 && l < WX) static_for<WY>([&](auto r) { pl[1][r][l] = col[r]; });
-  wave_sync();
-  if (h == 0 && l < WX) {
-    static_for<WY>([&](auto r) {
-      const float2 pb = pl[1][r][l];
-      col[r] = make_float2(col[r].x - pb.y, col[r].y + pb.x);
-    });
-    fft_reg<WY, true>(col);
-    static_for<WY>([&](auto r) { pl[0][r][l] = col[r]; });
-  }
+  wave_sync();  // every spectrum read is done before plane 0 is reused
+  // plane 0 as eo[h][y][kx], y < WY/2: E (h = 0) and w^y O (h = 1)
+  float2* const eo = &pl[0][0][0];
+  if (l < WX) static_for<HY>([&](auto k) { eo[(h * HY + k) * SX + l] = q[k]; });
   wave_sync();
 
-  // 5. inverse row FFTs -> shifted correlation planes (in plane 1)
+  // 5. row IFFTs, shared the same way: lane (h, y) transforms the columns
+  //    kx = 2t + h of row y of X, the pair exchanges halves through LDS and
+  //    lane h writes output columns [h*WX/2, (h+1)*WX/2) of both correlation
+  //    planes (Re -> window a, Im -> window b), scaled and fftshifted.
   float* const cc0 = reinterpret_cast<float*>(&pl[1][0][0]);
-  if (h == 0 && l < WY) {
-    float2 x[WX];
-    static_for<WX>([&](auto c) { x[c] = pl[0][l][c]; });
-    fft_reg<WX, true>(x);
-    const int rs = (l + WY / 2) % WY;
-    static_for<WX>([&](auto cc) {
-      constexpr int c = cc;
-      constexpr int cs = (c + WX / 2) % WX;
-      cc0[rs * SX + cs] = (fabsf(x[c].x) * 0.25f) / (float)NPIX;
-      cc0[WY * SX + rs * SX + cs] = (fabsf(x[c].y) * 0.25f) / (float)NPIX;
+  float2 r[HX];
+  const int yy = l < HY ? l : l - HY;
+  if (l < WY) {
+    const float sg = l < HY ? 1.f : -1.f;
+    static_for<HX>([&](auto tc) {
+      constexpr int t = tc;
+      const float2 e = eo[yy * SX + 2 * t + h], o = eo[(HY + yy) * SX + 2 * t + h];
+      r[t] = make_float2(fmaf(sg, o.x, e.x), fmaf(sg, o.y, e.y));
+    });
+    fft_reg<HX, true>(r);
+    if (h) static_for<HX>([&](auto kc) { r[kc] = twiddle<kc, WX, true>(r[kc]); });
+  }
+  wave_sync();  // every read of eo is done before plane 0 is reused
+  // plane 0 as xb[h][k][y]: E' (h = 0) and w^k O' (h = 1) of row y
+  float2* const xb = &pl[0][0][0];
+  if (l < WY) static_for<HX>([&](auto k) { xb[(h * HX + k) * WY + l] = r[k]; });
+  wave_sync();
+  if (l < WY) {
+    const float sh = h ? -1.f : 1.f;
+    const int rs = (l + HY) % WY;
+    static_for<HX>([&](auto kc) {
+      constexpr int k = kc;
+      const float2 p = xb[((1 - h) * HX + k) * WY + l];
+      // h = 0: E' + w^k O';  h = 1: E' - w^k O'
+      const float re = fmaf(sh, r[k].x, p.x), im = fmaf(sh, r[k].y, p.y);
+      const int cs = h ? k : k + HX;
+      cc0[rs * SX + cs] = (fabsf(re) * 0.25f) / (float)NPIX;
+      cc0[WY * SX + rs * SX + cs] = (fabsf(im) * 0.25f) / (float)NPIX;
     });
   }
   wave_sync();
