@@ -43,6 +43,8 @@ struct GridBuf {
   uint8_t* fv = nullptr;
   float *sx = nullptr, *sy = nullptr;  // ping-pong partner of the filtered field
   uint8_t* sv = nullptr;
+  float *ux = nullptr, *uy = nullptr;  // second intermediate set of the median passes
+  uint8_t* uv = nullptr;
   float *ex = nullptr, *ey = nullptr;  // extended field (nx+6)*(ny+6)
   float* deform = nullptr;             // deformed frame 2 [ny_aug][nx_aug]
   float2 *twx = nullptr, *twy = nullptr;   // generic-path twiddles
@@ -310,8 +312,17 @@ int run_xcorr(pivx_ctx* c, int gi, bool deformed) {
 }
 
 // MedianFilter_Smoothing_p1/p2 (cu:1451-1545, 1766-1863) writing into (ox, oy, ov).
+// Median passes, hole filling and smoothing of one grid into (ox, oy, ov).
+// Buffers are planned so that no pass needs a copy: the last median pass
+// writes the valid flags to ov and the components to T0 = (ox, oy) when the
+// number of smoothing passes is even, else to the partner (sx, sy), so that
+// the smoothing ping-pong ends in (ox, oy); earlier median passes alternate
+// between two other buffer sets.  With `ext` set and at least one smoothing
+// pass, the last smoothing pass also writes the extended field (ext_x, ext_y)
+// and *ext_done is set.
 int run_filter(pivx_ctx* c, int gi, int npass_median, int min_nneigh, float insert_thr, int npass_smooth, int iffill,
-               float* ox, float* oy, uint8_t* ov) {
+               float* ox, float* oy, uint8_t* ov, float* ext_x = nullptr, float* ext_y = nullptr,
+               bool* ext_done = nullptr) {
   GridBuf& G = c->grid[gi];
   const pivx_geometry& g = G.g;
   const int nt = g.nt_vec;
@@ -320,21 +331,30 @@ int run_filter(pivx_ctx* c, int gi, int npass_median, int min_nneigh, float inse
   const int wym = gi == 0 ? c->p.wy1_median : c->p.wy2_median;
   const int wxs = gi == 0 ? c->p.wx1_smooth : c->p.wx2_smooth;
   const int wys = gi == 0 ? c->p.wy1_smooth : c->p.wy2_smooth;
+  if (ext_done) *ext_done = false;
+  const bool smooth_odd = npass_smooth > 0 && npass_smooth % 2 == 1;
+  float* t0x = smooth_odd ? G.sx : ox;
+  float* t0y = smooth_odd ? G.sy : oy;
+  const bool fill = iffill == 1;
+  const bool listed = fill && npass_median > 0;  // the last median pass lists the holes
   int rc;
   tic(c);
   if (npass_median <= 0) {
-    rc = check_launch(c, launch_prep(G.dx, G.dy, G.dv, K, ox, oy, ov, nt, c->stream), "median_filter_data_prep");
+    rc = check_launch(c, launch_prep(G.dx, G.dy, G.dv, K, t0x, t0y, ov, nt, c->stream), "median_filter_data_prep");
     if (rc) return rc;
   } else {
+    // intermediate sets: I1 = (ux, uy, uv), I2 = whichever of (ox, oy) / (sx, sy) is not T0, with sv
+    float* i2x = smooth_odd ? ox : G.sx;
+    float* i2y = smooth_odd ? oy : G.sy;
     const float* srcx = G.dx;
     const float* srcy = G.dy;
     const uint8_t* srcv = G.dv;
     int sstride = K;
     for (int p = 0; p < npass_median; ++p) {
-      const bool to_out = ((npass_median - 1 - p) % 2) == 0;
-      float* tx = to_out ? ox : G.sx;
-      float* ty = to_out ? oy : G.sy;
-      uint8_t* tv = to_out ? ov : G.sv;
+      const int back = npass_median - 1 - p;  // 0 for the last pass
+      float* tx = back == 0 ? t0x : (back % 2 == 1 ? G.ux : i2x);
+      float* ty = back == 0 ? t0y : (back % 2 == 1 ? G.uy : i2y);
+      uint8_t* tv = back == 0 ? ov : (back % 2 == 1 ? G.uv : G.sv);
       MedArgs m{};
       m.sx = srcx;
       m.sy = srcy;
@@ -356,8 +376,10 @@ int run_filter(pivx_ctx* c, int gi, int npass_median, int min_nneigh, float inse
       m.min_nneigh = min_nneigh;
       m.thr = insert_thr;
       m.eps = kEpsilon;
-      m.accepted = (p == npass_median - 1) ? G.accepted : nullptr;
+      m.accepted = back == 0 ? G.accepted : nullptr;
       m.scratch = G.med_scratch;
+      m.hole_list = (back == 0 && listed) ? c->list_a : nullptr;
+      m.nholes = (back == 0 && listed) ? c->nholes : nullptr;
       rc = check_launch(c, launch_median(m, c->stream), "median_filtering");
       if (rc) return rc;
       srcx = tx;
@@ -367,10 +389,10 @@ int run_filter(pivx_ctx* c, int gi, int npass_median, int min_nneigh, float inse
     }
   }
   toc(c, gi, 2);
-  if (iffill == 1) {
+  if (fill) {
     FillArgs f{};
-    f.dx = ox;
-    f.dy = oy;
+    f.dx = t0x;
+    f.dy = t0y;
     f.dv = ov;
     f.nx_vec = g.nx_vec;
     f.ny_vec = g.ny_vec;
@@ -381,6 +403,7 @@ int run_filter(pivx_ctx* c, int gi, int npass_median, int min_nneigh, float inse
     f.tmp_f = c->tmp_f;
     f.nholes = c->nholes;
     f.sweeps = c->sweeps + gi;
+    f.prelisted = listed ? 1 : 0;
     tic(c);
     rc = check_launch(c, launch_fill(f, c->stream), "fill_blank_vecfield");
     toc(c, gi, 3);
@@ -388,14 +411,7 @@ int run_filter(pivx_ctx* c, int gi, int npass_median, int min_nneigh, float inse
   }
   if (npass_smooth > 0) {
     tic(c);
-    // ping-pong so that the last pass lands in (ox, oy)
-    float *cx = ox, *cy = oy;
-    if (npass_smooth % 2 == 1) {
-      rc = check_launch(c, launch_copy_field(ox, oy, nullptr, G.sx, G.sy, nullptr, nt, c->stream), "copy");
-      if (rc) return rc;
-      cx = G.sx;
-      cy = G.sy;
-    }
+    float *cx = t0x, *cy = t0y;
     for (int s = 0; s < npass_smooth; ++s) {
       float* nx_ = (cx == ox) ? G.sx : ox;
       float* ny_ = (cy == oy) ? G.sy : oy;
@@ -410,6 +426,12 @@ int run_filter(pivx_ctx* c, int gi, int npass_median, int min_nneigh, float inse
       sa.wy = wys;
       sa.hfwx = (wxs - 1) / 2;
       sa.hfwy = (wys - 1) / 2;
+      if (s == npass_smooth - 1 && ext_x) {
+        sa.ex = ext_x;
+        sa.ey = ext_y;
+        sa.border = kExBorder;
+        if (ext_done) *ext_done = true;
+      }
       rc = check_launch(c, launch_smooth(sa, c->stream), "smoothing_tophat");
       if (rc) return rc;
       cx = nx_;
@@ -444,6 +466,9 @@ int alloc_grid(pivx_ctx* c, int gi) {
   rc |= dalloc(c, &G.sx, nt);
   rc |= dalloc(c, &G.sy, nt);
   rc |= dalloc(c, &G.sv, nt);
+  rc |= dalloc(c, &G.ux, nt);
+  rc |= dalloc(c, &G.uy, nt);
+  rc |= dalloc(c, &G.uv, nt);
   const size_t ne = (size_t)(g.nx_vec + 2 * kExBorder) * (g.ny_vec + 2 * kExBorder);
   rc |= dalloc(c, &G.ex, ne);
   rc |= dalloc(c, &G.ey, ne);
@@ -643,8 +668,7 @@ int pivx_extend_filtered_vecfield_p2(pivx_ctx* c) {
   return extend(c, 1);
 }
 
-int pivx_upsampling_vector_field(pivx_ctx* c) {
-  if (!c) return fail(nullptr, PIVX_ERR_PARAM, "null handle");
+static int upsample(pivx_ctx* c, bool write_ext) {
   GridBuf& G1 = c->grid[0];
   GridBuf& G2 = c->grid[1];
   const pivx_geometry& g1 = G1.g;
@@ -654,27 +678,40 @@ int pivx_upsampling_vector_field(pivx_ctx* c) {
       launch_upsample(G2.fx, G2.fy, G1.ex, G1.ey, g1.nx_vec + 2 * kExBorder, g1.ny_vec + 2 * kExBorder,
                       (float)(g1.spacing_x / 2 - kExBorder * g1.spacing_x),
                       (float)(g1.spacing_y / 2 - kExBorder * g1.spacing_y), (float)g1.spacing_x, (float)g1.spacing_y,
-                      g2.spacing_x, g2.spacing_y, g2.nx_vec, g2.ny_vec, c->stream),
+                      g2.spacing_x, g2.spacing_y, g2.nx_vec, g2.ny_vec, write_ext ? G2.ex : nullptr,
+                      write_ext ? G2.ey : nullptr, kExBorder, c->stream),
       "Upsampling_vecfield_kernel_2d");
 }
 
+int pivx_upsampling_vector_field(pivx_ctx* c) {
+  if (!c) return fail(nullptr, PIVX_ERR_PARAM, "null handle");
+  return upsample(c, false);
+}
+
+// The per-pair sequence of main.cpp:288-375.  The extension of a field is
+// written by the kernel that produces the field (last smoothing pass,
+// upsampling) whenever it can be; extend() runs only where it could not.
 static int run_pair_enqueue(pivx_ctx* c, const pivx_filter_params* f1, const pivx_filter_params* f2) {
   int rc;
+  bool ext_ok = false;
+  GridBuf& G1 = c->grid[0];
+  GridBuf& G2 = c->grid[1];
   for (int j = 0; j < c->p.npass1; ++j) {
-    if (j > 0 && (rc = extend(c, 0))) return rc;
+    if (j > 0 && !ext_ok && (rc = extend(c, 0))) return rc;
     if ((rc = pivx_calc_cc_map_p1(c, j))) return rc;
-    if ((rc = run_filter(c, 0, f1->npass_median, f1->min_nneigh, f1->insert_thr, f1->npass_smooth, 1, c->grid[0].fx,
-                         c->grid[0].fy, c->grid[0].fv)))
+    if ((rc = run_filter(c, 0, f1->npass_median, f1->min_nneigh, f1->insert_thr, f1->npass_smooth, 1, G1.fx, G1.fy,
+                         G1.fv, G1.ex, G1.ey, &ext_ok)))
       return rc;
   }
   if (c->p.npass2 > 0) {
-    if ((rc = extend(c, 0))) return rc;
-    if ((rc = pivx_upsampling_vector_field(c))) return rc;
+    if (!ext_ok && (rc = extend(c, 0))) return rc;
+    if ((rc = upsample(c, true))) return rc;
+    ext_ok = true;
     for (int j = 0; j < c->p.npass2; ++j) {
-      if ((rc = extend(c, 1))) return rc;
+      if (!ext_ok && (rc = extend(c, 1))) return rc;
       if ((rc = pivx_calc_cc_map_p2(c))) return rc;
-      if ((rc = run_filter(c, 1, f2->npass_median, f2->min_nneigh, f2->insert_thr, f2->npass_smooth, 1, c->grid[1].fx,
-                           c->grid[1].fy, c->grid[1].fv)))
+      if ((rc = run_filter(c, 1, f2->npass_median, f2->min_nneigh, f2->insert_thr, f2->npass_smooth, 1, G2.fx, G2.fy,
+                           G2.fv, G2.ex, G2.ey, &ext_ok)))
         return rc;
     }
     // median-only re-run on the last raw candidates (main.cpp:363-375)

@@ -42,6 +42,8 @@ struct MedArgs {
   float thr, eps;
   int8_t* accepted;        // [nt] index of the accepted candidate, -1 if none (may be null)
   float* scratch;          // generic path only: 4*wx*wy floats per vector
+  int* hole_list;          // if set: indices of vectors left invalid are appended here
+  int* nholes;             //   (count in *nholes, which must be 0 before the pass)
 };
 
 struct FillArgs {
@@ -50,11 +52,13 @@ struct FillArgs {
   int* list_a; int* list_b; float* tmp_x; float* tmp_y; uint8_t* tmp_f;
   int* nholes;             // hole counter, 0 between fills
   int* sweeps;
+  int prelisted;           // holes already in list_a / *nholes (appended by the median pass)
 };
 
 struct SmoothArgs {
   const float* sx; const float* sy; float* dx; float* dy;
   int nx_vec, ny_vec, wx, wy, hfwx, hfwy;
+  float* ex; float* ey; int border;   // if ex is set: also write the border-replicated extension
 };
 
 // Launchers (return hipGetLastError() of the launch).
@@ -66,7 +70,7 @@ hipError_t launch_extend(const float* fx, const float* fy, float* ex, float* ey,
                          hipStream_t s);
 hipError_t launch_upsample(float* dx2, float* dy2, const float* ex1x, const float* ex1y, int nxe1, int nye1,
                            float x_st1, float y_st1, float sp1x, float sp1y, int sp2x, int sp2y, int nx2, int ny2,
-                           hipStream_t s);
+                           float* ex2x, float* ex2y, int border, hipStream_t s);
 hipError_t launch_median(const MedArgs& a, hipStream_t s);
 hipError_t launch_fill(const FillArgs& a, hipStream_t s);
 hipError_t launch_smooth(const SmoothArgs& a, hipStream_t s);

@@ -19,6 +19,7 @@
 //   organize_smooth_filter_data + smoothing_tophat cu:399-443 -> k_smooth
 #include "kernels.h"
 #include "fft_reg.h"
+#include "sortnet.h"
 #include <math.h>
 #include <type_traits>
 
@@ -1030,11 +1031,26 @@ __global__ void k_extend(const float* fx, const float* fy, float* ex, float* ey,
   }
 }
 
+// The extension entries (i, j) that replicate vector (vx, vy) — the same
+// values k_extend writes, from the thread that produced the vector.
+__device__ __forceinline__ void write_ext(float* ex, float* ey, int nx, int ny, int b, int vx, int vy, float x,
+                                          float y) {
+  const int nxe = nx + 2 * b;
+  const int i0 = vx == 0 ? 0 : vx + b, i1 = vx == nx - 1 ? nxe - 1 : vx + b;
+  const int j0 = vy == 0 ? 0 : vy + b, j1 = vy == ny - 1 ? ny + 2 * b - 1 : vy + b;
+  for (int j = j0; j <= j1; ++j)
+    for (int i = i0; i <= i1; ++i) {
+      ex[j * nxe + i] = x;
+      ey[j * nxe + i] = y;
+    }
+}
+
 // ============================================================== upsampling
 // Upsampling_vecfield_kernel_2d (cu:675-704): grid-2 predictor at image
 // position (k*sp2 + sp2/2) from the extended grid-1 field (origin x_st1).
 __global__ void k_upsample(float* dx2, float* dy2, const float* ex, const float* ey, int nxe, int nye, float x_st1,
-                           float y_st1, float sp1x, float sp1y, int sp2x, int sp2y, int nx2, int ny2) {
+                           float y_st1, float sp1x, float sp1y, int sp2x, int sp2y, int nx2, int ny2, float* ex2x,
+                           float* ex2y, int border) {
 #pragma clang fp contract(off)
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nx2 * ny2; t += gridDim.x * blockDim.x) {
     const int vy = t / nx2, vx = t - (t / nx2) * nx2;
@@ -1048,8 +1064,11 @@ __global__ void k_upsample(float* dx2, float* dy2, const float* ex, const float*
     ix = min(max(ix, 0), nxe - 2);
     iy = min(max(iy, 0), nye - 2);
     const int i1 = iy * nxe + ix, i2 = i1 + 1, i3 = i1 + nxe, i4 = i3 + 1;
-    dx2[t] = ce1 * ex[i1] + ce2 * ex[i2] + ce3 * ex[i3] + ce4 * ex[i4];
-    dy2[t] = ce1 * ey[i1] + ce2 * ey[i2] + ce3 * ey[i3] + ce4 * ey[i4];
+    const float ux = ce1 * ex[i1] + ce2 * ex[i2] + ce3 * ex[i3] + ce4 * ex[i4];
+    const float uy = ce1 * ey[i1] + ce2 * ey[i2] + ce3 * ey[i3] + ce4 * ey[i4];
+    dx2[t] = ux;
+    dy2[t] = uy;
+    if (ex2x) write_ext(ex2x, ex2y, nx2, ny2, border, vx, vy, ux, uy);
   }
 }
 
@@ -1127,6 +1146,7 @@ __device__ __forceinline__ void median_decide(const MedArgs& a, int t, bool vmed
   a.dy[t] = oy;
   a.dv[t] = ok ? 1 : 0;
   if (a.accepted) a.accepted[t] = (int8_t)acc;
+  if (a.hole_list && !ok) a.hole_list[atomicAdd(a.nholes, 1)] = t;
 }
 
 template <int NS>
@@ -1176,6 +1196,110 @@ __global__ void __launch_bounds__(128) k_median(MedArgs a) {
     ry = median_of<NS>(Y, n);
   }
   median_decide(a, t, vmed, mx, my, rx, ry);
+}
+
+// Compile-time median windows (3x3, 5x5, 7x7): two lanes per vector, lane
+// c = 0 ranks the x components and c = 1 the y components.  The M = wx*wy-1
+// neighbour slots (the window without its centre slot, in raster order) hold
+// valid values or +inf; a pruned odd-even merge network leaves the order
+// statistics 0..M/2 in place, which contain the median of any count of valid
+// neighbours.  Deviations |v - median| of the finite slots go through the same
+// network.  The lanes swap their two medians and both run the candidate test
+// of median_filtering (cu:225-253) on identical inputs.
+template <int M>
+__device__ __forceinline__ void select_net(float (&v)[MedianNet<M>::NS]) {
+  static_for<MedianNet<M>::net.n>([&](auto cc) {
+    constexpr int x = MedianNet<M>::net.a[cc], y = MedianNet<M>::net.b[cc];
+    const float lo = fminf(v[x], v[y]), hi = fmaxf(v[x], v[y]);
+    v[x] = lo;
+    v[y] = hi;
+  });
+}
+
+template <int M>
+__device__ __forceinline__ float median_low(const float (&v)[MedianNet<M>::NS], int n) {
+#pragma clang fp contract(off)
+  // order statistic idx of the sorted prefix 0..M/2
+  auto at = [&](int idx) {
+    float r = 0.f;
+    static_for<M / 2 + 1>([&](auto ic) { r = (ic == idx) ? v[ic] : r; });
+    return r;
+  };
+  if (n <= 0) return 0.f;  // reference reads outside the list here (cu:199); defined as 0
+  if (n % 2 == 0) return (at(n / 2) + at(n / 2 - 1)) / 2;
+  return at((n - 1) / 2);
+}
+
+template <int WXM, int WYM>
+__global__ void __launch_bounds__(256) k_median2(MedArgs a) {
+#pragma clang fp contract(off)
+  constexpr int M = WXM * WYM - 1;
+  constexpr int NS = MedianNet<M>::NS;
+  constexpr int SKIP = (WXM * WYM - 1) / 2;
+  constexpr int HX = (WXM - 1) / 2, HY = (WYM - 1) / 2;
+  const int nt = a.nx_vec * a.ny_vec;
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = g >> 1, c = g & 1;
+  const bool live = t < nt;
+  const int tt = live ? t : nt - 1;  // both lanes of a pair stay active for the swap
+  const int vy = tt / a.nx_vec, vx = tt - vy * a.nx_vec;
+  const float* src = c ? a.sy : a.sx;
+  float v[NS];
+  int n = 0;
+  static_for<NS>([&](auto kc) {
+    constexpr int k = kc;
+    if constexpr (k >= M) {
+      v[k] = INFINITY;
+    } else {
+      constexpr int s = k < SKIP ? k : k + 1;
+      const int qx = vx - HX + s % WXM, qy = vy - HY + s / WXM;
+      const bool inb = (unsigned)qx < (unsigned)a.nx_vec && (unsigned)qy < (unsigned)a.ny_vec;
+      const long long q = (long long)(inb ? qy * a.nx_vec + qx : 0) * a.sstride;
+      const bool ok = inb && a.sv[q];
+      const float val = src[q];
+      v[k] = ok ? val : INFINITY;
+      n += ok ? 1 : 0;
+    }
+  });
+  float med = 0.f, dev = 0.f;
+  const bool vmed = n >= a.min_nneigh;
+  if (vmed) {
+    select_net<M>(v);
+    med = median_low<M>(v, n);
+    static_for<M>([&](auto kc) { v[kc] = v[kc] < INFINITY ? fabsf(v[kc] - med) : INFINITY; });
+    select_net<M>(v);
+    dev = median_low<M>(v, n);
+  }
+  const float omed = __shfl_xor(med, 1), odev = __shfl_xor(dev, 1);
+  if (!live) return;
+  const float mx = c ? omed : med, my = c ? med : omed;
+  const float rx = c ? odev : dev, ry = c ? dev : odev;
+  float ox = 0.f, oy = 0.f;
+  bool okc = false;
+  int acc = -1;
+  if (vmed) {
+    for (int k = 0; k < a.K; ++k) {
+      const long long o = (long long)t * a.K + k;
+      if (!a.cv[o]) continue;
+      const float devix = fabsf(a.cx[o] - mx) / (rx + a.eps);
+      const float deviy = fabsf(a.cy[o] - my) / (ry + a.eps);
+      if (devix < a.thr && deviy < a.thr) {
+        okc = true;
+        ox = a.cx[o];
+        oy = a.cy[o];
+        acc = k;
+        break;
+      }
+    }
+  }
+  if (c == 0) {
+    a.dx[t] = ox;
+    a.dv[t] = okc ? 1 : 0;
+    if (a.accepted) a.accepted[t] = (int8_t)acc;
+    if (a.hole_list && !okc) a.hole_list[atomicAdd(a.nholes, 1)] = t;
+  } else {
+    a.dy[t] = oy;
+  }
 }
 
 // Windows larger than the register path: per-vector lists in global scratch,
@@ -1338,8 +1462,10 @@ __global__ void k_smooth(SmoothArgs a) {
         ++n;
       }
     }
-    a.dx[t] = sx / (float)n;
-    a.dy[t] = sy / (float)n;
+    const float ox = sx / (float)n, oy = sy / (float)n;
+    a.dx[t] = ox;
+    a.dy[t] = oy;
+    if (a.ex) write_ext(a.ex, a.ey, a.nx_vec, a.ny_vec, a.border, vx, vy, ox, oy);
   }
 }
 
@@ -1483,9 +1609,9 @@ hipError_t launch_extend(const float* fx, const float* fy, float* ex, float* ey,
 
 hipError_t launch_upsample(float* dx2, float* dy2, const float* ex1x, const float* ex1y, int nxe1, int nye1,
                            float x_st1, float y_st1, float sp1x, float sp1y, int sp2x, int sp2y, int nx2, int ny2,
-                           hipStream_t s) {
+                           float* ex2x, float* ex2y, int border, hipStream_t s) {
   hipLaunchKernelGGL(k_upsample, dim3(grid_for((long long)nx2 * ny2, 256)), dim3(256), 0, s, dx2, dy2, ex1x, ex1y,
-                     nxe1, nye1, x_st1, y_st1, sp1x, sp1y, sp2x, sp2y, nx2, ny2);
+                     nxe1, nye1, x_st1, y_st1, sp1x, sp1y, sp2x, sp2y, nx2, ny2, ex2x, ex2y, border);
   return hipGetLastError();
 }
 
@@ -1493,7 +1619,11 @@ hipError_t launch_median(const MedArgs& a, hipStream_t s) {
   const int nt = a.nx_vec * a.ny_vec;
   const int ntm = a.wx * a.wy;
   dim3 g(grid_for(nt, 128, 1 << 30)), b(128);
-  if (ntm <= 8) hipLaunchKernelGGL(k_median<8>, g, b, 0, s, a);
+  const dim3 g2(grid_for(2LL * nt, 256, 1 << 30)), b2(256);
+  if (a.wx == 5 && a.wy == 5) hipLaunchKernelGGL((k_median2<5, 5>), g2, b2, 0, s, a);
+  else if (a.wx == 3 && a.wy == 3) hipLaunchKernelGGL((k_median2<3, 3>), g2, b2, 0, s, a);
+  else if (a.wx == 7 && a.wy == 7) hipLaunchKernelGGL((k_median2<7, 7>), g2, b2, 0, s, a);
+  else if (ntm <= 8) hipLaunchKernelGGL(k_median<8>, g, b, 0, s, a);
   else if (ntm <= 16) hipLaunchKernelGGL(k_median<16>, g, b, 0, s, a);
   else if (ntm <= 32) hipLaunchKernelGGL(k_median<32>, g, b, 0, s, a);
   else if (ntm <= 64) hipLaunchKernelGGL(k_median<64>, g, b, 0, s, a);
@@ -1505,9 +1635,11 @@ hipError_t launch_median(const MedArgs& a, hipStream_t s) {
 }
 
 hipError_t launch_fill(const FillArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL(k_fill_list, dim3(grid_for((long long)a.nx_vec * a.ny_vec, 256, 2048)), dim3(256), 0, s, a);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
+  if (!a.prelisted) {
+    hipLaunchKernelGGL(k_fill_list, dim3(grid_for((long long)a.nx_vec * a.ny_vec, 256, 2048)), dim3(256), 0, s, a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
   hipLaunchKernelGGL(k_fill, dim3(1), dim3(1024), 0, s, a);
   return hipGetLastError();
 }

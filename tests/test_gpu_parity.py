@@ -152,7 +152,8 @@ def _random_candidates(rng, nt, K, outlier_frac=0.1):
 
 
 @pytest.mark.parametrize("win,npm,nps,fill", [((5, 5), 2, 1, 1), ((3, 3), 1, 2, 1), ((7, 5), 3, 0, 0),
-                                              ((9, 9), 1, 1, 1), ((4, 4), 2, 1, 1)])
+                                              ((9, 9), 1, 1, 1), ((4, 4), 2, 1, 1), ((7, 7), 3, 3, 1),
+                                              ((5, 5), 0, 1, 1), ((5, 5), 4, 2, 0)])
 def test_validation_bitexact(pivx, orc, win, npm, nps, fill):
     """MedianFilter_Smoothing_p2 on injected candidates: median passes, fill,
     smoothing, bit-exact against the FP32 transcription."""
