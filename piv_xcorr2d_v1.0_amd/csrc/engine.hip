@@ -269,7 +269,10 @@ int run_deform(pivx_ctx* c, int gi) {
   return rc;
 }
 
-int run_xcorr(pivx_ctx* c, int gi, bool deformed) {
+// With zero_pred set (first grid-1 pass) the predictor is 0 and the kernel
+// itself zeroes the filtered field, replacing set_float_zero_kernel
+// (cu:1287-1298).
+int run_xcorr(pivx_ctx* c, int gi, bool deformed, bool zero_pred = false) {
   GridBuf& G = c->grid[gi];
   const pivx_geometry& g = G.g;
   XcArgs a{};
@@ -295,8 +298,10 @@ int run_xcorr(pivx_ctx* c, int gi, bool deformed) {
   a.lx2 = c->p.dispx_lim2;
   a.ly1 = c->p.dispy_lim1;
   a.ly2 = c->p.dispy_lim2;
-  a.pred_x = G.fx;
-  a.pred_y = G.fy;
+  a.pred_x = zero_pred ? nullptr : G.fx;
+  a.pred_y = zero_pred ? nullptr : G.fy;
+  a.zero_x = zero_pred ? G.fx : nullptr;
+  a.zero_y = zero_pred ? G.fy : nullptr;
   a.disp_x = G.dx;
   a.disp_y = G.dy;
   a.valid = G.dv;
@@ -620,15 +625,8 @@ int pivx_calc_cc_map_p1(pivx_ctx* c, int idx_pass) {
   if (!c) return fail(nullptr, PIVX_ERR_PARAM, "null handle");
   if (!c->have_frames) return fail(c, PIVX_ERR_STATE, "no frames: call pivx_prep_img_data first");
   if (idx_pass < 0) return fail(c, PIVX_ERR_PARAM, "idx_pass must be >= 0");
-  GridBuf& G = c->grid[0];
   int rc;
-  if (idx_pass == 0) {
-    // set_float_zero_kernel on the grid-1 predictor (cu:1287-1298)
-    rc = check_launch(c, launch_zero_f32(G.fx, G.g.nt_vec, c->stream), "set_float_zero_kernel");
-    if (!rc) rc = check_launch(c, launch_zero_f32(G.fy, G.g.nt_vec, c->stream), "set_float_zero_kernel");
-    if (rc) return rc;
-    return run_xcorr(c, 0, false);
-  }
+  if (idx_pass == 0) return run_xcorr(c, 0, false, true);
   rc = run_deform(c, 0);
   if (rc) return rc;
   return run_xcorr(c, 0, true);

@@ -23,6 +23,8 @@ struct XcArgs {
   float* pk_val; int* pk_idx;                     // [nt_vec][K]
   int* ncand;                                     // [nt_vec] uncapped local-maximum count
   int chunks_x;           // blocks per vector row (fast path)
+  float* zero_x;          // if set: zeroed at every window (first-pass predictor reset)
+  float* zero_y;
 };
 
 struct DefArgs {

@@ -22,6 +22,7 @@
 #include "sortnet.h"
 #include <math.h>
 #include <type_traits>
+#include <algorithm>
 
 namespace pivx {
 
@@ -271,7 +272,10 @@ __global__ void __launch_bounds__((FastCfg<WX, WY, WPB>::NT)) k_xcorr_fast(XcArg
     cnt = wave_sum(cnt);
     const float px = a.pred_x ? a.pred_x[w] : 0.f;
     const float py = a.pred_y ? a.pred_y[w] : 0.f;
-    if (lane == 0) a.ncand[w] = cnt;
+    if (lane == 0) {
+      a.ncand[w] = cnt;
+      if (a.zero_x) a.zero_x[w] = a.zero_y[w] = 0.f;
+    }
     // Npeak_save rounds of arg-max; round k's winner is kept by lane k % 64
     // and all fits of a group of 64 rounds run in parallel afterwards.
     float pv = INFINITY;
@@ -531,7 +535,10 @@ __global__ void __launch_bounds__(64) k_xcorr_pair(XcArgs a) {
   }
   const int n = half_sum(__popc(bits));
   const int w = iy * a.nx_vec + ix;
-  if (win_ok && l == 0) a.ncand[w] = n;
+  if (win_ok && l == 0) {
+    a.ncand[w] = n;
+    if (a.zero_x) a.zero_x[w] = a.zero_y[w] = 0.f;
+  }
   const float px = (win_ok && a.pred_x) ? a.pred_x[w] : 0.f;
   const float py = (win_ok && a.pred_y) ? a.pred_y[w] : 0.f;
   // lane best over its column's maxima (rows ascend, so the strict compare
@@ -600,10 +607,8 @@ __global__ void __launch_bounds__(64) k_xcorr_single(XcArgs a) {
   __shared__ float2 pl[WY][SX];
 
   const int lane = threadIdx.x;
-  const int bid = xcd_remap(blockIdx.x, gridDim.x);
-  const int iy = bid / a.nx_vec;
-  const int ix = bid - iy * a.nx_vec;
-
+  const int w = blockIdx.x;
+  const int iy = w / a.nx_vec, ix = w - iy * a.nx_vec;
   // 1-2. row gather + forward row FFT
   if (lane < WY) {
     float2 x[WX];
@@ -694,8 +699,10 @@ __global__ void __launch_bounds__(64) k_xcorr_single(XcArgs a) {
     });
   }
   const int n = wave_sum(__popcll((unsigned long long)bits));
-  const int w = bid;
-  if (lane == 0) a.ncand[w] = n;
+  if (lane == 0) {
+    a.ncand[w] = n;
+    if (a.zero_x) a.zero_x[w] = a.zero_y[w] = 0.f;
+  }
   const float px = a.pred_x ? a.pred_x[w] : 0.f;
   const float py = a.pred_y ? a.pred_y[w] : 0.f;
   // lane best over its column's maxima (rows ascend, so the strict compare
@@ -897,6 +904,7 @@ __global__ void __launch_bounds__(256) k_xcorr_generic(XcArgs a, int WX, int WY,
     int t = 0;
     for (int q = 0; q < (int)((blockDim.x + 63) >> 6); ++q) t += cnt_s[q];
     a.ncand[w] = t;
+    if (a.zero_x) a.zero_x[w] = a.zero_y[w] = 0.f;
   }
   const float px = a.pred_x ? a.pred_x[w] : 0.f;
   const float py = a.pred_y ? a.pred_y[w] : 0.f;
