@@ -47,38 +47,57 @@ def config2(pivx):
     return ip, fp, sp
 
 
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(pivx, ip, fp, a, b, threads, budget_s):
-    """FP32 scalar transcription (the oracle's FP32 instance) timed on this
-    host: the full pair if it fits the time budget, else a vertical crop of
-    the same workload, reported per correlation window."""
+    """FP32 scalar transcription (the oracle's FP32 instance, OpenMP over
+    windows) timed on this host on horizontal crops of the same pair (all
+    passes of both grids on the crop): one warm-up, then the median of 5
+    timed runs, once with `threads` threads and once single-threaded.  Crop
+    heights are chosen from a short calibration so that the whole leg takes
+    about `budget_s` seconds.  Reported per correlation window."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import pyoracle  # noqa: E402  (CPU baseline leg only)
 
-    # calibrate on a 1/16-area crop (same windows, same passes)
-    rows = 4400 // 16
-    crop = pivx.init_params(**{k: getattr(ip, k) for k, _ in ip._fields_})
-    crop.ny = rows
-    t0 = time.perf_counter()
-    pyoracle.run_pair(a[:rows], b[:rows], crop, fp, fp, prec=32, threads=threads)
-    t_small = time.perf_counter() - t0
-    n_small = _windows(pivx, crop)
-    est_full = t_small * 16
-    if est_full <= budget_s:
+    def crop_params(rows):
+        c = pivx.init_params(**{k: getattr(ip, k) for k, _ in ip._fields_})
+        c.ny = rows
+        return c
+
+    def run(rows, nthr):
+        c = crop_params(rows)
         t0 = time.perf_counter()
-        pyoracle.run_pair(a, b, ip, fp, fp, prec=32, threads=threads)
-        dt = time.perf_counter() - t0
-        n = _windows(pivx, ip)
-        sample = f"whole 6600x4400 pair ({n} windows), FP32 port, {threads} threads"
-    else:
-        k = max(1, int(budget_s / max(t_small, 1e-3)))
-        rows = min(4400, rows * k)
-        crop.ny = rows
-        t0 = time.perf_counter()
-        pyoracle.run_pair(a[:rows], b[:rows], crop, fp, fp, prec=32, threads=threads)
-        dt = time.perf_counter() - t0
-        n = _windows(pivx, crop)
-        sample = f"6600x{rows} crop of the same pair ({n} windows, all passes), FP32 port, {threads} threads"
-    return dict(value=n / dt, unit="windows/s", cores=threads, kind="port", sample=sample, seconds=round(dt, 2))
+        pyoracle.run_pair(a[:rows], b[:rows], c, fp, fp, prec=32, threads=nthr)
+        return time.perf_counter() - t0, _windows(pivx, c)
+
+    def timed(rows, nthr):
+        run(rows, nthr)  # warm-up
+        ts = sorted(run(rows, nthr)[0] for _ in range(5))
+        return ts[2], _windows(pivx, crop_params(rows))
+
+    # calibration on a 256-row crop (4 grid-1 and 16 grid-2 window rows)
+    t_cal, n_cal = run(256, threads)
+    per_win_mt = t_cal / n_cal
+    rows_mt = int(min(4400, max(256, (0.6 * budget_s / 6) / per_win_mt / n_cal * 256)) // 64 * 64)
+    t_mt, n_mt = timed(rows_mt, threads)
+    t_cal1, n_cal1 = run(128, 1)
+    per_win_1 = t_cal1 / n_cal1
+    rows_1 = int(min(4400, max(128, (0.4 * budget_s / 6) / per_win_1 / n_cal1 * 128)) // 64 * 64)
+    t_1, n_1 = timed(rows_1, 1)
+    return dict(value=round(n_mt / t_mt, 1), unit="windows/s", cores=threads, kind="port",
+                sample=(f"6600x{rows_mt} crop of the config-2 pair ({n_mt} windows over all 6 passes), FP32 port, "
+                        f"{threads} threads, median of 5 after 1 warm-up"),
+                single_thread=dict(value=round(n_1 / t_1, 1), sample=f"6600x{rows_1} crop ({n_1} windows), 1 thread"),
+                seconds_per_run=round(t_mt, 3), hardware_concurrency=os.cpu_count(), cpu_model=_cpu_model())
 
 
 def _windows(pivx, ip):
