@@ -108,6 +108,23 @@ int pivx_prep_img_data(pivx_ctx* ctx, const uint8_t* img1, const uint8_t* img2, 
  * (device-to-device copy, no host staging). */
 int pivx_prep_img_data_device(pivx_ctx* ctx, const uint8_t* d_img1, const uint8_t* d_img2, size_t pitch);
 
+/* MHE contrast enhancement, replacing the mhe_gpu class (MHE7_4_gpu.h:7-40;
+ * mhe_gpu::init at MHE7_4_gpu.cu:149-212, processsing at :214-253).  With
+ * enable = 1 every frame later passed to pivx_prep_img_data[_device] is
+ * enhanced on the GPU before use, as main.cpp:267-283 does when Parameter.dat
+ * line 8 (ifmhe) is 1; ws and thresh are lines 9-10.  Pixels of the central
+ * region of ws x ws tiles are mapped through their tile's transfer table,
+ * pixels outside it become 0.  Rejects ws outside [1, min(nx, ny)] and
+ * thresholds that leave no counted pixel per tile (the reference divides by
+ * zero there).  enable = 0 switches the enhancement off. */
+int pivx_set_mhe(pivx_ctx* ctx, int enable, int ws, float thresh);
+
+/* One frame through the configured MHE (host in, host out, same pitch),
+ * without touching the handle's frames: the d_enhanced result of
+ * mhe_gpu::processsing (MHE7_4_gpu.cu:214-253) copied back as main.cpp:274
+ * does.  Requires a prior pivx_set_mhe(ctx, 1, ...). */
+int pivx_mhe_enhance(pivx_ctx* ctx, const uint8_t* img, uint8_t* out, size_t pitch);
+
 /* ------------------------------------------------------------------ passes */
 
 /* xcorr2d_multi::Calc_cc_map_p1 (cu:1284-1449): one grid-1 correlation pass.

@@ -35,6 +35,10 @@
 //                           mean
 //   orc_geometry         -- reference result files (I=206,J=137 / I=412,J=275
 //                           for 6600x4400 at spacing 32/16)
+//   orc_mhe              -- numpy brute force (per-tile np.bincount, 3x3 sums,
+//                           np.cumsum) on random frames; constant frames map
+//                           to 255; margins outside the tiles are 0; corner /
+//                           edge / interior neighbour counts on a 3x3 tiling
 #include "oracle.h"
 
 #include <algorithm>
@@ -838,6 +842,79 @@ int orc_run_pair(int prec, const orc_run* R, const uint8_t* f1, const uint8_t* f
   if (prec == 32)
     return run_pair<float>(R, f1, f2, r1x, r1y, r1v, o1x, o1y, o1v, r2x, r2y, r2v, o2x, o2y, o2v, m2x, m2y, m2v);
   return run_pair<double>(R, f1, f2, r1x, r1y, r1v, o1x, o1y, o1v, r2x, r2y, r2v, o2x, o2y, o2v, m2x, m2y, m2v);
+}
+
+// MHE contrast enhancement (MHE7_4_gpu.cu = mhe), restated step by step:
+// init geometry (mhe:149-178), neighbour counts (mhe:45-64), tile histograms
+// with a zero border ring (mhe:66-83, 216), 3x3 transfer sums (mhe:85-101),
+// inclusive prefix sum over the 256 levels (prescan_kernel writes the
+// exclusive scan shifted by one, mhe:5-43), enhanced values (mhe:103-119) and
+// the mapping of the tiled region (mhe:121-137); pixels outside it keep the 0
+// that d_enhanced is cleared to at init (mhe:197).  Integer arithmetic only;
+// the product c*255 is formed in 64 bits.
+int orc_mhe(const uint8_t* in, uint8_t* out, int nx, int ny, int ws, float thresh) {
+  if (ws < 1 || ws > nx || ws > ny) return 1;
+  int width, height, xs, ys;
+  if (nx % ws == 0 && ny % ws == 0) {
+    width = nx;
+    height = ny;
+    xs = 0;
+    ys = 0;
+  } else {
+    width = ws * (nx / ws);
+    xs = (nx - width) / 2;
+    height = ws * (ny / ws);
+    ys = (ny - height) / 2;
+  }
+  const int maxI = height / ws, maxJ = width / ws;
+  const int ntrash = (int)roundf((float)(ws * ws) * thresh);
+  const int nnontrash = ws * ws - ntrash;
+  if (nnontrash < 1) return 1;
+  std::vector<int> cnt((size_t)maxI * maxJ);
+  for (int j = 0; j < maxI; ++j)
+    for (int i = 0; i < maxJ; ++i) {
+      int c;
+      if (i >= 1 && i <= maxJ - 2 && j >= 1 && j <= maxI - 2) c = 9;
+      else if ((i == 0 && j == 0) || (i == 0 && j == maxI - 1) || (i == maxJ - 1 && j == 0) ||
+               (i == maxJ - 1 && j == maxI - 1))
+        c = 4;
+      else c = 6;
+      cnt[(size_t)j * maxJ + i] = c;
+    }
+  std::vector<int> levels((size_t)(maxI + 2) * (maxJ + 2) * 256, 0);
+  for (int idy = 0; idy < height; ++idy)
+    for (int idx = 0; idx < width; ++idx) {
+      const int v = in[(size_t)(idy + ys) * nx + xs + idx];
+      const int index = (maxJ + 2) * (idy / ws + 1) + 1 + idx / ws;
+      ++levels[(size_t)index * 256 + v];
+    }
+  std::vector<int> lut((size_t)maxI * maxJ * 256);
+  for (int ty = 0; ty < maxI; ++ty)
+    for (int tx = 0; tx < maxJ; ++tx) {
+      const int index = (ty + 1) * (maxJ + 2) + 1 + tx;
+      const int n = cnt[(size_t)ty * maxJ + tx];
+      long long cum = 0;
+      for (int p = 0; p < 256; ++p) {
+        long long t = 0;
+        for (int dy = -1; dy <= 1; ++dy)
+          for (int dx = -1; dx <= 1; ++dx) t += levels[(size_t)(index + dy * (maxJ + 2) + dx) * 256 + p];
+        cum += t;
+        long long c = cum - (long long)n * ntrash;
+        c = c >= 0 ? c : 0;
+        c = std::min<long long>(255, c * 255 / ((long long)n * nnontrash));
+        if (p == 0) c = 0;
+        if (p == 255) c = 255;
+        lut[((size_t)ty * maxJ + tx) * 256 + p] = (int)c;
+      }
+    }
+  std::memset(out, 0, (size_t)nx * ny);
+  for (int idy = 0; idy < height; ++idy)
+    for (int idx = 0; idx < width; ++idx) {
+      const size_t o = (size_t)(idy + ys) * nx + xs + idx;
+      const int tile = (idy / ws) * maxJ + idx / ws;
+      out[o] = (uint8_t)lut[(size_t)tile * 256 + in[o]];
+    }
+  return 0;
 }
 
 }  // extern "C"

@@ -109,6 +109,10 @@ int orc_run_pair(int prec, const orc_run* R, const uint8_t* f1, const uint8_t* f
                  uint8_t* raw1v, double* f1x, double* f1y, uint8_t* f1v, double* raw2x, double* raw2y, uint8_t* raw2v,
                  double* f2x, double* f2y, uint8_t* f2v, double* mo2x, double* mo2y, uint8_t* mo2v);
 
+/* MHE contrast enhancement of one nx*ny u8 frame (MHE7_4_gpu.cu:45-253).
+ * Returns 1 for a window or threshold the reference cannot run. */
+int orc_mhe(const uint8_t* in, uint8_t* out, int nx, int ny, int ws, float thresh);
+
 #ifdef __cplusplus
 }
 #endif

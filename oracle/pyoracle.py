@@ -58,6 +58,7 @@ def lib():
         L.orc_fill.argtypes = [C.c_int, _D, _D, _D, C.c_int, C.c_int, _D]
         L.orc_smooth.argtypes = [C.c_int, _D, _D, C.c_int, C.c_int, C.c_int, C.c_int, _D, _D]
         L.orc_run_pair.argtypes = [C.c_int, C.POINTER(OrcRun), _D, _D] + [_D] * 15
+        L.orc_mhe.argtypes = [_D, _D, C.c_int, C.c_int, C.c_int, C.c_float]
         _lib = L
     return _lib
 
@@ -193,6 +194,15 @@ def smooth(sx, sy, nvec, win, prec=64):
     dy = np.empty_like(dx)
     lib().orc_smooth(prec, _p(sx), _p(sy), nvec[0], nvec[1], win[0], win[1], _p(dx), _p(dy))
     return dx, dy
+
+
+def mhe(img, ws, thresh):
+    """MHE-enhanced copy of one u8 frame (MHE7_4_gpu.cu)."""
+    a = np.ascontiguousarray(img, np.uint8)
+    out = np.empty_like(a)
+    if lib().orc_mhe(_p(a), _p(out), a.shape[1], a.shape[0], ws, thresh) != 0:
+        raise ValueError("MHE window or threshold out of range")
+    return out
 
 
 def run_pair(f1, f2, ip, fp1, fp2, prec=64, threads=1):

@@ -92,6 +92,8 @@ _SIGS = {
     "pivx_get_geometry": [_P, C.c_int, C.POINTER(Geometry)],
     "pivx_prep_img_data": [_P, _P, _P, C.c_size_t],
     "pivx_prep_img_data_device": [_P, _P, _P, C.c_size_t],
+    "pivx_set_mhe": [_P, C.c_int, C.c_int, C.c_float],
+    "pivx_mhe_enhance": [_P, _P, _P, C.c_size_t],
     "pivx_calc_cc_map_p1": [_P, C.c_int],
     "pivx_median_filter_smoothing_p1": [_P, C.c_int, C.c_int, C.c_float, C.c_float, C.c_int],
     "pivx_extend_filtered_vecfield_p1": [_P],
@@ -223,6 +225,20 @@ class Xcorr2dMulti:
 
     def prep_img_data_device(self, dptr1: int, dptr2: int, pitch: int):
         self._c(lib().pivx_prep_img_data_device(self.h, C.c_void_p(dptr1), C.c_void_p(dptr2), pitch))
+
+    # ------------------------------------------------------------ MHE (mhe_gpu, MHE7_4_gpu.cu:149-253)
+    def set_mhe(self, enable: bool, ws: int = 40, thresh: float = 0.91):
+        """Enhance every later uploaded frame on the GPU (main.cpp:267-283)."""
+        self._c(lib().pivx_set_mhe(self.h, 1 if enable else 0, ws, thresh))
+
+    def mhe_enhance(self, img: np.ndarray) -> np.ndarray:
+        """One frame through the configured MHE; returns the enhanced frame."""
+        a = np.ascontiguousarray(img, dtype=np.uint8)
+        if a.shape != (self.p.ny, self.p.nx):
+            raise ValueError(f"frame must be ({self.p.ny}, {self.p.nx}) uint8")
+        out = np.empty_like(a)
+        self._c(lib().pivx_mhe_enhance(self.h, _ptr(a), _ptr(out), self.p.nx))
+        return out
 
     # ------------------------------------------------------------ passes (reference method names)
     def Calc_cc_map_p1(self, idx_pass: int):

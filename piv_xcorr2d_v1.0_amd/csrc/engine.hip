@@ -66,6 +66,11 @@ struct pivx_ctx {
   uint8_t* mo_v = nullptr;
   uint8_t *f1 = nullptr, *f2 = nullptr;    // frames [ny][nx]
   bool have_frames = false;
+  // MHE enhancement of uploaded frames (MHE7_4_gpu.cu), off unless pivx_set_mhe
+  bool mhe_on = false, mhe_ready = false;
+  MheArgs mhe{};
+  uint8_t* mhe_in = nullptr;    // frame staging (row pitch nx)
+  uint8_t* mhe_out = nullptr;   // output of pivx_mhe_enhance
   unsigned long long* counters = nullptr;  // [2]: cumulative out-of-buffer deformation taps per grid
   int* sweeps = nullptr;                   // [2]
   int* nholes = nullptr;                   // hole-list counter of the fill kernels
@@ -602,12 +607,29 @@ int pivx_get_geometry(const pivx_ctx* c, int grid, pivx_geometry* g) {
   return PIVX_OK;
 }
 
+// MHE of one frame already in c->mhe_in into `out` (device, row pitch nx).
+static int mhe_run(pivx_ctx* c, uint8_t* out) {
+  MheArgs m = c->mhe;
+  m.in = c->mhe_in;
+  m.out = out;
+  return check_launch(c, launch_mhe(m, c->stream), "mhe_gpu::processsing");
+}
+
 static int upload(pivx_ctx* c, const uint8_t* a, const uint8_t* b, size_t pitch, hipMemcpyKind kind) {
   if (!c || !a || !b) return fail(c, PIVX_ERR_PARAM, "null frame");
   if (pitch < (size_t)c->p.nx) return fail(c, PIVX_ERR_PARAM, "pitch %zu < nx %d", pitch, c->p.nx);
   HIPCHK(c, hipSetDevice(c->device));
-  HIPCHK(c, hipMemcpy2DAsync(c->f1, c->p.nx, a, pitch, c->p.nx, c->p.ny, kind, c->stream));
-  HIPCHK(c, hipMemcpy2DAsync(c->f2, c->p.nx, b, pitch, c->p.nx, c->p.ny, kind, c->stream));
+  if (c->mhe_on) {
+    // main.cpp:267-283: each frame is enhanced before prep_img_data
+    int rc;
+    HIPCHK(c, hipMemcpy2DAsync(c->mhe_in, c->p.nx, a, pitch, c->p.nx, c->p.ny, kind, c->stream));
+    if ((rc = mhe_run(c, c->f1))) return rc;
+    HIPCHK(c, hipMemcpy2DAsync(c->mhe_in, c->p.nx, b, pitch, c->p.nx, c->p.ny, kind, c->stream));
+    if ((rc = mhe_run(c, c->f2))) return rc;
+  } else {
+    HIPCHK(c, hipMemcpy2DAsync(c->f1, c->p.nx, a, pitch, c->p.nx, c->p.ny, kind, c->stream));
+    HIPCHK(c, hipMemcpy2DAsync(c->f2, c->p.nx, b, pitch, c->p.nx, c->p.ny, kind, c->stream));
+  }
   if (kind == hipMemcpyHostToDevice) HIPCHK(c, hipStreamSynchronize(c->stream));  // caller may reuse its buffers
   c->have_frames = true;
   return PIVX_OK;
@@ -619,6 +641,68 @@ int pivx_prep_img_data(pivx_ctx* c, const uint8_t* img1, const uint8_t* img2, si
 
 int pivx_prep_img_data_device(pivx_ctx* c, const uint8_t* d1, const uint8_t* d2, size_t pitch) {
   return upload(c, d1, d2, pitch, hipMemcpyDeviceToDevice);
+}
+
+// mhe_gpu::init (mhe:149-212) for the handle's frame size: tile grid, central
+// region and thresholds; the buffers are allocated once.
+int pivx_set_mhe(pivx_ctx* c, int enable, int ws, float thresh) {
+  if (!c) return fail(nullptr, PIVX_ERR_PARAM, "null handle");
+  if (!enable) {
+    c->mhe_on = false;
+    return PIVX_OK;
+  }
+  const int nx = c->p.nx, ny = c->p.ny;
+  if (ws < 1 || ws > nx || ws > ny)
+    return fail(c, PIVX_ERR_PARAM, "MHE window %d must lie in [1, min(nx, ny)] = [1, %d]", ws, std::min(nx, ny));
+  const int ntrash = (int)roundf((float)(ws * ws) * thresh);
+  const int nnontrash = ws * ws - ntrash;
+  if (nnontrash < 1)
+    return fail(c, PIVX_ERR_PARAM, "MHE threshold %g leaves no counted pixels in a %dx%d tile", thresh, ws, ws);
+  MheArgs m{};
+  m.nx = nx;
+  m.ny = ny;
+  m.ws = ws;
+  if (nx % ws == 0 && ny % ws == 0) {
+    m.xs = 0;
+    m.ys = 0;
+  } else {
+    m.xs = (nx - ws * (nx / ws)) / 2;
+    m.ys = (ny - ws * (ny / ws)) / 2;
+  }
+  m.tiles_x = nx / ws;
+  m.tiles_y = ny / ws;
+  m.ntrash = ntrash;
+  m.nnontrash = nnontrash;
+  HIPCHK(c, hipSetDevice(c->device));
+  const size_t nh = (size_t)(m.tiles_x + 2) * (m.tiles_y + 2) * 256;
+  if (c->mhe_ready && c->mhe.tiles_x == m.tiles_x && c->mhe.tiles_y == m.tiles_y) {
+    m.hist = c->mhe.hist;
+  } else {
+    int rc = dalloc(c, &m.hist, nh);
+    if (!rc && !c->mhe_in) rc = dalloc(c, &c->mhe_in, (size_t)nx * ny);
+    if (!rc && !c->mhe_out) rc = dalloc(c, &c->mhe_out, (size_t)nx * ny);
+    if (rc) return rc;
+    // the border ring of the histogram array stays 0 (d_levels' zero frame, mhe:216)
+    HIPCHK(c, hipMemsetAsync(m.hist, 0, nh * sizeof(uint32_t), c->stream));
+  }
+  c->mhe = m;
+  c->mhe_ready = true;
+  c->mhe_on = true;
+  return PIVX_OK;
+}
+
+// mhe_gpu::processsing + d_enhanced download (mhe:214-253) on one host frame.
+int pivx_mhe_enhance(pivx_ctx* c, const uint8_t* img, uint8_t* out, size_t pitch) {
+  if (!c || !img || !out) return fail(c, PIVX_ERR_PARAM, "null argument");
+  if (!c->mhe_ready) return fail(c, PIVX_ERR_STATE, "MHE not configured: call pivx_set_mhe first");
+  if (pitch < (size_t)c->p.nx) return fail(c, PIVX_ERR_PARAM, "pitch %zu < nx %d", pitch, c->p.nx);
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipMemcpy2DAsync(c->mhe_in, c->p.nx, img, pitch, c->p.nx, c->p.ny, hipMemcpyHostToDevice, c->stream));
+  int rc = mhe_run(c, c->mhe_out);
+  if (rc) return rc;
+  HIPCHK(c, hipMemcpy2DAsync(out, pitch, c->mhe_out, c->p.nx, c->p.nx, c->p.ny, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return PIVX_OK;
 }
 
 int pivx_calc_cc_map_p1(pivx_ctx* c, int idx_pass) {

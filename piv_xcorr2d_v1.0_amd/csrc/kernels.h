@@ -63,6 +63,17 @@ struct SmoothArgs {
   float* ex; float* ey; int border;   // if ex is set: also write the border-replicated extension
 };
 
+// MHE contrast enhancement of one u8 frame (MHE7_4_gpu.cu).  The central
+// tiles_x*ws by tiles_y*ws region (origin xs, ys) is split into ws x ws tiles;
+// every pixel there is mapped through its tile's transfer table, every pixel
+// outside it becomes 0.
+struct MheArgs {
+  const uint8_t* in; uint8_t* out; int nx, ny;
+  int ws, xs, ys, tiles_x, tiles_y;
+  int ntrash, nnontrash;
+  uint32_t* hist;          // [(tiles_y+2)*(tiles_x+2)][256]; the border ring is kept at 0
+};
+
 // Launchers (return hipGetLastError() of the launch).
 hipError_t launch_xcorr(const XcArgs& a, int ws_x, int ws_y, float2* gscratch, size_t gscratch_windows,
                         const float2* twx, const float2* twy, hipStream_t s);
@@ -84,5 +95,6 @@ hipError_t launch_copy_field(const float* sx, const float* sy, const uint8_t* sv
                              int nt, hipStream_t s);
 hipError_t launch_zero_u64(unsigned long long* p, size_t n, hipStream_t s);
 hipError_t launch_stream_copy(const void* src, void* dst, size_t bytes, hipStream_t s);
+hipError_t launch_mhe(const MheArgs& a, hipStream_t s);
 
 }  // namespace pivx

@@ -1508,6 +1508,86 @@ __global__ void k_zero_u64(unsigned long long* p, size_t n) {
   for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) p[t] = 0;
 }
 
+// ============================================================== MHE enhancement
+// MakeHist_kernel (mhe:66-83): 256-bin histogram of every tile, one workgroup
+// per tile with one sub-histogram per wave in LDS, written into the tile's
+// slot of the bordered histogram array.
+__global__ void __launch_bounds__(256) k_mhe_hist(MheArgs a) {
+  __shared__ uint32_t h[4][256];
+  const int t = threadIdx.x, wv = t >> 6;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) h[q][t] = 0;
+  __syncthreads();
+  const int tile = blockIdx.x;
+  const int ty = tile / a.tiles_x, tx = tile - ty * a.tiles_x;
+  const uint8_t* base = a.in + (long long)(a.ys + ty * a.ws) * a.nx + a.xs + tx * a.ws;
+  const int npix = a.ws * a.ws;
+  for (int p = t; p < npix; p += 256) {
+    const int r = p / a.ws, c = p - r * a.ws;
+    atomicAdd(&h[wv][base[(long long)r * a.nx + c]], 1u);
+  }
+  __syncthreads();
+  a.hist[((long long)(ty + 1) * (a.tiles_x + 2) + tx + 1) * 256 + t] = h[0][t] + h[1][t] + h[2][t] + h[3][t];
+}
+
+// CreateTransfer_kernel + prescan_kernel + Calc_Enhanced_value_kernel +
+// Enhanced_kernel (mhe:85-137, 5-43), one workgroup per tile, thread = grey
+// level p: the 3x3 tile neighbourhood's histogram count at p, its inclusive
+// prefix sum over p, then
+//   v = min(255, max(cum - n*Ntrash, 0) * 255 / (n * Nnontrash)), v(0) = 0, v(255) = 255
+// with n the tile's neighbour count of Calc_cnt_neigh_kernel (mhe:45-64).
+// The workgroup then maps its tile's pixels and zeroes the pixels outside the
+// tiled region that lie beside it (edge tiles own the margins), so every
+// frame pixel is written exactly once.
+__global__ void __launch_bounds__(256) k_mhe_map(MheArgs a) {
+  __shared__ int wsum[4];
+  __shared__ uint8_t lut[256];
+  const int p = threadIdx.x, lane = p & 63, wv = p >> 6;
+  const int tile = blockIdx.x;
+  const int ty = tile / a.tiles_x, tx = tile - ty * a.tiles_x;
+  const int row = a.tiles_x + 2;
+  const uint32_t* hc = a.hist + ((long long)(ty + 1) * row + tx + 1) * 256 + p;
+  int v = 0;
+#pragma unroll
+  for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+    for (int dx = -1; dx <= 1; ++dx) v += (int)hc[((long long)dy * row + dx) * 256];
+  // inclusive scan over the 256 levels
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int o = __shfl_up(v, off, 64);
+    if (lane >= off) v += o;
+  }
+  if (lane == 63) wsum[wv] = v;
+  __syncthreads();
+  for (int q = 0; q < wv; ++q) v += wsum[q];
+  const int mx = a.tiles_x, my = a.tiles_y;
+  int n;
+  if (tx >= 1 && tx <= mx - 2 && ty >= 1 && ty <= my - 2) n = 9;
+  else if ((tx == 0 && ty == 0) || (tx == 0 && ty == my - 1) || (tx == mx - 1 && ty == 0) || (tx == mx - 1 && ty == my - 1))
+    n = 4;
+  else n = 6;
+  long long c = (long long)v - (long long)n * a.ntrash;
+  c = c >= 0 ? c : 0;
+  c = c * 255 / ((long long)n * a.nnontrash);
+  int e = c < 255 ? (int)c : 255;
+  if (p == 0) e = 0;
+  if (p == 255) e = 255;
+  lut[p] = (uint8_t)e;
+  __syncthreads();
+  // rectangle owned by this tile: the tile, widened to the frame edge on edge tiles
+  const int x0t = a.xs + tx * a.ws, y0t = a.ys + ty * a.ws;
+  const int xlo = tx == 0 ? 0 : x0t, xhi = tx == mx - 1 ? a.nx : x0t + a.ws;
+  const int ylo = ty == 0 ? 0 : y0t, yhi = ty == my - 1 ? a.ny : y0t + a.ws;
+  const int w = xhi - xlo, npix = w * (yhi - ylo);
+  for (int q = p; q < npix; q += 256) {
+    const int r = q / w, y = ylo + r, x = xlo + (q - r * w);
+    const long long o = (long long)y * a.nx + x;
+    const bool in_tile = x >= x0t && x < x0t + a.ws && y >= y0t && y < y0t + a.ws;
+    a.out[o] = in_tile ? lut[a.in[o]] : (uint8_t)0;
+  }
+}
+
 // ============================================================== launchers
 
 static inline int grid_for(long long n, int block, int cap = 65536) {
@@ -1679,6 +1759,15 @@ hipError_t launch_stream_copy(const void* src, void* dst, size_t bytes, hipStrea
 }
 hipError_t launch_zero_u64(unsigned long long* p, size_t n, hipStream_t s) {
   hipLaunchKernelGGL(k_zero_u64, dim3(grid_for((long long)n, 256)), dim3(256), 0, s, p, n);
+  return hipGetLastError();
+}
+hipError_t launch_mhe(const MheArgs& a, hipStream_t s) {
+  const int tiles = a.tiles_x * a.tiles_y;
+  if (tiles < 1) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_mhe_hist, dim3(tiles), dim3(256), 0, s, a);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_mhe_map, dim3(tiles), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 

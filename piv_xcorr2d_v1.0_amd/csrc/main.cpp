@@ -7,9 +7,13 @@
 // main.cpp:327-376.  Windows-only pieces (MessageBox, getchar, MFC strings)
 // are replaced by messages on stderr and exit codes.
 //
+// When Parameter.dat line 8 (ifmhe) is 1, both frames of every pair are MHE
+// enhanced on the GPU with the window and threshold of lines 9-10, as
+// main.cpp:267-283 does.
+//
 // Extra flags (after the parameter file): --crlf writes CRLF line ends like
 // the reference's Windows text mode; --device N overrides Parameter.dat line 7;
-// --ignore-mhe runs with MHE switched off when line 8 asks for it.
+// --ignore-mhe runs with MHE switched off even when line 8 asks for it.
 #include "../../include/piv_xcorr2d.h"
 
 #include <sys/stat.h>
@@ -54,9 +58,6 @@ int main(int argc, char** argv) {
   pivx_parameter_file pf;
   int rc = pivx_parse_parameter_file(dir_para.c_str(), &pf);
   if (rc) return die(pivx_last_error(nullptr));  // "Can't open parameter file!" / "Parameter missing ..."
-  if (pf.ifmhe == 1 && !ignore_mhe)
-    return die("MHE contrast enhancement (Parameter.dat line 8 = 1) is not part of this build; "
-               "set line 8 to 0 or pass --ignore-mhe.");
   const int device = device_override >= 0 ? device_override : pf.device_id;
 
   std::vector<int> pairidx;
@@ -80,6 +81,11 @@ int main(int argc, char** argv) {
   if (pivx_init(&pf.init, device, &ctx) != PIVX_OK) {
     fprintf(stderr, "%s\n", pivx_last_error(nullptr));
     return die("Initialization of xcorr2d_multi class failed!");
+  }
+  if (pf.ifmhe == 1 && !ignore_mhe && pivx_set_mhe(ctx, 1, pf.mhe_ws, pf.mhe_thresh) != PIVX_OK) {
+    fprintf(stderr, "%s\n", pivx_last_error(ctx));
+    pivx_freemem(ctx);
+    return die("Initialization of mhe_gpu class failed!");
   }
   const size_t npix = (size_t)pf.init.nx * pf.init.ny;
   std::vector<uint8_t> img1(npix), img2(npix);

@@ -99,7 +99,43 @@ def test_cli_errors(pivx, tmp_path):
     (tmp_path / "pair index.txt").write_text("3")
     r = subprocess.run([pivx.CLI_PATH, str(par)], capture_output=True, text=True, timeout=60)
     assert r.returncode == 1 and "missing" in r.stderr
-    lines[7] = "1"  # MHE requested
+    # MHE requested with a window the reference cannot tile (line 9 = 0)
+    img = np.zeros((256, 256), np.uint8)
+    pivx.tiff_write8(str(tmp_path / "1_0003.tif"), img)
+    pivx.tiff_write8(str(tmp_path / "2_0003.tif"), img)
+    lines[7] = "1"
+    lines[8] = "0"
     _write_params(par, lines)
     r = subprocess.run([pivx.CLI_PATH, str(par)], capture_output=True, text=True, timeout=60)
-    assert r.returncode == 1 and "MHE" in r.stderr
+    assert r.returncode == 1 and "mhe_gpu" in r.stderr
+
+
+def test_cli_mhe_matches_enhanced_frames(pivx, orc, tmp_path):
+    """Parameter.dat line 8 = 1: the CLI enhances both frames (window 40,
+    threshold 0.91, lines 9-10) before the pair runs; the vectors equal a
+    library run on frames enhanced by the oracle's MHE."""
+    nx, ny = 640, 448
+    sp = pivx.synth_params(nx, ny, seed=77, flow=0, a0=3.3, a1=-1.7)
+    a, b = pivx.synth_pair(sp)
+    pivx.tiff_write8(str(tmp_path / "1_0005.tif"), a)
+    pivx.tiff_write8(str(tmp_path / "2_0005.tif"), b)
+    (tmp_path / "pair index.txt").write_text("5")
+    lines = _param_lines(tmp_path, nx, ny)
+    lines[7] = "1"
+    par = tmp_path / "Parameter.dat"
+    _write_params(par, lines)
+    r = subprocess.run([pivx.CLI_PATH, str(par)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    pf = pivx.parse_parameter_file(str(par))
+    assert (pf.ifmhe, pf.mhe_ws) == (1, 40)
+    ea, eb = orc.mhe(a, 40, pf.mhe_thresh), orc.mhe(b, 40, pf.mhe_thresh)
+    fp1 = pivx.FilterParams(pf.npass1_median, pf.min_nneigh_p1, pf.residual_threshold_p1, pf.insert_threshold_p1,
+                            pf.npass1_smooth)
+    fp2 = pivx.FilterParams(pf.npass2_median, pf.min_nneigh_p2, pf.residual_threshold_p2, pf.insert_threshold_p2,
+                            pf.npass2_smooth)
+    with pivx.Xcorr2dMulti(pf.init) as x:
+        x.prep_img_data(ea, eb)
+        x.run_pair(fp1, fp2)
+        f2 = x.filtered(2)
+    head, d = _read_tecplot(pivx.output_name(pf, 5, 3))
+    assert np.abs(d[:, 2] - f2[0]).max() <= 0.005 + 1e-6 and np.abs(d[:, 3] - f2[1]).max() <= 0.005 + 1e-6

@@ -357,3 +357,55 @@ def test_graph_replays_match_eager(pivx):
                 for u, w in zip(A, B):
                     np.testing.assert_array_equal(u, w)
     _report("graph replay", replays=6)
+
+
+# ---------------------------------------------------------------- MHE enhancement
+@pytest.mark.parametrize("nx,ny,ws,thr", [(6600, 4400, 40, 0.91), (1000, 700, 48, 0.5), (257, 129, 16, 0.0),
+                                          (300, 200, 200, 0.3), (640, 448, 40, 0.91)])
+def test_mhe_bitexact(pivx, orc, nx, ny, ws, thr):
+    """mhe_gpu::processsing on the GPU against the oracle's restatement:
+    integer work, bit-exact; covers divisible and non-divisible sizes and a
+    single-tile row."""
+    sp_ = pivx.synth_params(nx, ny, seed=nx + ws, flow=0, a0=1.0, a1=0.5)
+    a, _ = pivx.synth_pair(sp_)
+    p = config1_params(pivx, nx=nx, ny=ny, ws_x1=32, ws_y1=32, spacing_x1=32, spacing_y1=32, npass2=0)
+    with pivx.Xcorr2dMulti(p) as x:
+        x.set_mhe(True, ws, thr)
+        got = x.mhe_enhance(a)
+        got2 = x.mhe_enhance(255 - a)  # second frame through the same buffers
+    np.testing.assert_array_equal(got, orc.mhe(a, ws, thr))
+    np.testing.assert_array_equal(got2, orc.mhe(255 - a, ws, thr))
+    _report(f"mhe {nx}x{ny} ws={ws} thr={thr}", pixels=nx * ny, changed=int((got != a).sum()))
+
+
+def test_mhe_run_pair_equals_enhanced_frames(pivx, orc):
+    """With MHE switched on, run_pair equals run_pair on frames the oracle
+    enhanced beforehand (main.cpp:267-283 order: enhance, then prep)."""
+    nx, ny = 768, 512
+    sp_ = pivx.synth_params(nx, ny, seed=21, flow=1, a0=100.0, a1=5.0, a2=0.002)
+    a, b = pivx.synth_pair(sp_)
+    p = config1_params(pivx, nx=nx, ny=ny, ws_x1=64, ws_y1=64, spacing_x1=32, spacing_y1=32, npass1=2,
+                       ws_x2=32, ws_y2=32, spacing_x2=16, spacing_y2=16, npass2=2)
+    f = pivx.FilterParams(2, 5, 2.0, 2.0, 1)
+    with pivx.Xcorr2dMulti(p) as x:
+        x.set_mhe(True, 40, 0.91)
+        x.prep_img_data(a, b)
+        x.run_pair(f, f)
+        r1 = [x.raw(2), x.filtered(2), x.filtered(3)]
+    with pivx.Xcorr2dMulti(p) as y:
+        y.prep_img_data(orc.mhe(a, 40, 0.91), orc.mhe(b, 40, 0.91))
+        y.run_pair(f, f)
+        r2 = [y.raw(2), y.filtered(2), y.filtered(3)]
+    for A, B in zip(r1, r2):
+        for u, w in zip(A, B):
+            np.testing.assert_array_equal(u, w)
+
+
+def test_mhe_rejects_untileable_settings(pivx):
+    p = config1_params(pivx, nx=256, ny=128, npass2=0)
+    with pivx.Xcorr2dMulti(p) as x:
+        for ws, thr in ((0, 0.5), (129, 0.5), (16, 1.0)):
+            with pytest.raises(pivx.PivError):
+                x.set_mhe(True, ws, thr)
+        with pytest.raises(pivx.PivError):
+            x.mhe_enhance(np.zeros((128, 256), np.uint8))

@@ -425,3 +425,83 @@ def test_run_pair_composes_the_stages(orc, pivx):
     sx, sy = orc.smooth(sx, sy, nv, (3, 3))
     np.testing.assert_array_equal(out["filt1"][0], sx)
     np.testing.assert_array_equal(out["filt1"][2], sv)
+
+
+# ---------------------------------------------------------------- MHE (MHE7_4_gpu.cu)
+def _mhe_numpy(img, ws, thresh):
+    """Vectorised restatement: per-tile np.bincount histograms, a zero ring,
+    3x3 tile sums by slicing, np.cumsum over the levels, the enhanced-value
+    formula and the tile lookup; pixels outside the tiled region are 0."""
+    ny, nx = img.shape
+    if nx % ws == 0 and ny % ws == 0:
+        xs = ys = 0
+    else:
+        xs, ys = (nx - ws * (nx // ws)) // 2, (ny - ws * (ny // ws)) // 2
+    ti, tj = ny // ws, nx // ws
+    reg = img[ys:ys + ti * ws, xs:xs + tj * ws].astype(np.int64)
+    tiles = reg.reshape(ti, ws, tj, ws).transpose(0, 2, 1, 3).reshape(ti, tj, ws * ws)
+    hist = np.zeros((ti + 2, tj + 2, 256), np.int64)
+    for a in range(ti):
+        for b in range(tj):
+            hist[a + 1, b + 1] = np.bincount(tiles[a, b], minlength=256)
+    trans = sum(hist[1 + dy:1 + dy + ti, 1 + dx:1 + dx + tj] for dy in (-1, 0, 1) for dx in (-1, 0, 1))
+    cum = np.cumsum(trans, axis=2)
+    n = np.full((ti, tj), 6, np.int64)
+    n[1:ti - 1, 1:tj - 1] = 9
+    for a, b in ((0, 0), (ti - 1, 0), (0, tj - 1), (ti - 1, tj - 1)):
+        n[a, b] = 4
+    t = float(np.float32(ws * ws) * np.float32(thresh))
+    ntrash = int(np.floor(abs(t) + 0.5)) * (1 if t >= 0 else -1)  # C roundf: half away from zero
+    nn = ws * ws - ntrash
+    c = np.maximum(cum - n[..., None] * ntrash, 0) * 255 // (n[..., None] * nn)
+    lut = np.minimum(c, 255)
+    lut[..., 0] = 0
+    lut[..., 255] = 255
+    out = np.zeros_like(img)
+    ty, tx = np.meshgrid(np.arange(ti * ws) // ws, np.arange(tj * ws) // ws, indexing="ij")
+    out[ys:ys + ti * ws, xs:xs + tj * ws] = lut[ty, tx, reg]
+    return out
+
+
+@pytest.mark.parametrize("nx,ny,ws,thr", [(96, 64, 16, 0.91), (101, 77, 12, 0.5), (64, 40, 8, 0.0),
+                                          (50, 50, 50, 0.3), (120, 30, 20, 0.7)])
+def test_mhe_matches_numpy(orc, nx, ny, ws, thr):
+    rng = np.random.default_rng(nx + ny + ws)
+    img = np.clip(rng.normal(40, 30, (ny, nx)), 0, 255).astype(np.uint8)
+    img[rng.random((ny, nx)) < 0.05] = 255
+    got = orc.mhe(img, ws, thr)
+    np.testing.assert_array_equal(got, _mhe_numpy(img, ws, thr))
+
+
+def test_mhe_known_answers(orc):
+    # constant frame over a 2-D tiling: every level below the constant counts
+    # nothing, the constant's cumulative count is exactly n*ws^2 -> 255
+    img = np.full((48, 64), 77, np.uint8)
+    assert (orc.mhe(img, 16, 0.91) == 255).all()
+    # level 0 always maps to 0, level 255 to 255
+    img[:8, :8] = 0
+    img[-8:, -8:] = 255
+    out = orc.mhe(img, 16, 0.91)
+    assert (out[:8, :8] == 0).all() and (out[-8:, -8:] == 255).all()
+    # one tile: its neighbour count is 4 (corner rule) although it has no
+    # neighbours, so with threshold 0 level v of 64 distinct values maps to
+    # floor(v*255/(4*64))
+    img = (np.arange(64).reshape(8, 8) + 1).astype(np.uint8)
+    out = orc.mhe(img, 8, 0.0)
+    np.testing.assert_array_equal(out, (img.astype(int) * 255 // 256).astype(np.uint8))
+    # a single row of three tiles: end tiles count 4 neighbours, the middle one
+    # 6, while two and three tiles contribute -> a constant maps to 127 in all
+    img = np.full((10, 30), 9, np.uint8)
+    assert (orc.mhe(img, 10, 0.0) == 127).all()
+    # sizes not divisible by the window: the central region is tiled, the
+    # margins are 0
+    img = np.full((45, 70), 33, np.uint8)
+    out = orc.mhe(img, 20, 0.5)
+    ys, xs = (45 - 40) // 2, (70 - 60) // 2
+    assert (out[ys:ys + 40, xs:xs + 60] == 255).all()
+    out[ys:ys + 40, xs:xs + 60] = 0
+    assert (out == 0).all()
+    with pytest.raises(ValueError):
+        orc.mhe(img, 0, 0.5)
+    with pytest.raises(ValueError):
+        orc.mhe(img, 10, 1.0)
