@@ -189,6 +189,15 @@ def main():
         fill2_ms, _ = x.stage_ms(2, 3)
         smooth2_ms, _ = x.stage_ms(2, 4)
         x.enable_timing(False)
+        # MHE of both frames (main.cpp:267-283; off in this workload): timed
+        # separately on the same frames with the sample Parameter.dat settings
+        x.set_mhe(True, 40, 0.91)
+        x.enable_timing(True)
+        for _ in range(3):
+            x.prep_img_data(a, b)
+        mhe_ms, n_mhe = x.stage_ms(1, 6)
+        x.enable_timing(False)
+        x.set_mhe(False)
         copy_gbps = x.bandwidth_probe(1 << 30, 10)  # measured streaming-copy peak (read + write)
         st = x.stats()
         fdx, fdy, fv = x.filtered(2)
@@ -258,6 +267,8 @@ def main():
             "grid2_median": round(med2_ms, 4),
             "grid2_fill": round(fill2_ms, 4),
             "grid2_smooth": round(smooth2_ms, 4),
+            "mhe_per_frame": round(mhe_ms, 4),
+            "mhe_GBps": round(3 * 6600 * 4400 / (mhe_ms * 1e-3) / 1e9, 1) if mhe_ms > 0 else None,
         },
         "quality": {"grid2_rms_err_px": round(rms, 4), "overflow_windows_grid2": st["overflow_windows"][1],
                     "outliers_grid2": st["outliers"][1], "fill_sweeps_grid2": st["fill_sweeps"][1]},

@@ -200,10 +200,11 @@ int pivx_get_stats(pivx_ctx* ctx, pivx_stats* st);
 int pivx_set_filtered(pivx_ctx* ctx, int grid, const float* dx, const float* dy, const uint8_t* valid);
 int pivx_set_raw(pivx_ctx* ctx, int grid, const float* dx, const float* dy, const uint8_t* valid);
 
-/* Average device time (ms) of the kernels of the last pass of a stage, measured
- * with HIP events on the handle's stream when timing is enabled.
+/* Average device time (ms) per launch of a stage since timing was enabled,
+ * measured with HIP events on the handle's stream.
  * stage: 0 deformation, 1 correlation, 2 median, 3 fill, 4 smoothing,
- * 5 one whole pair replayed as a graph (grid argument ignored: use 1). */
+ * 5 one whole pair replayed as a graph, 6 MHE of one frame (for 5 and 6 the
+ * grid argument is ignored: use 1). */
 int pivx_enable_timing(pivx_ctx* ctx, int enable);
 int pivx_get_stage_ms(pivx_ctx* ctx, int grid, int stage, double* ms, int* launches);
 

@@ -612,7 +612,10 @@ static int mhe_run(pivx_ctx* c, uint8_t* out) {
   MheArgs m = c->mhe;
   m.in = c->mhe_in;
   m.out = out;
-  return check_launch(c, launch_mhe(m, c->stream), "mhe_gpu::processsing");
+  tic(c);
+  const int rc = check_launch(c, launch_mhe(m, c->stream), "mhe_gpu::processsing");
+  toc(c, 0, 6);
+  return rc;
 }
 
 static int upload(pivx_ctx* c, const uint8_t* a, const uint8_t* b, size_t pitch, hipMemcpyKind kind) {
@@ -996,7 +999,7 @@ int pivx_enable_timing(pivx_ctx* c, int enable) {
 }
 
 int pivx_get_stage_ms(pivx_ctx* c, int grid, int stage, double* ms, int* launches) {
-  if (!c || (grid != 1 && grid != 2) || stage < 0 || stage > 5) return fail(c, PIVX_ERR_PARAM, "bad argument");
+  if (!c || (grid != 1 && grid != 2) || stage < 0 || stage > 6) return fail(c, PIVX_ERR_PARAM, "bad argument");
   HIPCHK(c, hipStreamSynchronize(c->stream));
   double sum = 0.0;
   int n = 0;

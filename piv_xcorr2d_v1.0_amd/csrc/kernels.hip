@@ -1509,78 +1509,167 @@ __global__ void k_zero_u64(unsigned long long* p, size_t n) {
 }
 
 // ============================================================== MHE enhancement
-// MakeHist_kernel (mhe:66-83): 256-bin histogram of every tile, one workgroup
-// per tile with one sub-histogram per wave in LDS, written into the tile's
-// slot of the bordered histogram array.
+// One wave per tile, four tiles per 256-thread workgroup; no workgroup
+// barriers.  Tile rows are read 4 pixels per load when ws % 4 == 0, with up
+// to MHE_U loads of a lane in flight before their use.
+constexpr int MHE_SUB = 4;  // sub-histograms per wave (lanes l, l+4, ... share one)
+constexpr int MHE_U = 8;
+
+__device__ __forceinline__ int mhe_neigh_count(int tx, int ty, int mx, int my) {
+  // Calc_cnt_neigh_kernel (mhe:45-64)
+  if (tx >= 1 && tx <= mx - 2 && ty >= 1 && ty <= my - 2) return 9;
+  if ((tx == 0 && ty == 0) || (tx == 0 && ty == my - 1) || (tx == mx - 1 && ty == 0) || (tx == mx - 1 && ty == my - 1))
+    return 4;
+  return 6;
+}
+
+// MakeHist_kernel (mhe:66-83): the 256-bin histogram of the wave's tile,
+// written into the tile's slot of the bordered histogram array.
 __global__ void __launch_bounds__(256) k_mhe_hist(MheArgs a) {
-  __shared__ uint32_t h[4][256];
-  const int t = threadIdx.x, wv = t >> 6;
+  __shared__ uint32_t hs[4][MHE_SUB][257];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int tile = blockIdx.x * 4 + wv;
+  if (tile >= a.tiles_x * a.tiles_y) return;
+  uint32_t (*h)[257] = hs[wv];
+  const int sub = lane & (MHE_SUB - 1);
 #pragma unroll
-  for (int q = 0; q < 4; ++q) h[q][t] = 0;
-  __syncthreads();
-  const int tile = blockIdx.x;
+  for (int q = 0; q < MHE_SUB; ++q)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) h[q][lane + 64 * j] = 0;
+  wave_sync();
   const int ty = tile / a.tiles_x, tx = tile - ty * a.tiles_x;
   const uint8_t* base = a.in + (long long)(a.ys + ty * a.ws) * a.nx + a.xs + tx * a.ws;
-  const int npix = a.ws * a.ws;
-  for (int p = t; p < npix; p += 256) {
-    const int r = p / a.ws, c = p - r * a.ws;
-    atomicAdd(&h[wv][base[(long long)r * a.nx + c]], 1u);
+  if ((a.ws & 3) == 0) {
+    const int q4 = a.ws >> 2, nd = a.ws * q4;
+    auto word = [&](int d) {
+      const int r = d / q4, c = d - r * q4;
+      uint32_t w;
+      __builtin_memcpy(&w, base + (long long)r * a.nx + 4 * c, 4);
+      return w;
+    };
+    auto count = [&](uint32_t w) {
+      atomicAdd(&h[sub][w & 255u], 1u);
+      atomicAdd(&h[sub][(w >> 8) & 255u], 1u);
+      atomicAdd(&h[sub][(w >> 16) & 255u], 1u);
+      atomicAdd(&h[sub][w >> 24], 1u);
+    };
+    int d0 = lane;
+    for (; d0 + 64 * (MHE_U - 1) < nd; d0 += 64 * MHE_U) {
+      uint32_t w[MHE_U];
+#pragma unroll
+      for (int j = 0; j < MHE_U; ++j) w[j] = word(d0 + 64 * j);
+#pragma unroll
+      for (int j = 0; j < MHE_U; ++j) count(w[j]);
+    }
+    for (; d0 < nd; d0 += 64) count(word(d0));
+  } else {
+    const int npix = a.ws * a.ws;
+    for (int p = lane; p < npix; p += 64) {
+      const int r = p / a.ws, c = p - r * a.ws;
+      atomicAdd(&h[sub][base[(long long)r * a.nx + c]], 1u);
+    }
   }
-  __syncthreads();
-  a.hist[((long long)(ty + 1) * (a.tiles_x + 2) + tx + 1) * 256 + t] = h[0][t] + h[1][t] + h[2][t] + h[3][t];
+  wave_sync();
+  uint4 out;
+  uint32_t* o = reinterpret_cast<uint32_t*>(&out);
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    uint32_t sum = 0;
+#pragma unroll
+    for (int q = 0; q < MHE_SUB; ++q) sum += h[q][4 * lane + j];
+    o[j] = sum;
+  }
+  reinterpret_cast<uint4*>(a.hist + ((long long)(ty + 1) * (a.tiles_x + 2) + tx + 1) * 256)[lane] = out;
 }
 
 // CreateTransfer_kernel + prescan_kernel + Calc_Enhanced_value_kernel +
-// Enhanced_kernel (mhe:85-137, 5-43), one workgroup per tile, thread = grey
-// level p: the 3x3 tile neighbourhood's histogram count at p, its inclusive
-// prefix sum over p, then
+// Enhanced_kernel (mhe:85-137, 5-43), one wave per tile, lane l holding grey
+// levels 4l..4l+3: the 3x3 tile neighbourhood's histogram count at each level,
+// the inclusive prefix sum over the levels, then
 //   v = min(255, max(cum - n*Ntrash, 0) * 255 / (n * Nnontrash)), v(0) = 0, v(255) = 255
-// with n the tile's neighbour count of Calc_cnt_neigh_kernel (mhe:45-64).
-// The workgroup then maps its tile's pixels and zeroes the pixels outside the
-// tiled region that lie beside it (edge tiles own the margins), so every
-// frame pixel is written exactly once.
+// with n the tile's neighbour count.  The wave then maps its tile's pixels
+// and zeroes the pixels outside the tiled region that lie beside it (edge
+// tiles own the margins), so every frame pixel is written exactly once.
 __global__ void __launch_bounds__(256) k_mhe_map(MheArgs a) {
-  __shared__ int wsum[4];
-  __shared__ uint8_t lut[256];
-  const int p = threadIdx.x, lane = p & 63, wv = p >> 6;
-  const int tile = blockIdx.x;
+  __shared__ uint8_t luts[4][256];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int tile = blockIdx.x * 4 + wv;
+  if (tile >= a.tiles_x * a.tiles_y) return;
+  uint8_t* lut = luts[wv];
   const int ty = tile / a.tiles_x, tx = tile - ty * a.tiles_x;
   const int row = a.tiles_x + 2;
-  const uint32_t* hc = a.hist + ((long long)(ty + 1) * row + tx + 1) * 256 + p;
-  int v = 0;
+  const uint4* hc = reinterpret_cast<const uint4*>(a.hist + ((long long)(ty + 1) * row + tx + 1) * 256) + lane;
+  uint4 hv[9];
 #pragma unroll
   for (int dy = -1; dy <= 1; ++dy)
 #pragma unroll
-    for (int dx = -1; dx <= 1; ++dx) v += (int)hc[((long long)dy * row + dx) * 256];
-  // inclusive scan over the 256 levels
+    for (int dx = -1; dx <= 1; ++dx) hv[(dy + 1) * 3 + dx + 1] = hc[((long long)dy * row + dx) * 64];
+  int v[4] = {0, 0, 0, 0};
+#pragma unroll
+  for (int q = 0; q < 9; ++q) {
+    v[0] += (int)hv[q].x;
+    v[1] += (int)hv[q].y;
+    v[2] += (int)hv[q].z;
+    v[3] += (int)hv[q].w;
+  }
+  v[1] += v[0];
+  v[2] += v[1];
+  v[3] += v[2];
+  int run = v[3];
 #pragma unroll
   for (int off = 1; off < 64; off <<= 1) {
-    const int o = __shfl_up(v, off, 64);
-    if (lane >= off) v += o;
+    const int o = __shfl_up(run, off, 64);
+    if (lane >= off) run += o;
   }
-  if (lane == 63) wsum[wv] = v;
-  __syncthreads();
-  for (int q = 0; q < wv; ++q) v += wsum[q];
-  const int mx = a.tiles_x, my = a.tiles_y;
-  int n;
-  if (tx >= 1 && tx <= mx - 2 && ty >= 1 && ty <= my - 2) n = 9;
-  else if ((tx == 0 && ty == 0) || (tx == 0 && ty == my - 1) || (tx == mx - 1 && ty == 0) || (tx == mx - 1 && ty == my - 1))
-    n = 4;
-  else n = 6;
-  long long c = (long long)v - (long long)n * a.ntrash;
-  c = c >= 0 ? c : 0;
-  c = c * 255 / ((long long)n * a.nnontrash);
-  int e = c < 255 ? (int)c : 255;
-  if (p == 0) e = 0;
-  if (p == 255) e = 255;
-  lut[p] = (uint8_t)e;
-  __syncthreads();
+  const int before = run - v[3];
+  const int n = mhe_neigh_count(tx, ty, a.tiles_x, a.tiles_y);
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int p = 4 * lane + j;
+    long long c = (long long)(before + v[j]) - (long long)n * a.ntrash;
+    c = c >= 0 ? c : 0;
+    c = c * 255 / ((long long)n * a.nnontrash);
+    int e = c < 255 ? (int)c : 255;
+    if (p == 0) e = 0;
+    if (p == 255) e = 255;
+    lut[p] = (uint8_t)e;
+  }
+  wave_sync();
   // rectangle owned by this tile: the tile, widened to the frame edge on edge tiles
+  const int mx = a.tiles_x, my = a.tiles_y;
   const int x0t = a.xs + tx * a.ws, y0t = a.ys + ty * a.ws;
   const int xlo = tx == 0 ? 0 : x0t, xhi = tx == mx - 1 ? a.nx : x0t + a.ws;
   const int ylo = ty == 0 ? 0 : y0t, yhi = ty == my - 1 ? a.ny : y0t + a.ws;
-  const int w = xhi - xlo, npix = w * (yhi - ylo);
-  for (int q = p; q < npix; q += 256) {
+  const int w = xhi - xlo;
+  if (w == a.ws && yhi - ylo == a.ws && (a.ws & 3) == 0) {
+    // the rectangle is the tile itself: 4 pixels per load and store
+    const int q4 = a.ws >> 2, nd = a.ws * q4;
+    const long long ob = (long long)y0t * a.nx + x0t;
+    auto word = [&](int d) {
+      const int r = d / q4, c = d - r * q4;
+      uint32_t x;
+      __builtin_memcpy(&x, a.in + ob + (long long)r * a.nx + 4 * c, 4);
+      return x;
+    };
+    auto put = [&](int d, uint32_t x) {
+      const int r = d / q4, c = d - r * q4;
+      const uint32_t m = (uint32_t)lut[x & 255u] | ((uint32_t)lut[(x >> 8) & 255u] << 8) |
+                         ((uint32_t)lut[(x >> 16) & 255u] << 16) | ((uint32_t)lut[x >> 24] << 24);
+      __builtin_memcpy(a.out + ob + (long long)r * a.nx + 4 * c, &m, 4);
+    };
+    int d0 = lane;
+    for (; d0 + 64 * (MHE_U - 1) < nd; d0 += 64 * MHE_U) {
+      uint32_t x[MHE_U];
+#pragma unroll
+      for (int j = 0; j < MHE_U; ++j) x[j] = word(d0 + 64 * j);
+#pragma unroll
+      for (int j = 0; j < MHE_U; ++j) put(d0 + 64 * j, x[j]);
+    }
+    for (; d0 < nd; d0 += 64) put(d0, word(d0));
+    return;
+  }
+  const int npix = w * (yhi - ylo);
+  for (int q = lane; q < npix; q += 64) {
     const int r = q / w, y = ylo + r, x = xlo + (q - r * w);
     const long long o = (long long)y * a.nx + x;
     const bool in_tile = x >= x0t && x < x0t + a.ws && y >= y0t && y < y0t + a.ws;
@@ -1764,10 +1853,11 @@ hipError_t launch_zero_u64(unsigned long long* p, size_t n, hipStream_t s) {
 hipError_t launch_mhe(const MheArgs& a, hipStream_t s) {
   const int tiles = a.tiles_x * a.tiles_y;
   if (tiles < 1) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_mhe_hist, dim3(tiles), dim3(256), 0, s, a);
+  const dim3 g((tiles + 3) / 4), b(256);
+  hipLaunchKernelGGL(k_mhe_hist, g, b, 0, s, a);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_mhe_map, dim3(tiles), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(k_mhe_map, g, b, 0, s, a);
   return hipGetLastError();
 }
 

@@ -361,11 +361,12 @@ def test_graph_replays_match_eager(pivx):
 
 # ---------------------------------------------------------------- MHE enhancement
 @pytest.mark.parametrize("nx,ny,ws,thr", [(6600, 4400, 40, 0.91), (1000, 700, 48, 0.5), (257, 129, 16, 0.0),
-                                          (300, 200, 200, 0.3), (640, 448, 40, 0.91)])
+                                          (300, 200, 200, 0.3), (640, 448, 40, 0.91), (333, 222, 10, 0.6),
+                                          (128, 96, 7, 0.2), (97, 61, 4, 0.91)])
 def test_mhe_bitexact(pivx, orc, nx, ny, ws, thr):
     """mhe_gpu::processsing on the GPU against the oracle's restatement:
-    integer work, bit-exact; covers divisible and non-divisible sizes and a
-    single-tile row."""
+    integer work, bit-exact; covers divisible and non-divisible sizes, a
+    single-tile row, and windows not divisible by 4 (two-kernel path)."""
     sp_ = pivx.synth_params(nx, ny, seed=nx + ws, flow=0, a0=1.0, a1=0.5)
     a, _ = pivx.synth_pair(sp_)
     p = config1_params(pivx, nx=nx, ny=ny, ws_x1=32, ws_y1=32, spacing_x1=32, spacing_y1=32, npass2=0)
