@@ -100,6 +100,54 @@ def cpu_baseline(pivx, ip, fp, a, b, threads, budget_s):
                 seconds_per_run=round(t_mt, 3), hardware_concurrency=os.cpu_count(), cpu_model=_cpu_model())
 
 
+def cli_leg(pivx, npairs, timeout_s=300):
+    """End-to-end drop-in throughput: npairs 6600x4400 TIFF pairs (one
+    synthetic config-2 pair written under every pair index) through the
+    piv_xcorr2d command line
+    with a config-2 Parameter.dat (MHE off), once pipelined (default) and once
+    with --serial (read, upload, compute, write one after the other, as the
+    reference's main loop does).  Frames are written first, so reads come
+    from the page cache.  Pairs/s from the CLI's own summary line (first pair
+    start to last file written; process start-up excluded)."""
+    import re
+    import shutil
+    import subprocess
+    import tempfile
+
+    d = tempfile.mkdtemp(prefix="pivx_cli_")
+    try:
+        sp = pivx.synth_params(6600, 4400, seed=2025, density=0.05, flow=1, a0=300.0, a1=8.0, a2=0.002)
+        a, b = pivx.synth_pair(sp)
+        for k in range(1, npairs + 1):  # the same frames under every pair index
+            pivx.tiff_write8(os.path.join(d, f"1_{k:04d}.tif"), a)
+            pivx.tiff_write8(os.path.join(d, f"2_{k:04d}.tif"), b)
+        with open(os.path.join(d, "pair index.txt"), "w") as f:
+            f.write("\n".join(str(k) for k in range(1, npairs + 1)))
+        with open(os.path.join(REPO, "tests", "golden", "Parameter.dat"), "rb") as f:
+            lines = f.read().decode().split("\r\n")
+        lines[0:6] = [os.path.join(d, "1_"), ".tif", os.path.join(d, "2_"), ".tif", os.path.join(d, "pair index.txt"),
+                      os.path.join(d, "result") + "/"]
+        lines[7] = "0"                               # MHE off
+        lines[14], lines[15], lines[16] = "64", "64", "3"   # grid 1 spacing 64, 3 passes
+        lines[21] = "3"                              # grid 2: 3 passes
+        lines[28], lines[37] = "2", "2"              # insert threshold 2
+        par = os.path.join(d, "Parameter.dat")
+        with open(par, "wb") as f:
+            f.write("\r\n".join(lines).encode())
+        out = {"pairs": npairs}
+        for key, extra in (("pipelined", []), ("serial", ["--serial"])):
+            r = subprocess.run([pivx.CLI_PATH, par] + extra, capture_output=True, text=True, timeout=timeout_s)
+            m = re.search(r"Total: (\d+) pairs in ([0-9.eE+-]+) seconds", r.stdout)
+            if r.returncode != 0 or not m:
+                out[key] = {"error": (r.stderr or r.stdout)[-300:]}
+                continue
+            out[key] = {"seconds": round(float(m.group(2)), 3), "pairs_per_s": round(npairs / float(m.group(2)), 2)}
+            print(f"[bench] cli {key}: {npairs} pairs in {m.group(2)} s", file=sys.stderr, flush=True)
+        return out
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+
+
 def _windows(pivx, ip):
     g1 = _geom(ip.nx, ip.ny, ip.spacing_x1, ip.spacing_y1)
     g2 = _geom(ip.nx, ip.ny, ip.spacing_x2, ip.spacing_y2)
@@ -136,6 +184,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--eager", action="store_true", help="launch kernel by kernel instead of replaying a hipGraph")
     ap.add_argument("--cpu-budget", type=float, default=25.0, help="seconds of CPU baseline work")
+    ap.add_argument("--cli-pairs", type=int, default=16, help="pairs for the end-to-end CLI leg (0 = skip)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one GPU only (the PIV path runs one process on one MI355X); "
@@ -273,6 +322,8 @@ def main():
         "quality": {"grid2_rms_err_px": round(rms, 4), "overflow_windows_grid2": st["overflow_windows"][1],
                     "outliers_grid2": st["outliers"][1], "fill_sweeps_grid2": st["fill_sweeps"][1]},
     }
+    if args.cli_pairs > 0:
+        out["end_to_end_cli"] = cli_leg(pivx, args.cli_pairs)
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         cb = cpu_baseline(pivx, ip, fp, a, b, threads, args.cpu_budget)

@@ -125,6 +125,37 @@ int pivx_set_mhe(pivx_ctx* ctx, int enable, int ws, float thresh);
  * does.  Requires a prior pivx_set_mhe(ctx, 1, ...). */
 int pivx_mhe_enhance(pivx_ctx* ctx, const uint8_t* img, uint8_t* out, size_t pitch);
 
+/* ------------------------------------------------------------------ pipelined host I/O
+ * No reference counterpart: the reference serialises disk reads, row-by-row
+ * uploads, compute and file formatting per pair (main.cpp:227-376,
+ * cu:1271-1282, 1865-1973).  These calls let a driver read the next pair and
+ * write the previous pair's files while the GPU works. */
+
+/* Page-locked host memory (so that uploads and result copies overlap host
+ * work); NULL on failure. */
+void* pivx_host_alloc(size_t bytes);
+void pivx_host_free(void* p);
+
+/* pivx_prep_img_data without the final wait: the uploads (and the MHE, if
+ * enabled) are enqueued on the handle's stream.  The frames must stay
+ * unchanged until the stream has passed them (pivx_synchronize, or
+ * pivx_results_wait on a later fetch). */
+int pivx_prep_img_data_async(pivx_ctx* ctx, const uint8_t* img1, const uint8_t* img2, size_t pitch);
+
+/* Host snapshot of the fields write_vec_data_general reads (raw and filtered
+ * fields of both grids and the grid-2 median-only field), in page-locked
+ * memory.  fetch enqueues the copies on the handle's stream and returns;
+ * wait blocks until they have landed; write formats one of the five files
+ * (selection as in pivx_write_vec_data_general) from the snapshot only, so it
+ * may run on any host thread while the handle is used for the next pair.  A
+ * snapshot belongs to the handle that created it. */
+typedef struct pivx_results pivx_results;
+int pivx_results_create(pivx_ctx* ctx, pivx_results** out);
+int pivx_results_fetch(pivx_ctx* ctx, pivx_results* r);
+int pivx_results_wait(pivx_results* r);
+int pivx_results_write(const pivx_results* r, const char* path, int selection, int crlf);
+void pivx_results_destroy(pivx_results* r);
+
 /* ------------------------------------------------------------------ passes */
 
 /* xcorr2d_multi::Calc_cc_map_p1 (cu:1284-1449): one grid-1 correlation pass.

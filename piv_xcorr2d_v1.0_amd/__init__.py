@@ -93,6 +93,14 @@ _SIGS = {
     "pivx_prep_img_data": [_P, _P, _P, C.c_size_t],
     "pivx_prep_img_data_device": [_P, _P, _P, C.c_size_t],
     "pivx_set_mhe": [_P, C.c_int, C.c_int, C.c_float],
+    "pivx_prep_img_data_async": [_P, _P, _P, C.c_size_t],
+    "pivx_host_alloc": [C.c_size_t],
+    "pivx_host_free": [_P],
+    "pivx_results_create": [_P, C.POINTER(_P)],
+    "pivx_results_fetch": [_P, _P],
+    "pivx_results_wait": [_P],
+    "pivx_results_write": [_P, C.c_char_p, C.c_int, C.c_int],
+    "pivx_results_destroy": [_P],
     "pivx_mhe_enhance": [_P, _P, _P, C.c_size_t],
     "pivx_calc_cc_map_p1": [_P, C.c_int],
     "pivx_median_filter_smoothing_p1": [_P, C.c_int, C.c_int, C.c_float, C.c_float, C.c_int],
@@ -127,7 +135,8 @@ _SIGS = {
     "pivx_synth_pair": [C.POINTER(SynthParams), _P, _P],
     "pivx_synth_flow": [C.POINTER(SynthParams), _P, _P, C.c_int, _P, _P],
 }
-_RESTYPE = {"pivx_last_error": C.c_char_p}
+_RESTYPE = {"pivx_last_error": C.c_char_p, "pivx_host_alloc": C.c_void_p, "pivx_host_free": None,
+            "pivx_results_destroy": None}
 
 
 def lib():
@@ -225,6 +234,20 @@ class Xcorr2dMulti:
 
     def prep_img_data_device(self, dptr1: int, dptr2: int, pitch: int):
         self._c(lib().pivx_prep_img_data_device(self.h, C.c_void_p(dptr1), C.c_void_p(dptr2), pitch))
+
+    def prep_img_data_async(self, img1: np.ndarray, img2: np.ndarray):
+        """Enqueue the uploads without waiting; keep img1/img2 alive and
+        unchanged until synchronize() (pinned buffers overlap best)."""
+        a = np.ascontiguousarray(img1, dtype=np.uint8)
+        b = np.ascontiguousarray(img2, dtype=np.uint8)
+        if a.shape != (self.p.ny, self.p.nx) or b.shape != a.shape:
+            raise ValueError(f"frames must be ({self.p.ny}, {self.p.nx}) uint8")
+        self._c(lib().pivx_prep_img_data_async(self.h, _ptr(a), _ptr(b), self.p.nx))
+        self._keep = (a, b)
+
+    def results(self):
+        """A host result snapshot bound to this handle (see Results)."""
+        return Results(self)
 
     # ------------------------------------------------------------ MHE (mhe_gpu, MHE7_4_gpu.cu:149-253)
     def set_mhe(self, enable: bool, ws: int = 40, thresh: float = 0.91):
@@ -422,3 +445,37 @@ def synth_flow(sp: SynthParams, x, y):
     v = np.empty_like(x)
     _check(lib().pivx_synth_flow(C.byref(sp), _ptr(x), _ptr(y), x.size, _ptr(u), _ptr(v)))
     return u, v
+
+
+class Results:
+    """Page-locked snapshot of the fields the five output files need:
+    fetch() enqueues the copies, wait() blocks until they land, write()
+    formats one file from the snapshot (selection 0..4 as
+    write_vec_data_general) without touching the GPU."""
+
+    def __init__(self, x: "Xcorr2dMulti"):
+        self.x = x
+        r = _P()
+        x._c(lib().pivx_results_create(x.h, C.byref(r)))
+        self.r = r
+
+    def fetch(self):
+        self.x._c(lib().pivx_results_fetch(self.x.h, self.r))
+
+    def wait(self):
+        _check(lib().pivx_results_wait(self.r), None)
+
+    def write(self, path: str, selection: int, crlf: bool = False):
+        _check(lib().pivx_results_write(self.r, path.encode(), selection, 1 if crlf else 0), None)
+
+    def close(self):
+        if self.r:
+            lib().pivx_results_destroy(self.r)
+            self.r = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+

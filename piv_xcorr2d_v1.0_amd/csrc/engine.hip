@@ -618,7 +618,8 @@ static int mhe_run(pivx_ctx* c, uint8_t* out) {
   return rc;
 }
 
-static int upload(pivx_ctx* c, const uint8_t* a, const uint8_t* b, size_t pitch, hipMemcpyKind kind) {
+static int upload(pivx_ctx* c, const uint8_t* a, const uint8_t* b, size_t pitch, hipMemcpyKind kind,
+                  bool wait = true) {
   if (!c || !a || !b) return fail(c, PIVX_ERR_PARAM, "null frame");
   if (pitch < (size_t)c->p.nx) return fail(c, PIVX_ERR_PARAM, "pitch %zu < nx %d", pitch, c->p.nx);
   HIPCHK(c, hipSetDevice(c->device));
@@ -633,7 +634,7 @@ static int upload(pivx_ctx* c, const uint8_t* a, const uint8_t* b, size_t pitch,
     HIPCHK(c, hipMemcpy2DAsync(c->f1, c->p.nx, a, pitch, c->p.nx, c->p.ny, kind, c->stream));
     HIPCHK(c, hipMemcpy2DAsync(c->f2, c->p.nx, b, pitch, c->p.nx, c->p.ny, kind, c->stream));
   }
-  if (kind == hipMemcpyHostToDevice) HIPCHK(c, hipStreamSynchronize(c->stream));  // caller may reuse its buffers
+  if (kind == hipMemcpyHostToDevice && wait) HIPCHK(c, hipStreamSynchronize(c->stream));  // caller may reuse its buffers
   c->have_frames = true;
   return PIVX_OK;
 }
@@ -644,6 +645,20 @@ int pivx_prep_img_data(pivx_ctx* c, const uint8_t* img1, const uint8_t* img2, si
 
 int pivx_prep_img_data_device(pivx_ctx* c, const uint8_t* d1, const uint8_t* d2, size_t pitch) {
   return upload(c, d1, d2, pitch, hipMemcpyDeviceToDevice);
+}
+
+int pivx_prep_img_data_async(pivx_ctx* c, const uint8_t* img1, const uint8_t* img2, size_t pitch) {
+  return upload(c, img1, img2, pitch, hipMemcpyHostToDevice, false);
+}
+
+void* pivx_host_alloc(size_t bytes) {
+  void* p = nullptr;
+  if (hipHostMalloc(&p, bytes ? bytes : 1, hipHostMallocDefault) != hipSuccess) return nullptr;
+  return p;
+}
+
+void pivx_host_free(void* p) {
+  if (p) (void)hipHostFree(p);
 }
 
 // mhe_gpu::init (mhe:149-212) for the handle's frame size: tile grid, central
@@ -1045,6 +1060,118 @@ int pivx_bandwidth_probe(pivx_ctx* c, size_t bytes, int reps, double* gbps) {
   if (e != hipSuccess) return fail(c, PIVX_ERR_HIP, "bandwidth probe: %s", hipGetErrorString(e));
   *gbps = 2.0 * (double)bytes * reps / (ms * 1e-3) / 1e9;
   return PIVX_OK;
+}
+
+// Host snapshot of everything write_vec_data_general reads (cu:1865-1973),
+// in page-locked memory so that the fetch is one asynchronous copy train on
+// the handle's stream, and the files can be written by host threads while
+// the GPU runs the next pair.
+struct pivx_results {
+  int device = 0;
+  int K = 0;
+  pivx_geometry g[2]{};
+  std::vector<float> gx[2], gy[2];     // grid coordinates (fixed per handle)
+  float* raw_x[2] = {nullptr, nullptr};  // [nt][K]
+  float* raw_y[2] = {nullptr, nullptr};
+  float* fil_x[3] = {nullptr, nullptr, nullptr};  // grid 1, grid 2, grid-2 median-only
+  float* fil_y[3] = {nullptr, nullptr, nullptr};
+  hipEvent_t done = nullptr;
+  bool pending = false;
+};
+
+int pivx_results_create(pivx_ctx* c, pivx_results** out) {
+  if (!c || !out) return fail(c, PIVX_ERR_PARAM, "null argument");
+  HIPCHK(c, hipSetDevice(c->device));
+  pivx_results* r = new pivx_results();
+  r->device = c->device;
+  r->K = c->p.npeak_save;
+  bool ok = true;
+  for (int gi = 0; gi < 2; ++gi) {
+    r->g[gi] = c->grid[gi].g;
+    const size_t nt = (size_t)r->g[gi].nt_vec;
+    r->gx[gi].resize(nt);
+    r->gy[gi].resize(nt);
+    pivx_get_grid_coords(c, gi + 1, r->gx[gi].data(), r->gy[gi].data());
+    ok = ok && hipHostMalloc((void**)&r->raw_x[gi], nt * r->K * 4 + 4, hipHostMallocDefault) == hipSuccess;
+    ok = ok && hipHostMalloc((void**)&r->raw_y[gi], nt * r->K * 4 + 4, hipHostMallocDefault) == hipSuccess;
+    ok = ok && hipHostMalloc((void**)&r->fil_x[gi], nt * 4 + 4, hipHostMallocDefault) == hipSuccess;
+    ok = ok && hipHostMalloc((void**)&r->fil_y[gi], nt * 4 + 4, hipHostMallocDefault) == hipSuccess;
+  }
+  const size_t nt2 = (size_t)r->g[1].nt_vec;
+  ok = ok && hipHostMalloc((void**)&r->fil_x[2], nt2 * 4 + 4, hipHostMallocDefault) == hipSuccess;
+  ok = ok && hipHostMalloc((void**)&r->fil_y[2], nt2 * 4 + 4, hipHostMallocDefault) == hipSuccess;
+  ok = ok && hipEventCreateWithFlags(&r->done, hipEventDisableTiming) == hipSuccess;
+  if (!ok) {
+    pivx_results_destroy(r);
+    return fail(c, PIVX_ERR_HIP, "page-locked result buffers could not be allocated");
+  }
+  *out = r;
+  return PIVX_OK;
+}
+
+int pivx_results_fetch(pivx_ctx* c, pivx_results* r) {
+  if (!c || !r) return fail(c, PIVX_ERR_PARAM, "null argument");
+  if (r->device != c->device || r->K != c->p.npeak_save || r->g[0].nt_vec != c->grid[0].g.nt_vec ||
+      r->g[1].nt_vec != c->grid[1].g.nt_vec)
+    return fail(c, PIVX_ERR_PARAM, "result snapshot was created for another handle");
+  HIPCHK(c, hipSetDevice(c->device));
+  int rc = 0;
+  for (int gi = 0; gi < 2; ++gi) {
+    const GridBuf& G = c->grid[gi];
+    const size_t nt = (size_t)G.g.nt_vec;
+    rc |= d2h(c, r->raw_x[gi], G.dx, nt * r->K * 4);
+    rc |= d2h(c, r->raw_y[gi], G.dy, nt * r->K * 4);
+    rc |= d2h(c, r->fil_x[gi], G.fx, nt * 4);
+    rc |= d2h(c, r->fil_y[gi], G.fy, nt * 4);
+  }
+  const size_t nt2 = (size_t)c->grid[1].g.nt_vec;
+  rc |= d2h(c, r->fil_x[2], c->mo_x, nt2 * 4);
+  rc |= d2h(c, r->fil_y[2], c->mo_y, nt2 * 4);
+  if (rc) return rc;
+  HIPCHK(c, hipEventRecord(r->done, c->stream));
+  r->pending = true;
+  return PIVX_OK;
+}
+
+int pivx_results_wait(pivx_results* r) {
+  if (!r) return fail(nullptr, PIVX_ERR_PARAM, "null argument");
+  if (r->pending) {
+    const hipError_t e = hipEventSynchronize(r->done);
+    if (e != hipSuccess) return fail(nullptr, PIVX_ERR_HIP, "hipEventSynchronize: %s", hipGetErrorString(e));
+    r->pending = false;
+  }
+  return PIVX_OK;
+}
+
+int pivx_results_write(const pivx_results* r, const char* path, int selection, int crlf) {
+  if (!r || !path) return fail(nullptr, PIVX_ERR_PARAM, "null argument");
+  if (r->pending) return fail(nullptr, PIVX_ERR_STATE, "results not ready: call pivx_results_wait first");
+  const int gi = (selection == 0 || selection == 1) ? 0 : 1;
+  const pivx_geometry& g = r->g[gi];
+  int rc;
+  if (selection == 0 || selection == 2) {
+    rc = write_tecplot(path, g.nx_vec, g.ny_vec, r->gx[gi].data(), r->gy[gi].data(), r->raw_x[gi], r->raw_y[gi], r->K,
+                       true, crlf != 0);
+  } else if (selection == 1 || selection == 3 || selection == 4) {
+    const int f = selection == 4 ? 2 : gi;
+    rc = write_tecplot(path, g.nx_vec, g.ny_vec, r->gx[gi].data(), r->gy[gi].data(), r->fil_x[f], r->fil_y[f], 1,
+                       false, crlf != 0);
+  } else {
+    return fail(nullptr, PIVX_ERR_PARAM, "selection must be 0..4");
+  }
+  if (rc) return fail(nullptr, PIVX_ERR_IO, "Error occurs while the vector file is being written: %s", path);
+  return PIVX_OK;
+}
+
+void pivx_results_destroy(pivx_results* r) {
+  if (!r) return;
+  (void)hipSetDevice(r->device);
+  if (r->pending) (void)hipEventSynchronize(r->done);
+  for (float* p : {r->raw_x[0], r->raw_x[1], r->raw_y[0], r->raw_y[1], r->fil_x[0], r->fil_x[1], r->fil_x[2],
+                   r->fil_y[0], r->fil_y[1], r->fil_y[2]})
+    if (p) (void)hipHostFree(p);
+  if (r->done) (void)hipEventDestroy(r->done);
+  delete r;
 }
 
 int pivx_write_vec_data_general(pivx_ctx* c, const char* path, int selection, int crlf) {

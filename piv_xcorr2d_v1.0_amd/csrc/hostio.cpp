@@ -11,6 +11,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 namespace {
@@ -236,38 +237,87 @@ namespace pivx {
 void set_global_error(const char* msg) { g_io_error = msg; }
 const char* global_error() { return g_io_error.c_str(); }
 
+// "%.2f" of a float, byte for byte as printf / iostream fixed+setprecision(2)
+// print it: rint(x * 100) in double is the correctly rounded (ties to even)
+// value of x * 100 for every float x.  For x = k * 2^e (k < 2^24) the product
+// is exact when e >= -1; otherwise x * 100 lies at least 2^e away from any
+// tie while the double product's error is below 100 * k * 2^(e - 53) < 2^e.
+// The sign is printed for every negative x, so values rounding to zero print
+// as "-0.00" like printf.  Non-finite and huge values go through snprintf.
+static inline char* fmt_fixed2(char* p, float xf) {
+  const double x = xf;
+  if (!(std::fabs(x) < 1e15)) return p + snprintf(p, 64, "%.2f", x);
+  uint64_t v = (uint64_t)std::fabs(std::rint(x * 100.0));
+  if (std::signbit(x)) *p++ = '-';
+  uint64_t ip = v / 100;
+  const int fr = (int)(v % 100);
+  char tmp[24];
+  int n = 0;
+  do {
+    tmp[n++] = (char)('0' + ip % 10);
+    ip /= 10;
+  } while (ip);
+  while (n) *p++ = tmp[--n];
+  *p++ = '.';
+  *p++ = (char)('0' + fr / 10);
+  *p++ = (char)('0' + fr % 10);
+  return p;
+}
+
+// write_vec_data_general's text body (cu:1865-1973): rows are formatted in
+// parallel chunks (one per host thread) and written in order.
 int write_tecplot(const char* path, int nx_vec, int ny_vec, const float* gx, const float* gy, const float* dx,
                   const float* dy, int K, bool raw, bool crlf) {
   FILE* f = fopen(path, "wb");
   if (!f) return io_fail(PIVX_ERR_IO, std::string("cannot open ") + path);
   const char* nl = crlf ? "\r\n" : "\n";
-  std::string buf;
-  buf.reserve((size_t)nx_vec * ny_vec * (raw ? 72 : 32) + 256);
+  const size_t nll = crlf ? 2 : 1;
+  std::string head;
   char line[512];
-  buf += "TITLE = \"B00001\"";
-  buf += nl;
-  if (raw) buf += "VARIABLES = \"x\", \"y\", \"dx1\", \"dy1\", \"dx2\", \"dy2\", \"dx3\", \"dy3\", \"dx4\", \"dy4\"";
-  else buf += "VARIABLES = \"x\", \"y\", \"displacement-x\", \"displacement-y\"";
-  buf += nl;
+  head += "TITLE = \"B00001\"";
+  head += nl;
+  if (raw) head += "VARIABLES = \"x\", \"y\", \"dx1\", \"dy1\", \"dx2\", \"dy2\", \"dx3\", \"dy3\", \"dx4\", \"dy4\"";
+  else head += "VARIABLES = \"x\", \"y\", \"displacement-x\", \"displacement-y\"";
+  head += nl;
   snprintf(line, sizeof(line), "ZONE T=\"Frame 0\", I=%d, J=%d%s", nx_vec, ny_vec, nl);
-  buf += line;
+  head += line;
   const long long nt = (long long)nx_vec * ny_vec;
-  for (long long i = 0; i < nt; ++i) {
-    if (raw) {
-      double c[8];
-      for (int k = 0; k < 4; ++k) {
-        c[2 * k] = k < K ? (double)dx[i * K + k] : 0.0;
-        c[2 * k + 1] = k < K ? (double)dy[i * K + k] : 0.0;
+  const int ncol = raw ? 10 : 4;
+  auto format_rows = [&](long long r0, long long r1, std::string& out) {
+    out.resize((size_t)(r1 - r0) * (ncol * 24 + 4));
+    char* p = &out[0];
+    for (long long i = r0; i < r1; ++i) {
+      p = fmt_fixed2(p, gx[i]);
+      *p++ = '\t';
+      p = fmt_fixed2(p, gy[i]);
+      if (raw) {
+        for (int k = 0; k < 4; ++k) {
+          *p++ = '\t';
+          p = fmt_fixed2(p, k < K ? dx[i * K + k] : 0.f);
+          *p++ = '\t';
+          p = fmt_fixed2(p, k < K ? dy[i * K + k] : 0.f);
+        }
+      } else {
+        *p++ = '\t';
+        p = fmt_fixed2(p, dx[i]);
+        *p++ = '\t';
+        p = fmt_fixed2(p, dy[i]);
       }
-      snprintf(line, sizeof(line), "%.2f\t%.2f\t%.2f\t%.2f\t%.2f\t%.2f\t%.2f\t%.2f\t%.2f\t%.2f%s", (double)gx[i],
-               (double)gy[i], c[0], c[1], c[2], c[3], c[4], c[5], c[6], c[7], nl);
-    } else {
-      snprintf(line, sizeof(line), "%.2f\t%.2f\t%.2f\t%.2f%s", (double)gx[i], (double)gy[i], (double)dx[i],
-               (double)dy[i], nl);
+      memcpy(p, nl, nll);
+      p += nll;
     }
-    buf += line;
-  }
-  bool ok = fwrite(buf.data(), 1, buf.size(), f) == buf.size();
+    out.resize((size_t)(p - out.data()));
+  };
+  const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  const int nchunk = (int)std::max<long long>(1, std::min<long long>(hw, nt / 4096));
+  std::vector<std::string> parts(nchunk);
+  std::vector<std::thread> workers;
+  for (int c = 1; c < nchunk; ++c)
+    workers.emplace_back(format_rows, nt * c / nchunk, nt * (c + 1) / nchunk, std::ref(parts[c]));
+  format_rows(0, nt / nchunk, parts[0]);
+  for (auto& w : workers) w.join();
+  bool ok = fwrite(head.data(), 1, head.size(), f) == head.size();
+  for (const auto& part : parts) ok = ok && fwrite(part.data(), 1, part.size(), f) == part.size();
   ok = (fclose(f) == 0) && ok;
   return ok ? PIVX_OK : io_fail(PIVX_ERR_IO, std::string("write failed: ") + path);
 }

@@ -139,3 +139,29 @@ def test_cli_mhe_matches_enhanced_frames(pivx, orc, tmp_path):
         f2 = x.filtered(2)
     head, d = _read_tecplot(pivx.output_name(pf, 5, 3))
     assert np.abs(d[:, 2] - f2[0]).max() <= 0.005 + 1e-6 and np.abs(d[:, 3] - f2[1]).max() <= 0.005 + 1e-6
+
+
+def test_cli_pipelined_equals_serial(pivx, tmp_path):
+    """The pipelined driver (next pair read and previous pair written on host
+    threads while the GPU runs) writes the same bytes as the serial order."""
+    nx, ny = 512, 384
+    pairs = [2, 3, 4]
+    for k in pairs:
+        sp = pivx.synth_params(nx, ny, seed=300 + k, flow=1, a0=80.0, a1=4.0, a2=0.002)
+        a, b = pivx.synth_pair(sp)
+        pivx.tiff_write8(str(tmp_path / f"1_{k:04d}.tif"), a)
+        pivx.tiff_write8(str(tmp_path / f"2_{k:04d}.tif"), b)
+    (tmp_path / "pair index.txt").write_text("\n".join(str(k) for k in pairs))
+    outs = {}
+    for mode in ("pipelined", "serial"):
+        lines = _param_lines(tmp_path, nx, ny)
+        lines[5] = str(tmp_path / f"res_{mode}") + "/"
+        par = tmp_path / f"P_{mode}.dat"
+        _write_params(par, lines)
+        r = subprocess.run([pivx.CLI_PATH, str(par)] + (["--serial"] if mode == "serial" else []),
+                           capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        assert "Total: 3 pairs" in r.stdout
+        pf = pivx.parse_parameter_file(str(par))
+        outs[mode] = [open(pivx.output_name(pf, k, w), "rb").read() for k in pairs for w in range(5)]
+    assert outs["pipelined"] == outs["serial"]

@@ -410,3 +410,31 @@ def test_mhe_rejects_untileable_settings(pivx):
                 x.set_mhe(True, ws, thr)
         with pytest.raises(pivx.PivError):
             x.mhe_enhance(np.zeros((128, 256), np.uint8))
+
+
+def test_results_snapshot_survives_next_pair(pivx, tmp_path):
+    """A result snapshot fetched after pair 1 still holds pair 1 while pair 2
+    runs, and its files equal write_vec_data_general's bytes."""
+    nx, ny = 512, 384
+    p = config1_params(pivx, nx=nx, ny=ny, ws_x1=64, ws_y1=64, spacing_x1=32, spacing_y1=32, npass1=2,
+                       ws_x2=32, ws_y2=32, spacing_x2=16, spacing_y2=16, npass2=2)
+    f = pivx.FilterParams(2, 5, 2.0, 2.0, 1)
+    a1, b1 = pivx.synth_pair(pivx.synth_params(nx, ny, seed=8, flow=1, a0=80.0, a1=4.0, a2=0.002))
+    a2, b2 = pivx.synth_pair(pivx.synth_params(nx, ny, seed=9, flow=0, a0=2.0, a1=-1.0))
+    with pivx.Xcorr2dMulti(p) as x, x.results() as r:
+        x.prep_img_data(a1, b1)
+        x.run_pair(f, f)
+        direct = []
+        for sel in range(5):
+            path = str(tmp_path / f"d{sel}.dat")
+            x.write_vec_data_general(path, sel, crlf=True)
+            direct.append(open(path, "rb").read())
+        r.fetch()
+        x.prep_img_data_async(a2, b2)
+        x.run_pair(f, f)
+        r.wait()
+        for sel in range(5):
+            path = str(tmp_path / f"s{sel}.dat")
+            r.write(path, sel, crlf=True)
+            assert open(path, "rb").read() == direct[sel]
+        x.synchronize()

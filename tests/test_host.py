@@ -290,3 +290,26 @@ def test_synthetic_flows(pivx):
     X, Y = np.meshgrid(g, g)
     u, v = pivx.synth_flow(sp2, X, Y)
     assert abs(np.sqrt(u ** 2 + v ** 2).max() - 12.0) < 1e-3
+
+
+def test_tecplot_numbers_match_printf(pivx, tmp_path):
+    """The writer's number formatting equals C's "%.2f" (iostream fixed +
+    setprecision(2), cu:1879-1880) byte for byte: ties to even on the exact
+    binary value, "-0.00" for small negatives, wide values."""
+    rng = np.random.default_rng(5)
+    nx, ny = 97, 41
+    nt = nx * ny
+    v = rng.normal(0, 30, nt * 8).astype(np.float32)
+    edge = np.array([0.125, 0.375, -0.125, 0.005, -0.005, -0.004, 0.0, -0.0, 2.675, 1.005, 12345.675, -12345.675,
+                     0.045, 0.015, 1e6 + 0.125, -1e-30, 99.995, 0.5, -0.5, 3e9, 1e20, -2.5e-3], np.float32)
+    v[:edge.size] = edge
+    dx, dy = v[:nt * 4].reshape(nt, 4), v[nt * 4:].reshape(nt, 4)
+    gx = (rng.random(nt) * 6600).astype(np.float32)
+    gy = (rng.random(nt) * 4400).astype(np.float32)
+    p = str(tmp_path / "t.dat")
+    pivx.write_tecplot(p, nx, ny, gx, gy, dx, dy, raw=True, crlf=False)
+    rows = open(p, "rb").read().split(b"\n")[3:-1]
+    assert len(rows) == nt
+    for i in range(nt):
+        exp = "\t".join("%.2f" % float(x) for x in [gx[i], gy[i]] + [c for k in range(4) for c in (dx[i, k], dy[i, k])])
+        assert rows[i] == exp.encode(), (i, rows[i], exp)
