@@ -53,6 +53,14 @@ __device__ __forceinline__ void wave_argmax(float& bv, int& bi) {
   }
 }
 
+// |v| * 0.25 / N (the packed product carries a factor 4): one multiply when N
+// is a power of two (both forms are exact scalings), else the two-step form.
+template <int N>
+__device__ __forceinline__ float cc_scale(float v) {
+  if constexpr ((N & (N - 1)) == 0) return fabsf(v) * (0.25f / (float)N);
+  else return (fabsf(v) * 0.25f) / (float)N;
+}
+
 // Workgroups are dispatched round-robin over the 8 XCDs, each with its own
 // L2.  Renumber them so that every XCD walks one contiguous range of windows
 // (neighbouring windows share most of their pixels).
@@ -237,8 +245,8 @@ __global__ void __launch_bounds__((FastCfg<WX, WY, WPB>::NT)) k_xcorr_fast(XcArg
       constexpr int c = cc;
       constexpr int cs = (c + WX / 2) % WX;
       // |IFFT(4 conj(F1) F2)| * 0.25 / N == |IFFT(conj(F1) F2)| / N  (cu:542)
-      cca[rs * SX + cs] = (fabsf(x[c].x) * 0.25f) / (float)NPIX;
-      ccb[rs * SX + cs] = (fabsf(x[c].y) * 0.25f) / (float)NPIX;
+      cca[rs * SX + cs] = cc_scale<NPIX>(x[c].x);
+      ccb[rs * SX + cs] = cc_scale<NPIX>(x[c].y);
     });
   }
   group_sync<NT>();
@@ -372,9 +380,42 @@ __device__ __forceinline__ void load_u8_row(const uint8_t* p, float (&f)[WX]) {
   });
 }
 
+// Pixels s..s+3 of an image row (nx >= 4 pixels) as a packed u8 word, 0 for
+// pixels outside [0, nx): one 4-byte load at the clamped start, shifted into
+// place and masked.
+__device__ __forceinline__ uint32_t load_u8x4_edge(const uint8_t* row, int s, int nx) {
+  const int cs = min(max(s, 0), nx - 4);
+  uint32_t w;
+  __builtin_memcpy(&w, row + cs, 4);
+  const int sh = s - cs;  // pixel s + b is byte b + sh of w
+  const uint64_t wide = (uint64_t)w << 32;
+  const int rs = min(max(32 + 8 * sh, 0), 64);
+  const uint32_t moved = rs >= 64 ? 0u : (uint32_t)(wide >> rs);
+  const int lo = min(max(-s, 0), 4), hi = min(max(nx - s, 0), 4);  // inside bytes [lo, hi)
+  const uint64_t m = ((1ull << (8 * hi)) - 1ull) & ~((1ull << (8 * lo)) - 1ull);
+  return lo < hi ? moved & (uint32_t)m : 0u;
+}
+
+// A window row of u8 pixels starting at image column xi0 of image row yi, 0
+// outside the frame (rows crossing the frame edge).
+template <int WX>
+__device__ __forceinline__ void load_u8_row_edge(const uint8_t* img, long long pitch, int nx, int ny, int xi0, int yi,
+                                                 float (&f)[WX]) {
+  const bool row_in = (unsigned)yi < (unsigned)ny;
+  const uint8_t* row = img + (long long)min(max(yi, 0), ny - 1) * pitch;
+  static_for<WX / 4>([&](auto qc) {
+    constexpr int q = qc;
+    const uint32_t w = row_in ? load_u8x4_edge(row, xi0 + 4 * q, nx) : 0u;
+    f[4 * q + 0] = (float)(w & 255u);
+    f[4 * q + 1] = (float)((w >> 8) & 255u);
+    f[4 * q + 2] = (float)((w >> 16) & 255u);
+    f[4 * q + 3] = (float)(w >> 24);
+  });
+}
+
 // One window row of WX pixels starting at aug coordinates (ax0, ay):
 // x[c] = (frame 1, frame 2).  Rows fully inside the image use wide loads;
-// others read pixel by pixel with 0 outside the frame.
+// rows crossing the frame edge use masked 4-byte loads with 0 outside.
 template <int WX>
 __device__ __forceinline__ void load_row(const XcArgs& a, int ax0, int ay, float2 (&x)[WX]) {
   const int xi0 = ax0 - a.left_x, yi = ay - a.left_y;
@@ -384,10 +425,7 @@ __device__ __forceinline__ void load_row(const XcArgs& a, int ax0, int ay, float
   if (inside) {
     load_u8_row<WX>(a.f1 + (long long)yi * a.pitch1 + xi0, f1);
   } else {
-    static_for<WX>([&](auto cc) {
-      constexpr int c = cc;
-      f1[c] = (row_in && (unsigned)(xi0 + c) < (unsigned)a.nx) ? (float)a.f1[(long long)yi * a.pitch1 + xi0 + c] : 0.f;
-    });
+    load_u8_row_edge<WX>(a.f1, a.pitch1, a.nx, a.ny, xi0, yi, f1);
   }
   if (a.f2def) {
     const float* q = a.f2def + (long long)ay * a.def_pitch + ax0;
@@ -403,10 +441,7 @@ __device__ __forceinline__ void load_row(const XcArgs& a, int ax0, int ay, float
   } else if (inside) {
     load_u8_row<WX>(a.f2u8 + (long long)yi * a.pitch2 + xi0, f2);
   } else {
-    static_for<WX>([&](auto cc) {
-      constexpr int c = cc;
-      f2[c] = (row_in && (unsigned)(xi0 + c) < (unsigned)a.nx) ? (float)a.f2u8[(long long)yi * a.pitch2 + xi0 + c] : 0.f;
-    });
+    load_u8_row_edge<WX>(a.f2u8, a.pitch2, a.nx, a.ny, xi0, yi, f2);
   }
   static_for<WX>([&](auto c) { x[c] = make_float2(f1[c], f2[c]); });
 }
@@ -503,8 +538,8 @@ __global__ void __launch_bounds__(64) k_xcorr_pair(XcArgs a) {
       // h = 0: E' + w^k O';  h = 1: E' - w^k O'
       const float re = fmaf(sh, r[k].x, p.x), im = fmaf(sh, r[k].y, p.y);
       const int cs = h ? k : k + HX;
-      cc0[rs * SX + cs] = (fabsf(re) * 0.25f) / (float)NPIX;
-      cc0[WY * SX + rs * SX + cs] = (fabsf(im) * 0.25f) / (float)NPIX;
+      cc0[rs * SX + cs] = cc_scale<NPIX>(re);
+      cc0[WY * SX + rs * SX + cs] = cc_scale<NPIX>(im);
     });
   }
   wave_sync();
@@ -668,8 +703,8 @@ __global__ void __launch_bounds__(64) k_xcorr_single(XcArgs a) {
     static_for<WX>([&](auto xc) {
       constexpr int x = xc;
       constexpr int xs = (x + HX) % WX;
-      ccp[rs0 * SX + xs] = (fabsf(hrow[x].x) * 0.25f) / (float)NPIX;
-      ccp[rs1 * SX + xs] = (fabsf(hrow[x].y) * 0.25f) / (float)NPIX;
+      ccp[rs0 * SX + xs] = cc_scale<NPIX>(hrow[x].x);
+      ccp[rs1 * SX + xs] = cc_scale<NPIX>(hrow[x].y);
     });
   }
   wave_sync();
@@ -1288,13 +1323,15 @@ __global__ void __launch_bounds__(256) k_median2(MedArgs a) {
   if (vmed) {
     for (int k = 0; k < a.K; ++k) {
       const long long o = (long long)t * a.K + k;
-      if (!a.cv[o]) continue;
-      const float devix = fabsf(a.cx[o] - mx) / (rx + a.eps);
-      const float deviy = fabsf(a.cy[o] - my) / (ry + a.eps);
+      const bool cvk = a.cv[o] != 0;
+      const float cxk = a.cx[o], cyk = a.cy[o];
+      if (!cvk) continue;
+      const float devix = fabsf(cxk - mx) / (rx + a.eps);
+      const float deviy = fabsf(cyk - my) / (ry + a.eps);
       if (devix < a.thr && deviy < a.thr) {
         okc = true;
-        ox = a.cx[o];
-        oy = a.cy[o];
+        ox = cxk;
+        oy = cyk;
         acc = k;
         break;
       }
