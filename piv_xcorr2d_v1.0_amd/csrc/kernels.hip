@@ -366,17 +366,30 @@ __device__ __forceinline__ int half_sum(int v) {
   return v;
 }
 
-// WX consecutive u8 pixels as floats (4-byte chunk loads, any alignment).
+// WX consecutive u8 pixels as floats: 16-byte chunk loads (4-byte chunks for
+// a remainder of WX % 16), any alignment.
 template <int WX>
 __device__ __forceinline__ void load_u8_row(const uint8_t* p, float (&f)[WX]) {
-  static_for<WX / 4>([&](auto qc) {
-    constexpr int q = qc;
-    uint32_t w;
-    __builtin_memcpy(&w, p + 4 * q, 4);
+  auto put = [&](int q, uint32_t w) {
     f[4 * q + 0] = (float)(w & 255u);
     f[4 * q + 1] = (float)((w >> 8) & 255u);
     f[4 * q + 2] = (float)((w >> 16) & 255u);
     f[4 * q + 3] = (float)(w >> 24);
+  };
+  static_for<WX / 16>([&](auto qc) {
+    constexpr int q = qc;
+    uint4 w;
+    __builtin_memcpy(&w, p + 16 * q, 16);
+    put(4 * q + 0, w.x);
+    put(4 * q + 1, w.y);
+    put(4 * q + 2, w.z);
+    put(4 * q + 3, w.w);
+  });
+  static_for<(WX % 16) / 4>([&](auto qc) {
+    constexpr int q = (WX / 16) * 4 + qc;
+    uint32_t w;
+    __builtin_memcpy(&w, p + 4 * q, 4);
+    put(q, w);
   });
 }
 
