@@ -642,86 +642,174 @@ __global__ void __launch_bounds__(64) k_xcorr_pair(XcArgs a) {
 // ============================================================== wave-per-window kernel
 //
 // Windows up to 64x64: one 64-lane wave owns one window, no workgroup
-// barriers.  The product P = 4 conj(F1) F2 is Hermitian (the correlation is
-// real), so the inverse runs on half the spectrum: column IFFTs for
-// kx = 0..WX/2, then real-output row IFFTs two rows at a time
-// (H = G[y] + i G[y + WY/2], with G[y][-kx] = conj(G[y][kx])).
+// barriers.  The window's LDS footprint is about one real plane
+// (WY x (WX + 2) floats, 16.9 KB at 64x64) so that two windows share every
+// SIMD: the transpose between the forward row and column FFTs moves the real
+// parts, then the imaginary parts, through one float plane.
+//
+// The product P = 4 conj(F1) F2 is Hermitian (the correlation is real), so the
+// inverse runs on half the spectrum, every line shared by a lane pair (radix-2
+// decimation in time: half h transforms the odd or even samples, the odd half
+// is twiddled, the pair exchanges halves through LDS and combines):
+//   * column IFFTs for kx = 0..WX/2, where columns 0 and WX/2 (real after the
+//     transform) are packed into one: P[:, 0] + i P[:, WX/2];
+//   * real-output row IFFTs two rows at a time, H = G[y] + i G[y + WY/2] with
+//     G[y][-kx] = conj(G[y][kx]).
 template <int WX, int WY>
 __global__ void __launch_bounds__(64) k_xcorr_single(XcArgs a) {
   static_assert(WX <= 64 && WY <= 64 && WX % 4 == 0 && WY % 4 == 0, "single-wave kernel covers windows up to 64x64");
-  constexpr int SX = WX + 1;
+  constexpr int SX = WX + 1;   // real plane row stride (floats)
   constexpr int NPIX = WX * WY;
   constexpr int HX = WX / 2, HY = WY / 2;
-  __shared__ float2 pl[WY][SX];
+  constexpr int GS = HX + 1;   // G plane row stride (complex)
+  constexpr int NF = std::max(std::max(WY * SX, 2 * WY * GS), 4 * HX * HY);
+  __shared__ float lds[NF];
+  __shared__ float2 pnyq[WY];  // column WX/2 of P
 
   const int lane = threadIdx.x;
   const int w = blockIdx.x;
   const int iy = w / a.nx_vec, ix = w - iy * a.nx_vec;
-  // 1-2. row gather + forward row FFT
-  if (lane < WY) {
-    float2 x[WX];
-    load_row<WX>(a, ix * a.sp_x, iy * a.sp_y + lane, x);
-    fft_reg<WX, false>(x);
-    static_for<WX>([&](auto c) { pl[lane][c] = x[c]; });
-  }
-  wave_sync();
+  float* const sp = lds;
 
-  // 3. forward column FFTs
+  // 1-2. row gather + forward row FFT, then the transpose (Re, then Im)
   float2 col[WY];
+  {
+    float2 x[WX];
+    if (lane < WY) {
+      load_row<WX>(a, ix * a.sp_x, iy * a.sp_y + lane, x);
+      fft_reg<WX, false>(x);
+      static_for<WX>([&](auto c) { sp[lane * SX + c] = x[c].x; });
+    }
+    wave_sync();
+    if (lane < WX) static_for<WY>([&](auto r) { col[r].x = sp[r * SX + lane]; });
+    wave_sync();
+    if (lane < WY) static_for<WX>([&](auto c) { sp[lane * SX + c] = x[c].y; });
+    wave_sync();
+    if (lane < WX) static_for<WY>([&](auto r) { col[r].y = sp[r * SX + lane]; });
+  }
+
+  // 3. forward column FFTs (lane = column kx)
+  if (lane < WX) fft_reg<WY, false>(col);
+  wave_sync();
+
+  // 4. product and column IFFTs.  Lane (h4, t4) gathers Z[ky][t4] and the
+  //    mirror Z[-ky][-t4] for its rows ky = 2m + h4 (again Re, then Im),
+  //    lanes ky < WY the column WX/2 pair for pnyq.
+  const int h4 = lane / HX, t4 = lane - (lane / HX) * HX;  // valid for lane < WX
+  const int tm = (WX - t4) % WX;
+  const int ky0 = lane < WY ? lane : 0;
+  const int kym = (WY - ky0) % WY;
+  float2 own[HY], mir[HY];
+  float2 n0, n1;
+  if (lane < WX) static_for<WY>([&](auto r) { sp[r * SX + lane] = col[r].x; });
+  wave_sync();
   if (lane < WX) {
-    static_for<WY>([&](auto r) { col[r] = pl[r][lane]; });
-    fft_reg<WY, false>(col);
-    static_for<WY>([&](auto r) { pl[r][lane] = col[r]; });
-  }
-  wave_sync();
-
-  // 4. split, then inverse column FFTs of the non-negative half of the spectrum
-  if (lane <= HX) {
-    const int lm = (WX - lane) % WX;
-    static_for<WY>([&](auto rc) {
-      constexpr int r = rc;
-      constexpr int rm = (WY - r) % WY;
-      col[r] = xprod(col[r], pl[rm][lm]);
+    static_for<HY>([&](auto mc) {
+      constexpr int m = mc;
+      const int ky = 2 * m + h4;
+      own[m].x = sp[ky * SX + t4];
+      mir[m].x = sp[((WY - ky) % WY) * SX + tm];
     });
-    fft_reg<WY, true>(col);
   }
-  wave_sync();  // every mirror read is done before the plane is overwritten
-  if (lane <= HX) static_for<WY>([&](auto r) { pl[r][lane] = col[r]; });
+  if (lane < WY) {
+    n0.x = sp[ky0 * SX + HX];
+    n1.x = sp[kym * SX + HX];
+  }
+  wave_sync();
+  if (lane < WX) static_for<WY>([&](auto r) { sp[r * SX + lane] = col[r].y; });
+  wave_sync();
+  if (lane < WX) {
+    static_for<HY>([&](auto mc) {
+      constexpr int m = mc;
+      const int ky = 2 * m + h4;
+      own[m].y = sp[ky * SX + t4];
+      mir[m].y = sp[((WY - ky) % WY) * SX + tm];
+    });
+  }
+  if (lane < WY) {
+    n0.y = sp[ky0 * SX + HX];
+    n1.y = sp[kym * SX + HX];
+    pnyq[lane] = xprod(n0, n1);
+  }
+  wave_sync();
+  float2* const xb = reinterpret_cast<float2*>(lds);
+  if (lane < WX) {
+    // lanes t4 == 0 add i * P[ky][WX/2] (the broadcast read is harmless elsewhere)
+    const float f = t4 == 0 ? 1.f : 0.f;
+    float2 q[HY];
+    static_for<HY>([&](auto mc) {
+      constexpr int m = mc;
+      const float2 p = xprod(own[m], mir[m]);
+      const float2 pn = pnyq[2 * m + h4];
+      q[m] = make_float2(fmaf(-f, pn.y, p.x), fmaf(f, pn.x, p.y));
+    });
+    fft_reg<HY, true>(q);
+    if (h4) static_for<HY>([&](auto kc) { q[kc] = twiddle<kc, WY, true>(q[kc]); });
+    // xb[h][k][t]: E (h = 0) and w^k O (h = 1) of column t
+    static_for<HY>([&](auto k) { xb[(h4 * HY + k) * HX + t4] = q[k]; });
+    wave_sync();
+    float2 p[HY];
+    static_for<HY>([&](auto k) { p[k] = xb[((1 - h4) * HY + k) * HX + t4]; });
+    wave_sync();
+    // G rows k + h4 * HY of column t4: E + w^k O (h4 = 0), E - w^k O (h4 = 1)
+    const float sh = h4 ? -1.f : 1.f;
+    float2* const gs = reinterpret_cast<float2*>(lds);
+    static_for<HY>([&](auto k) {
+      gs[(h4 * HY + k) * GS + t4] = make_float2(fmaf(sh, q[k].x, p[k].x), fmaf(sh, q[k].y, p[k].y));
+    });
+  }
   wave_sync();
 
-  // 5. real-output row IFFTs, rows y and y + WY/2 packed in one transform
-  float* const ccp = reinterpret_cast<float*>(&pl[0][0]);
-  float2 hrow[WX];
-  if (lane < HY) {
-    static_for<WX>([&](auto xc) {
-      constexpr int x = xc;
-      float2 g0, g1;
-      if constexpr (x <= HX) {
-        g0 = pl[lane][x];
-        g1 = pl[lane + HY][x];
+  // 5. real-output row IFFTs: lane (h5, y5) transforms the samples kx = 2m + h5
+  //    of H = G[y5] + i G[y5 + WY/2]; the pair exchanges halves and lane h5
+  //    writes output columns [h5 * WX/2, (h5 + 1) * WX/2) of rows y5 (Im) and
+  //    y5 + WY/2 (Re), scaled and fftshifted.
+  const float2* const gs = reinterpret_cast<const float2*>(lds);
+  float* const ccp = lds;
+  const int h5 = lane / HY, y5 = lane - (lane / HY) * HY;  // valid for lane < WY
+  if (lane < WY) {
+    float2 hr[HX];
+    const float2* const g0r = gs + y5 * GS;
+    const float2* const g1r = gs + (y5 + HY) * GS;
+    static_for<HX>([&](auto mc) {
+      constexpr int m = mc;
+      if constexpr (m == 0) {
+        // kx = 0 (h5 = 0, real column 0) or kx = 1 (h5 = 1, direct)
+        const float2 g0 = g0r[h5], g1 = g1r[h5];
+        hr[m] = h5 ? make_float2(g0.x - g1.y, g0.y + g1.x) : make_float2(g0.x, g1.x);
+      } else if constexpr (2 * m < HX) {
+        const float2 g0 = g0r[2 * m + h5], g1 = g1r[2 * m + h5];
+        hr[m] = make_float2(g0.x - g1.y, g0.y + g1.x);
+      } else if constexpr (2 * m == HX) {
+        // kx = WX/2 (h5 = 0: imaginary part of the packed column 0) or
+        // kx = WX/2 + 1 (h5 = 1: conjugate of column WX/2 - 1)
+        const int c = h5 ? HX - 1 : 0;
+        const float2 g0 = g0r[c], g1 = g1r[c];
+        hr[m] = h5 ? make_float2(g0.x + g1.y, g1.x - g0.y) : make_float2(g0.y, g1.y);
       } else {
-        g0 = pl[lane][WX - x];
-        g1 = pl[lane + HY][WX - x];
-        g0.y = -g0.y;
-        g1.y = -g1.y;
+        const float2 g0 = g0r[WX - 2 * m - h5], g1 = g1r[WX - 2 * m - h5];
+        hr[m] = make_float2(g0.x + g1.y, g1.x - g0.y);
       }
-      hrow[x] = make_float2(g0.x - g1.y, g0.y + g1.x);
     });
-    fft_reg<WX, true>(hrow);
-  }
-  wave_sync();  // all of G is read before the correlation plane overwrites it
-  if (lane < HY) {
-    const int r0 = lane, r1 = lane + HY;
-    const int rs0 = (r0 + HY) % WY, rs1 = (r1 + HY) % WY;
-    static_for<WX>([&](auto xc) {
-      constexpr int x = xc;
-      constexpr int xs = (x + HX) % WX;
-      ccp[rs0 * SX + xs] = cc_scale<NPIX>(hrow[x].x);
-      ccp[rs1 * SX + xs] = cc_scale<NPIX>(hrow[x].y);
+    fft_reg<HX, true>(hr);
+    if (h5) static_for<HX>([&](auto kc) { hr[kc] = twiddle<kc, WX, true>(hr[kc]); });
+    wave_sync();  // all of G is read before the exchange overwrites it
+    float2* const xb2 = reinterpret_cast<float2*>(lds);
+    static_for<HX>([&](auto k) { xb2[(h5 * HX + k) * HY + y5] = hr[k]; });
+    wave_sync();
+    float2 p[HX];
+    static_for<HX>([&](auto k) { p[k] = xb2[((1 - h5) * HX + k) * HY + y5]; });
+    wave_sync();
+    const float sh = h5 ? -1.f : 1.f;
+    static_for<HX>([&](auto kc) {
+      constexpr int k = kc;
+      const float re = fmaf(sh, hr[k].x, p[k].x), im = fmaf(sh, hr[k].y, p[k].y);
+      const int xs = h5 ? k : k + HX;
+      ccp[(y5 + HY) * SX + xs] = cc_scale<NPIX>(re);
+      ccp[y5 * SX + xs] = cc_scale<NPIX>(im);
     });
   }
   wave_sync();
-
   // 6. peaks: lane l walks column l of the window, keeping a bit mask of the
   //    rows that are local maxima; Npeak_save rounds of 64-lane arg-max
   const float* cc = ccp;
