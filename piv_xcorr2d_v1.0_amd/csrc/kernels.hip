@@ -464,7 +464,9 @@ __global__ void __launch_bounds__(64) k_xcorr_pair(XcArgs a) {
   static_assert(WX <= 32 && WY <= 32 && WX % 4 == 0 && WY % 2 == 0, "pair kernel covers windows up to 32x32");
   constexpr int SX = WX + 1;
   constexpr int NPIX = WX * WY;
-  __shared__ float2 pl[2][WY][SX];
+  constexpr int HX = WX / 2, HY = WY / 2;
+  // one complex plane (= both windows' real planes): 8.4 KB at 32x32
+  __shared__ float lds[2 * WY * SX];
 
   const int lane = threadIdx.x;
   const int h = lane >> 5, l = lane & 31;
@@ -472,51 +474,64 @@ __global__ void __launch_bounds__(64) k_xcorr_pair(XcArgs a) {
   const int iy = bid / a.chunks_x;
   const int ix = (bid - iy * a.chunks_x) * 2 + h;
   const bool win_ok = ix < a.nx_vec;
+  float* const pr = lds + h * WY * SX;  // this half's real plane
 
-  // 1-2. row gather + forward row FFT
-  if (l < WY) {
+  // 1-2. row gather + forward row FFT, then the transpose (Re, then Im)
+  float2 col[WY];
+  {
     float2 x[WX];
-    if (win_ok) {
-      load_row<WX>(a, ix * a.sp_x, iy * a.sp_y + l, x);
-    } else {
-      static_for<WX>([&](auto c) { x[c] = make_float2(0.f, 0.f); });
+    if (l < WY) {
+      if (win_ok) {
+        load_row<WX>(a, ix * a.sp_x, iy * a.sp_y + l, x);
+      } else {
+        static_for<WX>([&](auto c) { x[c] = make_float2(0.f, 0.f); });
+      }
+      fft_reg<WX, false>(x);
+      static_for<WX>([&](auto c) { pr[l * SX + c] = x[c].x; });
     }
-    fft_reg<WX, false>(x);
-    static_for<WX>([&](auto c) { pl[h][l][c] = x[c]; });
+    wave_sync();
+    if (l < WX) static_for<WY>([&](auto r) { col[r].x = pr[r * SX + l]; });
+    wave_sync();
+    if (l < WY) static_for<WX>([&](auto c) { pr[l * SX + c] = x[c].y; });
+    wave_sync();
+    if (l < WX) static_for<WY>([&](auto r) { col[r].y = pr[r * SX + l]; });
   }
-  wave_sync();
 
   // 3. forward column FFTs (both windows)
-  float2 col[WY];
-  if (l < WX) {
-    static_for<WY>([&](auto r) { col[r] = pl[h][r][l]; });
-    fft_reg<WY, false>(col);
-    static_for<WY>([&](auto r) { pl[h][r][l] = col[r]; });
-  }
+  if (l < WX) fft_reg<WY, false>(col);
   wave_sync();
 
-  // 4. split + column IFFTs.  Q = Pa + i*Pb with P = 4 conj(F1) F2; every
-  //    column IFFT of length WY is shared by the lane pair (l, l + 32):
-  //    half h transforms the rows ky = 2t + h (length WY/2), then
-  //    X[y] = E[y] + w^y O[y] and X[y + WY/2] = E[y] - w^y O[y] (w = e^{2 pi i/WY}).
-  constexpr int HX = WX / 2, HY = WY / 2;
+  // 4. split + column IFFTs.  Q = Pa + i*Pb with P = 4 conj(F1) F2; the
+  //    spectra of windows a and b pass through the complex plane one after
+  //    the other.  Every column IFFT of length WY is shared by the lane pair
+  //    (l, l + 32): half h transforms the rows ky = 2t + h (length WY/2),
+  //    then X[y] = E[y] + w^y O[y] and X[y + WY/2] = E[y] - w^y O[y]
+  //    (w = e^{2 pi i/WY}).
+  float2* const pl = reinterpret_cast<float2*>(lds);
   float2 q[HY];
+  const int lm = (WX - l) % WX;
+  static_for<2>([&](auto wc) {
+    constexpr int wnd = wc;
+    if (h == wnd && l < WX) static_for<WY>([&](auto r) { pl[r * SX + l] = col[r]; });
+    wave_sync();
+    if (l < WX) {
+      static_for<HY>([&](auto tc) {
+        constexpr int t = tc;
+        const int ky = 2 * t + h;
+        const int km = t == 0 ? h * (WY - 1) : WY - ky;
+        const float2 p = xprod(pl[ky * SX + l], pl[km * SX + lm]);
+        if constexpr (wnd == 0) q[t] = p;
+        else q[t] = make_float2(q[t].x - p.y, q[t].y + p.x);
+      });
+    }
+    wave_sync();  // every read of this spectrum is done before the plane is reused
+  });
   if (l < WX) {
-    const int lm = (WX - l) % WX;
-    static_for<HY>([&](auto tc) {
-      constexpr int t = tc;
-      const int ky = 2 * t + h;
-      const int km = t == 0 ? h * (WY - 1) : WY - ky;
-      const float2 pa = xprod(pl[0][ky][l], pl[0][km][lm]);
-      const float2 pb = xprod(pl[1][ky][l], pl[1][km][lm]);
-      q[t] = make_float2(pa.x - pb.y, pa.y + pb.x);
-    });
     fft_reg<HY, true>(q);
     if (h) static_for<HY>([&](auto kc) { q[kc] = twiddle<kc, WY, true>(q[kc]); });
   }
-  wave_sync();  // every spectrum read is done before plane 0 is reused
-  // plane 0 as eo[h][y][kx], y < WY/2: E (h = 0) and w^y O (h = 1)
-  float2* const eo = &pl[0][0][0];
+  // plane as eo[h][y][kx], y < WY/2: E (h = 0) and w^y O (h = 1)
+  float2* const eo = pl;
   if (l < WX) static_for<HY>([&](auto k) { eo[(h * HY + k) * SX + l] = q[k]; });
   wave_sync();
 
@@ -524,7 +539,6 @@ __global__ void __launch_bounds__(64) k_xcorr_pair(XcArgs a) {
   //    kx = 2t + h of row y of X, the pair exchanges halves through LDS and
   //    lane h writes output columns [h*WX/2, (h+1)*WX/2) of both correlation
   //    planes (Re -> window a, Im -> window b), scaled and fftshifted.
-  float* const cc0 = reinterpret_cast<float*>(&pl[1][0][0]);
   float2 r[HX];
   const int yy = l < HY ? l : l - HY;
   if (l < WY) {
@@ -537,26 +551,28 @@ __global__ void __launch_bounds__(64) k_xcorr_pair(XcArgs a) {
     fft_reg<HX, true>(r);
     if (h) static_for<HX>([&](auto kc) { r[kc] = twiddle<kc, WX, true>(r[kc]); });
   }
-  wave_sync();  // every read of eo is done before plane 0 is reused
-  // plane 0 as xb[h][k][y]: E' (h = 0) and w^k O' (h = 1) of row y
-  float2* const xb = &pl[0][0][0];
-  if (l < WY) static_for<HX>([&](auto k) { xb[(h * HX + k) * WY + l] = r[k]; });
-  wave_sync();
+  wave_sync();  // every read of eo is done before the plane is reused
+  // plane as xb[h][k][y]: E' (h = 0) and w^k O' (h = 1) of row y
+  float2* const xb = pl;
+  float* const cc0 = lds;
   if (l < WY) {
+    static_for<HX>([&](auto k) { xb[(h * HX + k) * WY + l] = r[k]; });
+    wave_sync();
+    float2 p[HX];
+    static_for<HX>([&](auto k) { p[k] = xb[((1 - h) * HX + k) * WY + l]; });
+    wave_sync();  // the correlation planes overwrite xb
     const float sh = h ? -1.f : 1.f;
     const int rs = (l + HY) % WY;
     static_for<HX>([&](auto kc) {
       constexpr int k = kc;
-      const float2 p = xb[((1 - h) * HX + k) * WY + l];
       // h = 0: E' + w^k O';  h = 1: E' - w^k O'
-      const float re = fmaf(sh, r[k].x, p.x), im = fmaf(sh, r[k].y, p.y);
+      const float re = fmaf(sh, r[k].x, p[k].x), im = fmaf(sh, r[k].y, p[k].y);
       const int cs = h ? k : k + HX;
       cc0[rs * SX + cs] = cc_scale<NPIX>(re);
       cc0[WY * SX + rs * SX + cs] = cc_scale<NPIX>(im);
     });
   }
   wave_sync();
-
   // 6. peaks: half-wave h walks window h column l, keeping a bit mask of the
   //    rows of its column that are local maxima; Npeak_save rounds of 32-lane
   //    arg-max over the lanes' best maxima
