@@ -501,32 +501,43 @@ __global__ void __launch_bounds__(64) k_xcorr_pair(XcArgs a) {
   if (l < WX) fft_reg<WY, false>(col);
   wave_sync();
 
-  // 4. split + column IFFTs.  Q = Pa + i*Pb with P = 4 conj(F1) F2; the
-  //    spectra of windows a and b pass through the complex plane one after
-  //    the other.  Every column IFFT of length WY is shared by the lane pair
-  //    (l, l + 32): half h transforms the rows ky = 2t + h (length WY/2),
-  //    then X[y] = E[y] + w^y O[y] and X[y + WY/2] = E[y] - w^y O[y]
-  //    (w = e^{2 pi i/WY}).
+  // 4. split + column IFFTs.  Q = Pa + i*Pb with P = 4 conj(F1) F2.  Lane
+  //    (h, l) holds column l of window h and takes the mirror column -l from
+  //    lane (h, -l) by lane permutation (no LDS), forming its window's P for
+  //    all rows in place; one permlane32 swap per component then gives lane
+  //    (h, l) the rows ky = 2t + h of both windows.  Every column IFFT of
+  //    length WY is shared by the lane pair (l, l + 32): half h transforms
+  //    the rows ky = 2t + h (length WY/2), then X[y] = E[y] + w^y O[y] and
+  //    X[y + WY/2] = E[y] - w^y O[y] (w = e^{2 pi i/WY}).
   float2* const pl = reinterpret_cast<float2*>(lds);
   float2 q[HY];
-  const int lm = (WX - l) % WX;
-  static_for<2>([&](auto wc) {
-    constexpr int wnd = wc;
-    if (h == wnd && l < WX) static_for<WY>([&](auto r) { pl[r * SX + l] = col[r]; });
-    wave_sync();
-    if (l < WX) {
-      static_for<HY>([&](auto tc) {
-        constexpr int t = tc;
-        const int ky = 2 * t + h;
-        const int km = t == 0 ? h * (WY - 1) : WY - ky;
-        const float2 p = xprod(pl[ky * SX + l], pl[km * SX + lm]);
-        if constexpr (wnd == 0) q[t] = p;
-        else q[t] = make_float2(q[t].x - p.y, q[t].y + p.x);
-      });
-    }
-    wave_sync();  // every read of this spectrum is done before the plane is reused
-  });
   if (l < WX) {
+    const int src = h * 32 + (WX - l) % WX;
+    auto mirror = [&](float2 v) { return make_float2(__shfl(v.x, src, 64), __shfl(v.y, src, 64)); };
+    static_for<HY + 1>([&](auto kc) {
+      constexpr int ky = kc;
+      constexpr int km = (WY - ky) % WY;
+      if constexpr (ky == km) {
+        col[ky] = xprod(col[ky], mirror(col[ky]));
+      } else {
+        const float2 ma = mirror(col[km]), mb = mirror(col[ky]);
+        col[ky] = xprod(col[ky], ma);
+        col[km] = xprod(col[km], mb);
+      }
+    });
+    // lanes 32-63 of the even rows <-> lanes 0-31 of the odd rows
+    auto swap32 = [](float& lo, float& hi) {
+      const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(lo), __float_as_uint(hi), false, false);
+      lo = __uint_as_float(r[0]);
+      hi = __uint_as_float(r[1]);
+    };
+    static_for<HY>([&](auto tc) {
+      constexpr int t = tc;
+      float2 pa = col[2 * t], pb = col[2 * t + 1];
+      swap32(pa.x, pb.x);
+      swap32(pa.y, pb.y);
+      q[t] = make_float2(pa.x - pb.y, pa.y + pb.x);
+    });
     fft_reg<HY, true>(q);
     if (h) static_for<HY>([&](auto kc) { q[kc] = twiddle<kc, WY, true>(q[kc]); });
   }
