@@ -115,6 +115,36 @@ PIVX_HD float2 twiddle(float2 a) {
   }
 }
 
+// (a + w*b, a - w*b) with w = exp(sgn * 2*pi*i*M/N), the twiddled radix-2
+// butterfly.  For a nontrivial w = c + i*s it is written in tangent form
+// (w*b = c*(b.x - (s/c)*b.y, b.y + (s/c)*b.x), or the cotangent form when
+// |s| > |c|, so the ratio never exceeds 1): two fused multiply-adds for the
+// scaled product and four for the two outputs, six instructions where a
+// complex multiply and four additions take eight.
+template <int M, int N, bool INV>
+PIVX_HD void bfly2(float2 a, float2 b, float2& p, float2& q) {
+  constexpr int m = ((M % N) + N) % N;
+  if constexpr (m == 0 || 2 * m == N || 4 * m == N || 4 * m == 3 * N) {
+    const float2 wb = twiddle<m, N, INV>(b);
+    p = cadd(a, wb);
+    q = csub(a, wb);
+  } else {
+    constexpr CS w = cossin2pi(m, N);
+    constexpr double c = w.c, sn = INV ? w.s : -w.s;
+    if constexpr ((c < 0 ? -c : c) >= (sn < 0 ? -sn : sn)) {
+      constexpr float t = float(sn / c), cf = float(c);
+      const float u = fmaf(-t, b.y, b.x), v = fmaf(t, b.x, b.y);
+      p = make_float2(fmaf(cf, u, a.x), fmaf(cf, v, a.y));
+      q = make_float2(fmaf(-cf, u, a.x), fmaf(-cf, v, a.y));
+    } else {
+      constexpr float t = float(c / sn), sf = float(sn);
+      const float u = fmaf(t, b.x, -b.y), v = fmaf(t, b.y, b.x);
+      p = make_float2(fmaf(sf, u, a.x), fmaf(sf, v, a.y));
+      q = make_float2(fmaf(-sf, u, a.x), fmaf(-sf, v, a.y));
+    }
+  }
+}
+
 // ---------------------------------------------------------------- small DFTs
 template <int R, bool INV>
 PIVX_HD void dft_small(float2 (&v)[R]) {
@@ -164,23 +194,38 @@ constexpr int pick_radix(int rem) {
   return rem;
 }
 
+// Radix-2 and radix-4 stages run as twiddled radix-2 butterflies: a radix-4
+// butterfly with inputs twiddled by w^0, w^m, w^2m, w^3m is
+//   (v0, v2) -> v0 +- w^2m v2 = t0, t1;   (v1, v3) -> v1 +- w^2m v3 = s, d;
+//   X0, X2 = t0 +- w^m s;   X1, X3 = t1 +- (-+i) w^m d
+// where -+i is the radix-4 rotation, i.e. the twiddle index m + N/4.
 template <int N, int R, int Ns, bool INV>
 PIVX_HD void stockham_stage(const float2 (&x)[N], float2 (&y)[N]) {
   static_for<N / R>([&](auto jc) {
     constexpr int j = jc;
     constexpr int k = j % Ns;
-    float2 v[R];
-    static_for<R>([&](auto rc) {
-      constexpr int r = rc;
-      constexpr int m = (r * k * (N / (Ns * R))) % N;
-      v[r] = twiddle<m, N, INV>(x[j + r * (N / R)]);
-    });
-    dft_small<R, INV>(v);
+    constexpr int m = k * (N / (Ns * R));
     constexpr int idxD = (j / Ns) * Ns * R + k;
-    static_for<R>([&](auto rc) {
-      constexpr int r = rc;
-      y[idxD + r * Ns] = v[r];
-    });
+    if constexpr (R == 2) {
+      bfly2<m, N, INV>(x[j], x[j + N / 2], y[idxD], y[idxD + Ns]);
+    } else if constexpr (R == 4) {
+      float2 t0, t1, sv, dv;
+      bfly2<2 * m, N, INV>(x[j], x[j + 2 * (N / 4)], t0, t1);
+      bfly2<2 * m, N, INV>(x[j + N / 4], x[j + 3 * (N / 4)], sv, dv);
+      bfly2<m, N, INV>(t0, sv, y[idxD], y[idxD + 2 * Ns]);
+      bfly2<m + N / 4, N, INV>(t1, dv, y[idxD + Ns], y[idxD + 3 * Ns]);
+    } else {
+      float2 v[R];
+      static_for<R>([&](auto rc) {
+        constexpr int r = rc;
+        v[r] = twiddle<(r * m) % N, N, INV>(x[j + r * (N / R)]);
+      });
+      dft_small<R, INV>(v);
+      static_for<R>([&](auto rc) {
+        constexpr int r = rc;
+        y[idxD + r * Ns] = v[r];
+      });
+    }
   });
 }
 
