@@ -75,6 +75,7 @@ struct pivx_ctx {
   int* sweeps = nullptr;                   // [2]
   int* nholes = nullptr;                   // hole-list counter of the fill kernels
   int *list_a = nullptr, *list_b = nullptr;
+  int list_cap = 0;                        // entries of list_a / list_b
   float *tmp_x = nullptr, *tmp_y = nullptr;
   uint8_t* tmp_f = nullptr;
   std::vector<void*> allocs;
@@ -390,6 +391,7 @@ int run_filter(pivx_ctx* c, int gi, int npass_median, int min_nneigh, float inse
       m.scratch = G.med_scratch;
       m.hole_list = (back == 0 && listed) ? c->list_a : nullptr;
       m.nholes = (back == 0 && listed) ? c->nholes : nullptr;
+      m.hole_cap = c->list_cap;
       rc = check_launch(c, launch_median(m, c->stream), "median_filtering");
       if (rc) return rc;
       srcx = tx;
@@ -412,6 +414,7 @@ int run_filter(pivx_ctx* c, int gi, int npass_median, int min_nneigh, float inse
     f.tmp_y = c->tmp_y;
     f.tmp_f = c->tmp_f;
     f.nholes = c->nholes;
+    f.list_cap = c->list_cap;
     f.sweeps = c->sweeps + gi;
     f.prelisted = listed ? 1 : 0;
     tic(c);
@@ -563,6 +566,7 @@ int pivx_init(const pivx_init_params* p, int device_id, pivx_ctx** out) {
   rc |= dalloc(c, &c->counters, 2);
   rc |= dalloc(c, &c->sweeps, 2);
   rc |= dalloc(c, &c->nholes, 1);
+  c->list_cap = (int)ntmax;
   rc |= dalloc(c, &c->list_a, ntmax);
   rc |= dalloc(c, &c->list_b, ntmax);
   rc |= dalloc(c, &c->tmp_x, ntmax);

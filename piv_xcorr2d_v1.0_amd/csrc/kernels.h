@@ -46,6 +46,7 @@ struct MedArgs {
   float* scratch;          // generic path only: 4*wx*wy floats per vector
   int* hole_list;          // if set: indices of vectors left invalid are appended here
   int* nholes;             //   (count in *nholes, which must be 0 before the pass)
+  int hole_cap;            //   capacity of hole_list; appends beyond it are dropped
 };
 
 struct FillArgs {
@@ -53,6 +54,7 @@ struct FillArgs {
   int nx_vec, ny_vec;
   int* list_a; int* list_b; float* tmp_x; float* tmp_y; uint8_t* tmp_f;
   int* nholes;             // hole counter, 0 between fills
+  int list_cap;            // capacity of list_a and list_b
   int* sweeps;
   int prelisted;           // holes already in list_a / *nholes (appended by the median pass)
 };

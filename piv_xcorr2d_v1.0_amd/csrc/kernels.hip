@@ -1317,7 +1317,10 @@ __device__ __forceinline__ void median_decide(const MedArgs& a, int t, bool vmed
   a.dy[t] = oy;
   a.dv[t] = ok ? 1 : 0;
   if (a.accepted) a.accepted[t] = (int8_t)acc;
-  if (a.hole_list && !ok) a.hole_list[atomicAdd(a.nholes, 1)] = t;
+  if (a.hole_list && !ok) {
+    const int slot = atomicAdd(a.nholes, 1);
+    if (slot < a.hole_cap) a.hole_list[slot] = t;
+  }
 }
 
 template <int NS>
@@ -1469,7 +1472,10 @@ __global__ void __launch_bounds__(256) k_median2(MedArgs a) {
     a.dx[t] = ox;
     a.dv[t] = okc ? 1 : 0;
     if (a.accepted) a.accepted[t] = (int8_t)acc;
-    if (a.hole_list && !okc) a.hole_list[atomicAdd(a.nholes, 1)] = t;
+    if (a.hole_list && !okc) {
+      const int slot = atomicAdd(a.nholes, 1);
+      if (slot < a.hole_cap) a.hole_list[slot] = t;
+    }
   } else {
     a.dy[t] = oy;
   }
@@ -1541,14 +1547,17 @@ __global__ void __launch_bounds__(128) k_median_generic(MedArgs a) {
 __global__ void __launch_bounds__(256) k_fill_list(FillArgs a) {
   const int nt = a.nx_vec * a.ny_vec;
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nt; t += gridDim.x * blockDim.x)
-    if (!a.dv[t]) a.list_a[atomicAdd(a.nholes, 1)] = t;
+    if (!a.dv[t]) {
+      const int slot = atomicAdd(a.nholes, 1);
+      if (slot < a.list_cap) a.list_a[slot] = t;
+    }
 }
 
 __global__ void __launch_bounds__(1024) k_fill(FillArgs a) {
 #pragma clang fp contract(off)
   __shared__ int n_holes, n_next, n_filled;
   if (threadIdx.x == 0) {
-    n_holes = *a.nholes;
+    n_holes = min(*a.nholes, a.list_cap);
     n_filled = 0;
   }
   __syncthreads();
