@@ -347,6 +347,21 @@ int xcorr_pass(const orc_pass* P, double* dx, double* dy, uint8_t* vv, double* p
   return 0;
 }
 
+// ---------------------------------------------------------------- bilinear sums
+// The reference's bilinear sums as the fused multiply-add chains a CUDA build
+// forms from their source by default (--fmad=true; the rest of the oracle is
+// compiled with contraction off):
+//   x + c1*e1 + c2*e2 + c3*e3 + c4*e4 -> fma(c4,e4, fma(c3,e3, fma(c2,e2, fma(c1,e1, x))))
+//   c1*e1 + c2*e2 + c3*e3 + c4*e4     -> fma(c4,e4, fma(c3,e3, fma(c1,e1, c2*e2)))
+template <class T>
+inline T fma4(T c1, T e1, T c2, T e2, T c3, T e3, T c4, T e4, T x) {
+  return std::fma(c4, e4, std::fma(c3, e3, std::fma(c2, e2, std::fma(c1, e1, x))));
+}
+template <class T>
+inline T sum4(T c1, T e1, T c2, T e2, T c3, T e3, T c4, T e4) {
+  return std::fma(c4, e4, std::fma(c3, e3, std::fma(c1, e1, c2 * e2)));
+}
+
 // ---------------------------------------------------------------- deformation (A16)
 template <class T>
 void deform(const uint8_t* f2, int nx, int ny, int left_x, int left_y, int nx_aug, int ny_aug, const double* exx,
@@ -362,8 +377,8 @@ void deform(const uint8_t* f2, int nx, int ny, int left_x, int left_y, int nx_au
       ix = std::min(std::max(ix, 0), nxe - 2);
       iy = std::min(std::max(iy, 0), nye - 2);
       const int i1 = iy * nxe + ix, i2 = i1 + 1, i3 = i1 + nxe, i4 = i3 + 1;
-      T sx = (T)idx + ce1 * (T)exx[i1] + ce2 * (T)exx[i2] + ce3 * (T)exx[i3] + ce4 * (T)exx[i4];
-      T sy = (T)idy + ce1 * (T)exy[i1] + ce2 * (T)exy[i2] + ce3 * (T)exy[i3] + ce4 * (T)exy[i4];
+      T sx = fma4(ce1, (T)exx[i1], ce2, (T)exx[i2], ce3, (T)exx[i3], ce4, (T)exx[i4], (T)idx);
+      T sy = fma4(ce1, (T)exy[i1], ce2, (T)exy[i2], ce3, (T)exy[i3], ce4, (T)exy[i4], (T)idy);
       const int jx = (int)std::floor(sx), jy = (int)std::floor(sy);
       sx = sx - (T)jx;
       sy = sy - (T)jy;
@@ -376,7 +391,7 @@ void deform(const uint8_t* f2, int nx, int ny, int left_x, int left_y, int nx_au
         return (xi >= 0 && xi < nx && yi >= 0 && yi < ny) ? (T)f2[(long)yi * nx + xi] : (T)0;
       };
       out[(long)idy * nx_aug + idx] =
-          (double)(ce1 * px(jx, jy) + ce2 * px(jx + 1, jy) + ce3 * px(jx, jy + 1) + ce4 * px(jx + 1, jy + 1));
+          (double)sum4(ce1, px(jx, jy), ce2, px(jx + 1, jy), ce3, px(jx, jy + 1), ce4, px(jx + 1, jy + 1));
     }
 }
 
@@ -397,8 +412,8 @@ void upsample(const double* ex, const double* ey, int nxe, int nye, T x_st1, T y
       iy = std::min(std::max(iy, 0), nye - 2);
       const int i1 = iy * nxe + ix, i2 = i1 + 1, i3 = i1 + nxe, i4 = i3 + 1;
       const long t = (long)vy * nx2 + vx;
-      dx2[t] = (double)(ce1 * (T)ex[i1] + ce2 * (T)ex[i2] + ce3 * (T)ex[i3] + ce4 * (T)ex[i4]);
-      dy2[t] = (double)(ce1 * (T)ey[i1] + ce2 * (T)ey[i2] + ce3 * (T)ey[i3] + ce4 * (T)ey[i4]);
+      dx2[t] = (double)sum4(ce1, (T)ex[i1], ce2, (T)ex[i2], ce3, (T)ex[i3], ce4, (T)ex[i4]);
+      dy2[t] = (double)sum4(ce1, (T)ey[i1], ce2, (T)ey[i2], ce3, (T)ey[i3], ce4, (T)ey[i4]);
     }
 }
 

@@ -1105,7 +1105,11 @@ __global__ void __launch_bounds__(256) k_xcorr_generic(XcArgs a, int WX, int WY,
 // Kernel_Window_Deformation_2d (cu:608-652): for every aug pixel, bilinear
 // displacement from the extended filtered field, then bilinear sample of frame
 // 2 at the displaced position.  Out-of-frame taps read 0 (the reference's zero
-// padding); taps beyond the reference's padded buffer are counted.
+// padding); taps beyond the reference's padded buffer are counted.  The sums
+// x + c1*e1 + ... + c4*e4 (cu:631-632) and c1*v1 + ... + c4*v4 (cu:649) are the
+// fused multiply-add chains a CUDA build forms from that source by default
+// (--fmad=true): fma(c4, e4, fma(c3, e3, fma(c2, e2, fma(c1, e1, x)))) and
+// fma(c4, v4, fma(c3, v3, fma(c1, v1, c2*v2))).
 // One thread per aug column, DEF_ROWS consecutive rows.  The row terms
 // (cell row, fraction, 1 - fraction) are computed once per workgroup into
 // LDS and the column terms once per thread, with the same operations as the
@@ -1126,7 +1130,7 @@ __global__ void __launch_bounds__(256) k_deform(DefArgs a) {
     int iyf = (int)floorf(ay);
     ay = ay - (float)iyf;
     iyf = min(max(iyf, 0), a.nye - 2);
-    rowt[threadIdx.x] = make_float4(ay, 1 - ay, __int_as_float(iyf), 0.f);
+    rowt[threadIdx.x] = make_float4(ay, 1 - ay, __int_as_float(iyf), (float)idy);
   }
   __syncthreads();
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1140,6 +1144,11 @@ __global__ void __launch_bounds__(256) k_deform(DefArgs a) {
   ixf = min(max(ixf, 0), a.nxe - 2);
   int cell = -1;
   float ex1 = 0.f, ex2 = 0.f, ex3 = 0.f, ex4 = 0.f, ey1 = 0.f, ey2 = 0.f, ey3 = 0.f, ey4 = 0.f;
+  // A pixel's displacement is a convex combination of its cell's four field
+  // samples, so while every sample is below hfws - 2 in magnitude no tap can
+  // leave the reference's padded buffer and the per-pixel test is skipped.
+  const float oob_guard = (float)(min(a.hfws_x, a.hfws_y) - 2);
+  bool may_oob = true;
   const int nxm = a.nx - 1, nym = a.ny - 1;
   const unsigned pitch = (unsigned)a.pitch;
   const unsigned span_x = (unsigned)(a.nx_aug + 2 * a.hfws_x - 1), span_y = (unsigned)(a.ny_aug + 2 * a.hfws_y - 1);
@@ -1148,7 +1157,7 @@ __global__ void __launch_bounds__(256) k_deform(DefArgs a) {
     const float4 rt = rowt[r];
     const float ay = rt.x, ay1 = rt.y;
     const int iyf = __builtin_amdgcn_readfirstlane(__float_as_int(rt.z));
-    const float fidy = (float)(y0 + r);
+    const float fidy = rt.w;
     float ce1 = ax1 * ay1, ce2 = ax * ay1, ce3 = ax1 * ay, ce4 = ax * ay;
     if (iyf != cell) {
       cell = iyf;
@@ -1161,9 +1170,12 @@ __global__ void __launch_bounds__(256) k_deform(DefArgs a) {
       ey2 = a.ex_y[i1 + 1];
       ey3 = a.ex_y[i3];
       ey4 = a.ex_y[i3 + 1];
+      const float em = fmaxf(fmaxf(fmaxf(fabsf(ex1), fabsf(ex2)), fmaxf(fabsf(ex3), fabsf(ex4))),
+                             fmaxf(fmaxf(fabsf(ey1), fabsf(ey2)), fmaxf(fabsf(ey3), fabsf(ey4))));
+      may_oob = !(em < oob_guard);  // NaN samples keep the exact test
     }
-    float sx = fidx + ce1 * ex1 + ce2 * ex2 + ce3 * ex3 + ce4 * ex4;
-    float sy = fidy + ce1 * ey1 + ce2 * ey2 + ce3 * ey3 + ce4 * ey4;
+    float sx = fmaf(ce4, ex4, fmaf(ce3, ex3, fmaf(ce2, ex2, fmaf(ce1, ex1, fidx))));
+    float sy = fmaf(ce4, ey4, fmaf(ce3, ey3, fmaf(ce2, ey2, fmaf(ce1, ey1, fidy))));
     const int jx = (int)floorf(sx), jy = (int)floorf(sy);
     sx = sx - (float)jx;
     sy = sy - (float)jy;
@@ -1174,17 +1186,19 @@ __global__ void __launch_bounds__(256) k_deform(DefArgs a) {
     const int xi = jx - a.left_x, yi = jy - a.left_y;
     const bool x0 = (unsigned)xi <= (unsigned)nxm, x1 = (unsigned)(xi + 1) <= (unsigned)nxm;
     const bool yok0 = (unsigned)yi <= (unsigned)nym, yok1 = (unsigned)(yi + 1) <= (unsigned)nym;
-    const unsigned cx0 = (unsigned)min(max(xi, 0), nxm), cx1 = (unsigned)min(max(xi + 1, 0), nxm);
+    // taps outside the frame load pixel 0 and are zeroed below
+    const unsigned cx0 = x0 ? (unsigned)xi : 0u, cx1 = x1 ? (unsigned)(xi + 1) : 0u;
     // 24-bit products: frames hold fewer than 2^31 pixels and rows fewer than 2^24
-    const unsigned r0 = __umul24((unsigned)min(max(yi, 0), nym), pitch);
-    const unsigned r1 = __umul24((unsigned)min(max(yi + 1, 0), nym), pitch);
+    const unsigned r0 = __umul24(yok0 ? (unsigned)yi : 0u, pitch);
+    const unsigned r1 = __umul24(yok1 ? (unsigned)(yi + 1) : 0u, pitch);
     const float v1 = (float)a.f2[r0 + cx0], v2 = (float)a.f2[r0 + cx1];
     const float v3 = (float)a.f2[r1 + cx0], v4 = (float)a.f2[r1 + cx1];
     const float t1 = (yok0 && x0) ? v1 : 0.f, t2 = (yok0 && x1) ? v2 : 0.f;
     const float t3 = (yok1 && x0) ? v3 : 0.f, t4 = (yok1 && x1) ? v4 : 0.f;
     // taps outside the reference's aug_ex buffer ([-hfws, n_aug + hfws) in aug coordinates)
-    oob += ((unsigned)(jx + a.hfws_x) >= span_x || (unsigned)(jy + a.hfws_y) >= span_y) ? 1u : 0u;
-    *out = ce1 * t1 + ce2 * t2 + ce3 * t3 + ce4 * t4;
+    if (__builtin_expect(may_oob, 0))
+      oob += ((unsigned)(jx + a.hfws_x) >= span_x || (unsigned)(jy + a.hfws_y) >= span_y) ? 1u : 0u;
+    *out = fmaf(ce4, t4, fmaf(ce3, t3, fmaf(ce1, t1, ce2 * t2)));
   }
   if (oob) atomicAdd(a.oob, (unsigned long long)oob);
 }
@@ -1218,7 +1232,9 @@ __device__ __forceinline__ void write_ext(float* ex, float* ey, int nx, int ny, 
 
 // ============================================================== upsampling
 // Upsampling_vecfield_kernel_2d (cu:675-704): grid-2 predictor at image
-// position (k*sp2 + sp2/2) from the extended grid-1 field (origin x_st1).
+// position (k*sp2 + sp2/2) from the extended grid-1 field (origin x_st1).  The
+// bilinear sums (cu:700-701) are fma(c4, e4, fma(c3, e3, fma(c1, e1, c2*e2))),
+// the chain a CUDA build forms from that source by default (--fmad=true).
 __global__ void k_upsample(float* dx2, float* dy2, const float* ex, const float* ey, int nxe, int nye, float x_st1,
                            float y_st1, float sp1x, float sp1y, int sp2x, int sp2y, int nx2, int ny2, float* ex2x,
                            float* ex2y, int border) {
@@ -1235,8 +1251,8 @@ __global__ void k_upsample(float* dx2, float* dy2, const float* ex, const float*
     ix = min(max(ix, 0), nxe - 2);
     iy = min(max(iy, 0), nye - 2);
     const int i1 = iy * nxe + ix, i2 = i1 + 1, i3 = i1 + nxe, i4 = i3 + 1;
-    const float ux = ce1 * ex[i1] + ce2 * ex[i2] + ce3 * ex[i3] + ce4 * ex[i4];
-    const float uy = ce1 * ey[i1] + ce2 * ey[i2] + ce3 * ey[i3] + ce4 * ey[i4];
+    const float ux = fmaf(ce4, ex[i4], fmaf(ce3, ex[i3], fmaf(ce1, ex[i1], ce2 * ex[i2])));
+    const float uy = fmaf(ce4, ey[i4], fmaf(ce3, ey[i3], fmaf(ce1, ey[i1], ce2 * ey[i2])));
     dx2[t] = ux;
     dy2[t] = uy;
     if (ex2x) write_ext(ex2x, ex2y, nx2, ny2, border, vx, vy, ux, uy);

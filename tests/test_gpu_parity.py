@@ -105,6 +105,36 @@ def test_deformation_bitexact_and_deformed_pass(pivx, orc):
     check_pass("config1 deformed pass", gpu, ref, (32, 32), 4)
 
 
+@pytest.mark.parametrize("shift", [(20, -18), (-17, 23), (5, -3)])
+def test_deformation_out_of_buffer_taps(pivx, shift):
+    """The count of deformation taps beyond the reference's padded buffer
+    (aug coordinates outside [-hfws, n_aug + hfws - 1)).  An integer uniform
+    field shifts every aug pixel exactly (the bilinear weights of a spacing-16
+    grid are exact binary fractions summing to 1), so the count has a closed
+    form; shifts beyond the half window exercise the exact per-pixel test,
+    the small shift the per-cell bound that skips it."""
+    a, b = config1_pair(pivx, seed=4)
+    p = config1_params(pivx, npass1=2)
+    with pivx.Xcorr2dMulti(p) as x:
+        g = geom_dict(x.geometry(1))
+        nt = g["nx_vec"] * g["ny_vec"]
+        x.prep_img_data(a, b)
+        x.set_filtered(1, np.full(nt, shift[0], np.float32), np.full(nt, shift[1], np.float32))
+        x.extend_filtered_vecfield_p1()
+        x.Calc_cc_map_p1(1)
+        got = x.stats()["oob_taps"][0]
+    hfws = 16
+    jx = np.arange(g["nx_aug"]) + shift[0]
+    jy = np.arange(g["ny_aug"]) + shift[1]
+    out_x = (jx + hfws < 0) | (jx + hfws >= g["nx_aug"] + 2 * hfws - 1)
+    out_y = (jy + hfws < 0) | (jy + hfws >= g["ny_aug"] + 2 * hfws - 1)
+    want = int((out_x[None, :] | out_y[:, None]).sum())
+    _report("out-of-buffer taps", shift=shift, got=got, want=want)
+    assert got == want
+    if max(abs(shift[0]), abs(shift[1])) < hfws - 2:
+        assert want == 0
+
+
 @pytest.mark.parametrize("ws", [(16, 16), (24, 24), (48, 48), (64, 64), (64, 32), (32, 64), (32, 16), (16, 32),
                                 (48, 24)])
 def test_fast_window_shapes(pivx, orc, ws):
