@@ -303,6 +303,61 @@ def test_two_grid_stagewise(pivx, orc):
         del fp
 
 
+def test_config2_full_size_properties(pivx):
+    """The 6600x4400 Lamb-Oseen pair (vortex core 300 px, peak swirl 8 px,
+    shear 0.002) through the whole two-grid pipeline: grid 1 64x64 at 64 x3,
+    grid 2 32x32 at 16 x3, top-4, 5x5 median test x2, fill, 3x3 smoothing.
+    Size-independent properties: the grid-2 field against the analytic flow,
+    run-to-run bit identity, and bit identity of pairs run concurrently on
+    three engine contexts (own streams and pair graphs, enqueued back to back)
+    with the one-context result."""
+    p = pivx.init_params(nx=6600, ny=4400, nsec_cc=1, ws_x1=64, ws_y1=64, spacing_x1=64, spacing_y1=64, npass1=3,
+                         ws_x2=32, ws_y2=32, spacing_x2=16, spacing_y2=16, npass2=3, npeak_save=4,
+                         max_dispx_percent1=0.4, max_dispy_percent1=0.4, max_dispx_percent2=0.4,
+                         max_dispy_percent2=0.4)
+    fp = pivx.FilterParams(npass_median=2, min_nneigh=5, residual_thr=2.0, insert_thr=2.0, npass_smooth=1)
+    sp_ = pivx.synth_params(6600, 4400, seed=2024, density=0.05, flow=1, a0=300.0, a1=8.0, a2=0.002)
+    a, b = pivx.synth_pair(sp_)
+
+    def fields(x):
+        return [x.raw(1), x.filtered(1), x.raw(2), x.filtered(2), x.filtered(3)]
+
+    with pivx.Xcorr2dMulti(p) as x:
+        x.prep_img_data(a, b)
+        x.set_graph_mode(True)
+        x.run_pair(fp, fp)
+        ref = fields(x)
+        x.run_pair(fp, fp)
+        again = fields(x)
+        gx, gy = x.grid_coords(2)
+        st = x.stats()
+        pipes = [pivx.Xcorr2dMulti(p) for _ in range(3)]
+        try:
+            for q in pipes:
+                q.prep_img_data(a, b)
+                q.set_graph_mode(True)
+            for _ in range(2):
+                for q in pipes:
+                    q.run_pair(fp, fp)
+            conc = [fields(q) for q in pipes]
+        finally:
+            for q in pipes:
+                q.close()
+    for other in [again] + conc:
+        for A, B in zip(ref, other):
+            for u, w in zip(A, B):
+                np.testing.assert_array_equal(u, w)
+    u, v = pivx.synth_flow(sp_, gx, gy)
+    fx, fy, fv = ref[3]
+    inner = (gx > 64) & (gx < 6600 - 64) & (gy > 64) & (gy < 4400 - 64)
+    rms = float(np.sqrt(np.mean((fx[inner] - u[inner]) ** 2 + (fy[inner] - v[inner]) ** 2)))
+    valid = float(np.mean(fv[inner]))
+    _report("config2 full size", rms_px=f"{rms:.4f}", valid=f"{valid:.5f}", outliers=st["outliers"],
+            overflow_windows=st["overflow_windows"])
+    assert rms < 0.06
+    assert valid > 0.999
+
+
 def test_run_pair_matches_stepwise_and_is_deterministic(pivx):
     """pivx_run_pair reproduces the reference's main-loop sequence exactly
     (same result as calling the per-pass entry points) and is bit-identical
