@@ -169,6 +169,103 @@ def test_generic_window_shapes(pivx, orc, ws, sp):
     check_pass(f"generic {ws[0]}x{ws[1]}", gpu, ref, ws, 4, min_ok=0.7)
 
 
+@pytest.mark.parametrize("ws,sp", [((32, 32), (16, 16)), ((64, 64), (32, 32)), ((48, 24), (24, 12))])
+def test_noise_windows_near_ties(pivx, orc, ws, sp):
+    """Config-1 images with 5% of the windows carrying added N(0, 20) noise in
+    both frames (near-equal secondary peaks): outside the oracle's tie flags
+    every peak index and validity flag still matches and the sub-pixel
+    displacements agree within 1e-3 px."""
+    a, b = config1_pair(pivx, seed=31)
+    p = config1_params(pivx, ws_x1=ws[0], ws_y1=ws[1], spacing_x1=sp[0], spacing_y1=sp[1])
+    rng = np.random.default_rng(32)
+    with pivx.Xcorr2dMulti(p) as x:
+        g = geom_dict(x.geometry(1))
+        nxv, nyv = g["nx_vec"], g["ny_vec"]
+        noisy = rng.random(nxv * nyv) < 0.05
+        a = a.astype(np.float64)
+        b = b.astype(np.float64)
+        for w in np.flatnonzero(noisy):
+            # window w covers image columns [c - ws/2, c + ws/2) around c = sp/2 + k*sp
+            cx = sp[0] // 2 + (w % nxv) * sp[0]
+            cy = sp[1] // 2 + (w // nxv) * sp[1]
+            ys = slice(max(cy - ws[1] // 2, 0), max(cy + ws[1] // 2, 0))
+            xs = slice(max(cx - ws[0] // 2, 0), max(cx + ws[0] // 2, 0))
+            a[ys, xs] += rng.normal(0.0, 20.0, a[ys, xs].shape)
+            b[ys, xs] += rng.normal(0.0, 20.0, b[ys, xs].shape)
+        a = np.clip(np.rint(a), 0, 255).astype(np.uint8)
+        b = np.clip(np.rint(b), 0, 255).astype(np.uint8)
+        x.prep_img_data(a, b)
+        x.Calc_cc_map_p1(0)
+        gpu = gpu_pass(x, 1)
+    ref = orc.xcorr_pass(a, f2u8=b, geom=g, ws=ws, sp=sp, K=4, tie_rel=tie_margin(*ws))
+    check_pass(f"noise windows {ws[0]}x{ws[1]} ({int(noisy.sum())} noisy)", gpu, ref, ws, 4, min_ok=0.8)
+
+
+def test_uneven_batch_nonsquare_windows(pivx, orc):
+    """Pairs of uneven size (1024x1024, 2048x1536, 1280x720) with random
+    Fourier-mode flows (max 10 px, densities 0.02-0.08) through the two-grid
+    pipeline with non-square windows: 64x32 at 32x16, then 32x16 at 16x8.
+    Each pair runs on its own engine context, all contexts enqueued back to
+    back; every result equals the same pair run alone, the grid-2 field follows
+    the analytic flow, and the first grid-1 pass of the smallest pair matches
+    the oracle."""
+    sizes = [(1024, 1024, 0.05), (2048, 1536, 0.02), (1280, 720, 0.08)]
+    fp = pivx.FilterParams(npass_median=2, min_nneigh=5, residual_thr=2.0, insert_thr=2.0, npass_smooth=1)
+
+    def params(nx, ny):
+        return config1_params(pivx, nx=nx, ny=ny, ws_x1=64, ws_y1=32, spacing_x1=32, spacing_y1=16, npass1=2,
+                              ws_x2=32, ws_y2=16, spacing_x2=16, spacing_y2=8, npass2=2,
+                              max_dispx_percent1=0.4, max_dispy_percent1=0.4, max_dispx_percent2=0.4,
+                              max_dispy_percent2=0.4)
+
+    pairs = []
+    for i, (nx, ny, dens) in enumerate(sizes):
+        sp_ = pivx.synth_params(nx, ny, seed=700 + i, density=dens, flow=2, a0=10.0)
+        pairs.append((sp_, *pivx.synth_pair(sp_)))
+
+    def fields(x):
+        return [x.raw(1), x.filtered(1), x.raw(2), x.filtered(2), x.filtered(3)]
+
+    alone = []
+    for sp_, a, b in pairs:
+        with pivx.Xcorr2dMulti(params(sp_.nx, sp_.ny)) as x:
+            x.prep_img_data(a, b)
+            x.set_graph_mode(True)
+            x.run_pair(fp, fp)
+            alone.append((fields(x), x.grid_coords(2)))
+    ctxs = [pivx.Xcorr2dMulti(params(sp_.nx, sp_.ny)) for sp_, _, _ in pairs]
+    try:
+        for x, (sp_, a, b) in zip(ctxs, pairs):
+            x.prep_img_data(a, b)
+            x.set_graph_mode(True)
+        for x in ctxs:
+            x.run_pair(fp, fp)
+        batch = [fields(x) for x in ctxs]
+    finally:
+        for x in ctxs:
+            x.close()
+    for (ref, (gx, gy)), got, (sp_, a, b) in zip(alone, batch, pairs):
+        for A, B in zip(ref, got):
+            for u, w in zip(A, B):
+                np.testing.assert_array_equal(u, w)
+        u, v = pivx.synth_flow(sp_, gx, gy)
+        fx, fy, fv = ref[3]
+        inner = (gx > 48) & (gx < sp_.nx - 48) & (gy > 48) & (gy < sp_.ny - 48)
+        rms = float(np.sqrt(np.mean((fx[inner] - u[inner]) ** 2 + (fy[inner] - v[inner]) ** 2)))
+        _report("uneven batch", size=(sp_.nx, sp_.ny), density=sp_.density, rms_px=f"{rms:.4f}",
+                valid=f"{float(np.mean(fv[inner])):.4f}")
+        assert rms < 0.25
+        assert float(np.mean(fv[inner])) > 0.99
+    sp_, a, b = pairs[2]
+    with pivx.Xcorr2dMulti(params(sp_.nx, sp_.ny)) as x:
+        x.prep_img_data(a, b)
+        x.Calc_cc_map_p1(0)
+        gpu = gpu_pass(x, 1)
+        g = geom_dict(x.geometry(1))
+    ref = orc.xcorr_pass(a, f2u8=b, geom=g, ws=(64, 32), sp=(32, 16), K=4, tie_rel=tie_margin(64, 32))
+    check_pass("uneven batch 1280x720 grid-1 pass 0 (64x32)", gpu, ref, (64, 32), 4, min_ok=0.8)
+
+
 def _random_candidates(rng, nt, K, outlier_frac=0.1):
     dx = (2.0 + 0.2 * rng.standard_normal((nt, K))).astype(np.float32)
     dy = (-1.0 + 0.2 * rng.standard_normal((nt, K))).astype(np.float32)
