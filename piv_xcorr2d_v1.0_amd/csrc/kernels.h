@@ -89,13 +89,9 @@ hipError_t launch_upsample(float* dx2, float* dy2, const float* ex1x, const floa
 hipError_t launch_median(const MedArgs& a, hipStream_t s);
 hipError_t launch_fill(const FillArgs& a, hipStream_t s);
 hipError_t launch_smooth(const SmoothArgs& a, hipStream_t s);
-hipError_t launch_zero_f32(float* p, size_t n, hipStream_t s);
-// filter := candidate 0 (median_filter_data_prep, cu:484-495); copy of a field.
+// filter := candidate 0 (median_filter_data_prep, cu:484-495).
 hipError_t launch_prep(const float* cx, const float* cy, const uint8_t* cv, int K, float* dx, float* dy, uint8_t* dv,
                        int nt, hipStream_t s);
-hipError_t launch_copy_field(const float* sx, const float* sy, const uint8_t* sv, float* dx, float* dy, uint8_t* dv,
-                             int nt, hipStream_t s);
-hipError_t launch_zero_u64(unsigned long long* p, size_t n, hipStream_t s);
 hipError_t launch_stream_copy(const void* src, void* dst, size_t bytes, hipStream_t s);
 hipError_t launch_mhe(const MheArgs& a, hipStream_t s);
 

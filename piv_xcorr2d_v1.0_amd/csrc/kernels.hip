@@ -7,16 +7,20 @@
 //   find_local_max_2d cu:585-606, reduce_max_index_general/final cu:4-172,
 //   replace_local_max_2d cu:298-307, peak_subpixel_4peak_2d cu:309-350 and
 //   kernel_remove_nan cu:457-471
-//     -> k_xcorr_fast<WX,WY,WPB> (one fused launch per pass, windows resident in
-//        LDS, register FFTs) and k_xcorr_generic (any legal size, LDS or global
-//        planes).
+//     -> one fused launch per pass, register FFTs: k_xcorr_pair<WX,WY> (two
+//        windows per wave, shapes up to 32x32), k_xcorr_single<WX,WY> (one
+//        window per wave, up to 64x64), k_xcorr_fast<WX,WY,WPB> (16x16), and
+//        k_xcorr_generic (any legal size, LDS or global planes).
 //   Kernel_Window_Deformation_2d cu:608-652           -> k_deform
 //   extend_filtered_vecfield_p1/p2 cu:1547-1618       -> k_extend
 //   Upsampling_vecfield_kernel_2d cu:675-704          -> k_upsample
 //   median_filter_data_prep, organize_median_filter_data, median_validity,
-//   Calc_median, Calc_devi, median_filtering cu:174-296,484-495 -> k_median
-//   fill_blank_vecfield(_update) + host loop cu:352-397,1514-1528 -> k_fill
+//   Calc_median, Calc_devi, median_filtering cu:174-296,484-495 -> k_prep,
+//     k_median2<WX,WY> (3x3, 5x5, 7x7), k_median<NS> / k_median_generic
+//   fill_blank_vecfield(_update) + host loop cu:352-397,1514-1528 -> k_fill_list,
+//     k_fill
 //   organize_smooth_filter_data + smoothing_tophat cu:399-443 -> k_smooth
+//   MHE enhancement (MHE7_4_gpu.cu)                  -> k_mhe_hist, k_mhe_map
 #include "kernels.h"
 #include "fft_reg.h"
 #include "sortnet.h"
@@ -1667,9 +1671,6 @@ __global__ void k_smooth(SmoothArgs a) {
   }
 }
 
-__global__ void k_zero_f32(float* p, size_t n) {
-  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) p[t] = 0.f;
-}
 __global__ void k_prep(const float* cx, const float* cy, const uint8_t* cv, int K, float* dx, float* dy, uint8_t* dv,
                        int nt) {
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nt; t += gridDim.x * blockDim.x) {
@@ -1679,23 +1680,12 @@ __global__ void k_prep(const float* cx, const float* cy, const uint8_t* cv, int 
     dy[t] = cy[o];
   }
 }
-__global__ void k_copy_field(const float* sx, const float* sy, const uint8_t* sv, float* dx, float* dy, uint8_t* dv,
-                             int nt) {
-  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nt; t += gridDim.x * blockDim.x) {
-    dx[t] = sx[t];
-    dy[t] = sy[t];
-    if (sv) dv[t] = sv[t];
-  }
-}
 // Streaming copy used to measure the achievable HBM bandwidth and to calibrate
 // the FETCH_SIZE / WRITE_SIZE counters against a known byte count.
 __global__ void __launch_bounds__(256) k_stream_copy(const float4* __restrict__ src, float4* __restrict__ dst,
                                                      size_t n4) {
   for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n4; t += (size_t)gridDim.x * blockDim.x)
     dst[t] = src[t];
-}
-__global__ void k_zero_u64(unsigned long long* p, size_t n) {
-  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) p[t] = 0;
 }
 
 // ============================================================== MHE enhancement
@@ -2016,28 +2006,15 @@ hipError_t launch_smooth(const SmoothArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t launch_zero_f32(float* p, size_t n, hipStream_t s) {
-  hipLaunchKernelGGL(k_zero_f32, dim3(grid_for((long long)n, 256)), dim3(256), 0, s, p, n);
-  return hipGetLastError();
-}
 hipError_t launch_prep(const float* cx, const float* cy, const uint8_t* cv, int K, float* dx, float* dy, uint8_t* dv,
                        int nt, hipStream_t s) {
   hipLaunchKernelGGL(k_prep, dim3(grid_for(nt, 256)), dim3(256), 0, s, cx, cy, cv, K, dx, dy, dv, nt);
-  return hipGetLastError();
-}
-hipError_t launch_copy_field(const float* sx, const float* sy, const uint8_t* sv, float* dx, float* dy, uint8_t* dv,
-                             int nt, hipStream_t s) {
-  hipLaunchKernelGGL(k_copy_field, dim3(grid_for(nt, 256)), dim3(256), 0, s, sx, sy, sv, dx, dy, dv, nt);
   return hipGetLastError();
 }
 hipError_t launch_stream_copy(const void* src, void* dst, size_t bytes, hipStream_t s) {
   const size_t n4 = bytes / 16;
   hipLaunchKernelGGL(k_stream_copy, dim3(grid_for((long long)n4, 256, 256 * 64)), dim3(256), 0, s,
                      (const float4*)src, (float4*)dst, n4);
-  return hipGetLastError();
-}
-hipError_t launch_zero_u64(unsigned long long* p, size_t n, hipStream_t s) {
-  hipLaunchKernelGGL(k_zero_u64, dim3(grid_for((long long)n, 256)), dim3(256), 0, s, p, n);
   return hipGetLastError();
 }
 hipError_t launch_mhe(const MheArgs& a, hipStream_t s) {
