@@ -1118,10 +1118,16 @@ __global__ void __launch_bounds__(256) k_xcorr_generic(XcArgs a, int WX, int WY,
 // (cell row, fraction, 1 - fraction) are computed once per workgroup into
 // LDS and the column terms once per thread, with the same operations as the
 // per-pixel formula; the four field samples of a cell are reloaded only when
-// the row crosses into the next cell.  The four frame taps are loaded
-// unconditionally from clamped in-frame addresses (32-bit offsets from the
-// frame base) and zeroed when outside, so a pixel waits on one round trip.
+// the row crosses into the next cell.  Frame 2 is read through a buffer
+// resource that starts one byte before the frame and ends with it: a tap
+// outside the frame gets an offset beyond that span, which the hardware
+// returns as 0 without a memory access, so the four taps are unconditional
+// loads and a pixel waits on one round trip.  The range check covers the
+// vector offset only, so taps 2 and 4 (x + 1) share the vector offset of
+// taps 1 and 3 and add 1 as the scalar offset; with the base one byte early
+// every in-frame tap's vector offset is >= 0.
 constexpr int DEF_ROWS = 16;
+constexpr unsigned DEF_TAP_OUT = 0xffffff00u;  // beyond every frame (frames hold < 2^31 bytes)
 
 __global__ void __launch_bounds__(256) k_deform(DefArgs a) {
 #pragma clang fp contract(off)
@@ -1155,6 +1161,9 @@ __global__ void __launch_bounds__(256) k_deform(DefArgs a) {
   bool may_oob = true;
   const int nxm = a.nx - 1, nym = a.ny - 1;
   const unsigned pitch = (unsigned)a.pitch;
+  // byte b of the resource is frame byte b - 1 (byte 0 is never read)
+  const __amdgpu_buffer_rsrc_t f2r = __builtin_amdgcn_make_buffer_rsrc(const_cast<uint8_t*>(a.f2) - 1, (short)0,
+                                                                       (int)(pitch * (unsigned)a.ny + 1u), 0x00020000);
   const unsigned span_x = (unsigned)(a.nx_aug + 2 * a.hfws_x - 1), span_y = (unsigned)(a.ny_aug + 2 * a.hfws_y - 1);
   float* out = a.out + (long long)y0 * a.nx_aug + idx;
   for (int r = 0; r < nrow; ++r, out += a.nx_aug) {
@@ -1180,9 +1189,12 @@ __global__ void __launch_bounds__(256) k_deform(DefArgs a) {
     }
     float sx = fmaf(ce4, ex4, fmaf(ce3, ex3, fmaf(ce2, ex2, fmaf(ce1, ex1, fidx))));
     float sy = fmaf(ce4, ey4, fmaf(ce3, ey3, fmaf(ce2, ey2, fmaf(ce1, ey1, fidy))));
-    const int jx = (int)floorf(sx), jy = (int)floorf(sy);
-    sx = sx - (float)jx;
-    sy = sy - (float)jy;
+    // (float)(int)floor(s) == floor(s) for |s| < 2^31, so the fraction
+    // s - (float)(int)floor(s) of cu:637-638 is s - floor(s)
+    const float fx = floorf(sx), fy = floorf(sy);
+    const int jx = (int)fx, jy = (int)fy;
+    sx = sx - fx;
+    sy = sy - fy;
     ce1 = (1 - sx) * (1 - sy);
     ce2 = sx * (1 - sy);
     ce3 = (1 - sx) * sy;
@@ -1190,15 +1202,16 @@ __global__ void __launch_bounds__(256) k_deform(DefArgs a) {
     const int xi = jx - a.left_x, yi = jy - a.left_y;
     const bool x0 = (unsigned)xi <= (unsigned)nxm, x1 = (unsigned)(xi + 1) <= (unsigned)nxm;
     const bool yok0 = (unsigned)yi <= (unsigned)nym, yok1 = (unsigned)(yi + 1) <= (unsigned)nym;
-    // taps outside the frame load pixel 0 and are zeroed below
-    const unsigned cx0 = x0 ? (unsigned)xi : 0u, cx1 = x1 ? (unsigned)(xi + 1) : 0u;
-    // 24-bit products: frames hold fewer than 2^31 pixels and rows fewer than 2^24
-    const unsigned r0 = __umul24(yok0 ? (unsigned)yi : 0u, pitch);
-    const unsigned r1 = __umul24(yok1 ? (unsigned)(yi + 1) : 0u, pitch);
-    const float v1 = (float)a.f2[r0 + cx0], v2 = (float)a.f2[r0 + cx1];
-    const float v3 = (float)a.f2[r1 + cx0], v4 = (float)a.f2[r1 + cx1];
-    const float t1 = (yok0 && x0) ? v1 : 0.f, t2 = (yok0 && x1) ? v2 : 0.f;
-    const float t3 = (yok1 && x0) ? v3 : 0.f, t4 = (yok1 && x1) ? v4 : 0.f;
+    // resource offsets (frame offset + 1) of taps 3 and 1; exact whenever the
+    // tap or its x + 1 neighbour is inside the frame (24-bit product: rows
+    // and row counts are below 2^24)
+    const unsigned o3 = __umul24((unsigned)(yi + 1), pitch) + (unsigned)(xi + 1), o1 = o3 - pitch;
+    const unsigned q1 = (yok0 && x0) ? o1 : DEF_TAP_OUT, q2 = (yok0 && x1) ? o1 : DEF_TAP_OUT;
+    const unsigned q3 = (yok1 && x0) ? o3 : DEF_TAP_OUT, q4 = (yok1 && x1) ? o3 : DEF_TAP_OUT;
+    const float t1 = (float)__builtin_amdgcn_raw_buffer_load_b8(f2r, q1, 0, 0);
+    const float t2 = (float)__builtin_amdgcn_raw_buffer_load_b8(f2r, q2, 1, 0);
+    const float t3 = (float)__builtin_amdgcn_raw_buffer_load_b8(f2r, q3, 0, 0);
+    const float t4 = (float)__builtin_amdgcn_raw_buffer_load_b8(f2r, q4, 1, 0);
     // taps outside the reference's aug_ex buffer ([-hfws, n_aug + hfws) in aug coordinates)
     if (__builtin_expect(may_oob, 0))
       oob += ((unsigned)(jx + a.hfws_x) >= span_x || (unsigned)(jy + a.hfws_y) >= span_y) ? 1u : 0u;
