@@ -48,6 +48,37 @@ __device__ __forceinline__ bool peak_before(float v, int i, float bv, int bi) {
   return v > bv || (v == bv && i < bi);
 }
 
+// The same order as one unsigned 64-bit key: a peak value (>= 0, whose bits
+// order as unsigned integers) above the complemented pixel index, so that the
+// larger key is the earlier peak.  Key 0 stands for "no peak".
+__device__ __forceinline__ uint64_t peak_key(float v, int i) {
+  return ((uint64_t)__float_as_uint(v) << 32) | (uint32_t)~i;
+}
+__device__ __forceinline__ float key_value(uint64_t k) { return __uint_as_float((uint32_t)(k >> 32)); }
+__device__ __forceinline__ int key_index(uint64_t k) { return (int)~(uint32_t)k; }
+
+// Max of a key over the W lanes of each aligned W-lane group.
+template <int W>
+__device__ __forceinline__ uint64_t group_max_key(uint64_t k) {
+#pragma unroll
+  for (int off = W / 2; off > 0; off >>= 1) {
+    const uint32_t lo = __shfl_xor((uint32_t)k, off, W), hi = __shfl_xor((uint32_t)(k >> 32), off, W);
+    const uint64_t o = ((uint64_t)hi << 32) | lo;
+    k = o > k ? o : k;
+  }
+  return k;
+}
+
+// Rows lo..hi of a column as a bit mask, where lo = max(by, 1) and
+// hi = min(WY - 1 - by, WY - 2): the rows of the 4-neighbour walk inside the
+// search border (cu:596).
+template <typename Bits, int WY>
+__device__ __forceinline__ Bits search_rows(int by) {
+  const int lo = max(by, 1), hi = min(WY - 1 - by, WY - 2);
+  if (hi < lo) return 0;
+  return (((Bits)2 << hi) - 1) & ~(((Bits)1 << lo) - 1);
+}
+
 __device__ __forceinline__ void wave_argmax(float& bv, int& bi) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
@@ -355,15 +386,6 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-__device__ __forceinline__ void half_argmax(float& bv, int& bi) {
-#pragma unroll
-  for (int off = 16; off > 0; off >>= 1) {
-    float ov = __shfl_xor(bv, off, 32);
-    int oi = __shfl_xor(bi, off, 32);
-    if (peak_before(ov, oi, bv, bi)) { bv = ov; bi = oi; }
-  }
-}
-
 __device__ __forceinline__ int half_sum(int v) {
 #pragma unroll
   for (int off = 16; off > 0; off >>= 1) v += __shfl_xor(v, off, 32);
@@ -595,23 +617,29 @@ __global__ void __launch_bounds__(64) k_xcorr_pair(XcArgs a) {
   const bool col_ok = l < WX && l >= a.bx && l <= WX - 1 - a.bx;
   const int cl = l < WX ? l : WX - 1;
   const int lft = cl > 0 ? cl - 1 : 0, rgt = cl < WX - 1 ? cl + 1 : WX - 1;
+  // the search border (rows from a uniform mask, then the lane's column)
+  const uint32_t rows = search_rows<uint32_t, WY>(a.by);
+  const uint32_t keep = col_ok ? rows : 0u;
   uint32_t bits = 0;
   {
+    // c0 > 0 and c0 >= each neighbour (cu:597) as c0 >= max(up, dn, lv, rv,
+    // smallest positive float), the max taken over the bit patterns: the
+    // plane's values are all >= 0 (which order as their bits), or all NaN
+    // when a frame-2 value is NaN, where neither form finds a maximum
     float up = cc[cl];
     float cur = cc[SX + cl];
     static_for<WY - 2>([&](auto jc) {
       constexpr int j = jc + 1;
       const float dn = cc[(j + 1) * SX + cl];
-      const float c0 = cur;
       const float lv = cc[j * SX + lft], rv = cc[j * SX + rgt];
-      // non-short-circuit form keeps the walk branch-free
-      const bool is_max = col_ok & (j >= a.by) & (j <= WY - 1 - a.by) & (c0 > 0.f) & (c0 >= up) & (c0 >= dn) &
-                          (c0 >= lv) & (c0 >= rv);
-      bits |= (uint32_t)is_max << j;
+      const uint32_t m = max(max(max(__float_as_uint(up), __float_as_uint(dn)), __float_as_uint(lv)),
+                             max(__float_as_uint(rv), 1u));
+      bits |= (uint32_t)(cur >= __uint_as_float(m)) << j;
       up = cur;
       cur = dn;
     });
   }
+  bits &= keep;
   const int n = half_sum(__popc(bits));
   const int w = iy * a.nx_vec + ix;
   if (win_ok && l == 0) {
@@ -620,21 +648,16 @@ __global__ void __launch_bounds__(64) k_xcorr_pair(XcArgs a) {
   }
   const float px = (win_ok && a.pred_x) ? a.pred_x[w] : 0.f;
   const float py = (win_ok && a.pred_y) ? a.pred_y[w] : 0.f;
-  // lane best over its column's maxima (rows ascend, so the strict compare
-  // keeps the lower raster index on ties); the owner of a round's winner
-  // clears that row and refreshes
-  float mv;
-  int mi;
+  // lane best over its column's maxima as a peak key (every maximum is > 0,
+  // so key 0 is "none"); the owner of a round's winner clears that row and
+  // refreshes
+  uint64_t mk;
   auto refresh = [&]() {
-    mv = 0.f;
-    mi = 0x7fffffff;
+    mk = 0;
     for (uint32_t rb = bits; rb; rb &= rb - 1) {
       const int j = __builtin_ctz(rb);
-      const float val = cc[j * SX + cl];
-      if (val > mv) {
-        mv = val;
-        mi = j * WX + cl;
-      }
+      const uint64_t kj = peak_key(cc[j * SX + cl], j * WX + cl);
+      mk = kj > mk ? kj : mk;
     }
   };
   refresh();
@@ -643,16 +666,14 @@ __global__ void __launch_bounds__(64) k_xcorr_pair(XcArgs a) {
     float my_v = 0.f;
     int my_i = -1;
     for (int k = 0; k < kn; ++k) {
-      float bv = mv;
-      int bi = mi;
-      half_argmax(bv, bi);
-      if (bv > 0.f && bi == mi) {
-        bits &= ~((uint32_t)1 << (mi / WX));
+      const uint64_t best = group_max_key<32>(mk);
+      if (best != 0 && best == mk) {
+        bits &= ~((uint32_t)1 << (key_index(mk) / WX));
         refresh();
       }
-      if (l == k) {
-        my_v = bv;
-        my_i = bv > 0.f ? bi : -1;
+      if (l == k && best != 0) {
+        my_v = key_value(best);
+        my_i = key_index(best);
       }
     }
     if (win_ok && l < kn) {
@@ -848,23 +869,29 @@ __global__ void __launch_bounds__(64) k_xcorr_single(XcArgs a) {
   const bool col_ok = lane < WX && lane >= a.bx && lane <= WX - 1 - a.bx;
   const int cl = lane < WX ? lane : WX - 1;
   const int lft = cl > 0 ? cl - 1 : 0, rgt = cl < WX - 1 ? cl + 1 : WX - 1;
+  // the search border (rows from a uniform mask, then the lane's column)
+  const Bits rows = search_rows<Bits, WY>(a.by);
+  const Bits keep = col_ok ? rows : (Bits)0;
   Bits bits = 0;
   {
+    // c0 > 0 and c0 >= each neighbour (cu:597) as c0 >= max(up, dn, lv, rv,
+    // smallest positive float), the max taken over the bit patterns: the
+    // plane's values are all >= 0 (which order as their bits), or all NaN
+    // when a frame-2 value is NaN, where neither form finds a maximum
     float up = cc[cl];
     float cur = cc[SX + cl];
     static_for<WY - 2>([&](auto jc) {
       constexpr int j = jc + 1;
       const float dn = cc[(j + 1) * SX + cl];
-      const float c0 = cur;
       const float lv = cc[j * SX + lft], rv = cc[j * SX + rgt];
-      // non-short-circuit form keeps the walk branch-free
-      const bool is_max = col_ok & (j >= a.by) & (j <= WY - 1 - a.by) & (c0 > 0.f) & (c0 >= up) & (c0 >= dn) &
-                          (c0 >= lv) & (c0 >= rv);
-      bits |= (Bits)is_max << j;
+      const uint32_t m = max(max(max(__float_as_uint(up), __float_as_uint(dn)), __float_as_uint(lv)),
+                             max(__float_as_uint(rv), 1u));
+      bits |= (Bits)(cur >= __uint_as_float(m)) << j;
       up = cur;
       cur = dn;
     });
   }
+  bits &= keep;
   const int n = wave_sum(__popcll((unsigned long long)bits));
   if (lane == 0) {
     a.ncand[w] = n;
@@ -872,21 +899,16 @@ __global__ void __launch_bounds__(64) k_xcorr_single(XcArgs a) {
   }
   const float px = a.pred_x ? a.pred_x[w] : 0.f;
   const float py = a.pred_y ? a.pred_y[w] : 0.f;
-  // lane best over its column's maxima (rows ascend, so the strict compare
-  // keeps the lower raster index on ties); the owner of a round's winner
-  // clears that row and refreshes
-  float mv;
-  int mi;
+  // lane best over its column's maxima as a peak key (every maximum is > 0,
+  // so key 0 is "none"); the owner of a round's winner clears that row and
+  // refreshes
+  uint64_t mk;
   auto refresh = [&]() {
-    mv = 0.f;
-    mi = 0x7fffffff;
+    mk = 0;
     for (Bits rb = bits; rb; rb &= rb - 1) {
       const int j = __builtin_ctzll(rb);
-      const float val = cc[j * SX + cl];
-      if (val > mv) {
-        mv = val;
-        mi = j * WX + cl;
-      }
+      const uint64_t kj = peak_key(cc[j * SX + cl], j * WX + cl);
+      mk = kj > mk ? kj : mk;
     }
   };
   refresh();
@@ -895,16 +917,14 @@ __global__ void __launch_bounds__(64) k_xcorr_single(XcArgs a) {
     float my_v = 0.f;
     int my_i = -1;
     for (int k = 0; k < kn; ++k) {
-      float bv = mv;
-      int bi = mi;
-      wave_argmax(bv, bi);
-      if (bv > 0.f && bi == mi) {
-        bits &= ~((Bits)1 << (mi / WX));
+      const uint64_t best = group_max_key<64>(mk);
+      if (best != 0 && best == mk) {
+        bits &= ~((Bits)1 << (key_index(mk) / WX));
         refresh();
       }
-      if (lane == k) {
-        my_v = bv;
-        my_i = bv > 0.f ? bi : -1;
+      if (lane == k && best != 0) {
+        my_v = key_value(best);
+        my_i = key_index(best);
       }
     }
     if (lane < kn) {
