@@ -79,6 +79,53 @@ __device__ __forceinline__ Bits search_rows(int by) {
   return (((Bits)2 << hi) - 1) & ~(((Bits)1 << lo) - 1);
 }
 
+// 2 * w + (cond ? 1 : 0) as one add with carry whose carry-in is the lane
+// mask of cond.
+__device__ __forceinline__ uint32_t shift_in(uint32_t w, bool cond) {
+  const uint64_t m = __builtin_amdgcn_ballot_w64(cond);
+  uint32_t r;
+  uint64_t carry_out;
+  asm("v_addc_co_u32_e64 %0, %1, %2, %2, %3" : "=v"(r), "=s"(carry_out) : "v"(w), "s"(m));
+  return r;
+}
+
+// Rows 1..WY-2 of column cl of a correlation plane cc (row stride SX) that are
+// 4-neighbour maxima with a positive value, as a bit mask (bit j = row j).
+// "c0 > 0 and c0 >= each neighbour" (cu:597) is c0 >= max(up, dn, lv, rv,
+// smallest positive float) with the max taken over the bit patterns: a
+// window's plane holds values >= 0 (which order as their bits), or NaN
+// throughout when a frame-2 value is NaN, where neither form finds a maximum.
+// The walk runs upwards and shifts each row's flag in at bit 0, rows from 32
+// on into the high word.
+template <int WY, int SX, typename Bits>
+__device__ __forceinline__ Bits column_maxima(const float* cc, int cl, int lft, int rgt) {
+  static_assert(WY <= 8 * (int)sizeof(Bits), "one bit per row");
+  uint32_t lo = 0, hi = 0;
+  float dn = cc[(WY - 1) * SX + cl];
+  float cur = cc[(WY - 2) * SX + cl];
+  static_for<WY - 2>([&](auto jc) {
+    constexpr int j = WY - 2 - jc;  // WY-2 down to 1
+    const float up = cc[(j - 1) * SX + cl];
+    const float lv = cc[j * SX + lft], rv = cc[j * SX + rgt];
+    const uint32_t m = max(max(max(__float_as_uint(up), __float_as_uint(dn)), __float_as_uint(lv)),
+                           max(__float_as_uint(rv), 1u));
+    const bool is_max = cur >= __uint_as_float(m);
+    if constexpr (j >= 32) {
+      hi = shift_in(hi, is_max);  // row j at bit j - 32
+    } else {
+      lo = shift_in(lo, is_max);  // row j at bit j - 1
+    }
+    dn = cur;
+    cur = up;
+  });
+  lo <<= 1;
+  if constexpr (sizeof(Bits) == 8) {
+    return ((Bits)hi << 32) | lo;
+  } else {
+    return lo;
+  }
+}
+
 __device__ __forceinline__ void wave_argmax(float& bv, int& bi) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
@@ -507,11 +554,9 @@ __global__ void __launch_bounds__(64) k_xcorr_pair(XcArgs a) {
   {
     float2 x[WX];
     if (l < WY) {
-      if (win_ok) {
-        load_row<WX>(a, ix * a.sp_x, iy * a.sp_y + l, x);
-      } else {
-        static_for<WX>([&](auto c) { x[c] = make_float2(0.f, 0.f); });
-      }
+      // the unused second window of an odd row loads its neighbour (its
+      // results are never stored)
+      load_row<WX>(a, (win_ok ? ix : ix - 1) * a.sp_x, iy * a.sp_y + l, x);
       fft_reg<WX, false>(x);
       static_for<WX>([&](auto c) { pr[l * SX + c] = x[c].x; });
     }
@@ -599,14 +644,14 @@ __global__ void __launch_bounds__(64) k_xcorr_pair(XcArgs a) {
     static_for<HX>([&](auto k) { p[k] = xb[((1 - h) * HX + k) * WY + l]; });
     wave_sync();  // the correlation planes overwrite xb
     const float sh = h ? -1.f : 1.f;
-    const int rs = (l + HY) % WY;
+    // output row (l + WY/2) % WY, columns from h ? 0 : WX/2
+    float* const orow = cc0 + ((l + HY) % WY) * SX + (h ? 0 : HX);
     static_for<HX>([&](auto kc) {
       constexpr int k = kc;
       // h = 0: E' + w^k O';  h = 1: E' - w^k O'
       const float re = fmaf(sh, r[k].x, p[k].x), im = fmaf(sh, r[k].y, p[k].y);
-      const int cs = h ? k : k + HX;
-      cc0[rs * SX + cs] = cc_scale<NPIX>(re);
-      cc0[WY * SX + rs * SX + cs] = cc_scale<NPIX>(im);
+      orow[k] = cc_scale<NPIX>(re);
+      orow[WY * SX + k] = cc_scale<NPIX>(im);
     });
   }
   wave_sync();
@@ -620,26 +665,7 @@ __global__ void __launch_bounds__(64) k_xcorr_pair(XcArgs a) {
   // the search border (rows from a uniform mask, then the lane's column)
   const uint32_t rows = search_rows<uint32_t, WY>(a.by);
   const uint32_t keep = col_ok ? rows : 0u;
-  uint32_t bits = 0;
-  {
-    // c0 > 0 and c0 >= each neighbour (cu:597) as c0 >= max(up, dn, lv, rv,
-    // smallest positive float), the max taken over the bit patterns: the
-    // plane's values are all >= 0 (which order as their bits), or all NaN
-    // when a frame-2 value is NaN, where neither form finds a maximum
-    float up = cc[cl];
-    float cur = cc[SX + cl];
-    static_for<WY - 2>([&](auto jc) {
-      constexpr int j = jc + 1;
-      const float dn = cc[(j + 1) * SX + cl];
-      const float lv = cc[j * SX + lft], rv = cc[j * SX + rgt];
-      const uint32_t m = max(max(max(__float_as_uint(up), __float_as_uint(dn)), __float_as_uint(lv)),
-                             max(__float_as_uint(rv), 1u));
-      bits |= (uint32_t)(cur >= __uint_as_float(m)) << j;
-      up = cur;
-      cur = dn;
-    });
-  }
-  bits &= keep;
+  uint32_t bits = column_maxima<WY, SX, uint32_t>(cc, cl, lft, rgt) & keep;
   const int n = half_sum(__popc(bits));
   const int w = iy * a.nx_vec + ix;
   if (win_ok && l == 0) {
@@ -853,12 +879,13 @@ __global__ void __launch_bounds__(64) k_xcorr_single(XcArgs a) {
     static_for<HX>([&](auto k) { p[k] = xb2[((1 - h5) * HX + k) * HY + y5]; });
     wave_sync();
     const float sh = h5 ? -1.f : 1.f;
+    // output rows y5 (Im) and y5 + WY/2 (Re), columns from h5 ? 0 : WX/2
+    float* const orow = ccp + y5 * SX + (h5 ? 0 : HX);
     static_for<HX>([&](auto kc) {
       constexpr int k = kc;
       const float re = fmaf(sh, hr[k].x, p[k].x), im = fmaf(sh, hr[k].y, p[k].y);
-      const int xs = h5 ? k : k + HX;
-      ccp[(y5 + HY) * SX + xs] = cc_scale<NPIX>(re);
-      ccp[y5 * SX + xs] = cc_scale<NPIX>(im);
+      orow[HY * SX + k] = cc_scale<NPIX>(re);
+      orow[k] = cc_scale<NPIX>(im);
     });
   }
   wave_sync();
@@ -872,26 +899,7 @@ __global__ void __launch_bounds__(64) k_xcorr_single(XcArgs a) {
   // the search border (rows from a uniform mask, then the lane's column)
   const Bits rows = search_rows<Bits, WY>(a.by);
   const Bits keep = col_ok ? rows : (Bits)0;
-  Bits bits = 0;
-  {
-    // c0 > 0 and c0 >= each neighbour (cu:597) as c0 >= max(up, dn, lv, rv,
-    // smallest positive float), the max taken over the bit patterns: the
-    // plane's values are all >= 0 (which order as their bits), or all NaN
-    // when a frame-2 value is NaN, where neither form finds a maximum
-    float up = cc[cl];
-    float cur = cc[SX + cl];
-    static_for<WY - 2>([&](auto jc) {
-      constexpr int j = jc + 1;
-      const float dn = cc[(j + 1) * SX + cl];
-      const float lv = cc[j * SX + lft], rv = cc[j * SX + rgt];
-      const uint32_t m = max(max(max(__float_as_uint(up), __float_as_uint(dn)), __float_as_uint(lv)),
-                             max(__float_as_uint(rv), 1u));
-      bits |= (Bits)(cur >= __uint_as_float(m)) << j;
-      up = cur;
-      cur = dn;
-    });
-  }
-  bits &= keep;
+  Bits bits = column_maxima<WY, SX, Bits>(cc, cl, lft, rgt) & keep;
   const int n = wave_sum(__popcll((unsigned long long)bits));
   if (lane == 0) {
     a.ncand[w] = n;
