@@ -1155,7 +1155,9 @@ __global__ void __launch_bounds__(256) k_xcorr_generic(XcArgs a, int WX, int WY,
 // taps 1 and 3 and add 1 as the scalar offset; with the base one byte early
 // every in-frame tap's vector offset is >= 0.
 constexpr int DEF_ROWS = 16;
-constexpr unsigned DEF_TAP_OUT = 0xffffff00u;  // beyond every frame (frames hold < 2^31 bytes)
+// an offset beyond every frame (frames hold < 2^31 bytes), also after the +1
+// of taps 2 and 4; the bits of -2.0f, an inline constant of the select
+constexpr unsigned DEF_TAP_OUT = 0xc0000000u;
 
 __global__ void __launch_bounds__(256) k_deform(DefArgs a) {
 #pragma clang fp contract(off)
@@ -1193,9 +1195,17 @@ __global__ void __launch_bounds__(256) k_deform(DefArgs a) {
   const __amdgpu_buffer_rsrc_t f2r = __builtin_amdgcn_make_buffer_rsrc(const_cast<uint8_t*>(a.f2) - 1, (short)0,
                                                                        (int)(pitch * (unsigned)a.ny + 1u), 0x00020000);
   const unsigned span_x = (unsigned)(a.nx_aug + 2 * a.hfws_x - 1), span_y = (unsigned)(a.ny_aug + 2 * a.hfws_y - 1);
-  float* out = a.out + (long long)y0 * a.nx_aug + idx;
-  for (int r = 0; r < nrow; ++r, out += a.nx_aug) {
-    const float4 rt = rowt[r];
+  // this block's output rows as a buffer resource: row r of pixel idx at
+  // byte r * nx_aug * 4 (scalar offset) + idx * 4
+  const unsigned row_bytes = (unsigned)a.nx_aug * 4u;
+  const __amdgpu_buffer_rsrc_t outr = __builtin_amdgcn_make_buffer_rsrc(a.out + (long long)y0 * a.nx_aug, (short)0,
+                                                                        (int)(row_bytes * (unsigned)nrow), 0x00020000);
+  // row terms read at constant offsets from one base register
+  int rbase = 0;
+  asm volatile("" : "+v"(rbase));
+  const float4* const rowp = reinterpret_cast<const float4*>(reinterpret_cast<const char*>(rowt) + rbase);
+  auto row = [&](int r) {
+    const float4 rt = rowp[r];
     const float ay = rt.x, ay1 = rt.y;
     const int iyf = __builtin_amdgcn_readfirstlane(__float_as_int(rt.z));
     const float fidy = rt.w;
@@ -1243,7 +1253,14 @@ __global__ void __launch_bounds__(256) k_deform(DefArgs a) {
     // taps outside the reference's aug_ex buffer ([-hfws, n_aug + hfws) in aug coordinates)
     if (__builtin_expect(may_oob, 0))
       oob += ((unsigned)(jx + a.hfws_x) >= span_x || (unsigned)(jy + a.hfws_y) >= span_y) ? 1u : 0u;
-    *out = fmaf(ce4, t4, fmaf(ce3, t3, fmaf(ce1, t1, ce2 * t2)));
+    const float v = fmaf(ce4, t4, fmaf(ce3, t3, fmaf(ce1, t1, ce2 * t2)));
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), outr, (int)(idx * 4), (int)(r * row_bytes), 0);
+  };
+  // full blocks unrolled (row terms at constant LDS offsets)
+  if (nrow == DEF_ROWS) {
+    static_for<DEF_ROWS>([&](auto rc) { row(rc); });
+  } else {
+    for (int r = 0; r < nrow; ++r) row(r);
   }
   if (oob) atomicAdd(a.oob, (unsigned long long)oob);
 }
