@@ -152,6 +152,34 @@ def test_fast_window_shapes(pivx, orc, ws):
     check_pass(f"fast {ws[0]}x{ws[1]}", gpu, ref, ws, 4, min_ok=0.8)
 
 
+@pytest.mark.parametrize("ws,pct", [((32, 32), (0.15, 0.3)), ((64, 64), (0.3, 0.1)), ((48, 48), (0.05, 0.45)),
+                                    ((32, 32), (0.0, 0.4)), ((64, 32), (0.45, 0.0))])
+def test_search_border_shapes(pivx, orc, ws, pct):
+    """Peak search borders other than the default (cu:596): wide and
+    asymmetric ones, rows crossing row 32 of a 64-row window, and an empty
+    search region (border ws/2: no candidates anywhere).  Candidate counts
+    agree in 98 % of the windows outside ties, as in the default-border test
+    (a minor maximum whose neighbour is within rounding of it may count on
+    one side only), and are 0 everywhere for an empty region."""
+    a, b = config1_pair(pivx, seed=17, nx=384, ny=320, u=1.7, v=-2.4)
+    sp = (ws[0] // 2, ws[1] // 2)
+    p = config1_params(pivx, nx=384, ny=320, ws_x1=ws[0], ws_y1=ws[1], spacing_x1=sp[0], spacing_y1=sp[1],
+                       max_dispx_percent1=pct[0], max_dispy_percent1=pct[1])
+    with pivx.Xcorr2dMulti(p) as x:
+        x.prep_img_data(a, b)
+        x.Calc_cc_map_p1(0)
+        gpu = gpu_pass(x, 1)
+        g = geom_dict(x.geometry(1))
+    assert (g["border_x"], g["border_y"]) == (max(1, ws[0] // 2 - round(ws[0] * pct[0])),
+                                              max(1, ws[1] // 2 - round(ws[1] * pct[1])))
+    ref = orc.xcorr_pass(a, f2u8=b, geom=g, ws=ws, sp=sp, K=4, tie_rel=tie_margin(*ws))
+    check_pass(f"border {g['border_x']},{g['border_y']} at {ws[0]}x{ws[1]}", gpu, ref, ws, 4, min_ok=0.8)
+    ok = ref["tie"] == 0
+    assert (gpu["nc"][ok] == ref["ncand"][ok]).mean() > 0.98
+    if g["border_x"] * 2 >= ws[0] or g["border_y"] * 2 >= ws[1]:
+        assert int(np.asarray(gpu["nc"]).sum()) == 0 and (gpu["idx"] == -1).all()
+
+
 @pytest.mark.parametrize("ws,sp", [((20, 12), (10, 6)), ((36, 28), (18, 14)), ((44, 44), (22, 22)),
                                    ((128, 128), (64, 64)), ((8, 8), (4, 4))])
 def test_generic_window_shapes(pivx, orc, ws, sp):
