@@ -200,6 +200,19 @@ int pivx_run_pair(pivx_ctx* ctx, const pivx_filter_params* grid1, const pivx_fil
  * identical to the eager sequence. */
 int pivx_set_graph_mode(pivx_ctx* ctx, int on);
 
+/* Fused deformation (no reference counterpart).  Where the grid-1 windows do
+ * not overlap (spacing >= window) and take the wave-per-window correlation
+ * kernel (one side above 32, both at most 64), frame 2 can be deformed inside
+ * the correlation kernel's gather instead of into the deformed plane
+ * (Kernel_Window_Deformation_2d, cu:608-652, then a read of the plane); the
+ * correlation results are bit-identical, and the out-of-buffer tap count
+ * (pivx_stats) then covers the window pixels only (aug pixels outside every
+ * window reach no result and are not deformed).  mode 0:
+ * never; 1 (default): inside pivx_run_pair only; 2: also in
+ * pivx_calc_cc_map_p1.  A fused pass leaves the deformed plane of grid 1
+ * (pivx_get_deformed) as it was. */
+int pivx_set_fused_deform(pivx_ctx* ctx, int mode);
+
 /* Wait for all work of the handle. */
 int pivx_synchronize(pivx_ctx* ctx);
 
@@ -221,7 +234,9 @@ int pivx_get_filtered(pivx_ctx* ctx, int grid, float* dx, float* dy, uint8_t* va
  * index j*ws_x+i in the fftshifted window plane, and the uncapped local-maximum
  * count per window [nt_vec]. Any pointer may be NULL. */
 int pivx_get_peaks(pivx_ctx* ctx, int grid, float* value, int* index, int* ncand);
-/* The deformed frame 2 of the last deformed pass of `grid`, [ny_aug][nx_aug]. */
+/* The deformed frame 2 of the last deformed pass of `grid` that wrote it (a
+ * pass with fused deformation does not, see pivx_set_fused_deform),
+ * [ny_aug][nx_aug]. */
 int pivx_get_deformed(pivx_ctx* ctx, int grid, float* out);
 int pivx_get_stats(pivx_ctx* ctx, pivx_stats* st);
 

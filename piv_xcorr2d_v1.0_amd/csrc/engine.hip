@@ -47,6 +47,11 @@ struct GridBuf {
   uint8_t* uv = nullptr;
   float *ex = nullptr, *ey = nullptr;  // extended field (nx+6)*(ny+6)
   float* deform = nullptr;             // deformed frame 2 [ny_aug][nx_aug]
+  // Deformation fused into the gather of the correlation kernel (FusedDef):
+  // possible when the windows do not overlap and take the wave-per-window
+  // kernel; the interpolation terms per aug column and row.
+  bool fuse = false;
+  float4 *dcol = nullptr, *drow = nullptr;
   float2 *twx = nullptr, *twy = nullptr;   // generic-path twiddles
   float2* gscratch = nullptr;              // generic-path planes for big windows
   size_t gscratch_windows = 0;
@@ -66,6 +71,8 @@ struct pivx_ctx {
   uint8_t* mo_v = nullptr;
   uint8_t *f1 = nullptr, *f2 = nullptr;    // frames [ny][nx]
   bool have_frames = false;
+  bool in_pair = false;                    // inside pivx_run_pair's sequence
+  int fuse_mode = 1;                       // pivx_set_fused_deform
   // MHE enhancement of uploaded frames (MHE7_4_gpu.cu), off unless pivx_set_mhe
   bool mhe_on = false, mhe_ready = false;
   MheArgs mhe{};
@@ -243,8 +250,8 @@ int check_launch(pivx_ctx* c, hipError_t e, const char* what) {
   return PIVX_OK;
 }
 
-// Deformation of frame 2 by the extended field of grid gi (cu:1300-1301, 1624-1625).
-int run_deform(pivx_ctx* c, int gi) {
+// Deformation arguments of grid gi (cu:1300-1301, 1624-1625).
+DefArgs def_args(pivx_ctx* c, int gi) {
   GridBuf& G = c->grid[gi];
   const pivx_geometry& g = G.g;
   DefArgs d{};
@@ -268,6 +275,13 @@ int run_deform(pivx_ctx* c, int gi) {
   d.spx = (float)g.spacing_x;
   d.spy = (float)g.spacing_y;
   d.oob = c->counters + gi;
+  return d;
+}
+
+// Deformation of frame 2 by the extended field of grid gi.
+int run_deform(pivx_ctx* c, int gi) {
+  GridBuf& G = c->grid[gi];
+  const DefArgs d = def_args(c, gi);
   tic(c);
   const int rc = check_launch(c, launch_deform(d, c->stream), "Kernel_Window_Deformation_2d");
   toc(c, gi, 0);
@@ -278,13 +292,27 @@ int run_deform(pivx_ctx* c, int gi) {
 // With zero_pred set (first grid-1 pass) the predictor is 0 and the kernel
 // itself zeroes the filtered field, replacing set_float_zero_kernel
 // (cu:1287-1298).
-int run_xcorr(pivx_ctx* c, int gi, bool deformed, bool zero_pred = false) {
+// With fused set, frame 2 is deformed inside the correlation kernel's gather
+// (FusedDef) and the deformed plane is not written.
+int run_xcorr(pivx_ctx* c, int gi, bool deformed, bool zero_pred = false, bool fused = false) {
   GridBuf& G = c->grid[gi];
   const pivx_geometry& g = G.g;
   XcArgs a{};
   a.f1 = c->f1;
-  a.f2u8 = deformed ? nullptr : c->f2;
-  a.f2def = deformed ? G.deform : nullptr;
+  a.f2u8 = deformed && !fused ? nullptr : c->f2;
+  a.f2def = deformed && !fused ? G.deform : nullptr;
+  if (fused) {
+    a.fdef.col = G.dcol;
+    a.fdef.row = G.drow;
+    a.fdef.ex_x = G.ex;
+    a.fdef.ex_y = G.ey;
+    a.fdef.nxe = g.nx_vec + 2 * kExBorder;
+    a.fdef.hfws_x = g.ws_x / 2;
+    a.fdef.hfws_y = g.ws_y / 2;
+    a.fdef.nx_aug = g.nx_aug;
+    a.fdef.ny_aug = g.ny_aug;
+    a.fdef.oob = c->counters + gi;
+  }
   a.pitch1 = c->p.nx;
   a.pitch2 = c->p.nx;
   a.def_pitch = g.nx_aug;
@@ -519,6 +547,13 @@ int alloc_grid(pivx_ctx* c, int gi) {
   if ((long long)wxm * wym > 64) {
     if (dalloc(c, &G.med_scratch, nt * 2 * (size_t)wxm * wym)) return PIVX_ERR_HIP;
   }
+  // fused deformation: each aug pixel in at most one window
+  G.fuse = xcorr_fuses_deform(g.ws_x, g.ws_y) && g.spacing_x >= g.ws_x && g.spacing_y >= g.ws_y;
+  if (G.fuse) {
+    if (dalloc(c, &G.dcol, (size_t)g.nx_aug) || dalloc(c, &G.drow, (size_t)g.ny_aug)) return PIVX_ERR_HIP;
+    const int rc2 = check_launch(c, launch_deform_terms(def_args(c, gi), G.dcol, G.drow, c->stream), "deformation terms");
+    if (rc2) return rc2;
+  }
   return PIVX_OK;
 }
 
@@ -733,6 +768,10 @@ int pivx_calc_cc_map_p1(pivx_ctx* c, int idx_pass) {
   if (idx_pass < 0) return fail(c, PIVX_ERR_PARAM, "idx_pass must be >= 0");
   int rc;
   if (idx_pass == 0) return run_xcorr(c, 0, false, true);
+  // the deformation of non-overlapping windows runs in the correlation
+  // kernel (within pivx_run_pair by default, pivx_set_fused_deform)
+  if (c->grid[0].fuse && (c->fuse_mode == 2 || (c->fuse_mode == 1 && c->in_pair)))
+    return run_xcorr(c, 0, true, false, true);
   rc = run_deform(c, 0);
   if (rc) return rc;
   return run_xcorr(c, 0, true);
@@ -796,6 +835,11 @@ int pivx_upsampling_vector_field(pivx_ctx* c) {
 // written by the kernel that produces the field (last smoothing pass,
 // upsampling) whenever it can be; extend() runs only where it could not.
 static int run_pair_enqueue(pivx_ctx* c, const pivx_filter_params* f1, const pivx_filter_params* f2) {
+  struct InPair {
+    pivx_ctx* c;
+    explicit InPair(pivx_ctx* x) : c(x) { c->in_pair = true; }
+    ~InPair() { c->in_pair = false; }
+  } in_pair(c);
   int rc;
   bool ext_ok = false;
   GridBuf& G1 = c->grid[0];
@@ -876,6 +920,14 @@ int pivx_set_graph_mode(pivx_ctx* c, int on) {
   if (!c) return fail(nullptr, PIVX_ERR_PARAM, "null handle");
   c->use_graph = on != 0;
   if (!c->use_graph) drop_graph(c);
+  return PIVX_OK;
+}
+
+int pivx_set_fused_deform(pivx_ctx* c, int mode) {
+  if (!c) return fail(nullptr, PIVX_ERR_PARAM, "null handle");
+  if (mode < 0 || mode > 2) return fail(c, PIVX_ERR_PARAM, "fused deformation mode must be 0, 1 or 2 (got %d)", mode);
+  if (mode != c->fuse_mode) drop_graph(c);  // a captured pair holds the old choice
+  c->fuse_mode = mode;
   return PIVX_OK;
 }
 

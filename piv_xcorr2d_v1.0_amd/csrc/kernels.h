@@ -5,6 +5,19 @@
 
 namespace pivx {
 
+// Deformation fused into the gather of the wave-per-window kernel, for
+// windows that do not overlap (each aug pixel then belongs to one window and
+// is deformed once): frame 2 is sampled at the displaced position of every
+// window pixel with the arithmetic of k_deform instead of read from a
+// materialized plane.  col == nullptr: off.
+struct FusedDef {
+  const float4* col;      // per aug column: (ax, 1 - ax, ixf as int bits, (float)idx)
+  const float4* row;      // per aug row: (ay, 1 - ay, iyf as int bits, (float)idy)
+  const float* ex_x; const float* ex_y; int nxe;  // extended field of the grid
+  int hfws_x, hfws_y, nx_aug, ny_aug;             // for the out-of-buffer tap count
+  unsigned long long* oob;                        // cumulative out-of-buffer tap count
+};
+
 // Fused correlate-and-peak pass over all windows of one grid.
 struct XcArgs {
   const uint8_t* f1;      // frame 1, image coordinates, row pitch pitch1
@@ -25,6 +38,7 @@ struct XcArgs {
   int chunks_x;           // blocks per vector row (fast path)
   float* zero_x;          // if set: zeroed at every window (first-pass predictor reset)
   float* zero_y;
+  FusedDef fdef;          // frame 2 deformed in the gather (wave-per-window kernel only)
 };
 
 struct DefArgs {
@@ -81,6 +95,12 @@ hipError_t launch_xcorr(const XcArgs& a, int ws_x, int ws_y, float2* gscratch, s
                         const float2* twx, const float2* twy, hipStream_t s);
 bool xcorr_fast_supported(int ws_x, int ws_y);
 hipError_t launch_deform(const DefArgs& a, hipStream_t s);
+// The per-column and per-row interpolation terms of the deformation (as
+// k_deform computes them) into col[nx_aug] and row[ny_aug], for FusedDef.
+hipError_t launch_deform_terms(const DefArgs& a, float4* col, float4* row, hipStream_t s);
+// Whether launch_xcorr runs windows of this shape on the wave-per-window
+// kernel, the one that accepts FusedDef.
+bool xcorr_fuses_deform(int ws_x, int ws_y);
 hipError_t launch_extend(const float* fx, const float* fy, float* ex, float* ey, int nx, int ny, int border,
                          hipStream_t s);
 hipError_t launch_upsample(float* dx2, float* dy2, const float* ex1x, const float* ex1y, int nxe1, int nye1,

@@ -183,6 +183,135 @@ __device__ __forceinline__ bool subpixel_fit(const float* cc, int sx, int wsx, i
   return true;
 }
 
+// ============================================================== deformation per pixel
+// Kernel_Window_Deformation_2d (cu:608-652): for every aug pixel, bilinear
+// displacement from the extended filtered field, then bilinear sample of frame
+// 2 at the displaced position.  Out-of-frame taps read 0 (the reference's zero
+// padding); taps beyond the reference's padded buffer are counted.  The sums
+// x + c1*e1 + ... + c4*e4 (cu:631-632) and c1*v1 + ... + c4*v4 (cu:649) are the
+// fused multiply-add chains a CUDA build forms from that source by default
+// (--fmad=true): fma(c4, e4, fma(c3, e3, fma(c2, e2, fma(c1, e1, x)))) and
+// fma(c4, v4, fma(c3, v3, fma(c1, v1, c2*v2))).  Used by k_deform and by the
+// gather of k_xcorr_single when the deformation is fused into it.
+
+// Interpolation terms of aug coordinate v along one axis (cu:615-624): a =
+// (v - st) / sp, the cell floor(a) clamped to [0, n - 2] (a guard: the
+// extended field covers every aug pixel), the fraction a - floor(a) of the
+// unclamped cell.  Returns (fraction, 1 - fraction, cell as int bits, v).
+__device__ __forceinline__ float4 deform_terms(int v, float st, float sp, int n) {
+#pragma clang fp contract(off)
+  const float fv = (float)v;
+  float t = (fv - st) / sp;
+  int i = (int)floorf(t);
+  t = t - (float)i;
+  i = min(max(i, 0), n - 2);
+  return make_float4(t, 1 - t, __int_as_float(i), fv);
+}
+
+// The four field samples of a cell.  A pixel's displacement is a convex
+// combination of them, so while every sample is below hfws - 2 in magnitude
+// no tap can leave the reference's padded buffer and the per-pixel test is
+// skipped (may_oob false).
+struct DefCell {
+  float ex1, ex2, ex3, ex4, ey1, ey2, ey3, ey4;
+  bool may_oob;
+};
+
+__device__ __forceinline__ DefCell load_cell(const float* ex_x, const float* ex_y, int nxe, int ixf, int iyf,
+                                             float oob_guard) {
+  DefCell e;
+  const int i1 = iyf * nxe + ixf, i3 = i1 + nxe;
+  e.ex1 = ex_x[i1];
+  e.ex2 = ex_x[i1 + 1];
+  e.ex3 = ex_x[i3];
+  e.ex4 = ex_x[i3 + 1];
+  e.ey1 = ex_y[i1];
+  e.ey2 = ex_y[i1 + 1];
+  e.ey3 = ex_y[i3];
+  e.ey4 = ex_y[i3 + 1];
+  const float em = fmaxf(fmaxf(fmaxf(fabsf(e.ex1), fabsf(e.ex2)), fmaxf(fabsf(e.ex3), fabsf(e.ex4))),
+                         fmaxf(fmaxf(fabsf(e.ey1), fabsf(e.ey2)), fmaxf(fabsf(e.ey3), fabsf(e.ey4))));
+  e.may_oob = !(em < oob_guard);  // NaN samples keep the exact test
+  return e;
+}
+
+// Frame 2 for the taps.  It is read through a buffer resource that starts
+// one byte before the frame and ends with it: a tap outside the frame gets an
+// offset beyond that span, which the hardware returns as 0 without a memory
+// access, so the four taps are unconditional loads and a pixel waits on one
+// round trip.  The range check covers the vector offset only, so taps 2 and 4
+// (x + 1) share the vector offset of taps 1 and 3 and add 1 as the scalar
+// offset; with the base one byte early every in-frame tap's vector offset is
+// >= 0.
+struct DefFrame {
+  __amdgpu_buffer_rsrc_t f2r;
+  unsigned pitch;
+  int nxm, nym, left_x, left_y, hfws_x, hfws_y;
+  unsigned span_x, span_y;  // the reference's aug_ex buffer: [-hfws, n_aug + hfws - 1)
+};
+
+// an offset beyond every frame (frames hold < 2^31 bytes), also after the +1
+// of taps 2 and 4; the bits of -2.0f, an inline constant of the select
+constexpr unsigned DEF_TAP_OUT = 0xc0000000u;
+
+__device__ __forceinline__ DefFrame def_frame(const uint8_t* f2, long long pitch, int nx, int ny, int left_x, int left_y,
+                                              int hfws_x, int hfws_y, int nx_aug, int ny_aug) {
+  DefFrame F;
+  F.pitch = (unsigned)pitch;
+  // byte b of the resource is frame byte b - 1 (byte 0 is never read)
+  F.f2r = __builtin_amdgcn_make_buffer_rsrc(const_cast<uint8_t*>(f2) - 1, (short)0, (int)(F.pitch * (unsigned)ny + 1u),
+                                            0x00020000);
+  F.nxm = nx - 1;
+  F.nym = ny - 1;
+  F.left_x = left_x;
+  F.left_y = left_y;
+  F.hfws_x = hfws_x;
+  F.hfws_y = hfws_y;
+  F.span_x = (unsigned)(nx_aug + 2 * hfws_x - 1);
+  F.span_y = (unsigned)(ny_aug + 2 * hfws_y - 1);
+  return F;
+}
+
+// The deformed value of the aug pixel with column terms (ax, ax1, fidx) and
+// row terms (ay, ay1, fidy) in cell e; out-of-buffer taps are added to oob
+// (COUNT false: the caller has checked that the cell cannot produce any).
+template <bool COUNT = true>
+__device__ __forceinline__ float deform_px(const DefFrame& F, const DefCell& e, float ax, float ax1, float fidx, float ay,
+                                           float ay1, float fidy, unsigned& oob) {
+#pragma clang fp contract(off)
+  float ce1 = ax1 * ay1, ce2 = ax * ay1, ce3 = ax1 * ay, ce4 = ax * ay;
+  float sx = fmaf(ce4, e.ex4, fmaf(ce3, e.ex3, fmaf(ce2, e.ex2, fmaf(ce1, e.ex1, fidx))));
+  float sy = fmaf(ce4, e.ey4, fmaf(ce3, e.ey3, fmaf(ce2, e.ey2, fmaf(ce1, e.ey1, fidy))));
+  // (float)(int)floor(s) == floor(s) for |s| < 2^31, so the fraction
+  // s - (float)(int)floor(s) of cu:637-638 is s - floor(s)
+  const float fx = floorf(sx), fy = floorf(sy);
+  const int jx = (int)fx, jy = (int)fy;
+  sx = sx - fx;
+  sy = sy - fy;
+  ce1 = (1 - sx) * (1 - sy);
+  ce2 = sx * (1 - sy);
+  ce3 = (1 - sx) * sy;
+  ce4 = sx * sy;
+  const int xi = jx - F.left_x, yi = jy - F.left_y;
+  const bool x0 = (unsigned)xi <= (unsigned)F.nxm, x1 = (unsigned)(xi + 1) <= (unsigned)F.nxm;
+  const bool yok0 = (unsigned)yi <= (unsigned)F.nym, yok1 = (unsigned)(yi + 1) <= (unsigned)F.nym;
+  // resource offsets (frame offset + 1) of taps 3 and 1; exact whenever the
+  // tap or its x + 1 neighbour is inside the frame (24-bit product: rows and
+  // row counts are below 2^24)
+  const unsigned o3 = __umul24((unsigned)(yi + 1), F.pitch) + (unsigned)(xi + 1), o1 = o3 - F.pitch;
+  const unsigned q1 = (yok0 && x0) ? o1 : DEF_TAP_OUT, q2 = (yok0 && x1) ? o1 : DEF_TAP_OUT;
+  const unsigned q3 = (yok1 && x0) ? o3 : DEF_TAP_OUT, q4 = (yok1 && x1) ? o3 : DEF_TAP_OUT;
+  const float t1 = (float)__builtin_amdgcn_raw_buffer_load_b8(F.f2r, q1, 0, 0);
+  const float t2 = (float)__builtin_amdgcn_raw_buffer_load_b8(F.f2r, q2, 1, 0);
+  const float t3 = (float)__builtin_amdgcn_raw_buffer_load_b8(F.f2r, q3, 0, 0);
+  const float t4 = (float)__builtin_amdgcn_raw_buffer_load_b8(F.f2r, q4, 1, 0);
+  if constexpr (COUNT) {
+    if (__builtin_expect(e.may_oob, 0))
+      oob += ((unsigned)(jx + F.hfws_x) >= F.span_x || (unsigned)(jy + F.hfws_y) >= F.span_y) ? 1u : 0u;
+  }
+  return fmaf(ce4, t4, fmaf(ce3, t3, fmaf(ce1, t1, ce2 * t2)));
+}
+
 // Gather of one window pixel: frame-1 value and frame-2 value at aug
 // coordinates (ax, ay).  Image pixels outside the frame read as 0 (the
 // reference's zero-padded aug_ex buffers, cu:951-954, 1271-1282).
@@ -513,6 +642,11 @@ __device__ __forceinline__ void load_row(const XcArgs& a, int ax0, int ay, float
   } else {
     load_u8_row_edge<WX>(a.f1, a.pitch1, a.nx, a.ny, xi0, yi, f1);
   }
+  if (a.fdef.col) {
+    // frame 2 comes from the deformed window plane (deform_window)
+    static_for<WX>([&](auto c) { x[c].x = f1[c]; });
+    return;
+  }
   if (a.f2def) {
     const float* q = a.f2def + (long long)ay * a.def_pitch + ax0;
     static_for<WX / 4>([&](auto qc) {
@@ -530,6 +664,64 @@ __device__ __forceinline__ void load_row(const XcArgs& a, int ax0, int ay, float
     load_u8_row_edge<WX>(a.f2u8, a.pitch2, a.nx, a.ny, xi0, yi, f2);
   }
   static_for<WX>([&](auto c) { x[c] = make_float2(f1[c], f2[c]); });
+}
+
+// Frame 2 of the window at aug (ax0, ay0) deformed into the plane
+// p[r * SX + c] (FusedDef): lane = window column c, rows in order; the row
+// terms are wave-uniform and come through scalar loads, and the cell changes
+// with the row, as in k_deform.
+template <int WX, int WY, int SX>
+__device__ __forceinline__ void deform_window(const XcArgs& a, int ax0, int ay0, float* p, int lane) {
+  const FusedDef& d = a.fdef;
+  if (lane >= WX) return;
+  const float4 ct = d.col[ax0 + lane];
+  const int ixf = __float_as_int(ct.z);
+  const DefFrame F = def_frame(a.f2u8, a.pitch2, a.nx, a.ny, a.left_x, a.left_y, d.hfws_x, d.hfws_y, d.nx_aug, d.ny_aug);
+  const float oob_guard = (float)(min(d.hfws_x, d.hfws_y) - 2);
+  // the row terms through the constant address space: scalar loads
+  typedef const __attribute__((address_space(4))) float ConstF;
+  ConstF* const rows = (ConstF*)d.row + 4 * ay0;
+  static_assert(WY % 8 == 0, "rows in groups of 8");
+  unsigned oob = 0;
+  int cell = -1;
+  DefCell e{};
+  // rows in groups of 8: when a group lies in one cell whose samples stay
+  // below the out-of-buffer guard (the common case) the group has no branch
+  // and its 32 frame taps are in flight together
+  for (int r0 = 0; r0 < WY; r0 += 8) {
+    float ay[8], ay1[8], fidy[8];
+    int iyf[8];
+    static_for<8>([&](auto k) {
+      ay[k] = rows[4 * (r0 + k)];
+      ay1[k] = rows[4 * (r0 + k) + 1];
+      iyf[k] = __float_as_int(rows[4 * (r0 + k) + 2]);
+      fidy[k] = rows[4 * (r0 + k) + 3];
+    });
+    if (iyf[0] == iyf[7]) {  // cells are monotone in the row
+      if (iyf[0] != cell) {
+        cell = iyf[0];
+        e = load_cell(d.ex_x, d.ex_y, d.nxe, ixf, cell, oob_guard);
+      }
+      if (__builtin_expect(e.may_oob, 0)) {
+        static_for<8>([&](auto k) {
+          p[(r0 + k) * SX + lane] = deform_px<true>(F, e, ct.x, ct.y, ct.w, ay[k], ay1[k], fidy[k], oob);
+        });
+      } else {
+        static_for<8>([&](auto k) {
+          p[(r0 + k) * SX + lane] = deform_px<false>(F, e, ct.x, ct.y, ct.w, ay[k], ay1[k], fidy[k], oob);
+        });
+      }
+    } else {
+      static_for<8>([&](auto k) {
+        if (iyf[k] != cell) {
+          cell = iyf[k];
+          e = load_cell(d.ex_x, d.ex_y, d.nxe, ixf, cell, oob_guard);
+        }
+        p[(r0 + k) * SX + lane] = deform_px(F, e, ct.x, ct.y, ct.w, ay[k], ay1[k], fidy[k], oob);
+      });
+    }
+  }
+  if (oob) atomicAdd(d.oob, (unsigned long long)oob);
 }
 
 template <int WX, int WY>
@@ -749,12 +941,19 @@ __global__ void __launch_bounds__(64) k_xcorr_single(XcArgs a) {
   const int iy = w / a.nx_vec, ix = w - iy * a.nx_vec;
   float* const sp = lds;
 
+  // 0. fused deformation: frame 2 of the window into the plane
+  if (a.fdef.col) {
+    deform_window<WX, WY, SX>(a, ix * a.sp_x, iy * a.sp_y, sp, lane);
+    wave_sync();
+  }
   // 1-2. row gather + forward row FFT, then the transpose (Re, then Im)
   float2 col[WY];
   {
     float2 x[WX];
     if (lane < WY) {
       load_row<WX>(a, ix * a.sp_x, iy * a.sp_y + lane, x);
+      // each lane reads and then overwrites only its own row of the plane
+      if (a.fdef.col) static_for<WX>([&](auto c) { x[c].y = sp[lane * SX + c]; });
       fft_reg<WX, false>(x);
       static_for<WX>([&](auto c) { sp[lane * SX + c] = x[c].x; });
     }
@@ -1134,67 +1333,27 @@ __global__ void __launch_bounds__(256) k_xcorr_generic(XcArgs a, int WX, int WY,
 }
 
 // ============================================================== deformation
-// Kernel_Window_Deformation_2d (cu:608-652): for every aug pixel, bilinear
-// displacement from the extended filtered field, then bilinear sample of frame
-// 2 at the displaced position.  Out-of-frame taps read 0 (the reference's zero
-// padding); taps beyond the reference's padded buffer are counted.  The sums
-// x + c1*e1 + ... + c4*e4 (cu:631-632) and c1*v1 + ... + c4*v4 (cu:649) are the
-// fused multiply-add chains a CUDA build forms from that source by default
-// (--fmad=true): fma(c4, e4, fma(c3, e3, fma(c2, e2, fma(c1, e1, x)))) and
-// fma(c4, v4, fma(c3, v3, fma(c1, v1, c2*v2))).
-// One thread per aug column, DEF_ROWS consecutive rows.  The row terms
-// (cell row, fraction, 1 - fraction) are computed once per workgroup into
-// LDS and the column terms once per thread, with the same operations as the
-// per-pixel formula; the four field samples of a cell are reloaded only when
-// the row crosses into the next cell.  Frame 2 is read through a buffer
-// resource that starts one byte before the frame and ends with it: a tap
-// outside the frame gets an offset beyond that span, which the hardware
-// returns as 0 without a memory access, so the four taps are unconditional
-// loads and a pixel waits on one round trip.  The range check covers the
-// vector offset only, so taps 2 and 4 (x + 1) share the vector offset of
-// taps 1 and 3 and add 1 as the scalar offset; with the base one byte early
-// every in-frame tap's vector offset is >= 0.
+// One thread per aug column, DEF_ROWS consecutive rows.  The row terms are
+// computed once per workgroup into LDS and the column terms once per thread
+// (deform_terms); the four field samples of a cell are reloaded only when
+// the row crosses into the next cell; each pixel is deform_px.
 constexpr int DEF_ROWS = 16;
-// an offset beyond every frame (frames hold < 2^31 bytes), also after the +1
-// of taps 2 and 4; the bits of -2.0f, an inline constant of the select
-constexpr unsigned DEF_TAP_OUT = 0xc0000000u;
 
 __global__ void __launch_bounds__(256) k_deform(DefArgs a) {
-#pragma clang fp contract(off)
   __shared__ float4 rowt[DEF_ROWS];
   const int y0 = blockIdx.y * DEF_ROWS;
   const int nrow = min(DEF_ROWS, a.ny_aug - y0);
-  if (threadIdx.x < nrow) {
-    const int idy = y0 + threadIdx.x;
-    float ay = ((float)idy - a.y_st) / a.spy;
-    int iyf = (int)floorf(ay);
-    ay = ay - (float)iyf;
-    iyf = min(max(iyf, 0), a.nye - 2);
-    rowt[threadIdx.x] = make_float4(ay, 1 - ay, __int_as_float(iyf), (float)idy);
-  }
+  if (threadIdx.x < nrow) rowt[threadIdx.x] = deform_terms(y0 + (int)threadIdx.x, a.y_st, a.spy, a.nye);
   __syncthreads();
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= a.nx_aug) return;
-  unsigned oob = 0;
-  const float fidx = (float)idx;
-  const float ax0 = (fidx - a.x_st) / a.spx;
-  int ixf = (int)floorf(ax0);
-  const float ax = ax0 - (float)ixf;
-  const float ax1 = 1 - ax;
-  ixf = min(max(ixf, 0), a.nxe - 2);
-  int cell = -1;
-  float ex1 = 0.f, ex2 = 0.f, ex3 = 0.f, ex4 = 0.f, ey1 = 0.f, ey2 = 0.f, ey3 = 0.f, ey4 = 0.f;
-  // A pixel's displacement is a convex combination of its cell's four field
-  // samples, so while every sample is below hfws - 2 in magnitude no tap can
-  // leave the reference's padded buffer and the per-pixel test is skipped.
+  const float4 ct = deform_terms(idx, a.x_st, a.spx, a.nxe);
+  const int ixf = __float_as_int(ct.z);
+  const DefFrame F = def_frame(a.f2, a.pitch, a.nx, a.ny, a.left_x, a.left_y, a.hfws_x, a.hfws_y, a.nx_aug, a.ny_aug);
   const float oob_guard = (float)(min(a.hfws_x, a.hfws_y) - 2);
-  bool may_oob = true;
-  const int nxm = a.nx - 1, nym = a.ny - 1;
-  const unsigned pitch = (unsigned)a.pitch;
-  // byte b of the resource is frame byte b - 1 (byte 0 is never read)
-  const __amdgpu_buffer_rsrc_t f2r = __builtin_amdgcn_make_buffer_rsrc(const_cast<uint8_t*>(a.f2) - 1, (short)0,
-                                                                       (int)(pitch * (unsigned)a.ny + 1u), 0x00020000);
-  const unsigned span_x = (unsigned)(a.nx_aug + 2 * a.hfws_x - 1), span_y = (unsigned)(a.ny_aug + 2 * a.hfws_y - 1);
+  unsigned oob = 0;
+  int cell = -1;
+  DefCell e{};
   // this block's output rows as a buffer resource: row r of pixel idx at
   // byte r * nx_aug * 4 (scalar offset) + idx * 4
   const unsigned row_bytes = (unsigned)a.nx_aug * 4u;
@@ -1206,54 +1365,12 @@ __global__ void __launch_bounds__(256) k_deform(DefArgs a) {
   const float4* const rowp = reinterpret_cast<const float4*>(reinterpret_cast<const char*>(rowt) + rbase);
   auto row = [&](int r) {
     const float4 rt = rowp[r];
-    const float ay = rt.x, ay1 = rt.y;
     const int iyf = __builtin_amdgcn_readfirstlane(__float_as_int(rt.z));
-    const float fidy = rt.w;
-    float ce1 = ax1 * ay1, ce2 = ax * ay1, ce3 = ax1 * ay, ce4 = ax * ay;
     if (iyf != cell) {
       cell = iyf;
-      const int i1 = iyf * a.nxe + ixf, i3 = i1 + a.nxe;
-      ex1 = a.ex_x[i1];
-      ex2 = a.ex_x[i1 + 1];
-      ex3 = a.ex_x[i3];
-      ex4 = a.ex_x[i3 + 1];
-      ey1 = a.ex_y[i1];
-      ey2 = a.ex_y[i1 + 1];
-      ey3 = a.ex_y[i3];
-      ey4 = a.ex_y[i3 + 1];
-      const float em = fmaxf(fmaxf(fmaxf(fabsf(ex1), fabsf(ex2)), fmaxf(fabsf(ex3), fabsf(ex4))),
-                             fmaxf(fmaxf(fabsf(ey1), fabsf(ey2)), fmaxf(fabsf(ey3), fabsf(ey4))));
-      may_oob = !(em < oob_guard);  // NaN samples keep the exact test
+      e = load_cell(a.ex_x, a.ex_y, a.nxe, ixf, iyf, oob_guard);
     }
-    float sx = fmaf(ce4, ex4, fmaf(ce3, ex3, fmaf(ce2, ex2, fmaf(ce1, ex1, fidx))));
-    float sy = fmaf(ce4, ey4, fmaf(ce3, ey3, fmaf(ce2, ey2, fmaf(ce1, ey1, fidy))));
-    // (float)(int)floor(s) == floor(s) for |s| < 2^31, so the fraction
-    // s - (float)(int)floor(s) of cu:637-638 is s - floor(s)
-    const float fx = floorf(sx), fy = floorf(sy);
-    const int jx = (int)fx, jy = (int)fy;
-    sx = sx - fx;
-    sy = sy - fy;
-    ce1 = (1 - sx) * (1 - sy);
-    ce2 = sx * (1 - sy);
-    ce3 = (1 - sx) * sy;
-    ce4 = sx * sy;
-    const int xi = jx - a.left_x, yi = jy - a.left_y;
-    const bool x0 = (unsigned)xi <= (unsigned)nxm, x1 = (unsigned)(xi + 1) <= (unsigned)nxm;
-    const bool yok0 = (unsigned)yi <= (unsigned)nym, yok1 = (unsigned)(yi + 1) <= (unsigned)nym;
-    // resource offsets (frame offset + 1) of taps 3 and 1; exact whenever the
-    // tap or its x + 1 neighbour is inside the frame (24-bit product: rows
-    // and row counts are below 2^24)
-    const unsigned o3 = __umul24((unsigned)(yi + 1), pitch) + (unsigned)(xi + 1), o1 = o3 - pitch;
-    const unsigned q1 = (yok0 && x0) ? o1 : DEF_TAP_OUT, q2 = (yok0 && x1) ? o1 : DEF_TAP_OUT;
-    const unsigned q3 = (yok1 && x0) ? o3 : DEF_TAP_OUT, q4 = (yok1 && x1) ? o3 : DEF_TAP_OUT;
-    const float t1 = (float)__builtin_amdgcn_raw_buffer_load_b8(f2r, q1, 0, 0);
-    const float t2 = (float)__builtin_amdgcn_raw_buffer_load_b8(f2r, q2, 1, 0);
-    const float t3 = (float)__builtin_amdgcn_raw_buffer_load_b8(f2r, q3, 0, 0);
-    const float t4 = (float)__builtin_amdgcn_raw_buffer_load_b8(f2r, q4, 1, 0);
-    // taps outside the reference's aug_ex buffer ([-hfws, n_aug + hfws) in aug coordinates)
-    if (__builtin_expect(may_oob, 0))
-      oob += ((unsigned)(jx + a.hfws_x) >= span_x || (unsigned)(jy + a.hfws_y) >= span_y) ? 1u : 0u;
-    const float v = fmaf(ce4, t4, fmaf(ce3, t3, fmaf(ce1, t1, ce2 * t2)));
+    const float v = deform_px(F, e, ct.x, ct.y, ct.w, rt.x, rt.y, rt.w, oob);
     __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), outr, (int)(idx * 4), (int)(r * row_bytes), 0);
   };
   // full blocks unrolled (row terms at constant LDS offsets)
@@ -1263,6 +1380,13 @@ __global__ void __launch_bounds__(256) k_deform(DefArgs a) {
     for (int r = 0; r < nrow; ++r) row(r);
   }
   if (oob) atomicAdd(a.oob, (unsigned long long)oob);
+}
+
+// The per-column and per-row terms of k_deform as tables (FusedDef).
+__global__ void k_deform_terms(DefArgs a, float4* col, float4* row) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < a.nx_aug) col[t] = deform_terms(t, a.x_st, a.spx, a.nxe);
+  if (t < a.ny_aug) row[t] = deform_terms(t, a.y_st, a.spy, a.nye);
 }
 
 // ============================================================== field extension
@@ -1940,13 +2064,17 @@ static hipError_t launch_single(const XcArgs& a, hipStream_t s) {
 
 template <int WX, int WY, int WPB>
 static hipError_t launch_fast(const XcArgs& a0, hipStream_t s) {
-  if constexpr (WX <= 32 && WY <= 32 && WX + WY > 32) return launch_pair<WX, WY>(a0, s);
-  if constexpr (WX <= 64 && WY <= 64 && (WX > 32 || WY > 32)) return launch_single<WX, WY>(a0, s);
-  XcArgs a = a0;
-  a.chunks_x = (a.nx_vec + WPB - 1) / WPB;
-  const int blocks = a.chunks_x * a.ny_vec;
-  hipLaunchKernelGGL((k_xcorr_fast<WX, WY, WPB>), dim3(blocks), dim3(FastCfg<WX, WY, WPB>::NT), 0, s, a);
-  return hipGetLastError();
+  if constexpr (WX <= 32 && WY <= 32 && WX + WY > 32) {
+    return launch_pair<WX, WY>(a0, s);
+  } else if constexpr (WX <= 64 && WY <= 64 && (WX > 32 || WY > 32)) {
+    return launch_single<WX, WY>(a0, s);
+  } else {
+    XcArgs a = a0;
+    a.chunks_x = (a.nx_vec + WPB - 1) / WPB;
+    const int blocks = a.chunks_x * a.ny_vec;
+    hipLaunchKernelGGL((k_xcorr_fast<WX, WY, WPB>), dim3(blocks), dim3(FastCfg<WX, WY, WPB>::NT), 0, s, a);
+    return hipGetLastError();
+  }
 }
 
 // (WX, WY, windows per workgroup).  Shapes up to 32 wide run one 64-lane wave
@@ -1962,6 +2090,16 @@ static hipError_t launch_fast(const XcArgs& a0, hipStream_t s) {
   X(32, 16, 2)             \
   X(16, 32, 2)             \
   X(48, 24, 8)
+
+bool xcorr_fuses_deform(int ws_x, int ws_y) {
+  return xcorr_fast_supported(ws_x, ws_y) && ws_x <= 64 && ws_y <= 64 && (ws_x > 32 || ws_y > 32);
+}
+
+hipError_t launch_deform_terms(const DefArgs& a, float4* col, float4* row, hipStream_t s) {
+  const int n = a.nx_aug > a.ny_aug ? a.nx_aug : a.ny_aug;
+  hipLaunchKernelGGL(k_deform_terms, dim3(grid_for(n, 256)), dim3(256), 0, s, a, col, row);
+  return hipGetLastError();
+}
 
 bool xcorr_fast_supported(int ws_x, int ws_y) {
 #define PIVX_CASE(WX, WY, WPB) \

@@ -135,6 +135,38 @@ def test_deformation_out_of_buffer_taps(pivx, shift):
         assert want == 0
 
 
+@pytest.mark.parametrize("shift", [(40, -36), (-35, 45), (10, -6)])
+def test_fused_deformation_matches_plane(pivx, shift):
+    """Deformation fused into the correlation kernel (grid-1 64x64 windows at
+    spacing 64, pivx_set_fused_deform(2)) against the deformed-plane path
+    (mode 0): identical correlation results, and the out-of-buffer tap count
+    equals the closed form of an integer shift in both (the weights of a
+    spacing-64 grid are exact binary fractions summing to 1)."""
+    a, b = config1_pair(pivx, seed=4)
+    p = config1_params(pivx, ws_x1=64, ws_y1=64, spacing_x1=64, spacing_y1=64, npass1=2)
+    res = {}
+    for mode in (0, 2):
+        with pivx.Xcorr2dMulti(p) as x:
+            x.set_fused_deform(mode)
+            g = geom_dict(x.geometry(1))
+            nt = g["nx_vec"] * g["ny_vec"]
+            x.prep_img_data(a, b)
+            x.set_filtered(1, np.full(nt, shift[0], np.float32), np.full(nt, shift[1], np.float32))
+            x.extend_filtered_vecfield_p1()
+            x.Calc_cc_map_p1(1)
+            res[mode] = (x.stats()["oob_taps"][0], x.raw(1) + x.peaks(1))
+    hfws = 32
+    jx = np.arange(g["nx_aug"]) + shift[0]
+    jy = np.arange(g["ny_aug"]) + shift[1]
+    out_x = (jx + hfws < 0) | (jx + hfws >= g["nx_aug"] + 2 * hfws - 1)
+    out_y = (jy + hfws < 0) | (jy + hfws >= g["ny_aug"] + 2 * hfws - 1)
+    want = int((out_x[None, :] | out_y[:, None]).sum())
+    _report("fused deformation", shift=shift, oob_plane=res[0][0], oob_fused=res[2][0], want=want)
+    assert res[0][0] == want and res[2][0] == want
+    for u, w in zip(res[0][1], res[2][1]):
+        np.testing.assert_array_equal(u, w)
+
+
 @pytest.mark.parametrize("ws", [(16, 16), (24, 24), (48, 48), (64, 64), (64, 32), (32, 64), (32, 16), (16, 32),
                                 (48, 24)])
 def test_fast_window_shapes(pivx, orc, ws):
@@ -483,14 +515,18 @@ def test_config2_full_size_properties(pivx):
     assert valid > 0.999
 
 
-def test_run_pair_matches_stepwise_and_is_deterministic(pivx):
+@pytest.mark.parametrize("sp1", [32, 64])
+def test_run_pair_matches_stepwise_and_is_deterministic(pivx, sp1):
     """pivx_run_pair reproduces the reference's main-loop sequence exactly
     (same result as calling the per-pass entry points) and is bit-identical
-    across runs."""
+    across runs.  With grid-1 windows at spacing 64 (no overlap) run_pair
+    deforms frame 2 inside the correlation kernel while the per-pass entry
+    points write the deformed plane: the fields and the out-of-buffer tap
+    counts must still agree bit for bit."""
     nx, ny = 640, 480
     sp_ = pivx.synth_params(nx, ny, seed=4, flow=1, a0=100.0, a1=5.0, a2=0.002)
     a, b = pivx.synth_pair(sp_)
-    p = config1_params(pivx, nx=nx, ny=ny, ws_x1=64, ws_y1=64, spacing_x1=32, spacing_y1=32, npass1=2,
+    p = config1_params(pivx, nx=nx, ny=ny, ws_x1=64, ws_y1=64, spacing_x1=sp1, spacing_y1=sp1, npass1=2,
                        ws_x2=32, ws_y2=32, spacing_x2=16, spacing_y2=16, npass2=2,
                        max_dispx_percent1=0.4, max_dispy_percent1=0.4, max_dispx_percent2=0.4,
                        max_dispy_percent2=0.4)
@@ -500,6 +536,7 @@ def test_run_pair_matches_stepwise_and_is_deterministic(pivx):
         x.prep_img_data(a, b)
         x.run_pair(f1, f2)
         r1 = [x.raw(1), x.filtered(1), x.raw(2), x.filtered(2), x.filtered(3)]
+        oob1 = list(x.stats()["oob_taps"])
         x.run_pair(f1, f2)
         r2 = [x.raw(1), x.filtered(1), x.raw(2), x.filtered(2), x.filtered(3)]
     with pivx.Xcorr2dMulti(p) as y:
@@ -518,10 +555,13 @@ def test_run_pair_matches_stepwise_and_is_deterministic(pivx):
         s = [y.raw(1), y.filtered(1), y.raw(2), y.filtered(2)]
         y.MedianFilter_Smoothing_p2(2, 5, 2.0, 3.0, 0, 0)
         s.append(y.filtered(2))
+        oob_s = list(y.stats()["oob_taps"])
+    _report("run_pair vs per-pass", grid1_spacing=sp1, oob_taps=oob1)
     for A, B, S in zip(r1, r2, s):
         for u, w, z in zip(A, B, S):
             np.testing.assert_array_equal(u, w)
             np.testing.assert_array_equal(u, z)
+    assert oob1 == oob_s
 
 
 def test_graph_replays_match_eager(pivx):
