@@ -167,6 +167,36 @@ def test_fused_deformation_matches_plane(pivx, shift):
         np.testing.assert_array_equal(u, w)
 
 
+@pytest.mark.parametrize("ws", [(64, 64), (48, 48), (64, 32), (32, 64), (48, 24)])
+def test_fused_deformation_window_shapes(pivx, ws):
+    """Every window shape of the wave-per-window kernel with the deformation
+    fused in (spacing = window, a random predictor field, 448x352 frames, of
+    which the 48-wide windows leave a strip uncovered): raw candidates, peak
+    data and out-of-buffer counts identical to the deformed-plane path."""
+    a, b = config1_pair(pivx, seed=21, nx=448, ny=352, u=2.1, v=-1.6)
+    p = config1_params(pivx, nx=448, ny=352, ws_x1=ws[0], ws_y1=ws[1], spacing_x1=ws[0], spacing_y1=ws[1], npass1=2,
+                       max_dispx_percent1=0.4, max_dispy_percent1=0.4)
+    rng = np.random.default_rng(23)
+    res = {}
+    for mode in (0, 2):
+        with pivx.Xcorr2dMulti(p) as x:
+            x.set_fused_deform(mode)
+            g = geom_dict(x.geometry(1))
+            nt = g["nx_vec"] * g["ny_vec"]
+            if mode == 0:
+                fx = (2.1 + 0.7 * rng.standard_normal(nt)).astype(np.float32)
+                fy = (-1.6 + 0.7 * rng.standard_normal(nt)).astype(np.float32)
+            x.prep_img_data(a, b)
+            x.set_filtered(1, fx, fy)
+            x.extend_filtered_vecfield_p1()
+            x.Calc_cc_map_p1(1)
+            res[mode] = (x.stats()["oob_taps"][0], x.raw(1) + x.peaks(1))
+    _report("fused deformation shape", ws=ws, windows=nt, valid=int((res[2][1][2][:, 0] == 1).sum()))
+    assert res[0][0] == res[2][0]
+    for u, w in zip(res[0][1], res[2][1]):
+        np.testing.assert_array_equal(u, w)
+
+
 @pytest.mark.parametrize("ws", [(16, 16), (24, 24), (48, 48), (64, 64), (64, 32), (32, 64), (32, 16), (16, 32),
                                 (48, 24)])
 def test_fast_window_shapes(pivx, orc, ws):
