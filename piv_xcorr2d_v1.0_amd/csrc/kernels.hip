@@ -777,16 +777,22 @@ __global__ void __launch_bounds__(64) k_xcorr_pair(XcArgs a) {
   if (l < WX) {
     const int src = h * 32 + (WX - l) % WX;
     auto mirror = [&](float2 v) { return make_float2(__shfl(v.x, src, 64), __shfl(v.y, src, 64)); };
+    // rows 0..WY/2 from Z and the mirror column (the partner's rows
+    // WY-ky are still Z: only rows 0..WY/2 are overwritten, each after its
+    // own mirror read) ...
     static_for<HY + 1>([&](auto kc) {
       constexpr int ky = kc;
       constexpr int km = (WY - ky) % WY;
-      if constexpr (ky == km) {
-        col[ky] = xprod(col[ky], mirror(col[ky]));
-      } else {
-        const float2 ma = mirror(col[km]), mb = mirror(col[ky]);
-        col[ky] = xprod(col[ky], ma);
-        col[km] = xprod(col[km], mb);
-      }
+      col[ky] = xprod(col[ky], mirror(col[km]));
+    });
+    // ... and rows WY/2+1..WY-1 as conjugates of the partner's rows: the
+    // correlation is real, so P[-ky][-kx] = conj P[ky][kx], and xprod with
+    // its operands swapped is the exact conjugate (same rounded real part,
+    // negated imaginary part)
+    static_for<HY - 1>([&](auto kc) {
+      constexpr int ky = HY + 1 + kc;
+      const float2 pm = mirror(col[WY - ky]);
+      col[ky] = make_float2(pm.x, -pm.y);
     });
     // lanes 32-63 of the even rows <-> lanes 0-31 of the odd rows
     auto swap32 = [](float& lo, float& hi) {
