@@ -128,6 +128,26 @@ __device__ __forceinline__ Bits column_maxima(const float* cc, int cl, int lft, 
   }
 }
 
+// The two largest peak keys (k1 > k2, 0 = none) over the rows in `bits` of
+// column cl of a correlation plane cc (row stride SX, window width WX).
+template <typename Bits, int WX, int SX>
+__device__ __forceinline__ void best_two(const float* cc, Bits bits, int cl, uint64_t& k1, uint64_t& k2) {
+  k1 = 0;
+  k2 = 0;
+  for (Bits rb = bits; rb; rb &= rb - 1) {
+    int j;
+    if constexpr (sizeof(Bits) == 8) {
+      j = __builtin_ctzll(rb);
+    } else {
+      j = __builtin_ctz(rb);
+    }
+    const uint64_t kj = peak_key(cc[j * SX + cl], j * WX + cl);
+    const bool gt1 = kj > k1;
+    k2 = gt1 ? k1 : (kj > k2 ? kj : k2);
+    k1 = gt1 ? kj : k1;
+  }
+}
+
 __device__ __forceinline__ void wave_argmax(float& bv, int& bi) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
@@ -874,19 +894,12 @@ __global__ void __launch_bounds__(64) k_xcorr_pair(XcArgs a) {
   }
   const float px = (win_ok && a.pred_x) ? a.pred_x[w] : 0.f;
   const float py = (win_ok && a.pred_y) ? a.pred_y[w] : 0.f;
-  // lane best over its column's maxima as a peak key (every maximum is > 0,
-  // so key 0 is "none"); the owner of a round's winner clears that row and
-  // refreshes
-  uint64_t mk;
-  auto refresh = [&]() {
-    mk = 0;
-    for (uint32_t rb = bits; rb; rb &= rb - 1) {
-      const int j = __builtin_ctz(rb);
-      const uint64_t kj = peak_key(cc[j * SX + cl], j * WX + cl);
-      mk = kj > mk ? kj : mk;
-    }
-  };
-  refresh();
+  // the lane's best two maxima of its column as peak keys (every maximum is
+  // > 0, so key 0 is "none"); the owner of a round's winner clears that row
+  // and promotes its second, and a lane that wins again rescans its column
+  uint64_t mk, mk2;
+  best_two<uint32_t, WX, SX>(cc, bits, cl, mk, mk2);
+  bool have2 = true;
   for (int k0 = 0; k0 < a.K; k0 += 32) {
     const int kn = min(32, a.K - k0);
     float my_v = 0.f;
@@ -895,7 +908,13 @@ __global__ void __launch_bounds__(64) k_xcorr_pair(XcArgs a) {
       const uint64_t best = group_max_key<32>(mk);
       if (best != 0 && best == mk) {
         bits &= ~((uint32_t)1 << (key_index(mk) / WX));
-        refresh();
+        if (have2) {
+          mk = mk2;
+          have2 = false;
+        } else {
+          best_two<uint32_t, WX, SX>(cc, bits, cl, mk, mk2);
+          have2 = true;
+        }
       }
       if (l == k && best != 0) {
         my_v = key_value(best);
@@ -1114,19 +1133,12 @@ __global__ void __launch_bounds__(64) k_xcorr_single(XcArgs a) {
   }
   const float px = a.pred_x ? a.pred_x[w] : 0.f;
   const float py = a.pred_y ? a.pred_y[w] : 0.f;
-  // lane best over its column's maxima as a peak key (every maximum is > 0,
-  // so key 0 is "none"); the owner of a round's winner clears that row and
-  // refreshes
-  uint64_t mk;
-  auto refresh = [&]() {
-    mk = 0;
-    for (Bits rb = bits; rb; rb &= rb - 1) {
-      const int j = __builtin_ctzll(rb);
-      const uint64_t kj = peak_key(cc[j * SX + cl], j * WX + cl);
-      mk = kj > mk ? kj : mk;
-    }
-  };
-  refresh();
+  // the lane's best two maxima of its column as peak keys (every maximum is
+  // > 0, so key 0 is "none"); the owner of a round's winner clears that row
+  // and promotes its second, and a lane that wins again rescans its column
+  uint64_t mk, mk2;
+  best_two<Bits, WX, SX>(cc, bits, cl, mk, mk2);
+  bool have2 = true;
   for (int k0 = 0; k0 < a.K; k0 += 64) {
     const int kn = min(64, a.K - k0);
     float my_v = 0.f;
@@ -1135,7 +1147,13 @@ __global__ void __launch_bounds__(64) k_xcorr_single(XcArgs a) {
       const uint64_t best = group_max_key<64>(mk);
       if (best != 0 && best == mk) {
         bits &= ~((Bits)1 << (key_index(mk) / WX));
-        refresh();
+        if (have2) {
+          mk = mk2;
+          have2 = false;
+        } else {
+          best_two<Bits, WX, SX>(cc, bits, cl, mk, mk2);
+          have2 = true;
+        }
       }
       if (lane == k && best != 0) {
         my_v = key_value(best);
