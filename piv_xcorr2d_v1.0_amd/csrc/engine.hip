@@ -69,7 +69,8 @@ struct pivx_ctx {
   GridBuf grid[2];
   float *mo_x = nullptr, *mo_y = nullptr;  // grid-2 median-only field (run_pair)
   uint8_t* mo_v = nullptr;
-  uint8_t *f1 = nullptr, *f2 = nullptr;    // frames [ny][nx]
+  uint8_t *f1 = nullptr, *f2 = nullptr;    // frames: f1 [ny][nx]; f2 rows of pitch2 inside a zero border
+  long long pitch2 = 0;
   bool have_frames = false;
   bool in_pair = false;                    // inside pivx_run_pair's sequence
   int fuse_mode = 1;                       // pivx_set_fused_deform
@@ -259,7 +260,7 @@ DefArgs def_args(pivx_ctx* c, int gi) {
   d.nx_aug = g.nx_aug;
   d.ny_aug = g.ny_aug;
   d.f2 = c->f2;
-  d.pitch = c->p.nx;
+  d.pitch = c->pitch2;
   d.nx = c->p.nx;
   d.ny = c->p.ny;
   d.left_x = g.left_x;
@@ -314,7 +315,7 @@ int run_xcorr(pivx_ctx* c, int gi, bool deformed, bool zero_pred = false, bool f
     a.fdef.oob = c->counters + gi;
   }
   a.pitch1 = c->p.nx;
-  a.pitch2 = c->p.nx;
+  a.pitch2 = c->pitch2;
   a.def_pitch = g.nx_aug;
   a.nx = c->p.nx;
   a.ny = c->p.ny;
@@ -557,6 +558,16 @@ int alloc_grid(pivx_ctx* c, int gi) {
   return PIVX_OK;
 }
 
+// Zero border of frame 2 for grid g (pixels left/right, rows top/bottom):
+// the reference's aug_ex buffer of the grid, aug [-hfws, n_aug + hfws - 1)
+// in each axis (image = aug - left, cu:885-894), plus the row below it that
+// the deformation's unchecked tap resource spans.
+void frame2_border(const pivx_geometry& g, int nx, int ny, int& bx, int& by) {
+  const int hx = g.ws_x / 2, hy = g.ws_y / 2;
+  bx = std::max({0, hx + g.left_x, g.nx_aug + hx - 1 - g.left_x - nx});
+  by = std::max({0, hy + g.left_y, g.ny_aug + hy + 2 - g.left_y - ny});
+}
+
 }  // namespace
 
 // ====================================================================== C ABI
@@ -597,7 +608,21 @@ int pivx_init(const pivx_init_params* p, int device_id, pivx_ctx** out) {
   const size_t ntmax = (size_t)std::max(g1.nt_vec, g2.nt_vec);
   rc = 0;
   rc |= dalloc(c, &c->f1, npix);
-  rc |= dalloc(c, &c->f2, npix);
+  // frame 2 inside a zero border wide enough for both grids (the deformation
+  // reads taps of in-range cells from it without checks); cleared once here,
+  // uploads write the interior only
+  int bx1, by1, bx2, by2;
+  frame2_border(g1, p->nx, p->ny, bx1, by1);
+  frame2_border(g2, p->nx, p->ny, bx2, by2);
+  const int bx = (std::max(bx1, bx2) + 3) & ~3, by = std::max(by1, by2);
+  c->pitch2 = p->nx + 2LL * bx;
+  const size_t f2bytes = (size_t)c->pitch2 * ((size_t)p->ny + 2 * by);
+  uint8_t* f2buf = nullptr;
+  rc |= dalloc(c, &f2buf, f2bytes);
+  if (f2buf) {
+    c->f2 = f2buf + (size_t)by * c->pitch2 + bx;
+    if (hipMemsetAsync(f2buf, 0, f2bytes, c->stream) != hipSuccess) rc |= fail(c, PIVX_ERR_HIP, "frame border clear failed");
+  }
   rc |= dalloc(c, &c->counters, 2);
   rc |= dalloc(c, &c->sweeps, 2);
   rc |= dalloc(c, &c->nholes, 1);
@@ -646,11 +671,12 @@ int pivx_get_geometry(const pivx_ctx* c, int grid, pivx_geometry* g) {
   return PIVX_OK;
 }
 
-// MHE of one frame already in c->mhe_in into `out` (device, row pitch nx).
-static int mhe_run(pivx_ctx* c, uint8_t* out) {
+// MHE of one frame already in c->mhe_in into `out` (device, row pitch out_pitch).
+static int mhe_run(pivx_ctx* c, uint8_t* out, long long out_pitch) {
   MheArgs m = c->mhe;
   m.in = c->mhe_in;
   m.out = out;
+  m.out_pitch = out_pitch;
   tic(c);
   const int rc = check_launch(c, launch_mhe(m, c->stream), "mhe_gpu::processsing");
   toc(c, 0, 6);
@@ -666,12 +692,12 @@ static int upload(pivx_ctx* c, const uint8_t* a, const uint8_t* b, size_t pitch,
     // main.cpp:267-283: each frame is enhanced before prep_img_data
     int rc;
     HIPCHK(c, hipMemcpy2DAsync(c->mhe_in, c->p.nx, a, pitch, c->p.nx, c->p.ny, kind, c->stream));
-    if ((rc = mhe_run(c, c->f1))) return rc;
+    if ((rc = mhe_run(c, c->f1, c->p.nx))) return rc;
     HIPCHK(c, hipMemcpy2DAsync(c->mhe_in, c->p.nx, b, pitch, c->p.nx, c->p.ny, kind, c->stream));
-    if ((rc = mhe_run(c, c->f2))) return rc;
+    if ((rc = mhe_run(c, c->f2, c->pitch2))) return rc;
   } else {
     HIPCHK(c, hipMemcpy2DAsync(c->f1, c->p.nx, a, pitch, c->p.nx, c->p.ny, kind, c->stream));
-    HIPCHK(c, hipMemcpy2DAsync(c->f2, c->p.nx, b, pitch, c->p.nx, c->p.ny, kind, c->stream));
+    HIPCHK(c, hipMemcpy2DAsync(c->f2, c->pitch2, b, pitch, c->p.nx, c->p.ny, kind, c->stream));
   }
   if (kind == hipMemcpyHostToDevice && wait) HIPCHK(c, hipStreamSynchronize(c->stream));  // caller may reuse its buffers
   c->have_frames = true;
@@ -755,7 +781,7 @@ int pivx_mhe_enhance(pivx_ctx* c, const uint8_t* img, uint8_t* out, size_t pitch
   if (pitch < (size_t)c->p.nx) return fail(c, PIVX_ERR_PARAM, "pitch %zu < nx %d", pitch, c->p.nx);
   HIPCHK(c, hipSetDevice(c->device));
   HIPCHK(c, hipMemcpy2DAsync(c->mhe_in, c->p.nx, img, pitch, c->p.nx, c->p.ny, hipMemcpyHostToDevice, c->stream));
-  int rc = mhe_run(c, c->mhe_out);
+  int rc = mhe_run(c, c->mhe_out, c->p.nx);
   if (rc) return rc;
   HIPCHK(c, hipMemcpy2DAsync(out, pitch, c->mhe_out, c->p.nx, c->p.nx, c->p.ny, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));

@@ -41,10 +41,16 @@ struct XcArgs {
   FusedDef fdef;          // frame 2 deformed in the gather (wave-per-window kernel only)
 };
 
+// f2 (and XcArgs::f2u8 when frame 2 is deformed in the gather) points at
+// pixel (0, 0) of a frame stored inside a zero border: the bytes of aug
+// coordinates [-hfws_x, nx_aug + hfws_x) x [-hfws_y, ny_aug + hfws_y + 1)
+// (image = aug - left) are readable and 0 outside the image.  Taps of cells
+// whose field samples stay below the out-of-buffer guard are read from it
+// without checks.
 struct DefArgs {
   float* out; int nx_aug, ny_aug;
   const uint8_t* f2; long long pitch; int nx, ny; int left_x, left_y;
-  int hfws_x, hfws_y;     // only for counting taps outside the reference's padded buffer
+  int hfws_x, hfws_y;     // the reference's padded buffer: tap count and the zero border
   const float* ex_x; const float* ex_y; int nxe, nye;
   float x_st, y_st, spx, spy;
   unsigned long long* oob;   // cumulative count of taps beyond the reference's padded buffer
@@ -87,6 +93,7 @@ struct MheArgs {
   const uint8_t* in; uint8_t* out; int nx, ny;
   int ws, xs, ys, tiles_x, tiles_y;
   int ntrash, nnontrash;
+  long long out_pitch;     // row pitch of out (in has row pitch nx)
   uint32_t* hist;          // [(tiles_y+2)*(tiles_x+2)][256]; the border ring is kept at 0
 };
 

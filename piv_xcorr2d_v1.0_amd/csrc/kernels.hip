@@ -265,8 +265,13 @@ __device__ __forceinline__ DefCell load_cell(const float* ex_x, const float* ex_
 // (x + 1) share the vector offset of taps 1 and 3 and add 1 as the scalar
 // offset; with the base one byte early every in-frame tap's vector offset is
 // >= 0.
+// Cells that cannot leave the reference's padded buffer read their taps
+// through a second resource, f2f, whose byte (jy + hfws_y) * pitch + jx is
+// the frame byte at aug (jx, jy): the frame sits in a zero border that covers
+// that buffer, so those taps need no test (jy + hfws_y >= 1 and jx > -pitch
+// keep the offset positive).
 struct DefFrame {
-  __amdgpu_buffer_rsrc_t f2r;
+  __amdgpu_buffer_rsrc_t f2r, f2f;
   unsigned pitch;
   int nxm, nym, left_x, left_y, hfws_x, hfws_y;
   unsigned span_x, span_y;  // the reference's aug_ex buffer: [-hfws, n_aug + hfws - 1)
@@ -283,6 +288,9 @@ __device__ __forceinline__ DefFrame def_frame(const uint8_t* f2, long long pitch
   // byte b of the resource is frame byte b - 1 (byte 0 is never read)
   F.f2r = __builtin_amdgcn_make_buffer_rsrc(const_cast<uint8_t*>(f2) - 1, (short)0, (int)(F.pitch * (unsigned)ny + 1u),
                                             0x00020000);
+  // aug (0, -hfws_y); rows -hfws_y .. ny_aug + hfws_y of the zero border
+  F.f2f = __builtin_amdgcn_make_buffer_rsrc(const_cast<uint8_t*>(f2) - (long long)(left_y + hfws_y) * pitch - left_x,
+                                            (short)0, (int)(F.pitch * (unsigned)(ny_aug + 2 * hfws_y + 1)), 0x00020000);
   F.nxm = nx - 1;
   F.nym = ny - 1;
   F.left_x = left_x;
@@ -295,8 +303,9 @@ __device__ __forceinline__ DefFrame def_frame(const uint8_t* f2, long long pitch
 }
 
 // The deformed value of the aug pixel with column terms (ax, ax1, fidx) and
-// row terms (ay, ay1, fidy) in cell e; out-of-buffer taps are added to oob
-// (COUNT false: the caller has checked that the cell cannot produce any).
+// row terms (ay, ay1, fidy) in cell e; out-of-buffer taps are added to oob.
+// COUNT false: the caller has checked that the cell cannot produce any
+// (e.may_oob false), and the taps are read from the zero border unchecked.
 template <bool COUNT = true>
 __device__ __forceinline__ float deform_px(const DefFrame& F, const DefCell& e, float ax, float ax1, float fidx, float ay,
                                            float ay1, float fidy, unsigned& oob) {
@@ -314,6 +323,15 @@ __device__ __forceinline__ float deform_px(const DefFrame& F, const DefCell& e, 
   ce2 = sx * (1 - sy);
   ce3 = (1 - sx) * sy;
   ce4 = sx * sy;
+  if constexpr (!COUNT) {
+    // |displacement| < hfws - 2: every tap is inside the zero border
+    const unsigned o1 = __umul24((unsigned)(jy + F.hfws_y), F.pitch) + (unsigned)jx, o3 = o1 + F.pitch;
+    const float t1 = (float)__builtin_amdgcn_raw_buffer_load_b8(F.f2f, o1, 0, 0);
+    const float t2 = (float)__builtin_amdgcn_raw_buffer_load_b8(F.f2f, o1, 1, 0);
+    const float t3 = (float)__builtin_amdgcn_raw_buffer_load_b8(F.f2f, o3, 0, 0);
+    const float t4 = (float)__builtin_amdgcn_raw_buffer_load_b8(F.f2f, o3, 1, 0);
+    return fmaf(ce4, t4, fmaf(ce3, t3, fmaf(ce1, t1, ce2 * t2)));
+  }
   const int xi = jx - F.left_x, yi = jy - F.left_y;
   const bool x0 = (unsigned)xi <= (unsigned)F.nxm, x1 = (unsigned)(xi + 1) <= (unsigned)F.nxm;
   const bool yok0 = (unsigned)yi <= (unsigned)F.nym, yok1 = (unsigned)(yi + 1) <= (unsigned)F.nym;
@@ -327,10 +345,8 @@ __device__ __forceinline__ float deform_px(const DefFrame& F, const DefCell& e, 
   const float t2 = (float)__builtin_amdgcn_raw_buffer_load_b8(F.f2r, q2, 1, 0);
   const float t3 = (float)__builtin_amdgcn_raw_buffer_load_b8(F.f2r, q3, 0, 0);
   const float t4 = (float)__builtin_amdgcn_raw_buffer_load_b8(F.f2r, q4, 1, 0);
-  if constexpr (COUNT) {
-    if (__builtin_expect(e.may_oob, 0))
-      oob += ((unsigned)(jx + F.hfws_x) >= F.span_x || (unsigned)(jy + F.hfws_y) >= F.span_y) ? 1u : 0u;
-  }
+  if (__builtin_expect(e.may_oob, 0))
+    oob += ((unsigned)(jx + F.hfws_x) >= F.span_x || (unsigned)(jy + F.hfws_y) >= F.span_y) ? 1u : 0u;
   return fmaf(ce4, t4, fmaf(ce3, t3, fmaf(ce1, t1, ce2 * t2)));
 }
 
@@ -739,7 +755,8 @@ __device__ __forceinline__ void deform_window(const XcArgs& a, int ax0, int ay0,
           cell = iyf[k];
           e = load_cell(d.ex_x, d.ex_y, d.nxe, ixf, cell, oob_guard);
         }
-        p[(r0 + k) * SX + lane] = deform_px(F, e, ct.x, ct.y, ct.w, ay[k], ay1[k], fidy[k], oob);
+        p[(r0 + k) * SX + lane] = e.may_oob ? deform_px<true>(F, e, ct.x, ct.y, ct.w, ay[k], ay1[k], fidy[k], oob)
+                                            : deform_px<false>(F, e, ct.x, ct.y, ct.w, ay[k], ay1[k], fidy[k], oob);
       });
     }
   }
@@ -1396,12 +1413,38 @@ __global__ void __launch_bounds__(256) k_deform(DefArgs a) {
       cell = iyf;
       e = load_cell(a.ex_x, a.ex_y, a.nxe, ixf, iyf, oob_guard);
     }
-    const float v = deform_px(F, e, ct.x, ct.y, ct.w, rt.x, rt.y, rt.w, oob);
+    const float v = e.may_oob ? deform_px<true>(F, e, ct.x, ct.y, ct.w, rt.x, rt.y, rt.w, oob)
+                              : deform_px<false>(F, e, ct.x, ct.y, ct.w, rt.x, rt.y, rt.w, oob);
     __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), outr, (int)(idx * 4), (int)(r * row_bytes), 0);
   };
-  // full blocks unrolled (row terms at constant LDS offsets)
+  // full blocks unrolled (row terms at constant LDS offsets), in groups of 8
+  // rows: a group inside one cell whose samples stay below the guard has no
+  // branch, and its 32 frame taps per lane are in flight together
   if (nrow == DEF_ROWS) {
-    static_for<DEF_ROWS>([&](auto rc) { row(rc); });
+    static_for<DEF_ROWS / 8>([&](auto gc) {
+      constexpr int r0 = 8 * gc;
+      float4 rt[8];
+      static_for<8>([&](auto k) { rt[k] = rowp[r0 + k]; });
+      const int iy0 = __builtin_amdgcn_readfirstlane(__float_as_int(rt[0].z));
+      const int iy7 = __builtin_amdgcn_readfirstlane(__float_as_int(rt[7].z));
+      if (iy0 == iy7) {  // cells are monotone in the row
+        if (iy0 != cell) {
+          cell = iy0;
+          e = load_cell(a.ex_x, a.ex_y, a.nxe, ixf, iy0, oob_guard);
+        }
+        float v[8];
+        if (__builtin_expect(e.may_oob, 0)) {
+          static_for<8>([&](auto k) { v[k] = deform_px<true>(F, e, ct.x, ct.y, ct.w, rt[k].x, rt[k].y, rt[k].w, oob); });
+        } else {
+          static_for<8>([&](auto k) { v[k] = deform_px<false>(F, e, ct.x, ct.y, ct.w, rt[k].x, rt[k].y, rt[k].w, oob); });
+        }
+        static_for<8>([&](auto k) {
+          __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v[k]), outr, (int)(idx * 4), (int)((r0 + k) * row_bytes), 0);
+        });
+      } else {
+        static_for<8>([&](auto k) { row(r0 + k); });
+      }
+    });
   } else {
     for (int r = 0; r < nrow; ++r) row(r);
   }
@@ -2032,7 +2075,7 @@ __global__ void __launch_bounds__(256) k_mhe_map(MheArgs a) {
   if (w == a.ws && yhi - ylo == a.ws && (a.ws & 3) == 0) {
     // the rectangle is the tile itself: 4 pixels per load and store
     const int q4 = a.ws >> 2, nd = a.ws * q4;
-    const long long ob = (long long)y0t * a.nx + x0t;
+    const long long ob = (long long)y0t * a.nx + x0t, obo = (long long)y0t * a.out_pitch + x0t;
     auto word = [&](int d) {
       const int r = d / q4, c = d - r * q4;
       uint32_t x;
@@ -2043,7 +2086,7 @@ __global__ void __launch_bounds__(256) k_mhe_map(MheArgs a) {
       const int r = d / q4, c = d - r * q4;
       const uint32_t m = (uint32_t)lut[x & 255u] | ((uint32_t)lut[(x >> 8) & 255u] << 8) |
                          ((uint32_t)lut[(x >> 16) & 255u] << 16) | ((uint32_t)lut[x >> 24] << 24);
-      __builtin_memcpy(a.out + ob + (long long)r * a.nx + 4 * c, &m, 4);
+      __builtin_memcpy(a.out + obo + (long long)r * a.out_pitch + 4 * c, &m, 4);
     };
     int d0 = lane;
     for (; d0 + 64 * (MHE_U - 1) < nd; d0 += 64 * MHE_U) {
@@ -2061,7 +2104,7 @@ __global__ void __launch_bounds__(256) k_mhe_map(MheArgs a) {
     const int r = q / w, y = ylo + r, x = xlo + (q - r * w);
     const long long o = (long long)y * a.nx + x;
     const bool in_tile = x >= x0t && x < x0t + a.ws && y >= y0t && y < y0t + a.ws;
-    a.out[o] = in_tile ? lut[a.in[o]] : (uint8_t)0;
+    a.out[(long long)y * a.out_pitch + x] = in_tile ? lut[a.in[o]] : (uint8_t)0;
   }
 }
 
