@@ -359,10 +359,11 @@ int run_xcorr(pivx_ctx* c, int gi, bool deformed, bool zero_pred = false, bool f
 // the smoothing ping-pong ends in (ox, oy); earlier median passes alternate
 // between two other buffer sets.  With `ext` set and at least one smoothing
 // pass, the last smoothing pass also writes the extended field (ext_x, ext_y)
-// and *ext_done is set.
+// and *ext_done is set.  With (mx, my, mv) set, the field after the median
+// stage (before filling and smoothing) is also written there.
 int run_filter(pivx_ctx* c, int gi, int npass_median, int min_nneigh, float insert_thr, int npass_smooth, int iffill,
                float* ox, float* oy, uint8_t* ov, float* ext_x = nullptr, float* ext_y = nullptr,
-               bool* ext_done = nullptr) {
+               bool* ext_done = nullptr, float* mx = nullptr, float* my = nullptr, uint8_t* mv = nullptr) {
   GridBuf& G = c->grid[gi];
   const pivx_geometry& g = G.g;
   const int nt = g.nt_vec;
@@ -381,6 +382,7 @@ int run_filter(pivx_ctx* c, int gi, int npass_median, int min_nneigh, float inse
   tic(c);
   if (npass_median <= 0) {
     rc = check_launch(c, launch_prep(G.dx, G.dy, G.dv, K, t0x, t0y, ov, nt, c->stream), "median_filter_data_prep");
+    if (!rc && mx) rc = check_launch(c, launch_prep(G.dx, G.dy, G.dv, K, mx, my, mv, nt, c->stream), "median_filter_data_prep");
     if (rc) return rc;
   } else {
     // intermediate sets: I1 = (ux, uy, uv), I2 = whichever of (ox, oy) / (sx, sy) is not T0, with sv
@@ -407,6 +409,11 @@ int run_filter(pivx_ctx* c, int gi, int npass_median, int min_nneigh, float inse
       m.dx = tx;
       m.dy = ty;
       m.dv = tv;
+      if (back == 0) {
+        m.dx2 = mx;
+        m.dy2 = my;
+        m.dv2 = mv;
+      }
       m.nx_vec = g.nx_vec;
       m.ny_vec = g.ny_vec;
       m.wx = wxm;
@@ -881,16 +888,19 @@ static int run_pair_enqueue(pivx_ctx* c, const pivx_filter_params* f1, const piv
     if (!ext_ok && (rc = extend(c, 0))) return rc;
     if ((rc = upsample(c, true))) return rc;
     ext_ok = true;
+    // The median-only re-run of main.cpp:363-375 filters the last raw
+    // candidates with the same median settings and stops after the median
+    // passes: its field is the last pass's field before filling and
+    // smoothing, which that pass's last median launch also writes to mo.
     for (int j = 0; j < c->p.npass2; ++j) {
       if (!ext_ok && (rc = extend(c, 1))) return rc;
       if ((rc = pivx_calc_cc_map_p2(c))) return rc;
+      const bool last = j == c->p.npass2 - 1;
       if ((rc = run_filter(c, 1, f2->npass_median, f2->min_nneigh, f2->insert_thr, f2->npass_smooth, 1, G2.fx, G2.fy,
-                           G2.fv, G2.ex, G2.ey, &ext_ok)))
+                           G2.fv, G2.ex, G2.ey, &ext_ok, last ? c->mo_x : nullptr, last ? c->mo_y : nullptr,
+                           last ? c->mo_v : nullptr)))
         return rc;
     }
-    // median-only re-run on the last raw candidates (main.cpp:363-375)
-    if ((rc = run_filter(c, 1, f2->npass_median, f2->min_nneigh, f2->insert_thr, 0, 0, c->mo_x, c->mo_y, c->mo_v)))
-      return rc;
   }
   return PIVX_OK;
 }

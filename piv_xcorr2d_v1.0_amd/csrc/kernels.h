@@ -60,6 +60,7 @@ struct MedArgs {
   const float* sx; const float* sy; const uint8_t* sv; int sstride;   // neighbourhood field
   const float* cx; const float* cy; const uint8_t* cv; int K;         // raw candidates
   float* dx; float* dy; uint8_t* dv;                                  // output filtered field
+  float* dx2; float* dy2; uint8_t* dv2;  // optional second copy of the output (null: none)
   int nx_vec, ny_vec, wx, wy, hfwx, hfwy, min_nneigh;
   float thr, eps;
   int8_t* accepted;        // [nt] index of the accepted candidate, -1 if none (may be null)

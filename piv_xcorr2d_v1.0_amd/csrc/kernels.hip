@@ -1587,6 +1587,11 @@ __device__ __forceinline__ void median_decide(const MedArgs& a, int t, bool vmed
   a.dx[t] = ox;
   a.dy[t] = oy;
   a.dv[t] = ok ? 1 : 0;
+  if (a.dx2) {
+    a.dx2[t] = ox;
+    a.dy2[t] = oy;
+    a.dv2[t] = ok ? 1 : 0;
+  }
   if (a.accepted) a.accepted[t] = (int8_t)acc;
   if (a.hole_list && !ok) {
     const int slot = atomicAdd(a.nholes, 1);
@@ -1742,6 +1747,10 @@ __global__ void __launch_bounds__(256) k_median2(MedArgs a) {
   if (c == 0) {
     a.dx[t] = ox;
     a.dv[t] = okc ? 1 : 0;
+    if (a.dx2) {
+      a.dx2[t] = ox;
+      a.dv2[t] = okc ? 1 : 0;
+    }
     if (a.accepted) a.accepted[t] = (int8_t)acc;
     if (a.hole_list && !okc) {
       const int slot = atomicAdd(a.nholes, 1);
@@ -1749,6 +1758,7 @@ __global__ void __launch_bounds__(256) k_median2(MedArgs a) {
     }
   } else {
     a.dy[t] = oy;
+    if (a.dy2) a.dy2[t] = oy;
   }
 }
 
