@@ -205,6 +205,38 @@ __device__ __forceinline__ bool subpixel_fit(const float* cc, int sx, int wsx, i
   return true;
 }
 
+// subpixel_fit with its two axes on a lane pair: the even lane of the pair
+// fits x, the odd lane y (same operations as above), the pair exchanges the
+// coordinates and both apply the tests.  Lanes 2k and 2k + 1 of a W-lane
+// group hold the same peak (v at pixel idx, -1 / 0 for none); returns its
+// validity and writes the displacement on both lanes.
+template <int W>
+__device__ __forceinline__ bool subpixel_fit_pair(const float* cc, int sx, int wsx, int wsy, int idx, float v,
+                                                  float px, float py, float lx1, float lx2, float ly1, float ly2,
+                                                  bool yaxis, float& dx, float& dy) {
+#pragma clang fp contract(off)
+  const int i = idx % wsx, j = idx / wsx;
+  const bool fit = v > 0.f && i != 0 && i != wsx - 1 && j != 0 && j != wsy - 1;
+  float d = 0.f;
+  if (fit) {
+    const int c = j * sx + i, step = yaxis ? sx : 1;
+    const float f1 = logf(v);
+    const float f0 = logf(cc[c - step]);
+    const float f2 = logf(cc[c + step]);
+    d = (float)(yaxis ? j : i) + (f0 - f2) / (f0 + f2 - 2 * f1) / 2 - (float)(yaxis ? wsy : wsx) / 2 + (yaxis ? py : px);
+  }
+  const float od = __shfl_xor(d, 1, W);
+  const float x = yaxis ? od : d, y = yaxis ? d : od;
+  dx = 0.f;
+  dy = 0.f;
+  if (!fit) return false;
+  if (x < lx1 || x > lx2 || y < ly1 || y > ly2) return false;
+  if (isnan(x) || isnan(y) || isinf(x) || isinf(y)) return false;
+  dx = x;
+  dy = y;
+  return true;
+}
+
 // ============================================================== deformation per pixel
 // Kernel_Window_Deformation_2d (cu:608-652): for every aug pixel, bilinear
 // displacement from the extended filtered field, then bilinear sample of frame
@@ -917,8 +949,10 @@ __global__ void __launch_bounds__(64) k_xcorr_pair(XcArgs a) {
   uint64_t mk, mk2;
   best_two<uint32_t, WX, SX>(cc, bits, cl, mk, mk2);
   bool have2 = true;
-  for (int k0 = 0; k0 < a.K; k0 += 32) {
-    const int kn = min(32, a.K - k0);
+  // rounds in groups of 16: round k's winner is kept by lanes 2k and 2k + 1,
+  // which fit its x and its y
+  for (int k0 = 0; k0 < a.K; k0 += 16) {
+    const int kn = min(16, a.K - k0);
     float my_v = 0.f;
     int my_i = -1;
     for (int k = 0; k < kn; ++k) {
@@ -933,22 +967,24 @@ __global__ void __launch_bounds__(64) k_xcorr_pair(XcArgs a) {
           have2 = true;
         }
       }
-      if (l == k && best != 0) {
+      if ((l >> 1) == k && best != 0) {
         my_v = key_value(best);
         my_i = key_index(best);
       }
     }
-    if (win_ok && l < kn) {
-      const long long o = (long long)w * a.K + k0 + l;
-      float dx = 0.f, dy = 0.f;
-      bool ok = false;
-      if (my_v > 0.f)
-        ok = subpixel_fit(cc, SX, WX, WY, my_i % WX, my_i / WX, my_v, px, py, a.lx1, a.lx2, a.ly1, a.ly2, dx, dy);
-      a.disp_x[o] = dx;
-      a.disp_y[o] = dy;
-      a.valid[o] = ok ? 1 : 0;
-      a.pk_val[o] = my_v;
-      a.pk_idx[o] = my_i;
+    if (win_ok && l < 2 * kn) {
+      const long long o = (long long)w * a.K + k0 + (l >> 1);
+      const bool ya = (l & 1) != 0;
+      float dx, dy;
+      const bool ok = subpixel_fit_pair<32>(cc, SX, WX, WY, my_i, my_v, px, py, a.lx1, a.lx2, a.ly1, a.ly2, ya, dx, dy);
+      if (ya) {
+        a.disp_y[o] = dy;
+        a.pk_idx[o] = my_i;
+      } else {
+        a.disp_x[o] = dx;
+        a.valid[o] = ok ? 1 : 0;
+        a.pk_val[o] = my_v;
+      }
     }
   }
 }
@@ -1156,8 +1192,10 @@ __global__ void __launch_bounds__(64) k_xcorr_single(XcArgs a) {
   uint64_t mk, mk2;
   best_two<Bits, WX, SX>(cc, bits, cl, mk, mk2);
   bool have2 = true;
-  for (int k0 = 0; k0 < a.K; k0 += 64) {
-    const int kn = min(64, a.K - k0);
+  // rounds in groups of 32: round k's winner is kept by lanes 2k and 2k + 1,
+  // which fit its x and its y
+  for (int k0 = 0; k0 < a.K; k0 += 32) {
+    const int kn = min(32, a.K - k0);
     float my_v = 0.f;
     int my_i = -1;
     for (int k = 0; k < kn; ++k) {
@@ -1172,22 +1210,24 @@ __global__ void __launch_bounds__(64) k_xcorr_single(XcArgs a) {
           have2 = true;
         }
       }
-      if (lane == k && best != 0) {
+      if ((lane >> 1) == k && best != 0) {
         my_v = key_value(best);
         my_i = key_index(best);
       }
     }
-    if (lane < kn) {
-      const long long o = (long long)w * a.K + k0 + lane;
-      float dx = 0.f, dy = 0.f;
-      bool ok = false;
-      if (my_v > 0.f)
-        ok = subpixel_fit(cc, SX, WX, WY, my_i % WX, my_i / WX, my_v, px, py, a.lx1, a.lx2, a.ly1, a.ly2, dx, dy);
-      a.disp_x[o] = dx;
-      a.disp_y[o] = dy;
-      a.valid[o] = ok ? 1 : 0;
-      a.pk_val[o] = my_v;
-      a.pk_idx[o] = my_i;
+    if (lane < 2 * kn) {
+      const long long o = (long long)w * a.K + k0 + (lane >> 1);
+      const bool ya = (lane & 1) != 0;
+      float dx, dy;
+      const bool ok = subpixel_fit_pair<64>(cc, SX, WX, WY, my_i, my_v, px, py, a.lx1, a.lx2, a.ly1, a.ly2, ya, dx, dy);
+      if (ya) {
+        a.disp_y[o] = dy;
+        a.pk_idx[o] = my_i;
+      } else {
+        a.disp_x[o] = dx;
+        a.valid[o] = ok ? 1 : 0;
+        a.pk_val[o] = my_v;
+      }
     }
   }
 }
