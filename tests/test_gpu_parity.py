@@ -135,6 +135,55 @@ def test_deformation_out_of_buffer_taps(pivx, shift):
         assert want == 0
 
 
+@pytest.mark.parametrize("ws,big", [(32, (13.0, 24.0)), (64, (29.0, 45.0))])
+def test_deformation_mixed_cells_bitexact(pivx, orc, ws, big):
+    """A field of small vectors (every tap inside frame 2's zero border, read
+    without range tests) with scattered vectors beyond hfws - 2 (cells whose
+    taps may leave the reference's padded buffer, read with the checked
+    path; the corner vector sends taps beyond it), so that both kinds of cell
+    share waves.  The deformed plane equals the FP32 transcription bit for
+    bit; at 64x64 windows on a spacing of 64 the deformation fused into the
+    correlation kernel gives the same correlation results and out-of-buffer
+    count as the deformed plane."""
+    a, b = config1_pair(pivx, seed=5)
+    p = config1_params(pivx, ws_x1=ws, ws_y1=ws, spacing_x1=ws // 2 if ws == 32 else ws,
+                       spacing_y1=ws // 2 if ws == 32 else ws, npass1=2)
+    rng = np.random.default_rng(29)
+    res = {}
+    for mode in ((0,) if ws == 32 else (0, 2)):
+        with pivx.Xcorr2dMulti(p) as x:
+            x.set_fused_deform(mode)
+            g = geom_dict(x.geometry(1))
+            nvx, nvy = g["nx_vec"], g["ny_vec"]
+            nt = nvx * nvy
+            if mode == 0:
+                fx = (2.0 + rng.standard_normal(nt)).astype(np.float32)
+                fy = (-1.0 + rng.standard_normal(nt)).astype(np.float32)
+                pick = rng.random(nt) < 0.15
+                mag = rng.uniform(big[0], big[1], size=(2, nt)) * rng.choice([-1.0, 1.0], size=(2, nt))
+                fx[pick] = mag[0, pick]
+                fy[pick] = mag[1, pick]
+                fx[0] = fy[0] = -big[1]
+            x.prep_img_data(a, b)
+            x.set_filtered(1, fx, fy)
+            x.extend_filtered_vecfield_p1()
+            x.Calc_cc_map_p1(1)
+            res[mode] = (x.stats()["oob_taps"][0], x.raw(1) + x.peaks(1))
+            if mode == 0:
+                d = x.deformed(1)
+    ex, ey = orc.extend(fx.reshape(nvy, nvx), fy.reshape(nvy, nvx))
+    sp = ws // 2 if ws == 32 else ws
+    d32 = orc.deform(b, g, (ws, ws), (sp, sp), ex, ey, prec=32)
+    _report("deformation mixed cells", ws=ws, large=int(pick.sum()), oob=res[0][0],
+            max_diff_f32=float(np.abs(d - d32).max()))
+    np.testing.assert_array_equal(d, d32.astype(np.float32))
+    assert res[0][0] > 0
+    if 2 in res:
+        assert res[2][0] == res[0][0]
+        for u, w in zip(res[0][1], res[2][1]):
+            np.testing.assert_array_equal(u, w)
+
+
 @pytest.mark.parametrize("shift", [(40, -36), (-35, 45), (10, -6)])
 def test_fused_deformation_matches_plane(pivx, shift):
     """Deformation fused into the correlation kernel (grid-1 64x64 windows at
