@@ -708,13 +708,16 @@ def test_mhe_bitexact(pivx, orc, nx, ny, ws, thr):
     _report(f"mhe {nx}x{ny} ws={ws} thr={thr}", pixels=nx * ny, changed=int((got != a).sum()))
 
 
-def test_mhe_run_pair_equals_enhanced_frames(pivx, orc):
+@pytest.mark.parametrize("sp1", [32, 64])
+def test_mhe_run_pair_equals_enhanced_frames(pivx, orc, sp1):
     """With MHE switched on, run_pair equals run_pair on frames the oracle
-    enhanced beforehand (main.cpp:267-283 order: enhance, then prep)."""
+    enhanced beforehand (main.cpp:267-283 order: enhance, then prep).  The
+    enhanced frame 2 is written into its zero-bordered buffer; at a grid-1
+    spacing of 64 run_pair also deforms it inside the correlation kernel."""
     nx, ny = 768, 512
     sp_ = pivx.synth_params(nx, ny, seed=21, flow=1, a0=100.0, a1=5.0, a2=0.002)
     a, b = pivx.synth_pair(sp_)
-    p = config1_params(pivx, nx=nx, ny=ny, ws_x1=64, ws_y1=64, spacing_x1=32, spacing_y1=32, npass1=2,
+    p = config1_params(pivx, nx=nx, ny=ny, ws_x1=64, ws_y1=64, spacing_x1=sp1, spacing_y1=sp1, npass1=2,
                        ws_x2=32, ws_y2=32, spacing_x2=16, spacing_y2=16, npass2=2)
     f = pivx.FilterParams(2, 5, 2.0, 2.0, 1)
     with pivx.Xcorr2dMulti(p) as x:
