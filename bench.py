@@ -29,7 +29,8 @@ import time
 REPO = os.path.dirname(os.path.abspath(__file__))
 
 F_WIN = {(32, 32): 3 * 5 * 1024 * 10 + 10 * 1024, (64, 64): 3 * 5 * 4096 * 12 + 10 * 4096}  # flops per window
-PEAK_FP32_TFLOPS = 157.3
+PEAK_FP32_TFLOPS = 157.3          # MI355X FP32 vector peak, which needs packed FP32 (v_pk_fma_f32)
+UNPACKED_FP32_TFLOPS = 78.6       # the same pipe issuing one FP32 FMA per lane per cycle (this build: no packed FP32)
 PEAK_HBM_GBS = 8000.0
 
 
@@ -319,13 +320,16 @@ def main():
             "eager_ms_per_pair": round(dt_eager * 1000.0 / args.steps, 4),
         },
         "roofline": {
-            "bound": "mfma",
+            "bound": "valu",
             "kernel": "k_xcorr_pair<32,32> (grid-2 fused correlate-and-peak)",
             "achieved": round(achieved, 3),
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
             "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
+            "unpacked_fp32_peak": UNPACKED_FP32_TFLOPS,
             "traffic": traffic,
+            "traffic_source": "calibrated FETCH_SIZE + WRITE_SIZE per launch from the committed rocprofv3 counter "
+                              "passes (profiles/traffic.json), not measured in this run",
             "avg_launch_ms": round(corr2_ms, 5),
             "launches": n_corr2,
             "flops_per_window": f_win,

@@ -76,6 +76,10 @@ typedef struct pivx_stats {
   long long outliers[2];          /* last median pass: vectors whose first candidate was not the one accepted */
   int fill_sweeps[2];             /* last hole filling: sweeps run, counted like the reference's loop */
   long long oob_taps[2];          /* cumulative since init: deformation taps beyond the reference's padded buffer (read as 0) */
+  long long hole_list_drops;      /* cumulative since init: invalid vectors the hole filling could not list (0 unless
+                                     the hole counter was left stale; pivx_synchronize returns PIVX_ERR_STATE when it
+                                     finds new ones) */
+  int holes_pending;              /* hole counter between filter calls: 0 whenever the stream is idle */
 } pivx_stats;
 
 /* ------------------------------------------------------------------ lifecycle */
@@ -95,6 +99,11 @@ int pivx_freemem(pivx_ctx* ctx);
 const char* pivx_last_error(const pivx_ctx* ctx);
 
 int pivx_get_geometry(const pivx_ctx* ctx, int grid, pivx_geometry* g);
+
+/* The geometry pivx_init derives for `grid` (cu:751-894), computed on the host
+ * without a device (no reference counterpart: the reference derives it inside
+ * init).  Applies pivx_init's parameter checks and returns their errors. */
+int pivx_compute_geometry(const pivx_init_params* p, int grid, pivx_geometry* g);
 
 /* ------------------------------------------------------------------ inputs */
 

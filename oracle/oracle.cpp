@@ -235,9 +235,22 @@ struct Cand {
   int i;
 };
 
+// Largest change of the 3-point fit d = (L0 - L2) / (L0 + L2 - 2 L1) / 2
+// (L = log of the three plane values f0, f1, f2 along one axis) when each
+// value moves by at most delta: first-order bound sum_i |dd/dL_i| * delta / f_i.
+inline double fit_sensitivity(double f0, double f1, double f2, double delta) {
+  if (!(f0 > 0 && f1 > 0 && f2 > 0)) return 0.0;  // log of a non-positive value: NaN, the vector is invalid
+  const double L0 = std::log(f0), L1 = std::log(f1), L2 = std::log(f2);
+  const double N = L0 - L2, D = L0 + L2 - 2 * L1;
+  if (D == 0) return INFINITY;
+  const double d0 = (D - N) / (2 * D * D), d2 = (-D - N) / (2 * D * D), d1 = N / (D * D);
+  return std::fabs(d0) * delta / f0 + std::fabs(d1) * delta / f1 + std::fabs(d2) * delta / f2;
+}
+
 template <class T>
 void peaks_and_fit(const T* cc, int wsx, int wsy, int bx, int by, int K, T px, T py, T lx1, T lx2, T ly1, T ly2,
-                   double tie_rel, double* dxo, double* dyo, uint8_t* vo, double* pvo, int* pio, int* nc, uint8_t* tie) {
+                   double tie_rel, double fit_rel, double fit_tol, double* dxo, double* dyo, uint8_t* vo, double* pvo, int* pio,
+                   int* nc, uint8_t* tie) {
   std::vector<Cand> c;
   for (int j = by; j <= wsy - 1 - by; ++j)
     for (int i = bx; i <= wsx - 1 - bx; ++i) {
@@ -298,7 +311,23 @@ void peaks_and_fit(const T* cc, int wsx, int wsy, int bx, int by, int K, T px, T
           if (std::fabs(v) < delta) t = true;
         }
     }
-    *tie = t ? 1 : 0;
+    // an ill-conditioned fit (a nearly flat peak): values within fit_rel *
+    // peak move the sub-pixel displacement of a valid candidate by more than
+    // fit_tol
+    bool flat = false;
+    const double fdelta = fit_rel * top;
+    if (!t && fdelta > 0 && fit_tol > 0) {
+      for (int k = 0; k < K && k < (int)c.size() && !flat; ++k) {
+        if (vo && !vo[k]) continue;
+        const int i = c[k].i % wsx, j = c[k].i / wsx;
+        if (i == 0 || i == wsx - 1 || j == 0 || j == wsy - 1) continue;
+        const double f1 = (double)cc[j * wsx + i];
+        const double sx = fit_sensitivity((double)cc[j * wsx + i - 1], f1, (double)cc[j * wsx + i + 1], fdelta);
+        const double sy = fit_sensitivity((double)cc[(j - 1) * wsx + i], f1, (double)cc[(j + 1) * wsx + i], fdelta);
+        if (std::max(sx, sy) > fit_tol) flat = true;
+      }
+    }
+    *tie = t ? 1 : (flat ? 2 : 0);
   }
 }
 
@@ -340,7 +369,7 @@ int xcorr_pass(const orc_pass* P, double* dx, double* dy, uint8_t* vv, double* p
       const T px = P->pred_x ? (T)P->pred_x[w] : (T)0, py = P->pred_y ? (T)P->pred_y[w] : (T)0;
       const long o = (long)w * K;
       peaks_and_fit<T>(cc.data(), P->wsx, P->wsy, P->bx, P->by, K, px, py, (T)P->lx1, (T)P->lx2, (T)P->ly1,
-                       (T)P->ly2, P->tie_rel, dx ? dx + o : nullptr, dy ? dy + o : nullptr, vv ? vv + o : nullptr,
+                       (T)P->ly2, P->tie_rel, P->fit_rel, P->fit_tol, dx ? dx + o : nullptr, dy ? dy + o : nullptr, vv ? vv + o : nullptr,
                        pv ? pv + o : nullptr, pi ? pi + o : nullptr, nc ? nc + w : nullptr, tie ? tie + w : nullptr);
     }
   }
@@ -752,6 +781,10 @@ int run_pair(const orc_run* R, const uint8_t* f1, const uint8_t* f2, double* r1x
 }  // namespace
 
 extern "C" {
+
+double orc_fit_sensitivity(double f0, double f1, double f2, double delta) {
+  return fit_sensitivity(f0, f1, f2, delta);
+}
 
 int orc_geometry(int nx, int ny, int nsec, int wsx, int wsy, int spx, int spy, float pctx, float pcty, int* out) {
   if (spx < 1 || spy < 1 || !out) return 1;

@@ -50,16 +50,24 @@ typedef struct orc_pass {
   double tie_rel;           /* relative tie margin for the tie flags */
   int threads;              /* OpenMP threads (<= 0: 1) */
   int win_begin, win_end;   /* window range [begin, end); end <= 0 means all */
+  double fit_tol;           /* > 0: also flag windows whose sub-pixel fit moves by more than fit_tol px */
+  double fit_rel;           /*   when the plane values move by fit_rel * peak (nearly flat peaks) */
 } orc_pass;
 
 /* One correlation pass (A3-A15): uncapped 4-neighbour local maxima, top-K in
  * (value desc, raster index asc) order, 3-point Gaussian fit with predictor
  * and limits, NaN/Inf -> invalid.  Outputs [nt][K] (pk_idx -1 when no
  * candidate), ncand [nt], tie [nt] = 1 when the window's top-K order or
- * local-maximum set is within tie_rel * peak of flipping.  Any output may be
+ * local-maximum set is within tie_rel * peak of flipping, 2 when instead a
+ * valid candidate's fit is ill-conditioned (see fit_tol).  Any output may be
  * NULL. */
 int orc_xcorr_pass(int prec, const orc_pass* P, double* disp_x, double* disp_y, uint8_t* valid, double* pk_val,
                    int* pk_idx, int* ncand, uint8_t* tie);
+
+/* First-order bound on how far the 3-point Gaussian fit of one axis
+ * (cu:334-340) moves when each of its three plane values f0 (left), f1 (peak),
+ * f2 (right) moves by at most delta; used for the ill-conditioned-fit flag. */
+double orc_fit_sensitivity(double f0, double f1, double f2, double delta);
 
 /* Kernel_Window_Deformation_2d (cu:608-652); taps outside the image read 0. */
 int orc_deform(int prec, const uint8_t* f2, int nx, int ny, int left_x, int left_y, int nx_aug, int ny_aug,

@@ -26,7 +26,8 @@ class OrcPass(C.Structure):
                 ("bx", C.c_int), ("by", C.c_int), ("K", C.c_int),
                 ("lx1", C.c_double), ("lx2", C.c_double), ("ly1", C.c_double), ("ly2", C.c_double),
                 ("pred_x", C.c_void_p), ("pred_y", C.c_void_p), ("method", C.c_int), ("tie_rel", C.c_double),
-                ("threads", C.c_int), ("win_begin", C.c_int), ("win_end", C.c_int)]
+                ("threads", C.c_int), ("win_begin", C.c_int), ("win_end", C.c_int), ("fit_tol", C.c_double),
+                ("fit_rel", C.c_double)]
 
 
 class OrcRun(C.Structure):
@@ -59,6 +60,8 @@ def lib():
         L.orc_smooth.argtypes = [C.c_int, _D, _D, C.c_int, C.c_int, C.c_int, C.c_int, _D, _D]
         L.orc_run_pair.argtypes = [C.c_int, C.POINTER(OrcRun), _D, _D] + [_D] * 15
         L.orc_mhe.argtypes = [_D, _D, C.c_int, C.c_int, C.c_int, C.c_float]
+        L.orc_fit_sensitivity.argtypes = [C.c_double] * 4
+        L.orc_fit_sensitivity.restype = C.c_double
         _lib = L
     return _lib
 
@@ -99,8 +102,14 @@ def fft1d(x, sign=-1, prec=64):
 
 
 def xcorr_pass(f1, f2u8=None, f2def=None, *, geom, ws, sp, K, lims=(-100, 100, -100, 100), pred=None, method=0,
-               tie_rel=0.0, prec=64, threads=1, win_range=None):
-    """One correlation pass of a grid.  geom: dict from geometry(); ws, sp: (x, y)."""
+               tie_rel=0.0, prec=64, threads=1, win_range=None, fit_tol=1e-3, fit_rel=None):
+    """One correlation pass of a grid.  geom: dict from geometry(); ws, sp: (x, y).
+    tie [nt]: 1 where values within tie_rel * peak can change the candidates,
+    2 where values within fit_rel * peak (default: one FP32 rounding per FFT
+    stage, 2^-23 * log2(N), when tie_rel is set) can instead move a valid
+    sub-pixel fit by more than fit_tol px."""
+    if fit_rel is None:
+        fit_rel = 2.0 ** -23 * np.log2(ws[0] * ws[1]) if tie_rel > 0 else 0.0
     f1 = np.ascontiguousarray(f1, np.uint8)
     ny, nx = f1.shape
     f2u8 = None if f2u8 is None else np.ascontiguousarray(f2u8, np.uint8)
@@ -114,7 +123,8 @@ def xcorr_pass(f1, f2u8=None, f2def=None, *, geom, ws, sp, K, lims=(-100, 100, -
                 left_y=geom["left_y"], nx_vec=geom["nx_vec"], ny_vec=geom["ny_vec"], bx=geom["border_x"],
                 by=geom["border_y"], K=K, lx1=lims[0], lx2=lims[1], ly1=lims[2], ly2=lims[3], pred_x=_p(px),
                 pred_y=_p(py), method=method, tie_rel=tie_rel, threads=threads,
-                win_begin=0 if win_range is None else win_range[0], win_end=0 if win_range is None else win_range[1])
+                win_begin=0 if win_range is None else win_range[0], win_end=0 if win_range is None else win_range[1],
+                fit_tol=fit_tol, fit_rel=fit_rel)
     dx = np.zeros((nt, K))
     dy = np.zeros((nt, K))
     v = np.zeros((nt, K), np.uint8)
@@ -125,6 +135,11 @@ def xcorr_pass(f1, f2u8=None, f2def=None, *, geom, ws, sp, K, lims=(-100, 100, -
     rc = lib().orc_xcorr_pass(prec, C.byref(P), _p(dx), _p(dy), _p(v), _p(pv), _p(pi), _p(nc), _p(tie))
     assert rc == 0
     return dict(dx=dx, dy=dy, valid=v, pk_val=pv, pk_idx=pi, ncand=nc, tie=tie)
+
+
+def fit_sensitivity(f0, f1, f2, delta):
+    """Bound on the change of one axis's 3-point fit when its values move by delta."""
+    return lib().orc_fit_sensitivity(f0, f1, f2, delta)
 
 
 def deform(f2, geom, ws, sp, ex, ey, prec=64):

@@ -58,7 +58,8 @@ class FilterParams(C.Structure):
 
 class Stats(C.Structure):
     _fields_ = [("overflow_windows", C.c_longlong * 2), ("outliers", C.c_longlong * 2),
-                ("fill_sweeps", C.c_int * 2), ("oob_taps", C.c_longlong * 2)]
+                ("fill_sweeps", C.c_int * 2), ("oob_taps", C.c_longlong * 2), ("hole_list_drops", C.c_longlong),
+                ("holes_pending", C.c_int)]
 
 
 class ParameterFile(C.Structure):
@@ -90,6 +91,7 @@ _SIGS = {
     "pivx_freemem": [_P],
     "pivx_last_error": [_P],
     "pivx_get_geometry": [_P, C.c_int, C.POINTER(Geometry)],
+    "pivx_compute_geometry": [C.POINTER(InitParams), C.c_int, C.POINTER(Geometry)],
     "pivx_prep_img_data": [_P, _P, _P, C.c_size_t],
     "pivx_prep_img_data_device": [_P, _P, _P, C.c_size_t],
     "pivx_set_mhe": [_P, C.c_int, C.c_int, C.c_float],
@@ -351,7 +353,8 @@ class Xcorr2dMulti:
         s = Stats()
         self._c(lib().pivx_get_stats(self.h, C.byref(s)))
         return dict(overflow_windows=list(s.overflow_windows), outliers=list(s.outliers),
-                    fill_sweeps=list(s.fill_sweeps), oob_taps=list(s.oob_taps))
+                    fill_sweeps=list(s.fill_sweeps), oob_taps=list(s.oob_taps),
+                    hole_list_drops=int(s.hole_list_drops), holes_pending=int(s.holes_pending))
 
     def set_filtered(self, grid, dx, dy, valid=None):
         dx = np.ascontiguousarray(dx, np.float32)
@@ -382,6 +385,13 @@ class Xcorr2dMulti:
 
 
 # ---------------------------------------------------------------- host utilities
+def compute_geometry(params: InitParams, grid: int) -> Geometry:
+    """The geometry pivx_init derives for a grid, without a device."""
+    g = Geometry()
+    _check(lib().pivx_compute_geometry(C.byref(params), grid, C.byref(g)))
+    return g
+
+
 def parse_parameter_file(path: str) -> ParameterFile:
     pf = ParameterFile()
     _check(lib().pivx_parse_parameter_file(path.encode(), C.byref(pf)))

@@ -82,6 +82,8 @@ struct pivx_ctx {
   unsigned long long* counters = nullptr;  // [2]: cumulative out-of-buffer deformation taps per grid
   int* sweeps = nullptr;                   // [2]
   int* nholes = nullptr;                   // hole-list counter of the fill kernels
+  unsigned int* drops = nullptr;           // holes counted beyond the list capacity since the last check
+  long long drops_total = 0;               // host total of *drops, reported by pivx_get_stats
   int *list_a = nullptr, *list_b = nullptr;
   int list_cap = 0;                        // entries of list_a / list_b
   float *tmp_x = nullptr, *tmp_y = nullptr;
@@ -380,11 +382,21 @@ int run_filter(pivx_ctx* c, int gi, int npass_median, int min_nneigh, float inse
   const bool listed = fill && npass_median > 0;  // the last median pass lists the holes
   int rc;
   tic(c);
+  // The hole counter must be 0 when the holes are listed (by the last median
+  // pass, or by k_fill_list without median passes).  k_fill zeroes it after
+  // use; the launch ahead of the listing one zeroes it as well, so that no
+  // earlier sequence (an interrupted pair, a caller's partial sequence) can
+  // leave a stale count behind.
   if (npass_median <= 0) {
-    rc = check_launch(c, launch_prep(G.dx, G.dy, G.dv, K, t0x, t0y, ov, nt, c->stream), "median_filter_data_prep");
-    if (!rc && mx) rc = check_launch(c, launch_prep(G.dx, G.dy, G.dv, K, mx, my, mv, nt, c->stream), "median_filter_data_prep");
+    rc = check_launch(c, launch_prep(G.dx, G.dy, G.dv, K, t0x, t0y, ov, nt, fill ? c->nholes : nullptr, c->stream),
+                      "median_filter_data_prep");
+    if (!rc && mx)
+      rc = check_launch(c, launch_prep(G.dx, G.dy, G.dv, K, mx, my, mv, nt, nullptr, c->stream),
+                        "median_filter_data_prep");
     if (rc) return rc;
   } else {
+    if (listed && npass_median == 1 && (rc = check_launch(c, launch_zero_ints(c->nholes, 1, c->stream), "hole counter")))
+      return rc;
     // intermediate sets: I1 = (ux, uy, uv), I2 = whichever of (ox, oy) / (sx, sy) is not T0, with sv
     float* i2x = smooth_odd ? ox : G.sx;
     float* i2y = smooth_odd ? oy : G.sy;
@@ -428,6 +440,7 @@ int run_filter(pivx_ctx* c, int gi, int npass_median, int min_nneigh, float inse
       m.hole_list = (back == 0 && listed) ? c->list_a : nullptr;
       m.nholes = (back == 0 && listed) ? c->nholes : nullptr;
       m.hole_cap = c->list_cap;
+      m.reset_count = (listed && p == 0 && back > 0) ? c->nholes : nullptr;
       rc = check_launch(c, launch_median(m, c->stream), "median_filtering");
       if (rc) return rc;
       srcx = tx;
@@ -451,6 +464,7 @@ int run_filter(pivx_ctx* c, int gi, int npass_median, int min_nneigh, float inse
     f.tmp_f = c->tmp_f;
     f.nholes = c->nholes;
     f.list_cap = c->list_cap;
+    f.drops = c->drops;
     f.sweeps = c->sweeps + gi;
     f.prelisted = listed ? 1 : 0;
     tic(c);
@@ -633,6 +647,7 @@ int pivx_init(const pivx_init_params* p, int device_id, pivx_ctx** out) {
   rc |= dalloc(c, &c->counters, 2);
   rc |= dalloc(c, &c->sweeps, 2);
   rc |= dalloc(c, &c->nholes, 1);
+  rc |= dalloc(c, &c->drops, 1);
   c->list_cap = (int)ntmax;
   rc |= dalloc(c, &c->list_a, ntmax);
   rc |= dalloc(c, &c->list_b, ntmax);
@@ -647,6 +662,7 @@ int pivx_init(const pivx_init_params* p, int device_id, pivx_ctx** out) {
   if (rc == 0 && hipMemsetAsync(c->counters, 0, 2 * sizeof(unsigned long long), c->stream) != hipSuccess) rc = PIVX_ERR_HIP;
   if (rc == 0 && hipMemsetAsync(c->sweeps, 0, 2 * sizeof(int), c->stream) != hipSuccess) rc = PIVX_ERR_HIP;
   if (rc == 0 && hipMemsetAsync(c->nholes, 0, sizeof(int), c->stream) != hipSuccess) rc = PIVX_ERR_HIP;
+  if (rc == 0 && hipMemsetAsync(c->drops, 0, sizeof(unsigned int), c->stream) != hipSuccess) rc = PIVX_ERR_HIP;
   if (rc == 0 && hipStreamSynchronize(c->stream) != hipSuccess) rc = fail(c, PIVX_ERR_HIP, "init synchronisation failed");
   if (rc) {
     std::string msg = c->err;
@@ -669,6 +685,17 @@ int pivx_freemem(pivx_ctx* c) {
   for (hipEvent_t e : c->tl.pool) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
+  return PIVX_OK;
+}
+
+int pivx_compute_geometry(const pivx_init_params* p, int grid, pivx_geometry* g) {
+  if (!p || !g || (grid != 1 && grid != 2)) return fail(nullptr, PIVX_ERR_PARAM, "bad argument");
+  const int rc = validate(p);
+  if (rc) return rc;
+  *g = grid == 1 ? make_geometry(p->nx, p->ny, p->nsec_cc, p->ws_x1, p->ws_y1, p->spacing_x1, p->spacing_y1,
+                                 p->max_dispx_percent1, p->max_dispy_percent1)
+                 : make_geometry(p->nx, p->ny, p->nsec_cc, p->ws_x2, p->ws_y2, p->spacing_x2, p->spacing_y2,
+                                 p->max_dispx_percent2, p->max_dispy_percent2);
   return PIVX_OK;
 }
 
@@ -967,10 +994,24 @@ int pivx_set_fused_deform(pivx_ctx* c, int mode) {
   return PIVX_OK;
 }
 
+// Moves the device's dropped-hole count into drops_total; a non-zero count is
+// reported once, as an error of the call that finds it.
+static int check_drops(pivx_ctx* c) {
+  unsigned int d = 0;
+  HIPCHK(c, hipMemcpyAsync(&d, c->drops, sizeof(d), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (d == 0) return PIVX_OK;
+  const unsigned int zero = 0;
+  HIPCHK(c, hipMemcpyAsync(c->drops, &zero, sizeof(zero), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->drops_total += d;
+  return fail(c, PIVX_ERR_STATE, "hole list overflow: %u invalid vectors were not filled", d);
+}
+
 int pivx_synchronize(pivx_ctx* c) {
   if (!c) return fail(nullptr, PIVX_ERR_PARAM, "null handle");
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  return PIVX_OK;
+  return check_drops(c);
 }
 
 int pivx_get_dims(const pivx_ctx* c, int grid, int* nx_vec, int* ny_vec) {
@@ -1046,7 +1087,9 @@ int pivx_get_stats(pivx_ctx* c, pivx_stats* st) {
   if (!c || !st) return fail(c, PIVX_ERR_PARAM, "bad argument");
   unsigned long long h[2];
   int sw[2];
-  int rc = d2h(c, h, c->counters, sizeof(h)) | d2h(c, sw, c->sweeps, sizeof(sw));
+  int pending = 0;
+  int rc = d2h(c, h, c->counters, sizeof(h)) | d2h(c, sw, c->sweeps, sizeof(sw)) |
+           d2h(c, &pending, c->nholes, sizeof(pending));
   if (rc) return rc;
   std::vector<int> nc[2];
   std::vector<int8_t> acc[2];
@@ -1057,8 +1100,10 @@ int pivx_get_stats(pivx_ctx* c, pivx_stats* st) {
     rc = d2h(c, nc[g].data(), c->grid[g].ncand, nt * sizeof(int)) | d2h(c, acc[g].data(), c->grid[g].accepted, nt);
     if (rc) return rc;
   }
-  rc = pivx_synchronize(c);
-  if (rc) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  (void)check_drops(c);  // counted into drops_total; the stats carry it instead of an error
+  st->hole_list_drops = c->drops_total;
+  st->holes_pending = pending;
   for (int g = 0; g < 2; ++g) {
     const int cap = c->grid[g].g.npeak_slots;
     long long ov = 0, out = 0;

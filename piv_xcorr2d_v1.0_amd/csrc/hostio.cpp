@@ -12,6 +12,7 @@
 #include <cstring>
 #include <string>
 #include <thread>
+#include <stdexcept>
 #include <vector>
 
 namespace {
@@ -95,10 +96,25 @@ struct Reader {
 };
 
 // getImHeadTiff (ImageIOPkg.cpp:124-258): walks every IFD; the last IFD's
-// fields win, as in the reference's loop.
+// fields win, as in the reference's loop.  Counts and offsets read from the
+// file are checked against the file size before anything is sized by them,
+// and a short read ends the walk.
+int tiff_parse_checked(const char* path, TiffInfo& ti, FILE** keep);
+
 int tiff_parse(const char* path, TiffInfo& ti, FILE** keep) {
+  try {
+    return tiff_parse_checked(path, ti, keep);
+  } catch (const std::exception& e) {
+    return io_fail(PIVX_ERR_FORMAT, std::string("malformed TIFF: ") + e.what());
+  }
+}
+
+int tiff_parse_checked(const char* path, TiffInfo& ti, FILE** keep) {
   FILE* f = fopen(path, "rb");
   if (!f) return io_fail(PIVX_ERR_IO, std::string("cannot open ") + path);
+  long fsize = -1;
+  if (fseek(f, 0, SEEK_END) == 0) fsize = ftell(f);
+  if (fsize < 8 || fseek(f, 0, SEEK_SET) != 0) { fclose(f); return io_fail(PIVX_ERR_FORMAT, "short TIFF header"); }
   unsigned char hdr[4];
   if (fread(hdr, 1, 4, f) != 4) { fclose(f); return io_fail(PIVX_ERR_FORMAT, "short TIFF header"); }
   bool big;
@@ -109,29 +125,36 @@ int tiff_parse(const char* path, TiffInfo& ti, FILE** keep) {
   if (r.u16(2) != 42) { fclose(f); return io_fail(PIVX_ERR_FORMAT, "TIFF identifier is not 42"); }
   uint32_t ifd = r.u32(4);
   int guard = 0;
-  while (ifd != 0 && r.ok && guard++ < 1024) {
+  bool bad = false;
+  while (ifd != 0 && r.ok && !bad && guard++ < 1024) {
+    if ((long long)ifd + 2 > fsize) { bad = true; break; }
     uint32_t n = r.u16(ifd);
-    for (uint32_t i = 0; i < n && r.ok; ++i) {
+    if ((long long)ifd + 2 + 12LL * n + 4 > fsize) { bad = true; break; }
+    for (uint32_t i = 0; i < n && r.ok && !bad; ++i) {
       long ent = (long)ifd + 2 + 12 * (long)i;
-      uint32_t tag = r.u16(ent), count = r.u32(ent + 4);
+      uint32_t tag = r.u16(ent), type = r.u16(ent + 2), count = r.u32(ent + 4);
       switch (tag) {
         case 256: ti.width = (int)r.value(ent, 0); break;
         case 257: ti.height = (int)r.value(ent, 0); break;
         case 258: ti.bps = (int)r.value(ent, 0); break;
         case 259: ti.compression = (int)r.value(ent, 0); break;
         case 273:
-          ti.offs.resize(count);
-          for (uint32_t k = 0; k < count; ++k) ti.offs[k] = r.value(ent, (int)k);
+        case 279: {
+          // a strip table of `count` SHORT/LONG entries must lie inside the file
+          const long long sz = type == 3 ? 2 : 4;
+          if (count == 0 || (long long)count * sz > fsize) { bad = true; break; }
+          std::vector<uint32_t>& t = tag == 273 ? ti.offs : ti.counts;
+          t.resize(count);
+          for (uint32_t k = 0; k < count && r.ok; ++k) t[k] = r.value(ent, (int)k);
           break;
-        case 279:
-          ti.counts.resize(count);
-          for (uint32_t k = 0; k < count; ++k) ti.counts[k] = r.value(ent, (int)k);
-          break;
+        }
         default: break;
       }
     }
+    if (bad || !r.ok) break;
     ifd = r.u32((long)ifd + 2 + 12 * (long)n);
   }
+  if (bad) { fclose(f); return io_fail(PIVX_ERR_FORMAT, "TIFF directory points outside the file"); }
   if (!r.ok) { fclose(f); return io_fail(PIVX_ERR_FORMAT, "truncated TIFF"); }
   if (keep) *keep = f;
   else fclose(f);
@@ -483,6 +506,8 @@ int pivx_tiff_read8(const char* path, uint8_t* buf, size_t cap) {
     pos += n;
   }
   fclose(f);
+  // a frame shorter than the buffer would leave stale pixels behind
+  if (pos < cap) return io_fail(PIVX_ERR_FORMAT, "TIFF strips hold fewer bytes than the frame");
   return PIVX_OK;
 }
 

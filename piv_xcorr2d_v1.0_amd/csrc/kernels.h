@@ -67,7 +67,8 @@ struct MedArgs {
   float* scratch;          // generic path only: 4*wx*wy floats per vector
   int* hole_list;          // if set: indices of vectors left invalid are appended here
   int* nholes;             //   (count in *nholes, which must be 0 before the pass)
-  int hole_cap;            //   capacity of hole_list; appends beyond it are dropped
+  int hole_cap;            //   capacity of hole_list; appends beyond it are not stored (k_fill counts them)
+  int* reset_count;        // if set: the launch writes 0 here (the hole counter, ahead of a later listing pass)
 };
 
 struct FillArgs {
@@ -76,6 +77,7 @@ struct FillArgs {
   int* list_a; int* list_b; float* tmp_x; float* tmp_y; uint8_t* tmp_f;
   int* nholes;             // hole counter, 0 between fills
   int list_cap;            // capacity of list_a and list_b
+  unsigned int* drops;     // holes counted beyond list_cap (never filled) are added here
   int* sweeps;
   int prelisted;           // holes already in list_a / *nholes (appended by the median pass)
 };
@@ -117,9 +119,12 @@ hipError_t launch_upsample(float* dx2, float* dy2, const float* ex1x, const floa
 hipError_t launch_median(const MedArgs& a, hipStream_t s);
 hipError_t launch_fill(const FillArgs& a, hipStream_t s);
 hipError_t launch_smooth(const SmoothArgs& a, hipStream_t s);
-// filter := candidate 0 (median_filter_data_prep, cu:484-495).
+// filter := candidate 0 (median_filter_data_prep, cu:484-495); reset_count,
+// if set, is zeroed by the launch.
 hipError_t launch_prep(const float* cx, const float* cy, const uint8_t* cv, int K, float* dx, float* dy, uint8_t* dv,
-                       int nt, hipStream_t s);
+                       int nt, int* reset_count, hipStream_t s);
+// p[0..n) = 0 by a kernel (no memset node in a captured pair graph).
+hipError_t launch_zero_ints(int* p, int n, hipStream_t s);
 hipError_t launch_stream_copy(const void* src, void* dst, size_t bytes, hipStream_t s);
 hipError_t launch_mhe(const MheArgs& a, hipStream_t s);
 

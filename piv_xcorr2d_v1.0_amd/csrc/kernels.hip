@@ -1644,6 +1644,7 @@ __global__ void __launch_bounds__(128) k_median(MedArgs a) {
 #pragma clang fp contract(off)
   const int nt = a.nx_vec * a.ny_vec;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a.reset_count && t == 0) *a.reset_count = 0;
   if (t >= nt) return;
   const int vy = t / a.nx_vec, vx = t - (t / a.nx_vec) * a.nx_vec;
   const int ntm = a.wx * a.wy;
@@ -1731,6 +1732,7 @@ __global__ void __launch_bounds__(256) k_median2(MedArgs a) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   const int t = g >> 1, c = g & 1;
   const bool live = t < nt;
+  if (a.reset_count && g == 0) *a.reset_count = 0;
   const int tt = live ? t : nt - 1;  // both lanes of a pair stay active for the swap
   const int vy = tt / a.nx_vec, vx = tt - vy * a.nx_vec;
   const float* src = c ? a.sy : a.sx;
@@ -1824,6 +1826,7 @@ __global__ void __launch_bounds__(128) k_median_generic(MedArgs a) {
 #pragma clang fp contract(off)
   const int nt = a.nx_vec * a.ny_vec;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a.reset_count && t == 0) *a.reset_count = 0;
   if (t >= nt) return;
   const int vy = t / a.nx_vec, vx = t - (t / a.nx_vec) * a.nx_vec;
   const int ntm = a.wx * a.wy;
@@ -1864,7 +1867,11 @@ __global__ void __launch_bounds__(128) k_median_generic(MedArgs a) {
 // sweep, cu:1527).  Each sweep first computes every hole's 3x3 mean of valid
 // neighbours (raster sum order), then applies all fills at once.
 // Hole list of the field (any order: each sweep's result is independent of
-// the order holes are visited in).  *nholes is 0 on entry; k_fill resets it.
+// the order holes are visited in).  *nholes is 0 on entry (the launch ahead of
+// the listing one zeroes it); k_fill resets it as well.  A count beyond
+// list_cap cannot arise from a pass that starts at 0 (list_cap >= every
+// grid's vector count); k_fill adds any excess to *drops, which the host
+// reports as an error.
 __global__ void __launch_bounds__(256) k_fill_list(FillArgs a) {
   const int nt = a.nx_vec * a.ny_vec;
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nt; t += gridDim.x * blockDim.x)
@@ -1878,7 +1885,9 @@ __global__ void __launch_bounds__(1024) k_fill(FillArgs a) {
 #pragma clang fp contract(off)
   __shared__ int n_holes, n_next, n_filled;
   if (threadIdx.x == 0) {
-    n_holes = min(*a.nholes, a.list_cap);
+    const int counted = *a.nholes;
+    n_holes = min(counted, a.list_cap);
+    if (counted > a.list_cap && a.drops) atomicAdd(a.drops, (unsigned int)(counted - a.list_cap));
     n_filled = 0;
   }
   __syncthreads();
@@ -1973,7 +1982,8 @@ __global__ void k_smooth(SmoothArgs a) {
 }
 
 __global__ void k_prep(const float* cx, const float* cy, const uint8_t* cv, int K, float* dx, float* dy, uint8_t* dv,
-                       int nt) {
+                       int nt, int* reset_count) {
+  if (reset_count && blockIdx.x == 0 && threadIdx.x == 0) *reset_count = 0;
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nt; t += gridDim.x * blockDim.x) {
     const long long o = (long long)t * K;
     dv[t] = cv[o];
@@ -1981,6 +1991,10 @@ __global__ void k_prep(const float* cx, const float* cy, const uint8_t* cv, int 
     dy[t] = cy[o];
   }
 }
+__global__ void k_zero_ints(int* p, int n) {
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) p[t] = 0;
+}
+
 // Streaming copy used to measure the achievable HBM bandwidth and to calibrate
 // the FETCH_SIZE / WRITE_SIZE counters against a known byte count.
 __global__ void __launch_bounds__(256) k_stream_copy(const float4* __restrict__ src, float4* __restrict__ dst,
@@ -2322,8 +2336,13 @@ hipError_t launch_smooth(const SmoothArgs& a, hipStream_t s) {
 }
 
 hipError_t launch_prep(const float* cx, const float* cy, const uint8_t* cv, int K, float* dx, float* dy, uint8_t* dv,
-                       int nt, hipStream_t s) {
-  hipLaunchKernelGGL(k_prep, dim3(grid_for(nt, 256)), dim3(256), 0, s, cx, cy, cv, K, dx, dy, dv, nt);
+                       int nt, int* reset_count, hipStream_t s) {
+  hipLaunchKernelGGL(k_prep, dim3(grid_for(nt, 256)), dim3(256), 0, s, cx, cy, cv, K, dx, dy, dv, nt, reset_count);
+  return hipGetLastError();
+}
+
+hipError_t launch_zero_ints(int* p, int n, hipStream_t s) {
+  hipLaunchKernelGGL(k_zero_ints, dim3(grid_for(n, 256)), dim3(256), 0, s, p, n);
   return hipGetLastError();
 }
 hipError_t launch_stream_copy(const void* src, void* dst, size_t bytes, hipStream_t s) {

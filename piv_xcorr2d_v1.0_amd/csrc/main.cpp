@@ -124,6 +124,13 @@ int main(int argc, char** argv) {
   auto read_pair = [&](int k, int s) -> std::string {
     const std::string a = frame_name(k, 0), b = frame_name(k, 1);
     if (!file_exists(a) || !file_exists(b)) return "An image for crosss-correlation is missing!";
+    for (const std::string* fn : {&a, &b}) {
+      int fw = 0, fh = 0, fb = 0;
+      if (pivx_tiff_info(fn->c_str(), &fw, &fh, &fb)) return pivx_last_error(nullptr);
+      if (fw != pf.init.nx || fh != pf.init.ny)
+        return *fn + ": frame is " + std::to_string(fw) + "x" + std::to_string(fh) + ", Parameter.dat gives " +
+               std::to_string(pf.init.nx) + "x" + std::to_string(pf.init.ny);
+    }
     // the two frames are read concurrently
     auto second = std::async(std::launch::async, [&]() -> std::string {
       return pivx_tiff_read8(b.c_str(), slot[s][1], npix) ? pivx_last_error(nullptr) : "";

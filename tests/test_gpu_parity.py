@@ -5,7 +5,11 @@ Bar (written per assertion below):
   * integer peak index, candidate order and validity flags bit-exact, except
     in windows the FP64 oracle flags as ties (two candidate values, or a
     candidate and its neighbour, closer than tie_margin * peak);
-  * sub-pixel displacements within 1e-3 px;
+  * sub-pixel displacements within 1e-3 px, except in windows the oracle
+    flags as ill-conditioned fits (nearly flat peaks, where plane values
+    moved by one FP32 rounding per FFT stage, 2^-23 * log2(N) * peak, move
+    the 3-point fit by more than 1e-3 px);
+  * at most 3% of the windows excluded by either flag;
   * FP32 element-wise stages (deformation, upsampling, median test, fill,
     smoothing) bit-exact against the oracle's FP32 transcription on identical
     inputs; the deformed frame within 0.05 grey levels of the FP64 oracle.
@@ -24,18 +28,20 @@ def _report(name, **kw):
     print(f"[parity] {name}: " + ", ".join(f"{k}={v}" for k, v in kw.items()), flush=True)
 
 
-def check_pass(name, gpu, ref, ws, K, min_ok=0.9):
+def check_pass(name, gpu, ref, ws, K, min_ok=0.97):
     """gpu: dict(val, idx, dx, dy, v); ref: oracle xcorr_pass output."""
-    ok = ref["tie"] == 0
+    ok = ref["tie"] == 0       # neither flag: everything checked
+    ordered = ref["tie"] != 1  # no value tie: indices, order and validity checked
     frac_ok = float(ok.mean())
-    bad_idx = int((gpu["idx"][ok] != ref["pk_idx"][ok]).any(axis=1).sum())
-    bad_valid = int((gpu["v"][ok] != ref["valid"][ok]).any(axis=1).sum())
+    bad_idx = int((gpu["idx"][ordered] != ref["pk_idx"][ordered]).any(axis=1).sum())
+    bad_valid = int((gpu["v"][ordered] != ref["valid"][ordered]).any(axis=1).sum())
     both = ok[:, None] & (gpu["v"] == 1) & (ref["valid"] == 1)
     ex = float(np.abs(gpu["dx"] - ref["dx"])[both].max()) if both.any() else 0.0
     ey = float(np.abs(gpu["dy"] - ref["dy"])[both].max()) if both.any() else 0.0
     peak = max(float(ref["pk_val"].max()), 1.0)
     relv = float((np.abs(gpu["val"] - ref["pk_val"]) / peak)[ok].max()) if ok.any() else 0.0
-    _report(name, windows=len(ok), tie_excluded=int((~ok).sum()), idx_mismatch=bad_idx, valid_mismatch=bad_valid,
+    _report(name, windows=len(ok), tie_excluded=int((ref["tie"] == 1).sum()),
+            flat_fit_excluded=int((ref["tie"] == 2).sum()), idx_mismatch=bad_idx, valid_mismatch=bad_valid,
             max_subpix_err=f"{max(ex, ey):.2e}", max_rel_peak_err=f"{relv:.2e}")
     assert frac_ok >= min_ok, f"too many tie-excluded windows ({1 - frac_ok:.1%})"
     assert bad_idx == 0, "peak indices differ outside ties"
@@ -260,7 +266,7 @@ def test_fast_window_shapes(pivx, orc, ws):
         gpu = gpu_pass(x, 1)
         g = geom_dict(x.geometry(1))
     ref = orc.xcorr_pass(a, f2u8=b, geom=g, ws=ws, sp=sp, K=4, tie_rel=tie_margin(*ws))
-    check_pass(f"fast {ws[0]}x{ws[1]}", gpu, ref, ws, 4, min_ok=0.8)
+    check_pass(f"fast {ws[0]}x{ws[1]}", gpu, ref, ws, 4)
 
 
 @pytest.mark.parametrize("ws,pct", [((32, 32), (0.15, 0.3)), ((64, 64), (0.3, 0.1)), ((48, 48), (0.05, 0.45)),
@@ -284,7 +290,7 @@ def test_search_border_shapes(pivx, orc, ws, pct):
     assert (g["border_x"], g["border_y"]) == (max(1, ws[0] // 2 - round(ws[0] * pct[0])),
                                               max(1, ws[1] // 2 - round(ws[1] * pct[1])))
     ref = orc.xcorr_pass(a, f2u8=b, geom=g, ws=ws, sp=sp, K=4, tie_rel=tie_margin(*ws))
-    check_pass(f"border {g['border_x']},{g['border_y']} at {ws[0]}x{ws[1]}", gpu, ref, ws, 4, min_ok=0.8)
+    check_pass(f"border {g['border_x']},{g['border_y']} at {ws[0]}x{ws[1]}", gpu, ref, ws, 4)
     ok = ref["tie"] == 0
     assert (gpu["nc"][ok] == ref["ncand"][ok]).mean() > 0.98
     if g["border_x"] * 2 >= ws[0] or g["border_y"] * 2 >= ws[1]:
@@ -295,9 +301,11 @@ def test_search_border_shapes(pivx, orc, ws, pct):
                                    ((128, 128), (64, 64)), ((8, 8), (4, 4))])
 def test_generic_window_shapes(pivx, orc, ws, sp):
     """Shapes outside the compiled set take the generic kernel (runtime radix
-    plan; 128x128 uses global-memory planes)."""
-    a, b = config1_pair(pivx, seed=5, nx=320, ny=256, u=1.4, v=-2.2)
-    p = config1_params(pivx, nx=320, ny=256, ws_x1=ws[0], ws_y1=ws[1], spacing_x1=sp[0], spacing_y1=sp[1],
+    plan; 128x128 uses global-memory planes, on a larger frame so that the
+    excluded fraction is measured over enough windows)."""
+    nx, ny = (1024, 768) if ws[0] >= 128 else (320, 256)
+    a, b = config1_pair(pivx, seed=5, nx=nx, ny=ny, u=1.4, v=-2.2)
+    p = config1_params(pivx, nx=nx, ny=ny, ws_x1=ws[0], ws_y1=ws[1], spacing_x1=sp[0], spacing_y1=sp[1],
                        max_dispx_percent1=0.4, max_dispy_percent1=0.4)
     with pivx.Xcorr2dMulti(p) as x:
         x.prep_img_data(a, b)
@@ -305,7 +313,10 @@ def test_generic_window_shapes(pivx, orc, ws, sp):
         gpu = gpu_pass(x, 1)
         g = geom_dict(x.geometry(1))
     ref = orc.xcorr_pass(a, f2u8=b, geom=g, ws=ws, sp=sp, K=4, tie_rel=tie_margin(*ws))
-    check_pass(f"generic {ws[0]}x{ws[1]}", gpu, ref, ws, 4, min_ok=0.7)
+    # a 128x128 window holds 16 times the local maxima of a 32x32 one, so
+    # near-equal candidate / neighbour values are more frequent (7 of 192
+    # windows measured): 95% there
+    check_pass(f"generic {ws[0]}x{ws[1]}", gpu, ref, ws, 4, min_ok=0.95 if ws[0] >= 128 else 0.97)
 
 
 @pytest.mark.parametrize("ws,sp", [((32, 32), (16, 16)), ((64, 64), (32, 32)), ((48, 24), (24, 12))])
@@ -337,7 +348,7 @@ def test_noise_windows_near_ties(pivx, orc, ws, sp):
         x.Calc_cc_map_p1(0)
         gpu = gpu_pass(x, 1)
     ref = orc.xcorr_pass(a, f2u8=b, geom=g, ws=ws, sp=sp, K=4, tie_rel=tie_margin(*ws))
-    check_pass(f"noise windows {ws[0]}x{ws[1]} ({int(noisy.sum())} noisy)", gpu, ref, ws, 4, min_ok=0.8)
+    check_pass(f"noise windows {ws[0]}x{ws[1]} ({int(noisy.sum())} noisy)", gpu, ref, ws, 4)
 
 
 def test_uneven_batch_nonsquare_windows(pivx, orc):
@@ -402,7 +413,7 @@ def test_uneven_batch_nonsquare_windows(pivx, orc):
         gpu = gpu_pass(x, 1)
         g = geom_dict(x.geometry(1))
     ref = orc.xcorr_pass(a, f2u8=b, geom=g, ws=(64, 32), sp=(32, 16), K=4, tie_rel=tie_margin(64, 32))
-    check_pass("uneven batch 1280x720 grid-1 pass 0 (64x32)", gpu, ref, (64, 32), 4, min_ok=0.8)
+    check_pass("uneven batch 1280x720 grid-1 pass 0 (64x32)", gpu, ref, (64, 32), 4)
 
 
 def _random_candidates(rng, nt, K, outlier_frac=0.1):
@@ -519,7 +530,7 @@ def test_two_grid_stagewise(pivx, orc):
                     ref = orc.xcorr_pass(a, f2def=d.astype(np.float64), **kw)
                 else:
                     ref = orc.xcorr_pass(a, f2u8=b, **kw)
-                check_pass(f"two-grid g{grid} pass{j}", gpu, ref, wss[grid], 4, min_ok=0.85)
+                check_pass(f"two-grid g{grid} pass{j}", gpu, ref, wss[grid], 4)
                 # validation on the GPU's raw candidates
                 if grid == 1:
                     x.MedianFilter_Smoothing_p1(2, 5, 2.0, 2.0, 1)

@@ -64,6 +64,48 @@ def test_init_accepts_boundary_window_sizes_before_the_device_check(pivx):
 
 
 # ---------------------------------------------------------------- Parameter.dat
+GEOMETRY_CASES = [
+    # (nx, ny, ws_x, ws_y, sp_x, sp_y, pct_x, pct_y, nsec)
+    (512, 512, 32, 32, 16, 16, 0.5, 0.5, 1),        # config 1
+    (6600, 4400, 64, 64, 64, 64, 0.4, 0.4, 1),      # config 2, grid 1
+    (6600, 4400, 64, 64, 32, 32, 0.4, 0.4, 1),      # sample Parameter.dat grid 1
+    (6600, 4400, 32, 32, 16, 16, 0.4, 0.4, 1),      # config 2, grid 2
+    (600, 400, 32, 32, 16, 16, 0.4, 0.4, 1),        # odd vector-column count (37)
+    (588, 400, 24, 24, 12, 12, 0.4, 0.4, 1),
+    (600, 400, 16, 32, 8, 16, 0.4, 0.4, 1),
+    (584, 300, 16, 16, 8, 8, 0.3, 0.45, 1),
+    (2048, 2048, 32, 32, 8, 8, 0.4, 0.4, 1),        # 75% overlap
+    (4096, 2048, 64, 32, 32, 16, 0.4, 0.4, 1),      # non-square windows
+    (1000, 1000, 32, 32, 16, 16, 0.4, 0.4, 3),      # sections
+    (301, 177, 20, 12, 50, 9, 0.25, 0.1, 2),        # spacing beyond the window
+    (97, 61, 4, 8, 3, 5, 0.4, 0.4, 7),
+]
+
+
+@pytest.mark.parametrize("case", GEOMETRY_CASES)
+def test_engine_geometry_equals_oracle(pivx, orc, case):
+    """The geometry the engine derives at init (pads, aug sizes, borders,
+    slot cap, sections) equals the oracle's, which the reference result
+    files pin at 6600x4400 (test_oracle.py)."""
+    nx, ny, wx, wy, sx, sy, px, py, nsec = case
+    ip = pivx.init_params(nx=nx, ny=ny, nsec_cc=nsec, ws_x1=wx, ws_y1=wy, spacing_x1=sx, spacing_y1=sy,
+                          max_dispx_percent1=px, max_dispy_percent1=py, ws_x2=wx, ws_y2=wy, spacing_x2=sx,
+                          spacing_y2=sy, max_dispx_percent2=px, max_dispy_percent2=py)
+    want = orc.geometry(nx, ny, nsec, wx, wy, sx, sy, px, py)
+    for grid in (1, 2):
+        g = pivx.compute_geometry(ip, grid)
+        got = {k: getattr(g, k) for k in want}
+        assert got == want, (case, grid)
+        assert (g.ws_x, g.ws_y, g.spacing_x, g.spacing_y, g.nt_vec) == (wx, wy, sx, sy, g.nx_vec * g.ny_vec)
+
+
+def test_compute_geometry_applies_init_checks(pivx):
+    with pytest.raises(pivx.PivError, match="not divisible by 4"):
+        pivx.compute_geometry(pivx.init_params(ws_x1=30), 1)
+    with pytest.raises(pivx.PivError):
+        pivx.compute_geometry(pivx.init_params(), 3)
+
+
 def test_parameter_file_sample_parses(pivx):
     """The reference's 50-line CRLF Parameter.dat (no trailing newline)."""
     pf = pivx.parse_parameter_file(os.path.join(GOLDEN, "Parameter.dat"))
@@ -248,6 +290,41 @@ def test_tiff_reader_strips_and_byte_order(pivx, tmp_path, big, strips, short):
     p.write_bytes(_tiff_bytes(img, big, strips, short))
     assert pivx.tiff_info(str(p)) == (41, 29, 8)
     np.testing.assert_array_equal(pivx.tiff_read8(str(p), 41, 29), img)
+
+
+def test_tiff_rejects_corrupt_directories_and_short_strips(pivx, tmp_path):
+    """Counts and offsets taken from the file are bounded by the file size:
+    a strip table claiming 2^32-1 entries, an IFD beyond the end of the file
+    and an entry count running past it are format errors (not a huge
+    allocation or a long seek loop), and strips holding fewer bytes than the
+    frame are an error rather than a partly stale frame."""
+    img = np.random.default_rng(2).integers(0, 256, (29, 41), dtype=np.uint8)
+    good = bytearray(_tiff_bytes(img, False, 5, False))
+    n_ent = struct.unpack("<H", good[8:10])[0]
+    cases = {}
+    for k in range(n_ent):  # strip-offset entry: count -> 0xffffffff
+        o = 10 + 12 * k
+        if struct.unpack("<H", good[o:o + 2])[0] == 273:
+            b = bytearray(good)
+            b[o + 4:o + 8] = struct.pack("<I", 0xFFFFFFFF)
+            cases["huge strip count"] = b
+    b = bytearray(good)
+    b[4:8] = struct.pack("<I", len(good) + 100)
+    cases["IFD beyond the file"] = b
+    b = bytearray(good)
+    b[8:10] = struct.pack("<H", 0xFFFF)
+    cases["entry count beyond the file"] = b
+    assert len(cases) == 3
+    for name, data in cases.items():
+        p = tmp_path / "bad.tif"
+        p.write_bytes(bytes(data))
+        with pytest.raises(pivx.PivError) as ei:
+            pivx.tiff_read8(str(p), 41, 29)
+        assert ei.value.code == 5, name
+    p = tmp_path / "short.tif"
+    p.write_bytes(_tiff_bytes(img[:20], False, 2, False))  # 20 rows for a 29-row frame
+    with pytest.raises(pivx.PivError, match="fewer bytes"):
+        pivx.tiff_read8(str(p), 41, 29)
 
 
 def test_tiff_rejects_non_tiff(pivx, tmp_path):

@@ -110,6 +110,36 @@ def test_gaussian_subpixel_is_exact(orc):
         assert r["valid"][0, 0] == 1
 
 
+@pytest.mark.parametrize("f", [(0.6, 1.0, 0.8), (0.999, 1.0, 0.9985), (120.0, 400.0, 390.0), (5.0, 5.0, 1.0)])
+def test_fit_sensitivity_bounds_finite_differences(orc, f):
+    """The ill-conditioned-fit bound equals the sum of the fit's partial
+    derivatives (numpy central differences of the cu:334-340 formula) times
+    the relative perturbation of each value, and bounds the change seen under
+    every sign pattern of a small perturbation."""
+    import itertools
+
+    def fit(a, b, c):
+        la, lb, lc = np.log(a), np.log(b), np.log(c)
+        return (la - lc) / (la + lc - 2 * lb) / 2
+
+    delta = 1e-7 * max(f)
+    got = orc.fit_sensitivity(*f, delta)
+    want = 0.0
+    for i in range(3):
+        h = 1e-6 * f[i]
+        up, dn = list(f), list(f)
+        up[i] += h
+        dn[i] -= h
+        want += abs((fit(*up) - fit(*dn)) / (2 * h)) * delta
+    assert got == pytest.approx(want, rel=1e-4)
+    base = fit(*f)
+    worst = max(abs(fit(*(np.array(f) + delta * np.array(sg))) - base) for sg in itertools.product((-1, 1), repeat=3))
+    assert worst <= got * (1 + 1e-3)
+    assert worst >= got * 0.99  # first order: the corner pattern attains the bound
+    # a flat peak is ill-conditioned, a sharp Gaussian peak is not
+    assert orc.fit_sensitivity(0.9999, 1.0, 0.99985, 1e-6) > 1e-3 > orc.fit_sensitivity(0.4, 1.0, 0.5, 1e-6)
+
+
 def _peaks_numpy(cc, bx, by, K, pred, lims):
     """Brute-force restatement of local maxima, ranking and the 3-point fit."""
     wsy, wsx = cc.shape
@@ -425,6 +455,73 @@ def test_run_pair_composes_the_stages(orc, pivx):
     sx, sy = orc.smooth(sx, sy, nv, (3, 3))
     np.testing.assert_array_equal(out["filt1"][0], sx)
     np.testing.assert_array_equal(out["filt1"][2], sv)
+
+
+def test_run_pair_two_grid_composes_the_stages(orc, pivx):
+    """orc_run_pair's two-grid branch (main:289-375) equals the stage
+    functions composed by hand: grid-1 passes (pass 0 undeformed with a zero
+    predictor, later passes deformed by the extended filtered field), the
+    upsampled predictor, deformed grid-2 passes, and the median-only re-run
+    of the last grid-2 candidates (no fill, no smoothing)."""
+    nx, ny = 256, 192
+    sp = pivx.synth_params(nx, ny, seed=19, flow=1, a0=200.0, a1=4.0, a2=0.002)
+    a, b = pivx.synth_pair(sp)
+    ip = pivx.init_params(nx=nx, ny=ny, ws_x1=64, ws_y1=64, spacing_x1=32, spacing_y1=32, npass1=2, ws_x2=32,
+                          ws_y2=32, spacing_x2=16, spacing_y2=16, npass2=2)
+    fp1 = pivx.FilterParams(2, 5, 2.0, 2.0, 1)
+    fp2 = pivx.FilterParams(1, 4, 2.0, 3.0, 2)
+    out = orc.run_pair(a, b, ip, fp1, fp2)
+    geo = [orc.geometry(nx, ny, 1, 64, 64, 32, 32, 0.4, 0.4), orc.geometry(nx, ny, 1, 32, 32, 16, 16, 0.4, 0.4)]
+    ws, sps = [(64, 64), (32, 32)], [(32, 32), (16, 16)]
+
+    def filt(r, nv, f, fill=True, smooth=True):
+        cand = (r["dx"].ravel(), r["dy"].ravel(), r["valid"].ravel())
+        sx, sy, sv = r["dx"][:, 0], r["dy"][:, 0], r["valid"][:, 0]
+        for _ in range(f.npass_median):
+            sx, sy, sv, _m = orc.median_pass((sx, sy, sv), cand, 4, nv, (5, 5), f.min_nneigh, f.insert_thr)
+        if fill:
+            sx, sy, sv, _s = orc.fill(sx, sy, sv, nv)
+        if smooth:
+            for _ in range(f.npass_smooth):
+                sx, sy = orc.smooth(sx, sy, nv, (3, 3))
+        return sx, sy, sv
+
+    nv = [(g["nx_vec"], g["ny_vec"]) for g in geo]
+    fx = np.zeros(nv[0][0] * nv[0][1])
+    fy = np.zeros_like(fx)
+    for j in range(2):
+        kw = dict(geom=geo[0], ws=ws[0], sp=sps[0], K=4, pred=(fx, fy))
+        if j == 0:
+            r = orc.xcorr_pass(a, f2u8=b, **kw)
+        else:
+            ex, ey = orc.extend(fx.reshape(nv[0][1], nv[0][0]), fy.reshape(nv[0][1], nv[0][0]))
+            r = orc.xcorr_pass(a, f2def=orc.deform(b, geo[0], ws[0], sps[0], ex, ey), **kw)
+        fx, fy, fv = filt(r, nv[0], fp1)
+    np.testing.assert_array_equal(out["raw1"][0], r["dx"])
+    np.testing.assert_array_equal(out["filt1"][0], fx)
+    np.testing.assert_array_equal(out["filt1"][2], fv)
+    ex, ey = orc.extend(fx.reshape(nv[0][1], nv[0][0]), fy.reshape(nv[0][1], nv[0][0]))
+    gx, gy = orc.upsample(ex, ey, sps[0], sps[1], nv[1])
+    for j in range(2):
+        ex, ey = orc.extend(gx.reshape(nv[1][1], nv[1][0]), gy.reshape(nv[1][1], nv[1][0]))
+        r = orc.xcorr_pass(a, f2def=orc.deform(b, geo[1], ws[1], sps[1], ex, ey), geom=geo[1], ws=ws[1], sp=sps[1],
+                           K=4, pred=(gx, gy))
+        gx, gy, gv = filt(r, nv[1], fp2)
+    np.testing.assert_array_equal(out["raw2"][0], r["dx"])
+    np.testing.assert_array_equal(out["raw2"][2], r["valid"])
+    np.testing.assert_array_equal(out["filt2"][0], gx)
+    np.testing.assert_array_equal(out["filt2"][1], gy)
+    np.testing.assert_array_equal(out["filt2"][2], gv)
+    mx, my, mv = filt(r, nv[1], fp2, fill=False, smooth=False)
+    np.testing.assert_array_equal(out["mo2"][0], mx)
+    np.testing.assert_array_equal(out["mo2"][2], mv)
+    # the composed field follows the imposed flow (the branch is not vacuous)
+    cx = 8 + 16 * np.arange(nv[1][0])
+    cy = 8 + 16 * np.arange(nv[1][1])
+    X, Y = np.meshgrid(cx, cy)
+    u, v = pivx.synth_flow(sp, X.ravel(), Y.ravel())
+    inner = ((X > 32) & (X < nx - 32) & (Y > 32) & (Y < ny - 32)).ravel()
+    assert np.sqrt(np.mean((gx[inner] - u[inner]) ** 2 + (gy[inner] - v[inner]) ** 2)) < 0.2
 
 
 # ---------------------------------------------------------------- MHE (MHE7_4_gpu.cu)
