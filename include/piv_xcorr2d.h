@@ -180,6 +180,30 @@ int pivx_median_filter_smoothing_p1(pivx_ctx* ctx, int npass_median, int min_nne
 /* xcorr2d_multi::extend_filtered_vecfield_p1 (cu:1547-1576). */
 int pivx_extend_filtered_vecfield_p1(pivx_ctx* ctx);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the validation stage of pivx_median_filter_smoothing_p2
+ * (median passes, optional hole filling, smoothing) on caller device buffers,
+ * with no handle and no allocation.  Inputs: raw candidates
+ * [nx_vec*ny_vec][npeak_save] (dx, dy, valid) in device memory of the current
+ * device; outputs: the filtered field [nx_vec*ny_vec] in device memory.  The
+ * other arguments mean what they mean for pivx_init (median and smoothing
+ * windows) and pivx_median_filter_smoothing_p2.  Work memory is a static
+ * device array of the library (about 4.6 MB per GPU, held until the process
+ * ends) covering at most PIVX_NOALLOC_MAX_VECTORS vectors; larger fields, and
+ * median windows of more than 64 vectors, are rejected with PIVX_ERR_PARAM.
+ * All work is enqueued on `stream` (a hipStream_t; NULL = the null stream)
+ * and the call returns without waiting.  Every call on one GPU shares that
+ * array: calls that overlap in time on one GPU (other streams, other host
+ * threads) race and must be serialised by the caller.  Errors are reported
+ * through pivx_last_error(NULL). */
+#define PIVX_NOALLOC_MAX_VECTORS 131072
+int pivx_median_filter_smoothing_noalloc(const float* d_cand_x, const float* d_cand_y, const uint8_t* d_cand_valid,
+                                         int nx_vec, int ny_vec, int npeak_save, int wx_median, int wy_median,
+                                         int npass_median, int min_nneigh, float residual_thr, float insert_thr,
+                                         int wx_smooth, int wy_smooth, int npass_smooth, int iffill, float* d_out_x,
+                                         float* d_out_y, uint8_t* d_out_valid, void* stream);
+
 /* xcorr2d_multi::Upsampling_vector_field (cu:1578-1587): bilinear grid-1 ->
  * grid-2 predictor. */
 int pivx_upsampling_vector_field(pivx_ctx* ctx);

@@ -107,6 +107,9 @@ _SIGS = {
     "pivx_calc_cc_map_p1": [_P, C.c_int],
     "pivx_median_filter_smoothing_p1": [_P, C.c_int, C.c_int, C.c_float, C.c_float, C.c_int],
     "pivx_extend_filtered_vecfield_p1": [_P],
+    "pivx_median_filter_smoothing_noalloc": [_P, _P, _P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                             C.c_int, C.c_float, C.c_float, C.c_int, C.c_int, C.c_int, C.c_int,
+                                             _P, _P, _P, _P],
     "pivx_upsampling_vector_field": [_P],
     "pivx_extend_filtered_vecfield_p2": [_P],
     "pivx_calc_cc_map_p2": [_P],
@@ -382,6 +385,22 @@ class Xcorr2dMulti:
         n = C.c_int()
         self._c(lib().pivx_get_stage_ms(self.h, grid, stage, C.byref(ms), C.byref(n)))
         return ms.value, n.value
+
+
+NOALLOC_MAX_VECTORS = 131072  # PIVX_NOALLOC_MAX_VECTORS
+
+
+def median_filter_smoothing_noalloc(d_cand_x: int, d_cand_y: int, d_cand_valid: int, nx_vec: int, ny_vec: int,
+                                    npeak_save: int, wx_median: int, wy_median: int, npass_median: int,
+                                    min_nneigh: int, residual_thr: float, insert_thr: float, wx_smooth: int,
+                                    wy_smooth: int, npass_smooth: int, iffill: int, d_out_x: int, d_out_y: int,
+                                    d_out_valid: int, stream: int = 0):
+    """pivx_median_filter_smoothing_noalloc on device addresses (ints):
+    enqueues the validation stage on `stream` without allocating."""
+    _check(lib().pivx_median_filter_smoothing_noalloc(
+        C.c_void_p(d_cand_x), C.c_void_p(d_cand_y), C.c_void_p(d_cand_valid), nx_vec, ny_vec, npeak_save, wx_median,
+        wy_median, npass_median, min_nneigh, residual_thr, insert_thr, wx_smooth, wy_smooth, npass_smooth, iffill,
+        C.c_void_p(d_out_x), C.c_void_p(d_out_y), C.c_void_p(d_out_valid), C.c_void_p(stream)))
 
 
 # ---------------------------------------------------------------- host utilities

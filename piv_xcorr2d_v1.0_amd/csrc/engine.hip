@@ -21,6 +21,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -233,12 +234,13 @@ int next_event(pivx_ctx::Timeline& t) {
   return (int)t.used++;
 }
 void tic(pivx_ctx* c) {
-  if (!c->timing) return;
+  if (!c || !c->timing) return;
   pivx_ctx::Timeline& t = c->tl;
   t.start = next_event(t);
   if (t.start >= 0) (void)hipEventRecord(t.pool[t.start], c->stream);
 }
 void toc(pivx_ctx* c, int gi, int stage) {
+  if (!c) return;
   pivx_ctx::Timeline& t = c->tl;
   if (!c->timing || t.start < 0) return;
   const int e = next_event(t);
@@ -353,6 +355,22 @@ int run_xcorr(pivx_ctx* c, int gi, bool deformed, bool zero_pred = false, bool f
   return rc;
 }
 
+// Everything one filter sequence reads and works in: the raw candidates, the
+// field and window sizes, and the work buffers (owned by a handle, or carved
+// from the static scratch of pivx_filter_field_noalloc).
+struct FilterJob {
+  const float* cx; const float* cy; const uint8_t* cv; int K;  // raw candidates [nt][K]
+  int nx_vec, ny_vec;
+  int wxm, wym, wxs, wys;                                        // median and smoothing windows
+  float *sx, *sy; uint8_t* sv;                                   // ping-pong partner of the output
+  float *ux, *uy; uint8_t* uv;                                   // second intermediate set
+  int *list_a, *list_b; float *tmp_x, *tmp_y; uint8_t* tmp_f;    // hole lists and pending fills
+  int* nholes; unsigned int* drops; int* sweeps; int list_cap;
+  float* med_scratch;                                            // generic median lists (wx*wy > 64) or null
+  int8_t* accepted;                                              // accepted candidate per vector or null
+  hipStream_t stream;
+};
+
 // MedianFilter_Smoothing_p1/p2 (cu:1451-1545, 1766-1863) writing into (ox, oy, ov).
 // Median passes, hole filling and smoothing of one grid into (ox, oy, ov).
 // Buffers are planned so that no pass needs a copy: the last median pass
@@ -362,22 +380,19 @@ int run_xcorr(pivx_ctx* c, int gi, bool deformed, bool zero_pred = false, bool f
 // between two other buffer sets.  With `ext` set and at least one smoothing
 // pass, the last smoothing pass also writes the extended field (ext_x, ext_y)
 // and *ext_done is set.  With (mx, my, mv) set, the field after the median
-// stage (before filling and smoothing) is also written there.
-int run_filter(pivx_ctx* c, int gi, int npass_median, int min_nneigh, float insert_thr, int npass_smooth, int iffill,
-               float* ox, float* oy, uint8_t* ov, float* ext_x = nullptr, float* ext_y = nullptr,
-               bool* ext_done = nullptr, float* mx = nullptr, float* my = nullptr, uint8_t* mv = nullptr) {
-  GridBuf& G = c->grid[gi];
-  const pivx_geometry& g = G.g;
-  const int nt = g.nt_vec;
-  const int K = c->p.npeak_save;
-  const int wxm = gi == 0 ? c->p.wx1_median : c->p.wx2_median;
-  const int wym = gi == 0 ? c->p.wy1_median : c->p.wy2_median;
-  const int wxs = gi == 0 ? c->p.wx1_smooth : c->p.wx2_smooth;
-  const int wys = gi == 0 ? c->p.wy1_smooth : c->p.wy2_smooth;
+// stage (before filling and smoothing) is also written there.  c may be null
+// (no stage timing; errors go to the context-free error text).
+int enqueue_filter(pivx_ctx* c, int gi, const FilterJob& J, int npass_median, int min_nneigh, float insert_thr,
+                   int npass_smooth, int iffill, float* ox, float* oy, uint8_t* ov, float* ext_x, float* ext_y,
+                   bool* ext_done, float* mx, float* my, uint8_t* mv) {
+  const int nt = J.nx_vec * J.ny_vec;
+  const int K = J.K;
+  const int wxm = J.wxm, wym = J.wym, wxs = J.wxs, wys = J.wys;
+  const hipStream_t stream = J.stream;
   if (ext_done) *ext_done = false;
   const bool smooth_odd = npass_smooth > 0 && npass_smooth % 2 == 1;
-  float* t0x = smooth_odd ? G.sx : ox;
-  float* t0y = smooth_odd ? G.sy : oy;
+  float* t0x = smooth_odd ? J.sx : ox;
+  float* t0y = smooth_odd ? J.sy : oy;
   const bool fill = iffill == 1;
   const bool listed = fill && npass_median > 0;  // the last median pass lists the holes
   int rc;
@@ -388,35 +403,35 @@ int run_filter(pivx_ctx* c, int gi, int npass_median, int min_nneigh, float inse
   // earlier sequence (an interrupted pair, a caller's partial sequence) can
   // leave a stale count behind.
   if (npass_median <= 0) {
-    rc = check_launch(c, launch_prep(G.dx, G.dy, G.dv, K, t0x, t0y, ov, nt, fill ? c->nholes : nullptr, c->stream),
+    rc = check_launch(c, launch_prep(J.cx, J.cy, J.cv, K, t0x, t0y, ov, nt, fill ? J.nholes : nullptr, stream),
                       "median_filter_data_prep");
     if (!rc && mx)
-      rc = check_launch(c, launch_prep(G.dx, G.dy, G.dv, K, mx, my, mv, nt, nullptr, c->stream),
+      rc = check_launch(c, launch_prep(J.cx, J.cy, J.cv, K, mx, my, mv, nt, nullptr, stream),
                         "median_filter_data_prep");
     if (rc) return rc;
   } else {
-    if (listed && npass_median == 1 && (rc = check_launch(c, launch_zero_ints(c->nholes, 1, c->stream), "hole counter")))
+    if (listed && npass_median == 1 && (rc = check_launch(c, launch_zero_ints(J.nholes, 1, stream), "hole counter")))
       return rc;
     // intermediate sets: I1 = (ux, uy, uv), I2 = whichever of (ox, oy) / (sx, sy) is not T0, with sv
-    float* i2x = smooth_odd ? ox : G.sx;
-    float* i2y = smooth_odd ? oy : G.sy;
-    const float* srcx = G.dx;
-    const float* srcy = G.dy;
-    const uint8_t* srcv = G.dv;
+    float* i2x = smooth_odd ? ox : J.sx;
+    float* i2y = smooth_odd ? oy : J.sy;
+    const float* srcx = J.cx;
+    const float* srcy = J.cy;
+    const uint8_t* srcv = J.cv;
     int sstride = K;
     for (int p = 0; p < npass_median; ++p) {
       const int back = npass_median - 1 - p;  // 0 for the last pass
-      float* tx = back == 0 ? t0x : (back % 2 == 1 ? G.ux : i2x);
-      float* ty = back == 0 ? t0y : (back % 2 == 1 ? G.uy : i2y);
-      uint8_t* tv = back == 0 ? ov : (back % 2 == 1 ? G.uv : G.sv);
+      float* tx = back == 0 ? t0x : (back % 2 == 1 ? J.ux : i2x);
+      float* ty = back == 0 ? t0y : (back % 2 == 1 ? J.uy : i2y);
+      uint8_t* tv = back == 0 ? ov : (back % 2 == 1 ? J.uv : J.sv);
       MedArgs m{};
       m.sx = srcx;
       m.sy = srcy;
       m.sv = srcv;
       m.sstride = sstride;
-      m.cx = G.dx;
-      m.cy = G.dy;
-      m.cv = G.dv;
+      m.cx = J.cx;
+      m.cy = J.cy;
+      m.cv = J.cv;
       m.K = K;
       m.dx = tx;
       m.dy = ty;
@@ -426,8 +441,8 @@ int run_filter(pivx_ctx* c, int gi, int npass_median, int min_nneigh, float inse
         m.dy2 = my;
         m.dv2 = mv;
       }
-      m.nx_vec = g.nx_vec;
-      m.ny_vec = g.ny_vec;
+      m.nx_vec = J.nx_vec;
+      m.ny_vec = J.ny_vec;
       m.wx = wxm;
       m.wy = wym;
       m.hfwx = (wxm - 1) / 2;
@@ -435,13 +450,13 @@ int run_filter(pivx_ctx* c, int gi, int npass_median, int min_nneigh, float inse
       m.min_nneigh = min_nneigh;
       m.thr = insert_thr;
       m.eps = kEpsilon;
-      m.accepted = back == 0 ? G.accepted : nullptr;
-      m.scratch = G.med_scratch;
-      m.hole_list = (back == 0 && listed) ? c->list_a : nullptr;
-      m.nholes = (back == 0 && listed) ? c->nholes : nullptr;
-      m.hole_cap = c->list_cap;
-      m.reset_count = (listed && p == 0 && back > 0) ? c->nholes : nullptr;
-      rc = check_launch(c, launch_median(m, c->stream), "median_filtering");
+      m.accepted = back == 0 ? J.accepted : nullptr;
+      m.scratch = J.med_scratch;
+      m.hole_list = (back == 0 && listed) ? J.list_a : nullptr;
+      m.nholes = (back == 0 && listed) ? J.nholes : nullptr;
+      m.hole_cap = J.list_cap;
+      m.reset_count = (listed && p == 0 && back > 0) ? J.nholes : nullptr;
+      rc = check_launch(c, launch_median(m, stream), "median_filtering");
       if (rc) return rc;
       srcx = tx;
       srcy = ty;
@@ -455,20 +470,20 @@ int run_filter(pivx_ctx* c, int gi, int npass_median, int min_nneigh, float inse
     f.dx = t0x;
     f.dy = t0y;
     f.dv = ov;
-    f.nx_vec = g.nx_vec;
-    f.ny_vec = g.ny_vec;
-    f.list_a = c->list_a;
-    f.list_b = c->list_b;
-    f.tmp_x = c->tmp_x;
-    f.tmp_y = c->tmp_y;
-    f.tmp_f = c->tmp_f;
-    f.nholes = c->nholes;
-    f.list_cap = c->list_cap;
-    f.drops = c->drops;
-    f.sweeps = c->sweeps + gi;
+    f.nx_vec = J.nx_vec;
+    f.ny_vec = J.ny_vec;
+    f.list_a = J.list_a;
+    f.list_b = J.list_b;
+    f.tmp_x = J.tmp_x;
+    f.tmp_y = J.tmp_y;
+    f.tmp_f = J.tmp_f;
+    f.nholes = J.nholes;
+    f.list_cap = J.list_cap;
+    f.drops = J.drops;
+    f.sweeps = J.sweeps;
     f.prelisted = listed ? 1 : 0;
     tic(c);
-    rc = check_launch(c, launch_fill(f, c->stream), "fill_blank_vecfield");
+    rc = check_launch(c, launch_fill(f, stream), "fill_blank_vecfield");
     toc(c, gi, 3);
     if (rc) return rc;
   }
@@ -476,15 +491,15 @@ int run_filter(pivx_ctx* c, int gi, int npass_median, int min_nneigh, float inse
     tic(c);
     float *cx = t0x, *cy = t0y;
     for (int s = 0; s < npass_smooth; ++s) {
-      float* nx_ = (cx == ox) ? G.sx : ox;
-      float* ny_ = (cy == oy) ? G.sy : oy;
+      float* nx_ = (cx == ox) ? J.sx : ox;
+      float* ny_ = (cy == oy) ? J.sy : oy;
       SmoothArgs sa{};
       sa.sx = cx;
       sa.sy = cy;
       sa.dx = nx_;
       sa.dy = ny_;
-      sa.nx_vec = g.nx_vec;
-      sa.ny_vec = g.ny_vec;
+      sa.nx_vec = J.nx_vec;
+      sa.ny_vec = J.ny_vec;
       sa.wx = wxs;
       sa.wy = wys;
       sa.hfwx = (wxs - 1) / 2;
@@ -495,7 +510,7 @@ int run_filter(pivx_ctx* c, int gi, int npass_median, int min_nneigh, float inse
         sa.border = kExBorder;
         if (ext_done) *ext_done = true;
       }
-      rc = check_launch(c, launch_smooth(sa, c->stream), "smoothing_tophat");
+      rc = check_launch(c, launch_smooth(sa, stream), "smoothing_tophat");
       if (rc) return rc;
       cx = nx_;
       cy = ny_;
@@ -503,6 +518,43 @@ int run_filter(pivx_ctx* c, int gi, int npass_median, int min_nneigh, float inse
     toc(c, gi, 4);
   }
   return PIVX_OK;
+}
+
+int run_filter(pivx_ctx* c, int gi, int npass_median, int min_nneigh, float insert_thr, int npass_smooth, int iffill,
+               float* ox, float* oy, uint8_t* ov, float* ext_x = nullptr, float* ext_y = nullptr,
+               bool* ext_done = nullptr, float* mx = nullptr, float* my = nullptr, uint8_t* mv = nullptr) {
+  GridBuf& G = c->grid[gi];
+  FilterJob J{};
+  J.cx = G.dx;
+  J.cy = G.dy;
+  J.cv = G.dv;
+  J.K = c->p.npeak_save;
+  J.nx_vec = G.g.nx_vec;
+  J.ny_vec = G.g.ny_vec;
+  J.wxm = gi == 0 ? c->p.wx1_median : c->p.wx2_median;
+  J.wym = gi == 0 ? c->p.wy1_median : c->p.wy2_median;
+  J.wxs = gi == 0 ? c->p.wx1_smooth : c->p.wx2_smooth;
+  J.wys = gi == 0 ? c->p.wy1_smooth : c->p.wy2_smooth;
+  J.sx = G.sx;
+  J.sy = G.sy;
+  J.sv = G.sv;
+  J.ux = G.ux;
+  J.uy = G.uy;
+  J.uv = G.uv;
+  J.list_a = c->list_a;
+  J.list_b = c->list_b;
+  J.tmp_x = c->tmp_x;
+  J.tmp_y = c->tmp_y;
+  J.tmp_f = c->tmp_f;
+  J.nholes = c->nholes;
+  J.drops = c->drops;
+  J.sweeps = c->sweeps + gi;
+  J.list_cap = c->list_cap;
+  J.med_scratch = G.med_scratch;
+  J.accepted = G.accepted;
+  J.stream = c->stream;
+  return enqueue_filter(c, gi, J, npass_median, min_nneigh, insert_thr, npass_smooth, iffill, ox, oy, ov, ext_x, ext_y,
+                        ext_done, mx, my, mv);
 }
 
 int extend(pivx_ctx* c, int gi) {
@@ -589,6 +641,38 @@ void frame2_border(const pivx_geometry& g, int nx, int ny, int& bx, int& by) {
   by = std::max({0, hy + g.left_y, g.ny_aug + hy + 2 - g.left_y - ny});
 }
 
+}  // namespace
+
+// ---------------------------------------------------------------- allocation-free filter
+// Work memory of pivx_median_filter_smoothing_noalloc: a file-scope device
+// array (each GPU holds its own copy for the life of the process), sized for
+// PIVX_NOALLOC_MAX_VECTORS vectors: six float planes (two intermediate field
+// sets, pending fills), two int hole lists, three byte planes and a 256-byte
+// block of counters.  Host code never uses the array's name as an address:
+// the address of each device's copy comes from hipGetSymbolAddress, taken
+// once per device with that device current.
+constexpr size_t kNaN = PIVX_NOALLOC_MAX_VECTORS;
+constexpr size_t kNaBytes = 256 + 8 * 4 * kNaN + 3 * kNaN;
+static __device__ __align__(256) unsigned char scratch[kNaBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch[kMaxDevices] = {};
+
+int scratch_of_device(int dev, unsigned char** out) {
+  if (dev < 0 || dev >= kMaxDevices) return fail(nullptr, PIVX_ERR_PARAM, "device %d out of range", dev);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return fail(nullptr, PIVX_ERR_HIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+    g_scratch[dev] = static_cast<unsigned char*>(ptr);
+  }
+  *out = g_scratch[dev];
+  return PIVX_OK;
+}
 }  // namespace
 
 // ====================================================================== C ABI
@@ -859,6 +943,66 @@ int pivx_median_filter_smoothing_p2(pivx_ctx* c, int npass_median, int min_nneig
   if (!c) return fail(nullptr, PIVX_ERR_PARAM, "null handle");
   GridBuf& G = c->grid[1];
   return run_filter(c, 1, npass_median, min_nneigh, insert_thr, npass_smooth, iffill, G.fx, G.fy, G.fv);
+}
+
+int pivx_median_filter_smoothing_noalloc(const float* d_cand_x, const float* d_cand_y, const uint8_t* d_cand_valid,
+                                         int nx_vec, int ny_vec, int npeak_save, int wx_median, int wy_median,
+                                         int npass_median, int min_nneigh, float residual_thr, float insert_thr,
+                                         int wx_smooth, int wy_smooth, int npass_smooth, int iffill, float* d_out_x,
+                                         float* d_out_y, uint8_t* d_out_valid, void* stream) {
+  (void)residual_thr;  // read but never used by the reference
+  if (!d_cand_x || !d_cand_y || !d_cand_valid || !d_out_x || !d_out_y || !d_out_valid)
+    return fail(nullptr, PIVX_ERR_PARAM, "null buffer");
+  if (nx_vec < 1 || ny_vec < 1 || npeak_save < 1) return fail(nullptr, PIVX_ERR_PARAM, "bad field size");
+  const long long nt = (long long)nx_vec * ny_vec;
+  if (nt > (long long)PIVX_NOALLOC_MAX_VECTORS)
+    return fail(nullptr, PIVX_ERR_PARAM, "field of %lld vectors exceeds the allocation-free limit of %d vectors", nt,
+                PIVX_NOALLOC_MAX_VECTORS);
+  if (wx_median < 1 || wy_median < 1 || wx_smooth < 1 || wy_smooth < 1)
+    return fail(nullptr, PIVX_ERR_PARAM, "median/smoothing window sizes must be >= 1");
+  if (npass_median > 0 && wx_median * wy_median > 64)
+    return fail(nullptr, PIVX_ERR_PARAM,
+                "median window %dx%d exceeds the allocation-free limit of 64 vectors (larger windows need the "
+                "per-vector lists of pivx_median_filter_smoothing_p2)", wx_median, wy_median);
+  int dev = 0;
+  HIPCHK(nullptr, hipGetDevice(&dev));
+  unsigned char* base = nullptr;
+  int rc = scratch_of_device(dev, &base);
+  if (rc) return rc;
+  FilterJob J{};
+  J.cx = d_cand_x;
+  J.cy = d_cand_y;
+  J.cv = d_cand_valid;
+  J.K = npeak_save;
+  J.nx_vec = nx_vec;
+  J.ny_vec = ny_vec;
+  J.wxm = wx_median;
+  J.wym = wy_median;
+  J.wxs = wx_smooth;
+  J.wys = wy_smooth;
+  // counters first, then the planes (each a multiple of 256 bytes apart)
+  J.nholes = reinterpret_cast<int*>(base);
+  J.drops = reinterpret_cast<unsigned int*>(base + 64);
+  J.sweeps = reinterpret_cast<int*>(base + 128);
+  float* f = reinterpret_cast<float*>(base + 256);
+  J.sx = f;
+  J.sy = f + kNaN;
+  J.ux = f + 2 * kNaN;
+  J.uy = f + 3 * kNaN;
+  J.tmp_x = f + 4 * kNaN;
+  J.tmp_y = f + 5 * kNaN;
+  J.list_a = reinterpret_cast<int*>(f + 6 * kNaN);
+  J.list_b = reinterpret_cast<int*>(f + 7 * kNaN);
+  uint8_t* b8 = reinterpret_cast<uint8_t*>(f + 8 * kNaN);
+  J.sv = b8;
+  J.uv = b8 + kNaN;
+  J.tmp_f = b8 + 2 * kNaN;
+  J.list_cap = (int)kNaN;
+  J.med_scratch = nullptr;
+  J.accepted = nullptr;
+  J.stream = static_cast<hipStream_t>(stream);
+  return enqueue_filter(nullptr, 1, J, npass_median, min_nneigh, insert_thr, npass_smooth, iffill, d_out_x, d_out_y,
+                        d_out_valid, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
 }
 
 int pivx_extend_filtered_vecfield_p1(pivx_ctx* c) {
