@@ -153,6 +153,127 @@ def cli_leg(pivx, npairs, timeout_s=300):
         shutil.rmtree(d, ignore_errors=True)
 
 
+def learned_leg(pivx, npairs, epochs, neval, nspeed, pipelines):
+    """Config 5: the learned grid-1 predictor against the plain pipeline on
+    2048x2048 pairs (random Fourier-mode flows, max 12 px, density 0.05).
+    Plain: grid 1 64x64 at 32 x3 passes, grid 2 32x32 at 8 (75% overlap) x3.
+    Learned: the network field replaces grid 1 (plain grid 1 as the fallback
+    above 5% median-test outliers), then the same grid 2.  The network is
+    trained here, on the GPU, on `npairs` device-rendered pairs labelled by
+    the plain grid-1 passes; RMS errors against the analytic flow are over
+    `neval` held-out pairs (interior vectors); pairs/s are whole pairs with the
+    frames resident in HBM, spread over `pipelines` contexts replaying their
+    pair graphs.  Also reported: the learned pipeline with 2 grid-2 passes (the
+    lowest pass count whose RMS stays within plain + 0.02 px) and the plain
+    pipeline with 2 grid-2 passes."""
+    import numpy as np
+
+    n = 2048
+
+    def params(npass2):
+        return pivx.init_params(nx=n, ny=n, ws_x1=64, ws_y1=64, spacing_x1=32, spacing_y1=32, npass1=3, ws_x2=32,
+                                ws_y2=32, spacing_x2=8, spacing_y2=8, npass2=npass2)
+
+    fp = pivx.FilterParams(npass_median=2, min_nneigh=5, residual_thr=2.0, insert_thr=2.0, npass_smooth=1)
+    frac = 0.05
+    synth = pivx.synth_params(n, n, seed=100000, flow=2, a0=12.0)
+    tp = pivx.CnnTrainParams(npairs=npairs, epochs=epochs, batch=8, lr=2e-3, shuffle_seed=7, synth=synth, grid1=fp)
+    out = {"workload": "2048x2048 u8 pairs, Fourier-mode flows max 12 px; grid1 64x64@32 x3 (plain) or CNN; "
+                       "grid2 32x32@8 (75% overlap)",
+           "fallback_max_outlier_fraction": frac}
+    x3 = pivx.Xcorr2dMulti(params(3))
+    x2 = pivx.Xcorr2dMulti(params(2))
+    net = pivx.Cnn(0, seed=1)
+    try:
+        losses, rep = net.train(x3, tp)
+        out["training"] = {"pairs": npairs, "epochs": epochs, "batch": 8, "steps": rep.steps,
+                           "label_seconds": round(rep.gen_seconds, 2), "train_seconds": round(rep.train_seconds, 2),
+                           "loss_first_epoch_px2": round(float(losses[0]), 4),
+                           "loss_last_epoch_px2": round(float(losses[-1]), 4), "params": net.n}
+        print(f"[bench] learned: trained on {npairs} pairs x {epochs} epochs in {rep.train_seconds:.1f}s "
+              f"(labels {rep.gen_seconds:.1f}s), loss {losses[0]:.3f} -> {losses[-1]:.3f}", file=sys.stderr,
+              flush=True)
+        gx, gy = x3.grid_coords(2)
+        inner = (gx > 48) & (gx < n - 48) & (gy > 48) & (gy < n - 48)
+        acc = {k: [] for k in ("plain3", "learned3", "learned2", "plain2", "net_vs_plain_grid1")}
+        fallbacks = outl = 0
+
+        def rms(x, u, v):
+            fx, fy, _ = x.filtered(2)
+            return float(np.sqrt(np.mean((fx[inner] - u[inner]) ** 2 + (fy[inner] - v[inner]) ** 2)))
+
+        for i in range(neval):
+            sp = pivx.synth_params(n, n, seed=900000 + i, flow=2, a0=12.0)
+            u, v = pivx.synth_flow(sp, gx, gy)
+            x3.synth_pair_device(sp)
+            x3.run_pair(fp, fp)
+            acc["plain3"].append(rms(x3, u, v))
+            g1 = x3.filtered(1)
+            x3.run_pair_learned(net, fp, fp, frac)
+            acc["learned3"].append(rms(x3, u, v))
+            o, f = x3.learned_stats()
+            outl += o
+            fallbacks += f
+            n1 = x3.filtered(1)
+            acc["net_vs_plain_grid1"].append(float(np.sqrt(np.mean((n1[0] - g1[0]) ** 2 + (n1[1] - g1[1]) ** 2))))
+            x2.synth_pair_device(sp)
+            x2.run_pair_learned(net, fp, fp, frac)
+            acc["learned2"].append(rms(x2, u, v))
+            x2.run_pair(fp, fp)
+            acc["plain2"].append(rms(x2, u, v))
+        out["eval_pairs"] = neval
+        out["rms_err_px"] = {k: round(float(np.mean(v)), 5) for k, v in acc.items()}
+        out["rms_err_px_max"] = {k: round(float(np.max(v)), 5) for k, v in acc.items()}
+        out["fallbacks"] = fallbacks
+        out["network_field_outlier_fraction"] = round(outl / (neval * 64 * 64), 5)
+        print(f"[bench] learned: held-out RMS {out['rms_err_px']}", file=sys.stderr, flush=True)
+    finally:
+        x3.close()
+        x2.close()
+
+    def throughput(npass2, learned):
+        ctxs = [pivx.Xcorr2dMulti(params(npass2)) for _ in range(pipelines)]
+        try:
+            for j, c in enumerate(ctxs):
+                c.synth_pair_device(pivx.synth_params(n, n, seed=900000 + j, flow=2, a0=12.0))
+                c.set_graph_mode(True)
+
+            def run(c):
+                if learned:
+                    c.run_pair_learned(net, fp, fp, frac)
+                else:
+                    c.run_pair(fp, fp)
+
+            for c in ctxs:  # capture the graphs
+                run(c)
+                run(c)
+            for c in ctxs:
+                c.synchronize()
+            t0 = time.perf_counter()
+            for j in range(nspeed):
+                run(ctxs[j % pipelines])
+            for c in ctxs:
+                c.synchronize()
+            return nspeed / (time.perf_counter() - t0)
+        finally:
+            for c in ctxs:
+                c.close()
+
+    try:
+        pps = {"plain3": throughput(3, False), "learned3": throughput(3, True), "learned2": throughput(2, True),
+               "plain2": throughput(2, False)}
+    finally:
+        net.close()
+    out["pairs_per_s"] = {k: round(v, 1) for k, v in pps.items()}
+    r = out["rms_err_px"]
+    out["speedup_learned_vs_plain_3_grid2_passes"] = round(pps["learned3"] / pps["plain3"], 3)
+    out["speedup_learned_2_passes_vs_plain_3"] = round(pps["learned2"] / pps["plain3"], 3)
+    out["learned_within_plain_plus_0p02px"] = {"3_grid2_passes": r["learned3"] <= r["plain3"] + 0.02,
+                                               "2_grid2_passes": r["learned2"] <= r["plain3"] + 0.02}
+    print(f"[bench] learned: pairs/s {out['pairs_per_s']}", file=sys.stderr, flush=True)
+    return out
+
+
 def _windows(pivx, ip):
     g1 = _geom(ip.nx, ip.ny, ip.spacing_x1, ip.spacing_y1)
     g2 = _geom(ip.nx, ip.ny, ip.spacing_x2, ip.spacing_y2)
@@ -192,6 +313,10 @@ def main():
     ap.add_argument("--cli-pairs", type=int, default=16, help="pairs for the end-to-end CLI leg (0 = skip)")
     ap.add_argument("--pipelines", type=int, default=4,
                     help="engine contexts (each its own stream and pair graph) the pairs are spread over")
+    ap.add_argument("--learned-pairs", type=int, default=2000, help="training pairs of the learned leg (0 = skip)")
+    ap.add_argument("--learned-epochs", type=int, default=16)
+    ap.add_argument("--learned-eval", type=int, default=200, help="held-out pairs of the learned leg")
+    ap.add_argument("--learned-speed-pairs", type=int, default=200, help="pairs timed per learned-leg pipeline")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one GPU only (the PIV path runs one process on one MI355X); "
@@ -354,6 +479,9 @@ def main():
     }
     if args.cli_pairs > 0:
         out["end_to_end_cli"] = cli_leg(pivx, args.cli_pairs)
+    if args.learned_pairs > 0:
+        out["learned"] = learned_leg(pivx, args.learned_pairs, args.learned_epochs, args.learned_eval,
+                                     args.learned_speed_pairs, max(1, args.pipelines))
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         cb = cpu_baseline(pivx, ip, fp, a, b, threads, args.cpu_budget)

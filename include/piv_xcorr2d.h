@@ -355,6 +355,81 @@ int pivx_synth_pair(const pivx_synth_params* sp, uint8_t* img1, uint8_t* img2);
 int pivx_synth_flow(const pivx_synth_params* sp, const float* x, const float* y, int n,
                     float* u, float* v);
 
+/* Device rendering of a synthetic pair straight into the handle's frames (no
+ * reference counterpart): the particle model and flows of pivx_synth_pair
+ * with counter-based random numbers, so frames differ from the host
+ * generator's but follow the same flow (pivx_synth_flow) and render
+ * identically on every call.  sp->nx, sp->ny must equal the handle's frame
+ * size.  Enqueued on the handle's stream; the first call allocates two
+ * nx*ny 32-bit accumulators. */
+int pivx_synth_pair_device(pivx_ctx* ctx, const pivx_synth_params* sp);
+
+/* ------------------------------------------------------------------ learned grid-1 predictor
+ * No reference counterpart.  A 6-layer 3x3 convolutional network (channels
+ * 2-32-64-64-64-64-2, strides 2,2,2,1,1,1, leaky ReLU 0.1, 131,042
+ * parameters, bfloat16 weights and activations with FP32 accumulation on the
+ * matrix cores, FP32 master weights) maps the frame pair averaged over 4x4
+ * pixel blocks to the grid-1 displacement field, replacing the grid-1 passes
+ * of main.cpp:289-303 (Calc_cc_map_p1 / MedianFilter_Smoothing_p1) for a
+ * grid-1 spacing of 32x32 px.  It is trained on the GPU, by the project's own
+ * forward, backward and Adam kernels, on synthetic pairs labelled by the plain
+ * grid-1 passes. */
+typedef struct pivx_cnn pivx_cnn;
+int pivx_cnn_param_count(void);
+int pivx_cnn_create(int device_id, pivx_cnn** out);
+int pivx_cnn_destroy(pivx_cnn* net);
+/* He-normal weights (last layer scaled by 0.1), zero biases, from a seed. */
+int pivx_cnn_init_params(pivx_cnn* net, unsigned long long seed);
+/* FP32 master parameters, per layer W [cout][3*3][cin] then b [cout]. */
+int pivx_cnn_set_params(pivx_cnn* net, const float* params, int n);
+int pivx_cnn_get_params(pivx_cnn* net, float* params, int n);
+/* Host-side test entry points on a batch: x0 [batch][h][w][2] network input
+ * (rounded to bfloat16 on upload), out and labels [batch][2][(h/8)*(w/8)];
+ * w must be a multiple of 128 and h of 8.  pivx_cnn_gradient leaves the
+ * gradient on the device for pivx_cnn_adam_step and copies it to grad (if
+ * not NULL, pivx_cnn_param_count() floats). */
+int pivx_cnn_forward(pivx_cnn* net, const float* x0, int batch, int h, int w, float* out);
+int pivx_cnn_gradient(pivx_cnn* net, const float* x0, const float* labels, int batch, int h, int w, float* loss,
+                      float* grad);
+int pivx_cnn_adam_step(pivx_cnn* net, float lr);
+
+/* Training on synthetic pairs.  Pair i is rendered on the device with seed
+ * synth.seed + i into ctx's frames (ctx frame size = synth.nx x synth.ny,
+ * grid 1 at spacing 32x32 over nx a multiple of 512 and ny of 32), labelled
+ * by ctx's plain grid-1 passes with the `grid1` filter, and stored as network
+ * input; then `epochs` passes of minibatch Adam (cosine-decayed learning rate)
+ * over a seeded shuffle. */
+typedef struct pivx_cnn_train_params {
+  int npairs, epochs, batch;
+  float lr;
+  unsigned long long shuffle_seed;
+  pivx_synth_params synth;
+  pivx_filter_params grid1;
+} pivx_cnn_train_params;
+typedef struct pivx_cnn_train_report {
+  double gen_seconds, train_seconds;   /* wall time of labelling and of training */
+  int steps;
+  float first_epoch_loss, last_epoch_loss;   /* mean squared error, px^2 per component */
+} pivx_cnn_train_report;
+/* epoch_loss (optional): mean loss of every epoch. */
+int pivx_cnn_train(pivx_ctx* ctx, pivx_cnn* net, const pivx_cnn_train_params* tp, float* epoch_loss,
+                   pivx_cnn_train_report* report);
+
+/* The per-pair sequence of pivx_run_pair with the grid-1 passes replaced by
+ * the network: the network field becomes the grid-1 field, the normalised
+ * median test of main.cpp's grid-1 filter (its window, insert threshold and
+ * minimum neighbour count from `grid1`) counts the vectors it rejects, and
+ * when more than max_outlier_fraction of them are rejected the plain grid-1
+ * passes run instead (decided on the device; the grid-1 kernels are launched
+ * either way and return at once when not needed).  Then upsampling and the
+ * grid-2 passes as in pivx_run_pair.  Graph mode (pivx_set_graph_mode) applies.
+ * The network's parameters must not change while pairs using it are queued. */
+int pivx_run_pair_learned(pivx_ctx* ctx, const pivx_cnn* net, const pivx_filter_params* grid1,
+                          const pivx_filter_params* grid2, float max_outlier_fraction);
+/* Of the last learned pair: vectors of the network field rejected by the
+ * median test, and 1 when the plain grid-1 passes ran instead. */
+int pivx_get_learned_stats(pivx_ctx* ctx, int* outliers, int* fallback);
+
 #ifdef __cplusplus
 }
 #endif

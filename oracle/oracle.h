@@ -121,6 +121,16 @@ int orc_run_pair(int prec, const orc_run* R, const uint8_t* f1, const uint8_t* f
  * Returns 1 for a window or threshold the reference cannot run. */
 int orc_mhe(const uint8_t* in, uint8_t* out, int nx, int ny, int ws, float thresh);
 
+/* Learned grid-1 predictor (cnn_oracle.cpp).  params: the master layout of
+ * the GPU network, per layer W [cout][9][cin] then b [cout]; x0: network
+ * input [batch][h][w][2]; out and labels: [batch][2][h/8 * w/8].  bf16_storage
+ * rounds to bfloat16 wherever the GPU stores bfloat16.  grad (optional):
+ * dL/dparams of the mean squared error in the params layout. */
+int orc_cnn_param_count(void);
+int orc_cnn_forward(const double* params, const double* x0, int batch, int h, int w, int bf16_storage, double* out);
+int orc_cnn_grad(const double* params, const double* x0, const double* labels, int batch, int h, int w,
+                 int bf16_storage, double* loss, double* grad);
+
 #ifdef __cplusplus
 }
 #endif

@@ -60,6 +60,8 @@ def lib():
         L.orc_smooth.argtypes = [C.c_int, _D, _D, C.c_int, C.c_int, C.c_int, C.c_int, _D, _D]
         L.orc_run_pair.argtypes = [C.c_int, C.POINTER(OrcRun), _D, _D] + [_D] * 15
         L.orc_mhe.argtypes = [_D, _D, C.c_int, C.c_int, C.c_int, C.c_float]
+        L.orc_cnn_forward.argtypes = [_D, _D, C.c_int, C.c_int, C.c_int, C.c_int, _D]
+        L.orc_cnn_grad.argtypes = [_D, _D, _D, C.c_int, C.c_int, C.c_int, C.c_int, _D, _D]
         L.orc_fit_sensitivity.argtypes = [C.c_double] * 4
         L.orc_fit_sensitivity.restype = C.c_double
         _lib = L
@@ -135,6 +137,32 @@ def xcorr_pass(f1, f2u8=None, f2def=None, *, geom, ws, sp, K, lims=(-100, 100, -
     rc = lib().orc_xcorr_pass(prec, C.byref(P), _p(dx), _p(dy), _p(v), _p(pv), _p(pi), _p(nc), _p(tie))
     assert rc == 0
     return dict(dx=dx, dy=dy, valid=v, pk_val=pv, pk_idx=pi, ncand=nc, tie=tie)
+
+
+def cnn_param_count():
+    return lib().orc_cnn_param_count()
+
+
+def cnn_forward(params, x0, bf16=False):
+    """Network output [batch][2][h/8 * w/8] for input x0 [batch][h][w][2]."""
+    params = _f64(params)
+    x0 = _f64(x0)
+    b, h, w, _ = x0.shape
+    out = np.empty((b, 2, (h // 8) * (w // 8)))
+    assert lib().orc_cnn_forward(_p(params), _p(x0), b, h, w, 1 if bf16 else 0, _p(out)) == 0
+    return out
+
+
+def cnn_grad(params, x0, labels, bf16=False):
+    """(loss, dL/dparams) of the mean squared error against labels [batch][2][h/8 * w/8]."""
+    params = _f64(params)
+    x0 = _f64(x0)
+    labels = _f64(labels)
+    b, h, w, _ = x0.shape
+    loss = C.c_double()
+    g = np.empty(params.size)
+    assert lib().orc_cnn_grad(_p(params), _p(x0), _p(labels), b, h, w, 1 if bf16 else 0, C.byref(loss), _p(g)) == 0
+    return loss.value, g
 
 
 def fit_sensitivity(f0, f1, f2, delta):

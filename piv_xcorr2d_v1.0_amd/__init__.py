@@ -80,6 +80,18 @@ class SynthParams(C.Structure):
                 ("a2", C.c_float)]
 
 
+class CnnTrainParams(C.Structure):
+    """pivx_cnn_train_params."""
+
+    _fields_ = [("npairs", C.c_int), ("epochs", C.c_int), ("batch", C.c_int), ("lr", C.c_float),
+                ("shuffle_seed", C.c_ulonglong), ("synth", SynthParams), ("grid1", FilterParams)]
+
+
+class CnnTrainReport(C.Structure):
+    _fields_ = [("gen_seconds", C.c_double), ("train_seconds", C.c_double), ("steps", C.c_int),
+                ("first_epoch_loss", C.c_float), ("last_epoch_loss", C.c_float)]
+
+
 _P = C.c_void_p
 _F32P = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
 _U8P = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
@@ -139,6 +151,19 @@ _SIGS = {
     "pivx_tiff_read8": [C.c_char_p, _P, C.c_size_t],
     "pivx_tiff_write8": [C.c_char_p, _P, C.c_int, C.c_int],
     "pivx_synth_pair": [C.POINTER(SynthParams), _P, _P],
+    "pivx_synth_pair_device": [_P, C.POINTER(SynthParams)],
+    "pivx_cnn_param_count": [],
+    "pivx_cnn_create": [C.c_int, C.POINTER(_P)],
+    "pivx_cnn_destroy": [_P],
+    "pivx_cnn_init_params": [_P, C.c_ulonglong],
+    "pivx_cnn_set_params": [_P, _P, C.c_int],
+    "pivx_cnn_get_params": [_P, _P, C.c_int],
+    "pivx_cnn_forward": [_P, _P, C.c_int, C.c_int, C.c_int, _P],
+    "pivx_cnn_gradient": [_P, _P, _P, C.c_int, C.c_int, C.c_int, _P, _P],
+    "pivx_cnn_adam_step": [_P, C.c_float],
+    "pivx_cnn_train": [_P, _P, C.POINTER(CnnTrainParams), _P, C.POINTER(CnnTrainReport)],
+    "pivx_run_pair_learned": [_P, _P, C.POINTER(FilterParams), C.POINTER(FilterParams), C.c_float],
+    "pivx_get_learned_stats": [_P, C.POINTER(C.c_int), C.POINTER(C.c_int)],
     "pivx_synth_flow": [C.POINTER(SynthParams), _P, _P, C.c_int, _P, _P],
 }
 _RESTYPE = {"pivx_last_error": C.c_char_p, "pivx_host_alloc": C.c_void_p, "pivx_host_free": None,
@@ -295,6 +320,21 @@ class Xcorr2dMulti:
 
     def run_pair(self, f1: FilterParams, f2: FilterParams):
         self._c(lib().pivx_run_pair(self.h, C.byref(f1), C.byref(f2)))
+
+    def run_pair_learned(self, net: "Cnn", f1: FilterParams, f2: FilterParams, max_outlier_fraction: float = 0.05):
+        """The pair sequence with the grid-1 passes replaced by the network
+        (plain grid 1 as the fallback above the outlier fraction)."""
+        self._c(lib().pivx_run_pair_learned(self.h, net.h, C.byref(f1), C.byref(f2), max_outlier_fraction))
+
+    def learned_stats(self):
+        """(outliers of the network field, 1 if the plain grid 1 ran instead)."""
+        o, f = C.c_int(), C.c_int()
+        self._c(lib().pivx_get_learned_stats(self.h, C.byref(o), C.byref(f)))
+        return o.value, f.value
+
+    def synth_pair_device(self, sp: SynthParams):
+        """Render a synthetic pair on the GPU into this handle's frames."""
+        self._c(lib().pivx_synth_pair_device(self.h, C.byref(sp)))
 
     def set_graph_mode(self, on=True):
         self._c(lib().pivx_set_graph_mode(self.h, 1 if on else 0))
@@ -479,6 +519,69 @@ def synth_flow(sp: SynthParams, x, y):
     v = np.empty_like(x)
     _check(lib().pivx_synth_flow(C.byref(sp), _ptr(x), _ptr(y), x.size, _ptr(u), _ptr(v)))
     return u, v
+
+
+class Cnn:
+    """The learned grid-1 predictor (pivx_cnn_*): parameters live on the GPU."""
+
+    def __init__(self, device_id: int = 0, seed: int = None):
+        h = C.c_void_p()
+        _check(lib().pivx_cnn_create(device_id, C.byref(h)))
+        self.h = h
+        self.n = lib().pivx_cnn_param_count()
+        if seed is not None:
+            _check(lib().pivx_cnn_init_params(self.h, seed))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().pivx_cnn_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def params(self) -> np.ndarray:
+        p = np.empty(self.n, np.float32)
+        _check(lib().pivx_cnn_get_params(self.h, _ptr(p), self.n))
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        _check(lib().pivx_cnn_set_params(self.h, _ptr(p), p.size))
+
+    def forward(self, x0):
+        x0 = np.ascontiguousarray(x0, np.float32)
+        b, h, w, _ = x0.shape
+        out = np.empty((b, 2, (h // 8) * (w // 8)), np.float32)
+        _check(lib().pivx_cnn_forward(self.h, _ptr(x0), b, h, w, _ptr(out)))
+        return out
+
+    def gradient(self, x0, labels):
+        x0 = np.ascontiguousarray(x0, np.float32)
+        labels = np.ascontiguousarray(labels, np.float32)
+        b, h, w, _ = x0.shape
+        loss = C.c_float()
+        g = np.empty(self.n, np.float32)
+        _check(lib().pivx_cnn_gradient(self.h, _ptr(x0), _ptr(labels), b, h, w, C.byref(loss), _ptr(g)))
+        return loss.value, g
+
+    def adam_step(self, lr):
+        _check(lib().pivx_cnn_adam_step(self.h, lr))
+
+    def train(self, x: "Xcorr2dMulti", tp: CnnTrainParams):
+        losses = np.zeros(tp.epochs, np.float32)
+        rep = CnnTrainReport()
+        x._c(lib().pivx_cnn_train(x.h, self.h, C.byref(tp), _ptr(losses), C.byref(rep)))
+        return losses, rep
 
 
 class Results:

@@ -1,0 +1,607 @@
+// Kernels of the learned grid-1 predictor (see cnn.h): implicit-GEMM 3x3
+// convolutions on v_mfma_f32_16x16x32_bf16, forward and backward, plus the
+// loss, the Adam step, weight packing, input downsampling and the outlier
+// gate of the fallback.
+//
+// MFMA operand layout (16x16x32, D = A * B, A 16x32, B 32x16): lane l feeds
+// row l % 16 of A and column l % 16 of B with the eight K values
+// 8 * (l / 16) + 0..7, and receives D rows 4 * (l / 16) + 0..3 of column
+// l % 16.  Any fixed assignment of K values to (lane group, slot) gives the
+// same product as long as A and B use the same one.
+#include "cnn.h"
+
+#include <cmath>
+
+namespace pivx {
+
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((__vector_size__(16)));
+typedef float f32x4 __attribute__((__vector_size__(16)));
+typedef unsigned int u32x4 __attribute__((__vector_size__(16)));
+
+__device__ __forceinline__ float bf2f(uint16_t b) { return __builtin_bit_cast(float, (unsigned int)b << 16); }
+__device__ __forceinline__ uint16_t f2bf(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
+
+__device__ __forceinline__ bf16x8 ld8(const uint16_t* p) {
+  return __builtin_bit_cast(bf16x8, *reinterpret_cast<const u32x4*>(p));
+}
+__device__ __forceinline__ bf16x8 zero8() { return __builtin_bit_cast(bf16x8, u32x4{0u, 0u, 0u, 0u}); }
+__device__ __forceinline__ bf16x8 pack8(const uint16_t* v) {
+  u32x4 u;
+  u[0] = v[0] | ((unsigned int)v[1] << 16);
+  u[1] = v[2] | ((unsigned int)v[3] << 16);
+  u[2] = v[4] | ((unsigned int)v[5] << 16);
+  u[3] = v[6] | ((unsigned int)v[7] << 16);
+  return __builtin_bit_cast(bf16x8, u);
+}
+__device__ __forceinline__ f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+// gradient through the leaky ReLU, from the stored (bf16) layer output
+__device__ __forceinline__ uint16_t act_grad(uint16_t g, uint16_t post) {
+  return bf2f(post) > 0.f ? g : f2bf(bf2f(g) * kCnnLeak);
+}
+
+// Eight K values of the input patch of output pixel (oy, ox): k = tap * CIN + ci.
+template <int CIN, int S>
+__device__ __forceinline__ bf16x8 patch8(const uint16_t* in, int hin, int win, int oy, int ox, int k) {
+  if constexpr (CIN % 8 == 0) {
+    if (k >= 9 * CIN) return zero8();
+    const int tap = k / CIN, ci = k - tap * CIN;
+    const int ky = tap / 3, kx = tap - 3 * ky;
+    const int iy = oy * S + ky - 1, ix = ox * S + kx - 1;
+    if ((unsigned)iy >= (unsigned)hin || (unsigned)ix >= (unsigned)win) return zero8();
+    return ld8(in + ((long long)iy * win + ix) * CIN + ci);
+  } else {
+    uint16_t v[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int kk = k + i;
+      uint16_t x = 0;
+      if (kk < 9 * CIN) {
+        const int tap = kk / CIN, ci = kk - tap * CIN;
+        const int ky = tap / 3, kx = tap - 3 * ky;
+        const int iy = oy * S + ky - 1, ix = ox * S + kx - 1;
+        if ((unsigned)iy < (unsigned)hin && (unsigned)ix < (unsigned)win) x = in[((long long)iy * win + ix) * CIN + ci];
+      }
+      v[i] = x;
+    }
+    return pack8(v);
+  }
+}
+
+// Eight K values (k = tap * COUT + co) of the output-gradient patch that
+// reaches input pixel (iy, ix): output pixel o with o * S + tap - 1 = i.
+template <int COUT, int S, bool ACT>
+__device__ __forceinline__ bf16x8 gpatch8(const uint16_t* gout, const uint16_t* post, int hout, int wout, int iy,
+                                          int ix, int k) {
+  uint16_t v[8];
+  if constexpr (COUT % 8 == 0) {
+    if (k >= 9 * COUT) return zero8();
+    const int tap = k / COUT, co = k - tap * COUT;
+    const int ky = tap / 3, kx = tap - 3 * ky;
+    int oy = iy + 1 - ky, ox = ix + 1 - kx;
+    if (oy < 0 || ox < 0) return zero8();
+    if constexpr (S == 2) {
+      if ((oy & 1) || (ox & 1)) return zero8();
+      oy >>= 1;
+      ox >>= 1;
+    }
+    if (oy >= hout || ox >= wout) return zero8();
+    const long long o = ((long long)oy * wout + ox) * COUT + co;
+    if constexpr (!ACT) {
+      return ld8(gout + o);
+    } else {
+      const u32x4 g = *reinterpret_cast<const u32x4*>(gout + o);
+      const u32x4 p = *reinterpret_cast<const u32x4*>(post + o);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        v[2 * i] = act_grad((uint16_t)(g[i] & 0xffffu), (uint16_t)(p[i] & 0xffffu));
+        v[2 * i + 1] = act_grad((uint16_t)(g[i] >> 16), (uint16_t)(p[i] >> 16));
+      }
+      return pack8(v);
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int kk = k + i;
+      uint16_t x = 0;
+      if (kk < 9 * COUT) {
+        const int tap = kk / COUT, co = kk - tap * COUT;
+        const int ky = tap / 3, kx = tap - 3 * ky;
+        int oy = iy + 1 - ky, ox = ix + 1 - kx;
+        bool ok = oy >= 0 && ox >= 0;
+        if (S == 2) {
+          ok = ok && !(oy & 1) && !(ox & 1);
+          oy >>= 1;
+          ox >>= 1;
+        }
+        ok = ok && oy < hout && ox < wout;
+        if (ok) {
+          const long long o = ((long long)oy * wout + ox) * COUT + co;
+          x = ACT ? act_grad(gout[o], post[o]) : gout[o];
+        }
+      }
+      v[i] = x;
+    }
+    return pack8(v);
+  }
+}
+
+// ---------------------------------------------------------------- forward
+// One wave: MT output-channel tiles (of 16) of PT * 16 consecutive output
+// pixels of one row; the COUT / 16 / MT channel blocks of a pixel group go to
+// consecutive waves.  D[co][pixel] = W[co][k] * X[k][pixel], four waves per
+// workgroup.  The K loop is unrolled by three so that the loads of later
+// steps are in flight while the matrix cores work.
+template <int CIN, int COUT, int S, bool ACT, bool FINAL, int PT, int MT>
+__global__ void __launch_bounds__(256) k_conv_fwd(ConvFwdArgs a) {
+  constexpr int KP = cnn_kpad(CIN);
+  constexpr int NCB = cnn_pad16(COUT) / 16 / MT;  // channel blocks per pixel group
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int col = lane & 15, kg = lane >> 4;
+  const int gpr = a.wout / (16 * PT);
+  const long long gw = (long long)blockIdx.x * 4 + wave;
+  const long long g = gw / NCB;
+  const int cb = (int)(gw - g * NCB);
+  if (g >= (long long)a.batch * a.hout * gpr) return;  // whole wave; no barriers in this kernel
+  const int gx = (int)(g % gpr);
+  const long long r = g / gpr;
+  const int oy = (int)(r % a.hout), b = (int)(r / a.hout);
+  const int ox0 = gx * 16 * PT;
+  const uint16_t* wbase = a.w + (long long)cb * MT * 16 * KP;
+  const uint16_t* in = a.in + (long long)(a.batch_index ? a.batch_index[b] : b) * a.in_bstride;
+  f32x4 acc[MT][PT];
+#pragma unroll
+  for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+    for (int pt = 0; pt < PT; ++pt) acc[mt][pt] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 3
+  for (int k0 = 0; k0 < KP; k0 += 32) {
+    const int k = k0 + 8 * kg;
+    bf16x8 wa[MT], xb[PT];
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt) wa[mt] = ld8(wbase + (long long)(mt * 16 + col) * KP + k);
+#pragma unroll
+    for (int pt = 0; pt < PT; ++pt) xb[pt] = patch8<CIN, S>(in, a.hin, a.win, oy, ox0 + pt * 16 + col, k);
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+      for (int pt = 0; pt < PT; ++pt) acc[mt][pt] = mfma(wa[mt], xb[pt], acc[mt][pt]);
+  }
+#pragma unroll
+  for (int pt = 0; pt < PT; ++pt) {
+    const int ox = ox0 + pt * 16 + col;
+    if constexpr (FINAL) {
+      if (kg == 0) {
+        const long long o = (long long)b * a.out_bstride + (long long)oy * a.wout + ox;
+        a.out_x[o] = acc[0][pt][0] + a.bias[0];
+        a.out_y[o] = acc[0][pt][1] + a.bias[1];
+      }
+    } else {
+#pragma unroll
+      for (int mt = 0; mt < MT; ++mt) {
+        const int co0 = (cb * MT + mt) * 16 + 4 * kg;
+        if (co0 >= COUT) continue;
+        uint16_t h[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          float v = acc[mt][pt][j] + a.bias[co0 + j];
+          if (ACT) v = v > 0.f ? v : v * kCnnLeak;
+          h[j] = f2bf(v);
+        }
+        uint2 u;
+        u.x = h[0] | ((unsigned int)h[1] << 16);
+        u.y = h[2] | ((unsigned int)h[3] << 16);
+        *reinterpret_cast<uint2*>(a.out + (((long long)b * a.hout + oy) * a.wout + ox) * COUT + co0) = u;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------- backward, data
+// D[ci][input pixel] = WT[ci][k] * G[k][input pixel], k = tap * COUT + co.
+template <int CIN, int COUT, int S, bool ACT, int PT>
+__global__ void __launch_bounds__(256) k_conv_bwd_data(ConvBwdDataArgs a) {
+  constexpr int KP = cnn_kpad(COUT);
+  constexpr int MT = cnn_pad16(CIN) / 16;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int col = lane & 15, kg = lane >> 4;
+  const int gpr = a.win / (16 * PT);
+  const long long g = (long long)blockIdx.x * 4 + wave;
+  if (g >= (long long)a.batch * a.hin * gpr) return;
+  const int gx = (int)(g % gpr);
+  const long long r = g / gpr;
+  const int iy = (int)(r % a.hin), b = (int)(r / a.hin);
+  const int ix0 = gx * 16 * PT;
+  const long long so = (long long)b * a.hout * a.wout * COUT;
+  const uint16_t* gout = a.gout + so;
+  const uint16_t* post = ACT ? a.post + so : nullptr;
+  f32x4 acc[MT][PT];
+#pragma unroll
+  for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+    for (int pt = 0; pt < PT; ++pt) acc[mt][pt] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int k0 = 0; k0 < KP; k0 += 32) {
+    const int k = k0 + 8 * kg;
+    bf16x8 wa[MT], xb[PT];
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt) wa[mt] = ld8(a.wt + (long long)(mt * 16 + col) * KP + k);
+#pragma unroll
+    for (int pt = 0; pt < PT; ++pt)
+      xb[pt] = gpatch8<COUT, S, ACT>(gout, post, a.hout, a.wout, iy, ix0 + pt * 16 + col, k);
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+      for (int pt = 0; pt < PT; ++pt) acc[mt][pt] = mfma(wa[mt], xb[pt], acc[mt][pt]);
+  }
+#pragma unroll
+  for (int pt = 0; pt < PT; ++pt) {
+    const int ix = ix0 + pt * 16 + col;
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt) {
+      const int ci0 = mt * 16 + 4 * kg;
+      if (ci0 >= CIN) continue;
+      uint2 u;
+      u.x = f2bf(acc[mt][pt][0]) | ((unsigned int)f2bf(acc[mt][pt][1]) << 16);
+      u.y = f2bf(acc[mt][pt][2]) | ((unsigned int)f2bf(acc[mt][pt][3]) << 16);
+      *reinterpret_cast<uint2*>(a.gin + (((long long)b * a.hin + iy) * a.win + ix) * CIN + ci0) = u;
+    }
+  }
+}
+
+// ---------------------------------------------------------------- backward, weights
+// D[co][n] = Gpre[co][o] * X[o][n], n = tap * CIN + ci, K = the output pixels
+// o of one chunk (linear over [batch][hout][wout]).  Both operands are
+// gathered one pixel at a time (a lane's eight K values are eight pixels).
+// Grid: (chunks, groups of NTG column tiles), one wave per workgroup.
+template <int CIN, int COUT, int S, bool ACT, int NTG>
+__global__ void __launch_bounds__(64) k_conv_bwd_w(ConvBwdWArgs a) {
+  constexpr int KPI = cnn_kpad(CIN);
+  constexpr int MT = cnn_pad16(COUT) / 16;
+  const int lane = threadIdx.x, col = lane & 15, kg = lane >> 4;
+  const long long npx = (long long)a.batch * a.hout * a.wout;
+  const long long ob = (long long)blockIdx.x * a.chunk_px;
+  const long long oe = ob + a.chunk_px < npx ? ob + a.chunk_px : npx;
+  const int nt0 = blockIdx.y * NTG;
+  f32x4 acc[MT][NTG];
+#pragma unroll
+  for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+    for (int nt = 0; nt < NTG; ++nt) acc[mt][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const int hw = a.hout * a.wout;
+  for (long long o0 = ob; o0 < oe; o0 += 32) {
+    const long long ok = o0 + 8 * kg;
+    bf16x8 ga[MT], xb[NTG];
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt) {
+      const int co = mt * 16 + col;
+      uint16_t v[8];
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        const long long o = ok + i;
+        uint16_t x = 0;
+        if (o < oe && co < COUT) {
+          const long long q = o * COUT + co;
+          x = ACT ? act_grad(a.gout[q], a.post[q]) : a.gout[q];
+        }
+        v[i] = x;
+      }
+      ga[mt] = pack8(v);
+    }
+#pragma unroll
+    for (int nt = 0; nt < NTG; ++nt) {
+      const int n = (nt0 + nt) * 16 + col;
+      uint16_t v[8];
+      const int tap = n / CIN, ci = n - (n / CIN) * CIN;
+      const int ky = tap / 3, kx = tap - 3 * (tap / 3);
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        const long long o = ok + i;
+        uint16_t x = 0;
+        if (o < oe && n < 9 * CIN) {
+          const int b = (int)(o / hw);
+          const int rem = (int)(o - (long long)b * hw);
+          const int oy = rem / a.wout, ox = rem - (rem / a.wout) * a.wout;
+          const int iy = oy * S + ky - 1, ix = ox * S + kx - 1;
+          if ((unsigned)iy < (unsigned)a.hin && (unsigned)ix < (unsigned)a.win) {
+            const uint16_t* in = a.in + (long long)(a.batch_index ? a.batch_index[b] : b) * a.in_bstride;
+            x = in[((long long)iy * a.win + ix) * CIN + ci];
+          }
+        }
+        v[i] = x;
+      }
+      xb[nt] = pack8(v);
+    }
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+      for (int nt = 0; nt < NTG; ++nt) acc[mt][nt] = mfma(ga[mt], xb[nt], acc[mt][nt]);
+  }
+  float* part = a.partial + (long long)blockIdx.x * (MT * 16) * KPI;
+#pragma unroll
+  for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+    for (int nt = 0; nt < NTG; ++nt)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int co = mt * 16 + 4 * kg + j, n = (nt0 + nt) * 16 + col;
+        part[(long long)co * KPI + n] = acc[mt][nt][j];
+      }
+  // bias partial sums of the chunk (the same bf16 gradient values), one lane per channel
+  if (blockIdx.y == 0 && lane < COUT) {
+    float s = 0.f;
+    for (long long o = ob; o < oe; ++o) {
+      const long long q = o * COUT + lane;
+      s += bf2f(ACT ? act_grad(a.gout[q], a.post[q]) : a.gout[q]);
+    }
+    a.partial_b[(long long)blockIdx.x * COUT + lane] = s;
+  }
+}
+
+__global__ void k_wgrad_reduce(const float* partial, const float* partial_b, int nchunks, int cin, int cout, int kpi,
+                               int mrows, float* grad) {
+  const int nw = cout * 9 * cin;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < nw) {
+    const int co = t / (9 * cin), n = t - co * 9 * cin;  // n = tap * cin + ci
+    float s = 0.f;
+    for (int c = 0; c < nchunks; ++c) s += partial[((long long)c * mrows + co) * kpi + n];
+    grad[t] = s;
+  } else if (t < nw + cout) {
+    const int co = t - nw;
+    float s = 0.f;
+    for (int c = 0; c < nchunks; ++c) s += partial_b[(long long)c * cout + co];
+    grad[t] = s;
+  }
+}
+
+// ---------------------------------------------------------------- loss, optimizer
+constexpr int kLossBlock = 256;
+
+__global__ void __launch_bounds__(kLossBlock) k_mse(const float* out, long long out_bstride, const float* labels,
+                                                    const int* batch_index, int batch, int hw, uint16_t* gout,
+                                                    float* loss_part) {
+  __shared__ float red[kLossBlock];
+  const long long n = (long long)batch * 2 * hw;
+  const long long t = (long long)blockIdx.x * kLossBlock + threadIdx.x;
+  const float inv = 2.0f / (float)n;
+  float s = 0.f;
+  if (t < n) {
+    const int b = (int)(t / (2 * hw));
+    const int rem = (int)(t - (long long)b * 2 * hw);
+    const int c = rem / hw, p = rem - c * hw;
+    const float o = out[(long long)b * out_bstride + (long long)c * hw + p];
+    const long long li = (long long)(batch_index ? batch_index[b] : b) * 2 * hw + (long long)c * hw + p;
+    const float d = o - labels[li];
+    gout[((long long)b * hw + p) * 2 + c] = f2bf(d * inv);
+    s = d * d;
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = kLossBlock / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) loss_part[blockIdx.x] = red[0];
+}
+
+__global__ void __launch_bounds__(256) k_loss_final(const float* part, int n, double count, float* loss) {
+  __shared__ double red[256];
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) s += part[i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *loss = (float)(red[0] / count);
+}
+
+__global__ void k_adam(float* p, const float* g, float* m, float* v, int n, float lr, float b1, float b2, float eps,
+                       float c1, float c2) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const float gt = g[t];
+  const float mt = b1 * m[t] + (1.f - b1) * gt;
+  const float vt = b2 * v[t] + (1.f - b2) * gt * gt;
+  m[t] = mt;
+  v[t] = vt;
+  p[t] -= lr * (mt / c1) / (sqrtf(vt / c2) + eps);
+}
+
+__global__ void k_pack_layer(const float* w, int cin, int cout, uint16_t* wf, uint16_t* wt) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= cout * 9 * cin) return;
+  const int co = t / (9 * cin), r = t - co * 9 * cin, tap = r / cin, ci = r - tap * cin;
+  const uint16_t h = f2bf(w[t]);
+  wf[(long long)co * cnn_kpad(cin) + tap * cin + ci] = h;
+  wt[(long long)ci * cnn_kpad(cout) + tap * cout + co] = h;
+}
+
+// ---------------------------------------------------------------- input, gate
+__global__ void k_downsample(const uint8_t* f1, long long p1, const uint8_t* f2, long long p2, int w, int h,
+                             uint16_t* out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= w * h) return;
+  const int y = t / w, x = t - (t / w) * w;
+  unsigned int s1 = 0, s2 = 0;
+#pragma unroll
+  for (int r = 0; r < kCnnDown; ++r) {
+    const uchar4 a = *reinterpret_cast<const uchar4*>(f1 + (long long)(y * kCnnDown + r) * p1 + x * kCnnDown);
+    s1 += a.x + a.y + a.z + a.w;
+    const uint8_t* q = f2 + (long long)(y * kCnnDown + r) * p2 + x * kCnnDown;
+    s2 += q[0] + q[1] + q[2] + q[3];
+  }
+  const float inv = 1.0f / (kCnnDown * kCnnDown);
+  const uint16_t a = f2bf(((float)s1 * inv - kCnnMu) * kCnnInvSigma);
+  const uint16_t b = f2bf(((float)s2 * inv - kCnnMu) * kCnnInvSigma);
+  reinterpret_cast<unsigned int*>(out)[t] = a | ((unsigned int)b << 16);
+}
+
+__global__ void __launch_bounds__(256) k_gate(const int8_t* accepted, int n, int max_outliers, int* gate, int* count) {
+  __shared__ int red[256];
+  int c = 0;
+  for (int i = threadIdx.x; i < n; i += 256) c += accepted[i] < 0;
+  red[threadIdx.x] = c;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    *count = red[0];
+    *gate = red[0] > max_outliers ? 1 : 0;
+  }
+}
+
+__global__ void k_fill_u8(uint8_t* p, int n, uint8_t v) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) p[t] = v;
+}
+
+inline int blocks(long long n, int b) { return (int)((n + b - 1) / b); }
+
+template <int CIN, int COUT, int S, bool ACT, bool FINAL>
+hipError_t fwd(const ConvFwdArgs& a, hipStream_t s) {
+  constexpr int PT = 1;
+  constexpr int MT = 1;  // one 16-channel tile per wave: the small layers need the waves
+  constexpr int NCB = cnn_pad16(COUT) / 16 / MT;
+  if (a.wout % (16 * PT) != 0) return hipErrorInvalidValue;
+  const long long waves = (long long)a.batch * a.hout * (a.wout / (16 * PT)) * NCB;
+  hipLaunchKernelGGL((k_conv_fwd<CIN, COUT, S, ACT, FINAL, PT, MT>), dim3(blocks(waves, 4)), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+template <int CIN, int COUT, int S, bool ACT>
+hipError_t bwd_data(const ConvBwdDataArgs& a, hipStream_t s) {
+  constexpr int PT = 1;
+  if (a.win % (16 * PT) != 0) return hipErrorInvalidValue;
+  const long long groups = (long long)a.batch * a.hin * (a.win / (16 * PT));
+  hipLaunchKernelGGL((k_conv_bwd_data<CIN, COUT, S, ACT, PT>), dim3(blocks(groups, 4)), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+template <int CIN, int COUT, int S, bool ACT>
+hipError_t bwd_w(const ConvBwdWArgs& a, int nchunks, hipStream_t s) {
+  constexpr int NTG = 2;
+  constexpr int ntiles = cnn_kpad(CIN) / 16;
+  static_assert(ntiles % NTG == 0, "column tiles per group");
+  if (a.chunk_px % 32 != 0) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((k_conv_bwd_w<CIN, COUT, S, ACT, NTG>), dim3(nchunks, ntiles / NTG), dim3(64), 0, s, a);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+int cnn_param_count() {
+  int n = 0;
+  for (const CnnLayer& L : kCnnArch) n += L.cout * 9 * L.cin + L.cout;
+  return n;
+}
+
+int cnn_weight_offset(int layer) {
+  int n = 0;
+  for (int l = 0; l < layer; ++l) n += kCnnArch[l].cout * 9 * kCnnArch[l].cin + kCnnArch[l].cout;
+  return n;
+}
+
+hipError_t launch_conv_fwd(int layer, const ConvFwdArgs& a, hipStream_t s) {
+  switch (layer) {
+    case 0: return fwd<2, 32, 2, true, false>(a, s);
+    case 1: return fwd<32, 64, 2, true, false>(a, s);
+    case 2: return fwd<64, 64, 2, true, false>(a, s);
+    case 3: return fwd<64, 64, 1, true, false>(a, s);
+    case 4: return fwd<64, 64, 1, true, false>(a, s);
+    case 5: return fwd<64, 2, 1, false, true>(a, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_conv_bwd_data(int layer, const ConvBwdDataArgs& a, hipStream_t s) {
+  switch (layer) {
+    case 1: return bwd_data<32, 64, 2, true>(a, s);
+    case 2: return bwd_data<64, 64, 2, true>(a, s);
+    case 3: return bwd_data<64, 64, 1, true>(a, s);
+    case 4: return bwd_data<64, 64, 1, true>(a, s);
+    case 5: return bwd_data<64, 2, 1, false>(a, s);
+    default: return hipErrorInvalidValue;  // no gradient is needed for the network input
+  }
+}
+
+hipError_t launch_conv_bwd_w(int layer, const ConvBwdWArgs& a, int nchunks, hipStream_t s) {
+  switch (layer) {
+    case 0: return bwd_w<2, 32, 2, true>(a, nchunks, s);
+    case 1: return bwd_w<32, 64, 2, true>(a, nchunks, s);
+    case 2: return bwd_w<64, 64, 2, true>(a, nchunks, s);
+    case 3: return bwd_w<64, 64, 1, true>(a, nchunks, s);
+    case 4: return bwd_w<64, 64, 1, true>(a, nchunks, s);
+    case 5: return bwd_w<64, 2, 1, false>(a, nchunks, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_wgrad_reduce(int layer, const float* partial, const float* partial_b, int nchunks, float* grad,
+                               hipStream_t s) {
+  const CnnLayer& L = kCnnArch[layer];
+  const int n = L.cout * 9 * L.cin + L.cout;
+  hipLaunchKernelGGL(k_wgrad_reduce, dim3(blocks(n, 256)), dim3(256), 0, s, partial, partial_b, nchunks, L.cin,
+                     L.cout, cnn_kpad(L.cin), cnn_pad16(L.cout), grad);
+  return hipGetLastError();
+}
+
+int cnn_loss_blocks(int batch, int hw) { return blocks((long long)batch * 2 * hw, kLossBlock); }
+
+hipError_t launch_mse(const float* out, long long out_bstride, const float* labels, const int* batch_index, int batch,
+                      int hw, uint16_t* gout, float* loss_part, hipStream_t s) {
+  hipLaunchKernelGGL(k_mse, dim3(cnn_loss_blocks(batch, hw)), dim3(kLossBlock), 0, s, out, out_bstride, labels,
+                     batch_index, batch, hw, gout, loss_part);
+  return hipGetLastError();
+}
+
+hipError_t launch_loss_final(const float* loss_part, int n, double count, float* loss, hipStream_t s) {
+  hipLaunchKernelGGL(k_loss_final, dim3(1), dim3(256), 0, s, loss_part, n, count, loss);
+  return hipGetLastError();
+}
+
+hipError_t launch_adam(float* master, const float* grad, float* m, float* v, int n, float lr, float b1, float b2,
+                       float eps, long long t, hipStream_t s) {
+  const float c1 = 1.f - (float)std::pow((double)b1, (double)t);
+  const float c2 = 1.f - (float)std::pow((double)b2, (double)t);
+  hipLaunchKernelGGL(k_adam, dim3(blocks(n, 256)), dim3(256), 0, s, master, grad, m, v, n, lr, b1, b2, eps, c1, c2);
+  return hipGetLastError();
+}
+
+hipError_t launch_pack(const float* master, uint16_t* const* wf, uint16_t* const* wt, hipStream_t s) {
+  for (int l = 0; l < kCnnLayers; ++l) {
+    const CnnLayer& L = kCnnArch[l];
+    const int n = L.cout * 9 * L.cin;
+    hipLaunchKernelGGL(k_pack_layer, dim3(blocks(n, 256)), dim3(256), 0, s, master + cnn_weight_offset(l), L.cin,
+                       L.cout, wf[l], wt[l]);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+hipError_t launch_downsample(const uint8_t* f1, long long p1, const uint8_t* f2, long long p2, int nx, int ny,
+                             uint16_t* out, hipStream_t s) {
+  const int w = nx / kCnnDown, h = ny / kCnnDown;
+  if ((p1 & 3) != 0 || (reinterpret_cast<uintptr_t>(f1) & 3) != 0) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_downsample, dim3(blocks((long long)w * h, 256)), dim3(256), 0, s, f1, p1, f2, p2, w, h, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_gate(const int8_t* accepted, int n, int max_outliers, int* gate, int* count, hipStream_t s) {
+  hipLaunchKernelGGL(k_gate, dim3(1), dim3(256), 0, s, accepted, n, max_outliers, gate, count);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_u8(uint8_t* p, int n, uint8_t v, hipStream_t s) {
+  hipLaunchKernelGGL(k_fill_u8, dim3(blocks(n, 256)), dim3(256), 0, s, p, n, v);
+  return hipGetLastError();
+}
+
+}  // namespace pivx

@@ -14,9 +14,12 @@
 //   * everything is ordered on one HIP stream owned by the handle.
 #include "../../include/piv_xcorr2d.h"
 #include "kernels.h"
+#include "cnn.h"
 #include "hostio.h"
+#include "synth.h"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -84,6 +87,19 @@ struct pivx_ctx {
   int* sweeps = nullptr;                   // [2]
   int* nholes = nullptr;                   // hole-list counter of the fill kernels
   unsigned int* drops = nullptr;           // holes counted beyond the list capacity since the last check
+  const int* gate = nullptr;               // while set, grid kernels run only if *gate != 0 (learned fallback)
+  // learned pipeline (pivx_run_pair_learned): fallback flag and outlier count
+  // of the network field, network input and hidden activations (batch 1)
+  int* lgate = nullptr;
+  int* lcount = nullptr;
+  uint16_t* cx0 = nullptr;
+  uint16_t* cact[5] = {};
+  hipGraph_t lgraph = nullptr;
+  hipGraphExec_t lgexec = nullptr;
+  const pivx_cnn* lnet = nullptr;
+  pivx_filter_params lf[2] = {};
+  float lfrac = -1.f;
+  unsigned int *sacc1 = nullptr, *sacc2 = nullptr;  // accumulators of the device renderer
   long long drops_total = 0;               // host total of *drops, reported by pivx_get_stats
   int *list_a = nullptr, *list_b = nullptr;
   int list_cap = 0;                        // entries of list_a / list_b
@@ -280,6 +296,7 @@ DefArgs def_args(pivx_ctx* c, int gi) {
   d.spx = (float)g.spacing_x;
   d.spy = (float)g.spacing_y;
   d.oob = c->counters + gi;
+  d.gate = c->gate;
   return d;
 }
 
@@ -347,6 +364,7 @@ int run_xcorr(pivx_ctx* c, int gi, bool deformed, bool zero_pred = false, bool f
   a.pk_val = G.pkv;
   a.pk_idx = G.pki;
   a.ncand = G.ncand;
+  a.gate = c->gate;
   tic(c);
   hipError_t e = G.fast ? launch_xcorr(a, g.ws_x, g.ws_y, nullptr, 0, nullptr, nullptr, c->stream)
                         : launch_xcorr(a, g.ws_x, g.ws_y, G.gscratch, G.gscratch_windows, G.twx, G.twy, c->stream);
@@ -369,6 +387,7 @@ struct FilterJob {
   float* med_scratch;                                            // generic median lists (wx*wy > 64) or null
   int8_t* accepted;                                              // accepted candidate per vector or null
   hipStream_t stream;
+  const int* gate;                                               // launches run only if *gate != 0 (null: always)
 };
 
 // MedianFilter_Smoothing_p1/p2 (cu:1451-1545, 1766-1863) writing into (ox, oy, ov).
@@ -403,14 +422,15 @@ int enqueue_filter(pivx_ctx* c, int gi, const FilterJob& J, int npass_median, in
   // earlier sequence (an interrupted pair, a caller's partial sequence) can
   // leave a stale count behind.
   if (npass_median <= 0) {
-    rc = check_launch(c, launch_prep(J.cx, J.cy, J.cv, K, t0x, t0y, ov, nt, fill ? J.nholes : nullptr, stream),
+    rc = check_launch(c, launch_prep(J.cx, J.cy, J.cv, K, t0x, t0y, ov, nt, fill ? J.nholes : nullptr, stream, J.gate),
                       "median_filter_data_prep");
     if (!rc && mx)
-      rc = check_launch(c, launch_prep(J.cx, J.cy, J.cv, K, mx, my, mv, nt, nullptr, stream),
+      rc = check_launch(c, launch_prep(J.cx, J.cy, J.cv, K, mx, my, mv, nt, nullptr, stream, J.gate),
                         "median_filter_data_prep");
     if (rc) return rc;
   } else {
-    if (listed && npass_median == 1 && (rc = check_launch(c, launch_zero_ints(J.nholes, 1, stream), "hole counter")))
+    if (listed && npass_median == 1 &&
+        (rc = check_launch(c, launch_zero_ints(J.nholes, 1, stream, J.gate), "hole counter")))
       return rc;
     // intermediate sets: I1 = (ux, uy, uv), I2 = whichever of (ox, oy) / (sx, sy) is not T0, with sv
     float* i2x = smooth_odd ? ox : J.sx;
@@ -456,6 +476,7 @@ int enqueue_filter(pivx_ctx* c, int gi, const FilterJob& J, int npass_median, in
       m.nholes = (back == 0 && listed) ? J.nholes : nullptr;
       m.hole_cap = J.list_cap;
       m.reset_count = (listed && p == 0 && back > 0) ? J.nholes : nullptr;
+      m.gate = J.gate;
       rc = check_launch(c, launch_median(m, stream), "median_filtering");
       if (rc) return rc;
       srcx = tx;
@@ -480,6 +501,7 @@ int enqueue_filter(pivx_ctx* c, int gi, const FilterJob& J, int npass_median, in
     f.nholes = J.nholes;
     f.list_cap = J.list_cap;
     f.drops = J.drops;
+    f.gate = J.gate;
     f.sweeps = J.sweeps;
     f.prelisted = listed ? 1 : 0;
     tic(c);
@@ -494,6 +516,7 @@ int enqueue_filter(pivx_ctx* c, int gi, const FilterJob& J, int npass_median, in
       float* nx_ = (cx == ox) ? J.sx : ox;
       float* ny_ = (cy == oy) ? J.sy : oy;
       SmoothArgs sa{};
+      sa.gate = J.gate;
       sa.sx = cx;
       sa.sy = cy;
       sa.dx = nx_;
@@ -553,13 +576,14 @@ int run_filter(pivx_ctx* c, int gi, int npass_median, int min_nneigh, float inse
   J.med_scratch = G.med_scratch;
   J.accepted = G.accepted;
   J.stream = c->stream;
+  J.gate = c->gate;
   return enqueue_filter(c, gi, J, npass_median, min_nneigh, insert_thr, npass_smooth, iffill, ox, oy, ov, ext_x, ext_y,
                         ext_done, mx, my, mv);
 }
 
 int extend(pivx_ctx* c, int gi) {
   GridBuf& G = c->grid[gi];
-  return check_launch(c, launch_extend(G.fx, G.fy, G.ex, G.ey, G.g.nx_vec, G.g.ny_vec, kExBorder, c->stream),
+  return check_launch(c, launch_extend(G.fx, G.fy, G.ex, G.ey, G.g.nx_vec, G.g.ny_vec, kExBorder, c->stream, c->gate),
                       "extend_filtered_vecfield");
 }
 
@@ -759,6 +783,7 @@ int pivx_init(const pivx_init_params* p, int device_id, pivx_ctx** out) {
 }
 
 static void drop_graph(pivx_ctx* c);
+static void drop_learned_graph(pivx_ctx* c);
 
 int pivx_freemem(pivx_ctx* c) {
   if (!c) return PIVX_OK;
@@ -766,6 +791,7 @@ int pivx_freemem(pivx_ctx* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (void* q : c->allocs) (void)hipFree(q);
   drop_graph(c);
+  drop_learned_graph(c);
   for (hipEvent_t e : c->tl.pool) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -1038,23 +1064,32 @@ int pivx_upsampling_vector_field(pivx_ctx* c) {
 // The per-pair sequence of main.cpp:288-375.  The extension of a field is
 // written by the kernel that produces the field (last smoothing pass,
 // upsampling) whenever it can be; extend() runs only where it could not.
-static int run_pair_enqueue(pivx_ctx* c, const pivx_filter_params* f1, const pivx_filter_params* f2) {
-  struct InPair {
-    pivx_ctx* c;
-    explicit InPair(pivx_ctx* x) : c(x) { c->in_pair = true; }
-    ~InPair() { c->in_pair = false; }
-  } in_pair(c);
+struct InPair {
+  pivx_ctx* c;
+  explicit InPair(pivx_ctx* x) : c(x) { c->in_pair = true; }
+  ~InPair() { c->in_pair = false; }
+};
+
+// Grid-1 passes of main.cpp:289-303; *ext_ok tells whether the last
+// smoothing pass wrote the extended field.
+static int enqueue_grid1(pivx_ctx* c, const pivx_filter_params* f1, bool* ext_ok) {
   int rc;
-  bool ext_ok = false;
   GridBuf& G1 = c->grid[0];
-  GridBuf& G2 = c->grid[1];
+  *ext_ok = false;
   for (int j = 0; j < c->p.npass1; ++j) {
-    if (j > 0 && !ext_ok && (rc = extend(c, 0))) return rc;
+    if (j > 0 && !*ext_ok && (rc = extend(c, 0))) return rc;
     if ((rc = pivx_calc_cc_map_p1(c, j))) return rc;
     if ((rc = run_filter(c, 0, f1->npass_median, f1->min_nneigh, f1->insert_thr, f1->npass_smooth, 1, G1.fx, G1.fy,
-                         G1.fv, G1.ex, G1.ey, &ext_ok)))
+                         G1.fv, G1.ex, G1.ey, ext_ok)))
       return rc;
   }
+  return PIVX_OK;
+}
+
+// Upsampling and grid-2 passes of main.cpp:305-375.
+static int enqueue_grid2(pivx_ctx* c, const pivx_filter_params* f2, bool ext_ok) {
+  int rc;
+  GridBuf& G2 = c->grid[1];
   if (c->p.npass2 > 0) {
     if (!ext_ok && (rc = extend(c, 0))) return rc;
     if ((rc = upsample(c, true))) return rc;
@@ -1074,6 +1109,14 @@ static int run_pair_enqueue(pivx_ctx* c, const pivx_filter_params* f1, const piv
     }
   }
   return PIVX_OK;
+}
+
+static int run_pair_enqueue(pivx_ctx* c, const pivx_filter_params* f1, const pivx_filter_params* f2) {
+  InPair in_pair(c);
+  bool ext_ok = false;
+  const int rc = enqueue_grid1(c, f1, &ext_ok);
+  if (rc) return rc;
+  return enqueue_grid2(c, f2, ext_ok);
 }
 
 static void drop_graph(pivx_ctx* c) {
@@ -1126,14 +1169,20 @@ int pivx_run_pair(pivx_ctx* c, const pivx_filter_params* f1, const pivx_filter_p
 int pivx_set_graph_mode(pivx_ctx* c, int on) {
   if (!c) return fail(nullptr, PIVX_ERR_PARAM, "null handle");
   c->use_graph = on != 0;
-  if (!c->use_graph) drop_graph(c);
+  if (!c->use_graph) {
+    drop_graph(c);
+    drop_learned_graph(c);
+  }
   return PIVX_OK;
 }
 
 int pivx_set_fused_deform(pivx_ctx* c, int mode) {
   if (!c) return fail(nullptr, PIVX_ERR_PARAM, "null handle");
   if (mode < 0 || mode > 2) return fail(c, PIVX_ERR_PARAM, "fused deformation mode must be 0, 1 or 2 (got %d)", mode);
-  if (mode != c->fuse_mode) drop_graph(c);  // a captured pair holds the old choice
+  if (mode != c->fuse_mode) {  // a captured pair holds the old choice
+    drop_graph(c);
+    drop_learned_graph(c);
+  }
   c->fuse_mode = mode;
   return PIVX_OK;
 }
@@ -1475,6 +1524,674 @@ int pivx_write_vec_data_general(pivx_ctx* c, const char* path, int selection, in
     return fail(c, PIVX_ERR_PARAM, "selection must be 0..4");
   }
   if (rc) return fail(c, PIVX_ERR_IO, "Error occurs while the vector file is being written: %s", path);
+  return PIVX_OK;
+}
+
+}  // extern "C"
+
+// ====================================================================== learned grid-1 predictor
+struct pivx_cnn {
+  int device = 0;
+  int np = 0;
+  hipStream_t stream = nullptr;
+  float *master = nullptr, *m = nullptr, *v = nullptr, *grad = nullptr;
+  uint16_t* wf[kCnnLayers] = {};
+  uint16_t* wt[kCnnLayers] = {};
+  long long step = 0;
+  // batch workspace (training and the host test entry points)
+  int wb = 0, wh = 0, ww = 0;
+  uint16_t* act[kCnnLayers - 1] = {};
+  float* out = nullptr;
+  uint16_t* g[2] = {};
+  float *partial = nullptr, *partial_b = nullptr, *loss_part = nullptr, *loss = nullptr;
+  int* dindex = nullptr;
+  uint16_t* xin = nullptr;    // host-entry input batch
+  float* lab = nullptr;       // host-entry labels
+  std::vector<void*> allocs, ws;
+};
+
+namespace {
+
+constexpr int kCnnChunks = 256;  // weight-gradient partial sums per layer
+
+int cnn_fail(const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  set_global_error(buf);
+  return PIVX_ERR_PARAM;
+}
+
+template <class T>
+int nalloc(std::vector<void*>& list, T** p, size_t n) {
+  void* q = nullptr;
+  const size_t bytes = std::max<size_t>(n * sizeof(T), 256);
+  const hipError_t e = hipMalloc(&q, bytes);
+  if (e != hipSuccess) {
+    set_global_error((std::string("hipMalloc: ") + hipGetErrorString(e)).c_str());
+    return PIVX_ERR_HIP;
+  }
+  list.push_back(q);
+  *p = static_cast<T*>(q);
+  return PIVX_OK;
+}
+
+uint16_t host_bf16(float f) {  // round to nearest even
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  if ((u & 0x7f800000u) != 0x7f800000u) u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+// spatial size of layer l's input for a network input h x w
+void layer_dims(int h, int w, int l, int& hin, int& win, int& hout, int& wout) {
+  hin = h;
+  win = w;
+  for (int i = 0; i < l; ++i) {
+    hin /= kCnnArch[i].stride;
+    win /= kCnnArch[i].stride;
+  }
+  hout = hin / kCnnArch[l].stride;
+  wout = win / kCnnArch[l].stride;
+}
+
+bool cnn_shape_ok(int h, int w) { return h >= 8 && w >= 128 && h % 8 == 0 && w % 128 == 0; }
+
+// Forward pass over a batch: x0 sample b at x0 + (bidx ? bidx[b] : b) * xstride;
+// hidden activations into act[0..4]; output planes at out_x / out_y + b * ostride.
+int cnn_forward_enqueue(const pivx_cnn* net, const uint16_t* x0, long long xstride, const int* bidx, int B, int H,
+                        int W, uint16_t* const* act, float* out_x, float* out_y, long long ostride, hipStream_t s) {
+  for (int l = 0; l < kCnnLayers; ++l) {
+    const CnnLayer& L = kCnnArch[l];
+    ConvFwdArgs a{};
+    layer_dims(H, W, l, a.hin, a.win, a.hout, a.wout);
+    a.in = l == 0 ? x0 : act[l - 1];
+    a.in_bstride = l == 0 ? xstride : (long long)a.hin * a.win * L.cin;
+    a.batch_index = l == 0 ? bidx : nullptr;
+    a.batch = B;
+    a.w = net->wf[l];
+    a.bias = net->master + cnn_weight_offset(l) + L.cout * 9 * L.cin;
+    if (l < kCnnLayers - 1) {
+      a.out = act[l];
+    } else {
+      a.out_x = out_x;
+      a.out_y = out_y;
+      a.out_bstride = ostride;
+    }
+    const hipError_t e = launch_conv_fwd(l, a, s);
+    if (e != hipSuccess) return cnn_fail("network layer %d: %s", l, hipGetErrorString(e));
+  }
+  return PIVX_OK;
+}
+
+// Loss gradient in net->g[0] -> parameter gradient in net->grad.
+int cnn_backward_enqueue(pivx_cnn* net, const uint16_t* x0, long long xstride, const int* bidx, int B, int H, int W,
+                         hipStream_t s) {
+  int cur = 0;
+  for (int l = kCnnLayers - 1; l >= 0; --l) {
+    const CnnLayer& L = kCnnArch[l];
+    int hin, win, hout, wout;
+    layer_dims(H, W, l, hin, win, hout, wout);
+    const long long npx = (long long)B * hout * wout;
+    const int chunk = (int)(((npx + kCnnChunks - 1) / kCnnChunks + 31) / 32 * 32);
+    const int nchunks = (int)((npx + chunk - 1) / chunk);
+    ConvBwdWArgs w{};
+    w.in = l == 0 ? x0 : net->act[l - 1];
+    w.in_bstride = l == 0 ? xstride : (long long)hin * win * L.cin;
+    w.batch_index = l == 0 ? bidx : nullptr;
+    w.gout = net->g[cur];
+    w.post = L.act ? net->act[l] : nullptr;
+    w.partial = net->partial;
+    w.partial_b = net->partial_b;
+    w.batch = B;
+    w.hin = hin;
+    w.win = win;
+    w.hout = hout;
+    w.wout = wout;
+    w.chunk_px = chunk;
+    hipError_t e = launch_conv_bwd_w(l, w, nchunks, s);
+    if (e == hipSuccess) e = launch_wgrad_reduce(l, net->partial, net->partial_b, nchunks, net->grad + cnn_weight_offset(l), s);
+    if (e == hipSuccess && l > 0) {
+      ConvBwdDataArgs d{};
+      d.gout = net->g[cur];
+      d.post = L.act ? net->act[l] : nullptr;
+      d.wt = net->wt[l];
+      d.gin = net->g[cur ^ 1];
+      d.batch = B;
+      d.hin = hin;
+      d.win = win;
+      d.hout = hout;
+      d.wout = wout;
+      e = launch_conv_bwd_data(l, d, s);
+      cur ^= 1;
+    }
+    if (e != hipSuccess) return cnn_fail("network backward, layer %d: %s", l, hipGetErrorString(e));
+  }
+  return PIVX_OK;
+}
+
+// Batch workspace for B samples of H x W input (reallocated when larger).
+int cnn_workspace(pivx_cnn* net, int B, int H, int W) {
+  if (net->wb >= B && net->wh == H && net->ww == W) return PIVX_OK;
+  (void)hipStreamSynchronize(net->stream);
+  for (void* q : net->ws) (void)hipFree(q);
+  net->ws.clear();
+  net->wb = net->wh = net->ww = 0;
+  int rc = 0;
+  size_t gmax = (size_t)B * (H / 8) * (W / 8) * 2;
+  size_t pmax = 0;
+  for (int l = 0; l < kCnnLayers; ++l) {
+    int hin, win, hout, wout;
+    layer_dims(H, W, l, hin, win, hout, wout);
+    const size_t n = (size_t)B * hout * wout * kCnnArch[l].cout;
+    if (l < kCnnLayers - 1) rc |= nalloc(net->ws, &net->act[l], n);
+    gmax = std::max(gmax, n);
+    gmax = std::max(gmax, (size_t)B * hin * win * kCnnArch[l].cin);
+    pmax = std::max(pmax, (size_t)cnn_pad16(kCnnArch[l].cout) * cnn_kpad(kCnnArch[l].cin));
+  }
+  rc |= nalloc(net->ws, &net->out, (size_t)B * 2 * (H / 8) * (W / 8));
+  rc |= nalloc(net->ws, &net->g[0], gmax);
+  rc |= nalloc(net->ws, &net->g[1], gmax);
+  rc |= nalloc(net->ws, &net->partial, pmax * kCnnChunks);
+  rc |= nalloc(net->ws, &net->partial_b, (size_t)64 * kCnnChunks);
+  rc |= nalloc(net->ws, &net->loss_part, (size_t)cnn_loss_blocks(B, (H / 8) * (W / 8)));
+  rc |= nalloc(net->ws, &net->loss, 1);
+  rc |= nalloc(net->ws, &net->xin, (size_t)B * H * W * 2);
+  rc |= nalloc(net->ws, &net->lab, (size_t)B * 2 * (H / 8) * (W / 8));
+  if (rc) return PIVX_ERR_HIP;
+  net->wb = B;
+  net->wh = H;
+  net->ww = W;
+  return PIVX_OK;
+}
+
+int cnn_pack(pivx_cnn* net) {
+  const hipError_t e = launch_pack(net->master, net->wf, net->wt, net->stream);
+  if (e != hipSuccess) return cnn_fail("weight packing: %s", hipGetErrorString(e));
+  return PIVX_OK;
+}
+
+// One training step on the batch in net's workspace: forward, loss, backward, Adam.
+int cnn_step_enqueue(pivx_cnn* net, const uint16_t* x0, long long xstride, const int* bidx, const float* labels,
+                     int B, int H, int W, float lr, float* loss_slot, hipStream_t s) {
+  const int hw = (H / 8) * (W / 8);
+  int rc = cnn_forward_enqueue(net, x0, xstride, bidx, B, H, W, net->act, net->out, net->out + hw, 2LL * hw, s);
+  if (rc) return rc;
+  hipError_t e = launch_mse(net->out, 2LL * hw, labels, bidx, B, hw, net->g[0], net->loss_part, s);
+  if (e == hipSuccess)
+    e = launch_loss_final(net->loss_part, cnn_loss_blocks(B, hw), (double)B * 2 * hw, loss_slot, s);
+  if (e != hipSuccess) return cnn_fail("loss: %s", hipGetErrorString(e));
+  if ((rc = cnn_backward_enqueue(net, x0, xstride, bidx, B, H, W, s))) return rc;
+  if (lr > 0.f) {
+    net->step += 1;
+    e = launch_adam(net->master, net->grad, net->m, net->v, net->np, lr, 0.9f, 0.999f, 1e-8f, net->step, s);
+    if (e == hipSuccess) e = launch_pack(net->master, net->wf, net->wt, s);
+    if (e != hipSuccess) return cnn_fail("Adam step: %s", hipGetErrorString(e));
+  }
+  return PIVX_OK;
+}
+
+// The learned pipeline needs grid 1 at spacing 32x32 with one vector per
+// 32x32 block of the frame (the network's output cells).
+int learned_shape_check(const pivx_ctx* c) {
+  const pivx_geometry& g = c->grid[0].g;
+  if (g.spacing_x != kCnnDown * kCnnStride || g.spacing_y != kCnnDown * kCnnStride)
+    return cnn_fail("the learned grid-1 predictor needs grid-1 spacing %dx%d (got %dx%d)", kCnnDown * kCnnStride,
+                    kCnnDown * kCnnStride, g.spacing_x, g.spacing_y);
+  if (c->p.nx % 512 != 0 || c->p.ny % 32 != 0)
+    return cnn_fail("the learned grid-1 predictor needs nx a multiple of 512 and ny of 32 (got %dx%d)", c->p.nx,
+                    c->p.ny);
+  if (g.nx_vec != c->p.nx / 32 || g.ny_vec != c->p.ny / 32)
+    return cnn_fail("grid-1 vector count %dx%d does not match the network output %dx%d", g.nx_vec, g.ny_vec,
+                    c->p.nx / 32, c->p.ny / 32);
+  return PIVX_OK;
+}
+
+// per-handle buffers of the learned pipeline (allocated once, outside any capture)
+int learned_alloc(pivx_ctx* c) {
+  if (c->cx0) return PIVX_OK;
+  const int H = c->p.ny / kCnnDown, W = c->p.nx / kCnnDown;
+  int rc = 0;
+  rc |= dalloc(c, &c->lgate, 1);
+  rc |= dalloc(c, &c->lcount, 1);
+  for (int l = 0; l < kCnnLayers - 1; ++l) {
+    int hin, win, hout, wout;
+    layer_dims(H, W, l, hin, win, hout, wout);
+    rc |= dalloc(c, &c->cact[l], (size_t)hout * wout * kCnnArch[l].cout);
+  }
+  if (rc) return PIVX_ERR_HIP;
+  rc = dalloc(c, &c->cx0, (size_t)H * W * 2);
+  if (rc) return rc;
+  HIPCHK(c, hipMemsetAsync(c->lgate, 0, sizeof(int), c->stream));
+  HIPCHK(c, hipMemsetAsync(c->lcount, 0, sizeof(int), c->stream));
+  return PIVX_OK;
+}
+
+int learned_enqueue(pivx_ctx* c, const pivx_cnn* net, const pivx_filter_params* f1, const pivx_filter_params* f2,
+                    float max_frac) {
+  InPair in_pair(c);
+  GridBuf& G1 = c->grid[0];
+  const pivx_geometry& g1 = G1.g;
+  const int nt1 = g1.nt_vec;
+  const int H = c->p.ny / kCnnDown, W = c->p.nx / kCnnDown;
+  int rc;
+  // network field -> grid-1 field
+  hipError_t e = launch_downsample(c->f1, c->p.nx, c->f2, c->pitch2, c->p.nx, c->p.ny, c->cx0, c->stream);
+  if (e != hipSuccess) return fail(c, PIVX_ERR_HIP, "network input: %s", hipGetErrorString(e));
+  if ((rc = cnn_forward_enqueue(net, c->cx0, 0, nullptr, 1, H, W, c->cact, G1.fx, G1.fy, 0, c->stream))) {
+    c->err = global_error();
+    return rc;
+  }
+  if ((rc = check_launch(c, launch_fill_u8(G1.fv, nt1, 1, c->stream), "network field flags"))) return rc;
+  // universal outlier test of the network field: the grid-1 median test with
+  // the network vector as the only candidate; rejected vectors are counted
+  MedArgs m{};
+  m.sx = G1.fx;
+  m.sy = G1.fy;
+  m.sv = G1.fv;
+  m.sstride = 1;
+  m.cx = G1.fx;
+  m.cy = G1.fy;
+  m.cv = G1.fv;
+  m.K = 1;
+  m.dx = G1.ux;
+  m.dy = G1.uy;
+  m.dv = G1.uv;
+  m.nx_vec = g1.nx_vec;
+  m.ny_vec = g1.ny_vec;
+  m.wx = c->p.wx1_median;
+  m.wy = c->p.wy1_median;
+  m.hfwx = (m.wx - 1) / 2;
+  m.hfwy = (m.wy - 1) / 2;
+  m.min_nneigh = f1->min_nneigh;
+  m.thr = f1->insert_thr;
+  m.eps = kEpsilon;
+  m.accepted = G1.accepted;
+  m.scratch = G1.med_scratch;
+  if ((rc = check_launch(c, launch_median(m, c->stream), "network field median test"))) return rc;
+  const int max_out = (int)std::floor((double)max_frac * nt1);
+  if ((rc = check_launch(c, launch_gate(G1.accepted, nt1, max_out, c->lgate, c->lcount, c->stream), "fallback gate")))
+    return rc;
+  if ((rc = extend(c, 0))) return rc;  // the network field's extension (the fallback rewrites it)
+  // plain grid 1, run only when the gate says so
+  bool ext_ok = false;
+  c->gate = c->lgate;
+  rc = enqueue_grid1(c, f1, &ext_ok);
+  c->gate = nullptr;
+  if (rc) return rc;
+  // The extension now belongs to whichever field grid 1 holds: the network's
+  // (written above) or the fallback's (written by its last smoothing pass, or
+  // rewritten by the ungated extension in enqueue_grid2 when there is none).
+  return enqueue_grid2(c, f2, ext_ok);
+}
+
+}  // namespace
+
+static void drop_learned_graph(pivx_ctx* c) {
+  if (c->lgexec) (void)hipGraphExecDestroy(c->lgexec);
+  if (c->lgraph) (void)hipGraphDestroy(c->lgraph);
+  c->lgexec = nullptr;
+  c->lgraph = nullptr;
+}
+
+extern "C" {
+
+int pivx_synth_pair_device(pivx_ctx* c, const pivx_synth_params* sp) {
+  if (!c || !sp) return fail(c, PIVX_ERR_PARAM, "null argument");
+  if (sp->nx != c->p.nx || sp->ny != c->p.ny)
+    return fail(c, PIVX_ERR_PARAM, "synthetic frame size %dx%d differs from the handle's %dx%d", sp->nx, sp->ny,
+                c->p.nx, c->p.ny);
+  if (sp->flow < 0 || sp->flow > 2 || !(sp->sigma > 0.f)) return fail(c, PIVX_ERR_PARAM, "bad synthetic parameters");
+  HIPCHK(c, hipSetDevice(c->device));
+  const size_t npix = (size_t)c->p.nx * c->p.ny;
+  if (!c->sacc1) {
+    if (dalloc(c, &c->sacc1, npix) || dalloc(c, &c->sacc2, npix)) return PIVX_ERR_HIP;
+    HIPCHK(c, hipMemsetAsync(c->sacc1, 0, npix * 4, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->sacc2, 0, npix * 4, c->stream));
+  }
+  SynthRender r{};
+  r.nx = sp->nx;
+  r.ny = sp->ny;
+  r.seed = sp->seed;
+  r.nparticles = (long long)std::llround((double)sp->density * (sp->nx + 32.0) * (sp->ny + 32.0));
+  r.sigma = sp->sigma;
+  r.imin = sp->imin;
+  r.imax = sp->imax;
+  r.background = sp->background;
+  r.noise_sigma = sp->noise_sigma;
+  synth_flow_of(sp, &r.flow);
+  const int rc = check_launch(c, launch_synth(r, c->sacc1, c->sacc2, c->f1, c->p.nx, c->f2, c->pitch2, c->stream),
+                              "synthetic pair");
+  if (rc) return rc;
+  c->have_frames = true;
+  return PIVX_OK;
+}
+
+int pivx_cnn_param_count(void) { return cnn_param_count(); }
+
+int pivx_cnn_destroy(pivx_cnn* net) {
+  if (!net) return PIVX_OK;
+  (void)hipSetDevice(net->device);
+  if (net->stream) (void)hipStreamSynchronize(net->stream);
+  for (void* q : net->ws) (void)hipFree(q);
+  for (void* q : net->allocs) (void)hipFree(q);
+  if (net->stream) (void)hipStreamDestroy(net->stream);
+  delete net;
+  return PIVX_OK;
+}
+
+int pivx_cnn_create(int device_id, pivx_cnn** out) {
+  if (!out) return cnn_fail("null argument");
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || device_id < 0 || device_id >= ndev)
+    return cnn_fail("device id %d out of range", device_id);
+  if (hipSetDevice(device_id) != hipSuccess) return cnn_fail("hipSetDevice failed");
+  pivx_cnn* net = new pivx_cnn();
+  net->device = device_id;
+  net->np = cnn_param_count();
+  if (hipStreamCreateWithFlags(&net->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete net;
+    return cnn_fail("hipStreamCreate failed");
+  }
+  int rc = 0;
+  rc |= nalloc(net->allocs, &net->master, net->np);
+  rc |= nalloc(net->allocs, &net->m, net->np);
+  rc |= nalloc(net->allocs, &net->v, net->np);
+  rc |= nalloc(net->allocs, &net->grad, net->np);
+  for (int l = 0; l < kCnnLayers && !rc; ++l) {
+    const CnnLayer& L = kCnnArch[l];
+    const size_t nf = (size_t)cnn_pad16(L.cout) * cnn_kpad(L.cin), nt = (size_t)cnn_pad16(L.cin) * cnn_kpad(L.cout);
+    rc |= nalloc(net->allocs, &net->wf[l], nf);
+    rc |= nalloc(net->allocs, &net->wt[l], nt);
+    if (!rc && (hipMemsetAsync(net->wf[l], 0, nf * 2, net->stream) != hipSuccess ||
+                hipMemsetAsync(net->wt[l], 0, nt * 2, net->stream) != hipSuccess))
+      rc = PIVX_ERR_HIP;
+  }
+  if (!rc && (hipMemsetAsync(net->master, 0, (size_t)net->np * 4, net->stream) != hipSuccess ||
+              hipMemsetAsync(net->m, 0, (size_t)net->np * 4, net->stream) != hipSuccess ||
+              hipMemsetAsync(net->v, 0, (size_t)net->np * 4, net->stream) != hipSuccess ||
+              hipMemsetAsync(net->grad, 0, (size_t)net->np * 4, net->stream) != hipSuccess ||
+              hipStreamSynchronize(net->stream) != hipSuccess))
+    rc = PIVX_ERR_HIP;
+  if (rc) {
+    pivx_cnn_destroy(net);
+    return rc;
+  }
+  *out = net;
+  return PIVX_OK;
+}
+
+int pivx_cnn_set_params(pivx_cnn* net, const float* params, int n) {
+  if (!net || !params || n != net->np) return cnn_fail("parameter count must be %d", net ? net->np : 0);
+  (void)hipSetDevice(net->device);
+  if (hipMemcpyAsync(net->master, params, (size_t)n * 4, hipMemcpyHostToDevice, net->stream) != hipSuccess ||
+      hipMemsetAsync(net->m, 0, (size_t)n * 4, net->stream) != hipSuccess ||
+      hipMemsetAsync(net->v, 0, (size_t)n * 4, net->stream) != hipSuccess)
+    return cnn_fail("parameter upload failed");
+  net->step = 0;
+  int rc = cnn_pack(net);
+  if (rc) return rc;
+  if (hipStreamSynchronize(net->stream) != hipSuccess) return cnn_fail("parameter upload failed");
+  return PIVX_OK;
+}
+
+int pivx_cnn_get_params(pivx_cnn* net, float* params, int n) {
+  if (!net || !params || n != net->np) return cnn_fail("parameter count must be %d", net ? net->np : 0);
+  (void)hipSetDevice(net->device);
+  if (hipMemcpyAsync(params, net->master, (size_t)n * 4, hipMemcpyDeviceToHost, net->stream) != hipSuccess ||
+      hipStreamSynchronize(net->stream) != hipSuccess)
+    return cnn_fail("parameter download failed");
+  return PIVX_OK;
+}
+
+int pivx_cnn_init_params(pivx_cnn* net, unsigned long long seed) {
+  if (!net) return cnn_fail("null network");
+  std::vector<float> p((size_t)net->np, 0.f);
+  unsigned long long st = seed;
+  auto next = [&st]() {
+    st += 0x9e3779b97f4a7c15ULL;
+    unsigned long long z = st;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+    return (double)((z ^ (z >> 31)) >> 11) * (1.0 / 9007199254740992.0);
+  };
+  for (int l = 0; l < kCnnLayers; ++l) {
+    const CnnLayer& L = kCnnArch[l];
+    const double sd = std::sqrt(2.0 / (9.0 * L.cin)) * (l == kCnnLayers - 1 ? 0.1 : 1.0);
+    const int o = cnn_weight_offset(l);
+    for (int i = 0; i < L.cout * 9 * L.cin; ++i) {
+      double u1 = next(), u2 = next();
+      if (u1 < 1e-300) u1 = 1e-300;
+      p[o + i] = (float)(sd * std::sqrt(-2.0 * std::log(u1)) * std::cos(2.0 * M_PI * u2));
+    }
+  }
+  return pivx_cnn_set_params(net, p.data(), net->np);
+}
+
+static int cnn_upload_batch(pivx_cnn* net, const float* x0, const float* labels, int B, int H, int W) {
+  int rc = cnn_workspace(net, B, H, W);
+  if (rc) return rc;
+  std::vector<uint16_t> xb((size_t)B * H * W * 2);
+  for (size_t i = 0; i < xb.size(); ++i) xb[i] = host_bf16(x0[i]);
+  if (hipMemcpy(net->xin, xb.data(), xb.size() * 2, hipMemcpyHostToDevice) != hipSuccess)
+    return cnn_fail("input upload failed");
+  if (labels && hipMemcpy(net->lab, labels, (size_t)B * 2 * (H / 8) * (W / 8) * 4, hipMemcpyHostToDevice) != hipSuccess)
+    return cnn_fail("label upload failed");
+  return PIVX_OK;
+}
+
+int pivx_cnn_forward(pivx_cnn* net, const float* x0, int batch, int h, int w, float* out) {
+  if (!net || !x0 || !out || batch < 1) return cnn_fail("bad argument");
+  if (!cnn_shape_ok(h, w)) return cnn_fail("network input %dx%d: h must be a multiple of 8, w of 128", h, w);
+  (void)hipSetDevice(net->device);
+  int rc = cnn_upload_batch(net, x0, nullptr, batch, h, w);
+  if (rc) return rc;
+  const int hw = (h / 8) * (w / 8);
+  if ((rc = cnn_forward_enqueue(net, net->xin, (long long)h * w * 2, nullptr, batch, h, w, net->act, net->out,
+                                net->out + hw, 2LL * hw, net->stream)))
+    return rc;
+  if (hipMemcpyAsync(out, net->out, (size_t)batch * 2 * hw * 4, hipMemcpyDeviceToHost, net->stream) != hipSuccess ||
+      hipStreamSynchronize(net->stream) != hipSuccess)
+    return cnn_fail("network forward failed");
+  return PIVX_OK;
+}
+
+int pivx_cnn_gradient(pivx_cnn* net, const float* x0, const float* labels, int batch, int h, int w, float* loss,
+                      float* grad) {
+  if (!net || !x0 || !labels || batch < 1) return cnn_fail("bad argument");
+  if (!cnn_shape_ok(h, w)) return cnn_fail("network input %dx%d: h must be a multiple of 8, w of 128", h, w);
+  (void)hipSetDevice(net->device);
+  int rc = cnn_upload_batch(net, x0, labels, batch, h, w);
+  if (rc) return rc;
+  if ((rc = cnn_step_enqueue(net, net->xin, (long long)h * w * 2, nullptr, net->lab, batch, h, w, 0.f, net->loss,
+                             net->stream)))
+    return rc;
+  if (loss && hipMemcpyAsync(loss, net->loss, 4, hipMemcpyDeviceToHost, net->stream) != hipSuccess)
+    return cnn_fail("loss download failed");
+  if (grad && hipMemcpyAsync(grad, net->grad, (size_t)net->np * 4, hipMemcpyDeviceToHost, net->stream) != hipSuccess)
+    return cnn_fail("gradient download failed");
+  if (hipStreamSynchronize(net->stream) != hipSuccess) return cnn_fail("network gradient failed");
+  return PIVX_OK;
+}
+
+int pivx_cnn_adam_step(pivx_cnn* net, float lr) {
+  if (!net || !(lr > 0.f)) return cnn_fail("bad argument");
+  (void)hipSetDevice(net->device);
+  net->step += 1;
+  hipError_t e = launch_adam(net->master, net->grad, net->m, net->v, net->np, lr, 0.9f, 0.999f, 1e-8f, net->step,
+                             net->stream);
+  if (e == hipSuccess) e = launch_pack(net->master, net->wf, net->wt, net->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(net->stream);
+  if (e != hipSuccess) return cnn_fail("Adam step: %s", hipGetErrorString(e));
+  return PIVX_OK;
+}
+
+int pivx_cnn_train(pivx_ctx* c, pivx_cnn* net, const pivx_cnn_train_params* tp, float* epoch_loss,
+                   pivx_cnn_train_report* rep) {
+  if (!c || !net || !tp) return fail(c, PIVX_ERR_PARAM, "null argument");
+  if (net->device != c->device) return fail(c, PIVX_ERR_PARAM, "network and handle are on different devices");
+  if (tp->npairs < 1 || tp->epochs < 1 || tp->batch < 1 || tp->batch > tp->npairs || !(tp->lr > 0.f))
+    return fail(c, PIVX_ERR_PARAM, "bad training parameters");
+  int rc = learned_shape_check(c);
+  if (rc) {
+    c->err = global_error();
+    return rc;
+  }
+  if (c->p.npass1 < 1) return fail(c, PIVX_ERR_PARAM, "labels need at least one grid-1 pass");
+  HIPCHK(c, hipSetDevice(c->device));
+  const int N = tp->npairs, B = tp->batch;
+  const int H = c->p.ny / kCnnDown, W = c->p.nx / kCnnDown;
+  const int hw = (H / 8) * (W / 8);
+  const long long xstride = (long long)H * W * 2;
+  if ((rc = cnn_workspace(net, B, H, W))) return rc;
+  // the dataset: network inputs and plain grid-1 labels of every pair
+  std::vector<void*> data;
+  struct Release {
+    std::vector<void*>& d;
+    ~Release() {
+      for (void* q : d) (void)hipFree(q);
+    }
+  } release{data};
+  uint16_t* X = nullptr;
+  float* Y = nullptr;
+  int* perm = nullptr;
+  float* losses = nullptr;
+  const int steps = N / B;
+  if (nalloc(data, &X, (size_t)N * xstride) || nalloc(data, &Y, (size_t)N * 2 * hw) ||
+      nalloc(data, &perm, (size_t)steps * B) || nalloc(data, &losses, (size_t)tp->epochs * steps))
+    return fail(c, PIVX_ERR_HIP, "training data allocation failed");
+  const auto t0 = std::chrono::steady_clock::now();
+  GridBuf& G1 = c->grid[0];
+  c->gate = nullptr;
+  for (int i = 0; i < N; ++i) {
+    pivx_synth_params sp = tp->synth;
+    sp.seed = tp->synth.seed + (unsigned long long)i;
+    if ((rc = pivx_synth_pair_device(c, &sp))) return rc;
+    bool ext_ok = false;
+    {
+      InPair in_pair(c);
+      if ((rc = enqueue_grid1(c, &tp->grid1, &ext_ok))) return rc;
+    }
+    HIPCHK(c, hipMemcpyAsync(Y + (size_t)i * 2 * hw, G1.fx, (size_t)hw * 4, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(Y + (size_t)i * 2 * hw + hw, G1.fy, (size_t)hw * 4, hipMemcpyDeviceToDevice, c->stream));
+    const hipError_t e = launch_downsample(c->f1, c->p.nx, c->f2, c->pitch2, c->p.nx, c->p.ny, X + (size_t)i * xstride,
+                                           c->stream);
+    if (e != hipSuccess) return fail(c, PIVX_ERR_HIP, "network input: %s", hipGetErrorString(e));
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  const auto t1 = std::chrono::steady_clock::now();
+  // minibatch Adam over seeded shuffles, cosine-decayed learning rate
+  std::vector<int> order((size_t)N);
+  unsigned long long st = tp->shuffle_seed;
+  auto rnd = [&st]() {
+    st += 0x9e3779b97f4a7c15ULL;
+    unsigned long long z = st;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+    return z ^ (z >> 31);
+  };
+  const long long total = (long long)tp->epochs * steps;
+  long long t = 0;
+  for (int ep = 0; ep < tp->epochs; ++ep) {
+    for (int i = 0; i < N; ++i) order[i] = i;
+    for (int i = N - 1; i > 0; --i) std::swap(order[i], order[(size_t)(rnd() % (unsigned long long)(i + 1))]);
+    // the previous epoch's steps read perm: wait before overwriting it
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(perm, order.data(), (size_t)steps * B * sizeof(int), hipMemcpyHostToDevice));
+    for (int s = 0; s < steps; ++s, ++t) {
+      const float lr = (float)(0.5 * tp->lr * (1.0 + std::cos(M_PI * (double)t / (double)total)));
+      if ((rc = cnn_step_enqueue(net, X, xstride, perm + (size_t)s * B, Y, B, H, W, lr,
+                                 losses + (size_t)ep * steps + s, c->stream))) {
+        c->err = global_error();
+        return rc;
+      }
+    }
+  }
+  std::vector<float> lh((size_t)total);
+  HIPCHK(c, hipMemcpyAsync(lh.data(), losses, lh.size() * 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  const auto t2 = std::chrono::steady_clock::now();
+  std::vector<float> el((size_t)tp->epochs, 0.f);
+  for (int ep = 0; ep < tp->epochs; ++ep) {
+    double s = 0.0;
+    for (int k = 0; k < steps; ++k) s += lh[(size_t)ep * steps + k];
+    el[ep] = (float)(s / steps);
+    if (epoch_loss) epoch_loss[ep] = el[ep];
+  }
+  if (rep) {
+    rep->gen_seconds = std::chrono::duration<double>(t1 - t0).count();
+    rep->train_seconds = std::chrono::duration<double>(t2 - t1).count();
+    rep->steps = (int)total;
+    rep->first_epoch_loss = el.front();
+    rep->last_epoch_loss = el.back();
+  }
+  return PIVX_OK;
+}
+
+static bool same_learned(const pivx_ctx* c, const pivx_cnn* net, const pivx_filter_params* f1,
+                         const pivx_filter_params* f2, float frac) {
+  return c->lnet == net && same_filter(c->lf[0], *f1) && same_filter(c->lf[1], *f2) && c->lfrac == frac;
+}
+
+int pivx_run_pair_learned(pivx_ctx* c, const pivx_cnn* net, const pivx_filter_params* f1,
+                          const pivx_filter_params* f2, float max_outlier_fraction) {
+  if (!c || !net || !f1 || !f2) return fail(c, PIVX_ERR_PARAM, "null argument");
+  if (!c->have_frames) return fail(c, PIVX_ERR_STATE, "no frames: call pivx_prep_img_data first");
+  if (net->device != c->device) return fail(c, PIVX_ERR_PARAM, "network and handle are on different devices");
+  if (!(max_outlier_fraction >= 0.f)) return fail(c, PIVX_ERR_PARAM, "max_outlier_fraction must be >= 0");
+  int rc = learned_shape_check(c);
+  if (rc) {
+    c->err = global_error();
+    return rc;
+  }
+  HIPCHK(c, hipSetDevice(c->device));
+  if ((rc = learned_alloc(c))) return rc;
+  // the network's parameters were written on its own stream
+  HIPCHK(c, hipStreamSynchronize(net->stream));
+  if (!c->use_graph) return learned_enqueue(c, net, f1, f2, max_outlier_fraction);
+  if (!c->lgexec || !same_learned(c, net, f1, f2, max_outlier_fraction)) {
+    drop_learned_graph(c);
+    const bool timing = c->timing;
+    c->timing = false;
+    HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    rc = learned_enqueue(c, net, f1, f2, max_outlier_fraction);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(c->stream, &g);
+    c->timing = timing;
+    if (rc) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    if (e != hipSuccess) return fail(c, PIVX_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    c->lgraph = g;
+    e = hipGraphInstantiate(&c->lgexec, c->lgraph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+      drop_learned_graph(c);
+      return fail(c, PIVX_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+    }
+    c->lnet = net;
+    c->lf[0] = *f1;
+    c->lf[1] = *f2;
+    c->lfrac = max_outlier_fraction;
+  }
+  tic(c);
+  HIPCHK(c, hipGraphLaunch(c->lgexec, c->stream));
+  toc(c, 0, 5);
+  return PIVX_OK;
+}
+
+int pivx_get_learned_stats(pivx_ctx* c, int* outliers, int* fallback) {
+  if (!c) return fail(nullptr, PIVX_ERR_PARAM, "null handle");
+  if (!c->lgate) return fail(c, PIVX_ERR_STATE, "no learned pair has run on this handle");
+  int v[2] = {0, 0};
+  HIPCHK(c, hipMemcpyAsync(&v[0], c->lcount, 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(&v[1], c->lgate, 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (outliers) *outliers = v[0];
+  if (fallback) *fallback = v[1];
   return PIVX_OK;
 }
 

@@ -2,6 +2,7 @@
 // output naming, 8-bit TIFF strip I/O, Tecplot writer and the seeded
 // synthetic particle-pair generator.  Plain C++ (no GPU calls).
 #include "hostio.h"
+#include "synth.h"
 #include "../../include/piv_xcorr2d.h"
 
 #include <algorithm>
@@ -188,30 +189,8 @@ struct Rng {  // xoshiro256** seeded by splitmix64
   }
 };
 
-struct Flow {
-  int kind;
-  double a0, a1, a2, xc, yc, gamma;
-  double fp[8], fq[8], fa[8], fb[8], fphi[8], fpsi[8], scale;
-  void eval(double x, double y, double& u, double& v) const {
-    if (kind == 0) {
-      u = a0;
-      v = a1;
-    } else if (kind == 1) {
-      double dx = x - xc, dy = y - yc, r2 = dx * dx + dy * dy;
-      double ut_over_r = (r2 < 1e-12) ? gamma / (2 * M_PI * a0 * a0) : gamma / (2 * M_PI * r2) * (1 - std::exp(-r2 / (a0 * a0)));
-      u = -ut_over_r * dy + a2 * dy;
-      v = ut_over_r * dx;
-    } else {
-      u = v = 0;
-      for (int m = 0; m < 8; ++m) {
-        double ph = 2 * M_PI * (fp[m] * x + fq[m] * y);
-        u += fa[m] * std::sin(ph + fphi[m]);
-        v += fb[m] * std::cos(ph + fpsi[m]);
-      }
-      u *= scale;
-      v *= scale;
-    }
-  }
+struct Flow : pivx::SynthFlow {
+  void eval(double x, double y, double& u, double& v) const { pivx::synth_flow_eval(*this, x, y, u, v); }
 };
 
 Flow make_flow(const pivx_synth_params* sp) {
@@ -344,6 +323,8 @@ int write_tecplot(const char* path, int nx_vec, int ny_vec, const float* gx, con
   ok = (fclose(f) == 0) && ok;
   return ok ? PIVX_OK : io_fail(PIVX_ERR_IO, std::string("write failed: ") + path);
 }
+
+void synth_flow_of(const pivx_synth_params* sp, SynthFlow* out) { *out = make_flow(sp); }
 
 }  // namespace pivx
 

@@ -39,6 +39,7 @@ struct XcArgs {
   float* zero_x;          // if set: zeroed at every window (first-pass predictor reset)
   float* zero_y;
   FusedDef fdef;          // frame 2 deformed in the gather (wave-per-window kernel only)
+  const int* gate;         // if set: the launch does nothing unless *gate != 0
 };
 
 // f2 (and XcArgs::f2u8 when frame 2 is deformed in the gather) points at
@@ -54,6 +55,7 @@ struct DefArgs {
   const float* ex_x; const float* ex_y; int nxe, nye;
   float x_st, y_st, spx, spy;
   unsigned long long* oob;   // cumulative count of taps beyond the reference's padded buffer
+  const int* gate;         // if set: the launch does nothing unless *gate != 0
 };
 
 struct MedArgs {
@@ -69,6 +71,7 @@ struct MedArgs {
   int* nholes;             //   (count in *nholes, which must be 0 before the pass)
   int hole_cap;            //   capacity of hole_list; appends beyond it are not stored (k_fill counts them)
   int* reset_count;        // if set: the launch writes 0 here (the hole counter, ahead of a later listing pass)
+  const int* gate;         // if set: the launch does nothing unless *gate != 0
 };
 
 struct FillArgs {
@@ -80,12 +83,14 @@ struct FillArgs {
   unsigned int* drops;     // holes counted beyond list_cap (never filled) are added here
   int* sweeps;
   int prelisted;           // holes already in list_a / *nholes (appended by the median pass)
+  const int* gate;         // if set: the launch does nothing unless *gate != 0
 };
 
 struct SmoothArgs {
   const float* sx; const float* sy; float* dx; float* dy;
   int nx_vec, ny_vec, wx, wy, hfwx, hfwy;
   float* ex; float* ey; int border;   // if ex is set: also write the border-replicated extension
+  const int* gate;         // if set: the launch does nothing unless *gate != 0
 };
 
 // MHE contrast enhancement of one u8 frame (MHE7_4_gpu.cu).  The central
@@ -112,7 +117,7 @@ hipError_t launch_deform_terms(const DefArgs& a, float4* col, float4* row, hipSt
 // kernel, the one that accepts FusedDef.
 bool xcorr_fuses_deform(int ws_x, int ws_y);
 hipError_t launch_extend(const float* fx, const float* fy, float* ex, float* ey, int nx, int ny, int border,
-                         hipStream_t s);
+                         hipStream_t s, const int* gate = nullptr);
 hipError_t launch_upsample(float* dx2, float* dy2, const float* ex1x, const float* ex1y, int nxe1, int nye1,
                            float x_st1, float y_st1, float sp1x, float sp1y, int sp2x, int sp2y, int nx2, int ny2,
                            float* ex2x, float* ex2y, int border, hipStream_t s);
@@ -122,10 +127,15 @@ hipError_t launch_smooth(const SmoothArgs& a, hipStream_t s);
 // filter := candidate 0 (median_filter_data_prep, cu:484-495); reset_count,
 // if set, is zeroed by the launch.
 hipError_t launch_prep(const float* cx, const float* cy, const uint8_t* cv, int K, float* dx, float* dy, uint8_t* dv,
-                       int nt, int* reset_count, hipStream_t s);
+                       int nt, int* reset_count, hipStream_t s, const int* gate = nullptr);
 // p[0..n) = 0 by a kernel (no memset node in a captured pair graph).
-hipError_t launch_zero_ints(int* p, int n, hipStream_t s);
+hipError_t launch_zero_ints(int* p, int n, hipStream_t s, const int* gate = nullptr);
 hipError_t launch_stream_copy(const void* src, void* dst, size_t bytes, hipStream_t s);
 hipError_t launch_mhe(const MheArgs& a, hipStream_t s);
+// Device rendering of one synthetic pair (synth.hip) into frames f1, f2 (row
+// pitches p1, p2); acc1, acc2 are nx * ny zeroed accumulators, left zeroed.
+struct SynthRender;
+hipError_t launch_synth(const SynthRender& r, unsigned int* acc1, unsigned int* acc2, uint8_t* f1, long long p1,
+                        uint8_t* f2, long long p2, hipStream_t s);
 
 }  // namespace pivx

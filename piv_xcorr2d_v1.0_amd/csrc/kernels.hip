@@ -433,6 +433,7 @@ __device__ __forceinline__ void group_sync() {
 
 template <int WX, int WY, int WPB>
 __global__ void __launch_bounds__((FastCfg<WX, WY, WPB>::NT)) k_xcorr_fast(XcArgs a) {
+  if (a.gate && *a.gate == 0) return;  // learned pipeline: grid 1 runs only as the fallback
   static_assert(WPB % 2 == 0, "windows are processed in pairs");
   constexpr int NT = FastCfg<WX, WY, WPB>::NT;
   constexpr int SX = FastCfg<WX, WY, WPB>::SX;
@@ -797,6 +798,7 @@ __device__ __forceinline__ void deform_window(const XcArgs& a, int ax0, int ay0,
 
 template <int WX, int WY>
 __global__ void __launch_bounds__(64) k_xcorr_pair(XcArgs a) {
+  if (a.gate && *a.gate == 0) return;  // learned pipeline: grid 1 runs only as the fallback
   static_assert(WX <= 32 && WY <= 32 && WX % 4 == 0 && WY % 2 == 0, "pair kernel covers windows up to 32x32");
   constexpr int SX = WX + 1;
   constexpr int NPIX = WX * WY;
@@ -1007,6 +1009,7 @@ __global__ void __launch_bounds__(64) k_xcorr_pair(XcArgs a) {
 //     G[y][-kx] = conj(G[y][kx]).
 template <int WX, int WY>
 __global__ void __launch_bounds__(64) k_xcorr_single(XcArgs a) {
+  if (a.gate && *a.gate == 0) return;  // learned pipeline: grid 1 runs only as the fallback
   static_assert(WX <= 64 && WY <= 64 && WX % 4 == 0 && WY % 4 == 0, "single-wave kernel covers windows up to 64x64");
   constexpr int SX = WX + 1;   // real plane row stride (floats)
   constexpr int NPIX = WX * WY;
@@ -1322,6 +1325,7 @@ __device__ void block_argmax(float& bv, int& bi, float* sv, int* si) {
 
 __global__ void __launch_bounds__(256) k_xcorr_generic(XcArgs a, int WX, int WY, GenPlan plan, const float2* twx,
                                                        const float2* twy, float2* gscratch, int win0) {
+  if (a.gate && *a.gate == 0) return;  // learned pipeline: grid 1 runs only as the fallback
   extern __shared__ float2 dyn[];
   __shared__ float red_v[8];
   __shared__ int red_i[8];
@@ -1423,6 +1427,7 @@ __global__ void __launch_bounds__(256) k_xcorr_generic(XcArgs a, int WX, int WY,
 constexpr int DEF_ROWS = 16;
 
 __global__ void __launch_bounds__(256) k_deform(DefArgs a) {
+  if (a.gate && *a.gate == 0) return;  // learned pipeline: grid 1 runs only as the fallback
   __shared__ float4 rowt[DEF_ROWS];
   const int y0 = blockIdx.y * DEF_ROWS;
   const int nrow = min(DEF_ROWS, a.ny_aug - y0);
@@ -1501,7 +1506,9 @@ __global__ void k_deform_terms(DefArgs a, float4* col, float4* row) {
 // ============================================================== field extension
 // extend_filtered_vecfield_p1/p2 (cu:1547-1618): copy into an (nx+2b)x(ny+2b)
 // field with the border replicated b times (b = 3, cuh:190-191).
-__global__ void k_extend(const float* fx, const float* fy, float* ex, float* ey, int nx, int ny, int b) {
+__global__ void k_extend(const float* fx, const float* fy, float* ex, float* ey, int nx, int ny, int b,
+                         const int* gate) {
+  if (gate && *gate == 0) return;
   const int nxe = nx + 2 * b, nye = ny + 2 * b;
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nxe * nye; t += gridDim.x * blockDim.x) {
     const int j = t / nxe, i = t - (t / nxe) * nxe;
@@ -1642,6 +1649,7 @@ __device__ __forceinline__ void median_decide(const MedArgs& a, int t, bool vmed
 template <int NS>
 __global__ void __launch_bounds__(128) k_median(MedArgs a) {
 #pragma clang fp contract(off)
+  if (a.gate && *a.gate == 0) return;  // learned pipeline: grid 1 runs only as the fallback
   const int nt = a.nx_vec * a.ny_vec;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (a.reset_count && t == 0) *a.reset_count = 0;
@@ -1724,6 +1732,7 @@ __device__ __forceinline__ float median_low(const float (&v)[MedianNet<M>::NS], 
 template <int WXM, int WYM>
 __global__ void __launch_bounds__(256) k_median2(MedArgs a) {
 #pragma clang fp contract(off)
+  if (a.gate && *a.gate == 0) return;  // learned pipeline: grid 1 runs only as the fallback
   constexpr int M = WXM * WYM - 1;
   constexpr int NS = MedianNet<M>::NS;
   constexpr int SKIP = (WXM * WYM - 1) / 2;
@@ -1824,6 +1833,7 @@ __device__ float median_scratch(float* s, int n) {
 
 __global__ void __launch_bounds__(128) k_median_generic(MedArgs a) {
 #pragma clang fp contract(off)
+  if (a.gate && *a.gate == 0) return;  // learned pipeline: grid 1 runs only as the fallback
   const int nt = a.nx_vec * a.ny_vec;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (a.reset_count && t == 0) *a.reset_count = 0;
@@ -1873,6 +1883,7 @@ __global__ void __launch_bounds__(128) k_median_generic(MedArgs a) {
 // grid's vector count); k_fill adds any excess to *drops, which the host
 // reports as an error.
 __global__ void __launch_bounds__(256) k_fill_list(FillArgs a) {
+  if (a.gate && *a.gate == 0) return;  // learned pipeline: grid 1 runs only as the fallback
   const int nt = a.nx_vec * a.ny_vec;
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nt; t += gridDim.x * blockDim.x)
     if (!a.dv[t]) {
@@ -1883,6 +1894,7 @@ __global__ void __launch_bounds__(256) k_fill_list(FillArgs a) {
 
 __global__ void __launch_bounds__(1024) k_fill(FillArgs a) {
 #pragma clang fp contract(off)
+  if (a.gate && *a.gate == 0) return;  // learned pipeline: grid 1 runs only as the fallback
   __shared__ int n_holes, n_next, n_filled;
   if (threadIdx.x == 0) {
     const int counted = *a.nholes;
@@ -1958,6 +1970,7 @@ __global__ void __launch_bounds__(1024) k_fill(FillArgs a) {
 // in-bounds wx*wy window including the centre, ignoring validity; raster sum.
 __global__ void k_smooth(SmoothArgs a) {
 #pragma clang fp contract(off)
+  if (a.gate && *a.gate == 0) return;  // learned pipeline: grid 1 runs only as the fallback
   const int nt = a.nx_vec * a.ny_vec;
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nt; t += gridDim.x * blockDim.x) {
     const int vy = t / a.nx_vec, vx = t - (t / a.nx_vec) * a.nx_vec;
@@ -1982,7 +1995,8 @@ __global__ void k_smooth(SmoothArgs a) {
 }
 
 __global__ void k_prep(const float* cx, const float* cy, const uint8_t* cv, int K, float* dx, float* dy, uint8_t* dv,
-                       int nt, int* reset_count) {
+                       int nt, int* reset_count, const int* gate) {
+  if (gate && *gate == 0) return;
   if (reset_count && blockIdx.x == 0 && threadIdx.x == 0) *reset_count = 0;
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nt; t += gridDim.x * blockDim.x) {
     const long long o = (long long)t * K;
@@ -1991,7 +2005,8 @@ __global__ void k_prep(const float* cx, const float* cy, const uint8_t* cv, int 
     dy[t] = cy[o];
   }
 }
-__global__ void k_zero_ints(int* p, int n) {
+__global__ void k_zero_ints(int* p, int n, const int* gate) {
+  if (gate && *gate == 0) return;
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) p[t] = 0;
 }
 
@@ -2287,9 +2302,9 @@ hipError_t launch_deform(const DefArgs& a, hipStream_t s) {
 }
 
 hipError_t launch_extend(const float* fx, const float* fy, float* ex, float* ey, int nx, int ny, int border,
-                         hipStream_t s) {
+                         hipStream_t s, const int* gate) {
   const long long n = (long long)(nx + 2 * border) * (ny + 2 * border);
-  hipLaunchKernelGGL(k_extend, dim3(grid_for(n, 256)), dim3(256), 0, s, fx, fy, ex, ey, nx, ny, border);
+  hipLaunchKernelGGL(k_extend, dim3(grid_for(n, 256)), dim3(256), 0, s, fx, fy, ex, ey, nx, ny, border, gate);
   return hipGetLastError();
 }
 
@@ -2336,13 +2351,14 @@ hipError_t launch_smooth(const SmoothArgs& a, hipStream_t s) {
 }
 
 hipError_t launch_prep(const float* cx, const float* cy, const uint8_t* cv, int K, float* dx, float* dy, uint8_t* dv,
-                       int nt, int* reset_count, hipStream_t s) {
-  hipLaunchKernelGGL(k_prep, dim3(grid_for(nt, 256)), dim3(256), 0, s, cx, cy, cv, K, dx, dy, dv, nt, reset_count);
+                       int nt, int* reset_count, hipStream_t s, const int* gate) {
+  hipLaunchKernelGGL(k_prep, dim3(grid_for(nt, 256)), dim3(256), 0, s, cx, cy, cv, K, dx, dy, dv, nt, reset_count,
+                     gate);
   return hipGetLastError();
 }
 
-hipError_t launch_zero_ints(int* p, int n, hipStream_t s) {
-  hipLaunchKernelGGL(k_zero_ints, dim3(grid_for(n, 256)), dim3(256), 0, s, p, n);
+hipError_t launch_zero_ints(int* p, int n, hipStream_t s, const int* gate) {
+  hipLaunchKernelGGL(k_zero_ints, dim3(grid_for(n, 256)), dim3(256), 0, s, p, n, gate);
   return hipGetLastError();
 }
 hipError_t launch_stream_copy(const void* src, void* dst, size_t bytes, hipStream_t s) {
