@@ -246,6 +246,18 @@ int pivx_set_graph_mode(pivx_ctx* ctx, int on);
  * (pivx_get_deformed) as it was. */
 int pivx_set_fused_deform(pivx_ctx* ctx, int mode);
 
+/* Precision form of the correlation (no reference counterpart; replaces the
+ * FP32 cuFFT transforms of cu:922-931, 1334-1350 for the handle's passes):
+ * 0 = FP32 register FFTs (default), or a DFT-by-matrix contraction on the
+ * matrix cores (F_H z F_W, conj-multiply, F_H^H P F_W^H) with 1 = FP16 or
+ * 2 = BF16 operands, 3 = FP16 or 4 = BF16 operands split into hi + lo parts
+ * (three products each, about FP32 accuracy); FP32 accumulation in all.
+ * Modes 1-4 also evaluate the Gaussian fit of nearly flat peaks and the
+ * median test's normalised residuals in FP64.  Windows of both grids must be
+ * at most 64x64 with sides multiples of 8.  The fused grid-1 deformation is
+ * not used while a mode other than 0 is set. */
+int pivx_set_correlation_precision(pivx_ctx* ctx, int mode);
+
 /* Wait for all work of the handle. */
 int pivx_synchronize(pivx_ctx* ctx);
 

@@ -130,6 +130,7 @@ _SIGS = {
     "pivx_synchronize": [_P],
     "pivx_set_graph_mode": [_P, C.c_int],
     "pivx_set_fused_deform": [_P, C.c_int],
+    "pivx_set_correlation_precision": [_P, C.c_int],
     "pivx_write_vec_data_general": [_P, C.c_char_p, C.c_int, C.c_int],
     "pivx_get_dims": [_P, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)],
     "pivx_get_grid_coords": [_P, C.c_int, _P, _P],
@@ -338,6 +339,10 @@ class Xcorr2dMulti:
 
     def set_graph_mode(self, on=True):
         self._c(lib().pivx_set_graph_mode(self.h, 1 if on else 0))
+
+    def set_correlation_precision(self, mode):
+        """0 FP32 FFT; DFT-by-matrix on the matrix cores: 1 FP16, 2 BF16, 3 FP16 hi+lo, 4 BF16 hi+lo."""
+        self._c(lib().pivx_set_correlation_precision(self.h, int(mode)))
 
     def set_fused_deform(self, mode=1):
         """0: never; 1: inside run_pair (default); 2: also in Calc_cc_map_p1."""

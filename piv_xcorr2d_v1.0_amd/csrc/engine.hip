@@ -88,6 +88,8 @@ struct pivx_ctx {
   int* nholes = nullptr;                   // hole-list counter of the fill kernels
   unsigned int* drops = nullptr;           // holes counted beyond the list capacity since the last check
   const int* gate = nullptr;               // while set, grid kernels run only if *gate != 0 (learned fallback)
+  int prec = 0;                            // correlation precision form (pivx_set_correlation_precision)
+  DftTables dtab[2] = {};                  // its DFT-matrix tables per grid
   // learned pipeline (pivx_run_pair_learned): fallback flag and outlier count
   // of the network field, network input and hidden activations (batch 1)
   int* lgate = nullptr;
@@ -154,6 +156,14 @@ int dalloc(pivx_ctx* c, T** p, size_t n) {
   c->allocs.push_back(q);
   *p = static_cast<T*>(q);
   return PIVX_OK;
+}
+
+// float -> bfloat16 bits, round to nearest even
+uint16_t host_bf16(float f) {  // round to nearest even
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  if ((u & 0x7f800000u) != 0x7f800000u) u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
 }
 
 // Reference vector count along one axis (cu:771-777).
@@ -366,8 +376,10 @@ int run_xcorr(pivx_ctx* c, int gi, bool deformed, bool zero_pred = false, bool f
   a.ncand = G.ncand;
   a.gate = c->gate;
   tic(c);
-  hipError_t e = G.fast ? launch_xcorr(a, g.ws_x, g.ws_y, nullptr, 0, nullptr, nullptr, c->stream)
-                        : launch_xcorr(a, g.ws_x, g.ws_y, G.gscratch, G.gscratch_windows, G.twx, G.twy, c->stream);
+  hipError_t e;
+  if (c->prec) e = launch_xcorr_dft(a, g.ws_x, g.ws_y, c->prec, c->dtab[gi], c->stream);
+  else if (G.fast) e = launch_xcorr(a, g.ws_x, g.ws_y, nullptr, 0, nullptr, nullptr, c->stream);
+  else e = launch_xcorr(a, g.ws_x, g.ws_y, G.gscratch, G.gscratch_windows, G.twx, G.twy, c->stream);
   const int rc = check_launch(c, e, "correlation");
   toc(c, gi, 1);
   return rc;
@@ -388,6 +400,7 @@ struct FilterJob {
   int8_t* accepted;                                              // accepted candidate per vector or null
   hipStream_t stream;
   const int* gate;                                               // launches run only if *gate != 0 (null: always)
+  int fp64;                                                      // FP64 residual normalisation (precision form)
 };
 
 // MedianFilter_Smoothing_p1/p2 (cu:1451-1545, 1766-1863) writing into (ox, oy, ov).
@@ -477,6 +490,7 @@ int enqueue_filter(pivx_ctx* c, int gi, const FilterJob& J, int npass_median, in
       m.hole_cap = J.list_cap;
       m.reset_count = (listed && p == 0 && back > 0) ? J.nholes : nullptr;
       m.gate = J.gate;
+      m.fp64 = J.fp64;
       rc = check_launch(c, launch_median(m, stream), "median_filtering");
       if (rc) return rc;
       srcx = tx;
@@ -577,6 +591,7 @@ int run_filter(pivx_ctx* c, int gi, int npass_median, int min_nneigh, float inse
   J.accepted = G.accepted;
   J.stream = c->stream;
   J.gate = c->gate;
+  J.fp64 = c->prec != 0 ? 1 : 0;
   return enqueue_filter(c, gi, J, npass_median, min_nneigh, insert_thr, npass_smooth, iffill, ox, oy, ov, ext_x, ext_y,
                         ext_done, mx, my, mv);
 }
@@ -940,7 +955,7 @@ int pivx_calc_cc_map_p1(pivx_ctx* c, int idx_pass) {
   if (idx_pass == 0) return run_xcorr(c, 0, false, true);
   // the deformation of non-overlapping windows runs in the correlation
   // kernel (within pivx_run_pair by default, pivx_set_fused_deform)
-  if (c->grid[0].fuse && (c->fuse_mode == 2 || (c->fuse_mode == 1 && c->in_pair)))
+  if (c->grid[0].fuse && c->prec == 0 && (c->fuse_mode == 2 || (c->fuse_mode == 1 && c->in_pair)))
     return run_xcorr(c, 0, true, false, true);
   rc = run_deform(c, 0);
   if (rc) return rc;
@@ -1199,6 +1214,133 @@ static int check_drops(pivx_ctx* c) {
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->drops_total += d;
   return fail(c, PIVX_ERR_STATE, "hole list overflow: %u invalid vectors were not filled", d);
+}
+
+// Host tables of the DFT-by-matrix precision form for a W x H window: the
+// matrices k_xcorr_dft contracts with, each entry split into hi = round(x)
+// and lo = round(x - hi) in FP16 or BF16.
+static uint16_t host_f16(float f) {  // round to nearest even, with subnormals
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  const uint32_t sign = (u >> 16) & 0x8000u;
+  const float a = std::fabs(f);
+  if (!(a < 65520.f)) return (uint16_t)(sign | 0x7c00u);  // overflow (inputs never reach it)
+  if (a < 6.103515625e-05f) {                              // subnormal: multiples of 2^-24
+    const float q = std::nearbyint(a * 16777216.f);
+    return (uint16_t)(sign | (uint32_t)q);
+  }
+  uint32_t v = u & 0x7fffffffu;
+  const uint32_t lsb = (v >> 13) & 1u;
+  v += 0xfffu + lsb;
+  const uint32_t e = (v >> 23) - 127 + 15, m = (v >> 13) & 0x3ffu;
+  return (uint16_t)(sign | (e << 10) | m);
+}
+static float host_f16_back(uint16_t h) {
+  const uint32_t sign = (h & 0x8000u) << 16, e = (h >> 10) & 0x1fu, m = h & 0x3ffu;
+  float v;
+  if (e == 0) v = (float)m * 5.9604644775390625e-08f;
+  else {
+    const uint32_t u = ((e - 15 + 127) << 23) | (m << 13);
+    std::memcpy(&v, &u, 4);
+  }
+  uint32_t r;
+  std::memcpy(&r, &v, 4);
+  r |= sign;
+  std::memcpy(&v, &r, 4);
+  return v;
+}
+
+static int build_dft_tables(pivx_ctx* c, int gi, bool bf) {
+  const pivx_geometry& g = c->grid[gi].g;
+  const int W = g.ws_x, H = g.ws_y, PW = (W + 15) / 16 * 16, PH = (H + 15) / 16 * 16;
+  auto cs = [](int n, int j, int k, bool sine) {
+    const double ang = 2.0 * M_PI * (double)(((long long)j * k) % n) / (double)n;
+    return sine ? std::sin(ang) : std::cos(ang);
+  };
+  // A tables [PH][2PH]: (sa, ca) signs of the S and C halves: value(k, kk) = first half or second half
+  struct Spec { int n, pad; bool rowmajor; int first, second; };  // 1 = C, 2 = S, negative = negated, 0 = none
+  const Spec specs[7] = {{H, PH, true, 1, 2},  {H, PH, true, -2, 1},   // a1: [C S], [-S C]
+                         {W, PW, false, 1, 2}, {W, PW, false, -2, 1},  // b2: [C; S], [-S; C]
+                         {H, PH, true, 1, -2}, {H, PH, true, 2, 1},    // a3: [C -S], [S C]
+                         {W, PW, false, 1, -2}};                       // b4: [C; -S]
+  size_t total = 0;
+  for (const Spec& sp : specs) total += (size_t)sp.pad * 2 * sp.pad * 2;
+  std::vector<uint16_t> buf(total, 0);
+  size_t off = 0;
+  const uint16_t* base = nullptr;
+  uint16_t* dev = nullptr;
+  if (dalloc(c, &dev, total)) return PIVX_ERR_HIP;
+  base = dev;
+  const uint16_t* ptr[7][2];
+  for (int t = 0; t < 7; ++t) {
+    const Spec& sp = specs[t];
+    const int K = 2 * sp.pad;
+    uint16_t* hi = buf.data() + off;
+    uint16_t* lo = hi + (size_t)sp.pad * K;
+    for (int row = 0; row < sp.n; ++row)
+      for (int kk = 0; kk < K; ++kk) {
+        const int half = kk < sp.pad ? 0 : 1, idx = kk - half * sp.pad;
+        if (idx >= sp.n) continue;
+        const int which = half == 0 ? sp.first : sp.second;
+        // row-major tables hold M[row][idx]; column tables hold column `row` of M, entry idx: M[idx][row] (C, S symmetric)
+        double v = cs(sp.n, row, idx, std::abs(which) == 2);
+        if (which < 0) v = -v;
+        const float f = (float)v;
+        uint16_t h, l;
+        if (bf) {
+          h = host_bf16(f);
+          float hb;
+          const uint32_t hu = (uint32_t)h << 16;
+          std::memcpy(&hb, &hu, 4);
+          l = host_bf16(f - hb);
+        } else {
+          h = host_f16(f);
+          l = host_f16(f - host_f16_back(h));
+        }
+        hi[(size_t)row * K + kk] = h;
+        lo[(size_t)row * K + kk] = l;
+      }
+    ptr[t][0] = base + off;
+    ptr[t][1] = base + off + (size_t)sp.pad * K;
+    off += (size_t)sp.pad * K * 2;
+  }
+  HIPCHK(c, hipMemcpy(dev, buf.data(), total * sizeof(uint16_t), hipMemcpyHostToDevice));
+  DftTables& T = c->dtab[gi];
+  for (int o = 0; o < 2; ++o) {
+    T.a1h[o] = ptr[o][0];
+    T.a1l[o] = ptr[o][1];
+    T.b2h[o] = ptr[2 + o][0];
+    T.b2l[o] = ptr[2 + o][1];
+    T.a3h[o] = ptr[4 + o][0];
+    T.a3l[o] = ptr[4 + o][1];
+  }
+  T.b4h = ptr[6][0];
+  T.b4l = ptr[6][1];
+  return PIVX_OK;
+}
+
+int pivx_set_correlation_precision(pivx_ctx* c, int mode) {
+  if (!c) return fail(nullptr, PIVX_ERR_PARAM, "null handle");
+  if (mode < 0 || mode > 4) return fail(c, PIVX_ERR_PARAM, "precision mode must be 0..4 (got %d)", mode);
+  if (mode == c->prec) return PIVX_OK;
+  HIPCHK(c, hipSetDevice(c->device));
+  if (mode != 0) {
+    for (int gi = 0; gi < 2; ++gi) {
+      const pivx_geometry& g = c->grid[gi].g;
+      if (g.ws_x % 8 || g.ws_y % 8 || g.ws_x > 64 || g.ws_y > 64)
+        return fail(c, PIVX_ERR_PARAM,
+                    "the DFT-by-matrix precision form covers windows up to 64x64 with sides multiples of 8 "
+                    "(grid %d: %dx%d)", gi + 1, g.ws_x, g.ws_y);
+    }
+    for (int gi = 0; gi < 2; ++gi) {
+      const int rc = build_dft_tables(c, gi, mode == 2 || mode == 4);
+      if (rc) return rc;
+    }
+  }
+  drop_graph(c);
+  drop_learned_graph(c);
+  c->prec = mode;
+  return PIVX_OK;
 }
 
 int pivx_synchronize(pivx_ctx* c) {
@@ -1578,12 +1720,6 @@ int nalloc(std::vector<void*>& list, T** p, size_t n) {
   return PIVX_OK;
 }
 
-uint16_t host_bf16(float f) {  // round to nearest even
-  uint32_t u;
-  std::memcpy(&u, &f, 4);
-  if ((u & 0x7f800000u) != 0x7f800000u) u += 0x7fffu + ((u >> 16) & 1u);
-  return (uint16_t)(u >> 16);
-}
 
 // spatial size of layer l's input for a network input h x w
 void layer_dims(int h, int w, int l, int& hin, int& win, int& hout, int& wout) {

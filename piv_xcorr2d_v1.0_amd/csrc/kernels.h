@@ -71,6 +71,7 @@ struct MedArgs {
   int* nholes;             //   (count in *nholes, which must be 0 before the pass)
   int hole_cap;            //   capacity of hole_list; appends beyond it are not stored (k_fill counts them)
   int* reset_count;        // if set: the launch writes 0 here (the hole counter, ahead of a later listing pass)
+  int fp64;                // normalised residuals in FP64 (precision form)
   const int* gate;         // if set: the launch does nothing unless *gate != 0
 };
 
@@ -109,6 +110,19 @@ struct MheArgs {
 hipError_t launch_xcorr(const XcArgs& a, int ws_x, int ws_y, float2* gscratch, size_t gscratch_windows,
                         const float2* twx, const float2* twy, hipStream_t s);
 bool xcorr_fast_supported(int ws_x, int ws_y);
+// Precision form of the correlation: DFT-by-matrix on the matrix cores for
+// windows up to 64x64 (sides multiples of 8).  Tables of the constant DFT
+// matrices (bf16 or fp16 bits, hi and lo parts), laid out as k_xcorr_dft
+// reads them: a1 / a3 [pad16(H)][2 pad16(H)] row-major, b2 / b4
+// [pad16(W)][2 pad16(W)] (K contiguous per output column).
+struct DftTables {
+  const uint16_t* a1h[2]; const uint16_t* a1l[2];   // [C S], [-S C]   (step 1: Tr, Ti)
+  const uint16_t* b2h[2]; const uint16_t* b2l[2];   // [C; S], [-S; C] (step 2: Zr, Zi)
+  const uint16_t* a3h[2]; const uint16_t* a3l[2];   // [C -S], [S C]   (step 3: Ur, Ui)
+  const uint16_t* b4h; const uint16_t* b4l;         // [C; -S]         (step 4: cc)
+};
+// mode 1 FP16, 2 BF16, 3 FP16 hi+lo, 4 BF16 hi+lo
+hipError_t launch_xcorr_dft(const XcArgs& a, int ws_x, int ws_y, int mode, const DftTables& t, hipStream_t s);
 hipError_t launch_deform(const DefArgs& a, hipStream_t s);
 // The per-column and per-row interpolation terms of the deformation (as
 // k_deform computes them) into col[nx_aug] and row[ny_aug], for FusedDef.

@@ -205,6 +205,35 @@ __device__ __forceinline__ bool subpixel_fit(const float* cc, int sx, int wsx, i
   return true;
 }
 
+// subpixel_fit for the precision form: an axis whose FP32 log-ratio
+// denominator f0 + f2 - 2 f1 is below 1/16 in magnitude (a nearly flat peak,
+// where the ratio magnifies every rounding of the logarithms) is recomputed
+// with FP64 logarithms and FP64 ratio.
+__device__ __forceinline__ float fit_axis_fp64near(float v, float fm, float fp) {
+#pragma clang fp contract(off)
+  const float f1 = logf(v), f0 = logf(fm), f2 = logf(fp);
+  const float den = f0 + f2 - 2 * f1;
+  if (fabsf(den) >= 0.0625f) return (f0 - f2) / den / 2;
+  const double d1 = log((double)v), d0 = log((double)fm), d2 = log((double)fp);
+  return (float)((d0 - d2) / (d0 + d2 - 2.0 * d1) / 2.0);
+}
+
+__device__ __forceinline__ bool subpixel_fit_fp64near(const float* cc, int sx, int wsx, int wsy, int i, int j, float v,
+                                                      float px, float py, float lx1, float lx2, float ly1, float ly2,
+                                                      float& dx, float& dy) {
+#pragma clang fp contract(off)
+  dx = 0.f;
+  dy = 0.f;
+  if (i == 0 || i == wsx - 1 || j == 0 || j == wsy - 1 || !(v > 0.f)) return false;
+  const float x = (float)i + fit_axis_fp64near(v, cc[j * sx + i - 1], cc[j * sx + i + 1]) - (float)wsx / 2 + px;
+  const float y = (float)j + fit_axis_fp64near(v, cc[(j - 1) * sx + i], cc[(j + 1) * sx + i]) - (float)wsy / 2 + py;
+  if (x < lx1 || x > lx2 || y < ly1 || y > ly2) return false;
+  if (isnan(x) || isnan(y) || isinf(x) || isinf(y)) return false;
+  dx = x;
+  dy = y;
+  return true;
+}
+
 // subpixel_fit with its two axes on a lane pair: the even lane of the pair
 // fits x, the odd lane y (same operations as above), the pair exchanges the
 // coordinates and both apply the tests.  Lanes 2k and 2k + 1 of a W-lane
@@ -1323,47 +1352,14 @@ __device__ void block_argmax(float& bv, int& bi, float* sv, int* si) {
     }
 }
 
-__global__ void __launch_bounds__(256) k_xcorr_generic(XcArgs a, int WX, int WY, GenPlan plan, const float2* twx,
-                                                       const float2* twy, float2* gscratch, int win0) {
-  if (a.gate && *a.gate == 0) return;  // learned pipeline: grid 1 runs only as the fallback
-  extern __shared__ float2 dyn[];
-  __shared__ float red_v[8];
-  __shared__ int red_i[8];
+// Peak stage over a shifted correlation plane cc [WY][WX] in LDS, run by the
+// whole workgroup: 4-neighbour local maxima inside the search border
+// (cu:585-606) into cand, the uncapped count, Npeak_save rounds of arg-max
+// (value desc, raster index asc) and the Gaussian fit of each peak.  fit64:
+// the FP64 fit of nearly flat peaks (subpixel_fit_fp64near).
+__device__ void plane_peaks(const XcArgs& a, int w, float* cc, float* cand, int WX, int WY, float* red_v, int* red_i,
+                            bool fit64) {
   const int N = WX * WY;
-  const int w = win0 + blockIdx.x;
-  if (w >= a.nx_vec * a.ny_vec) return;
-  const int iy = w / a.nx_vec, ix = w - (w / a.nx_vec) * a.nx_vec;
-  float2* A = gscratch ? gscratch + (long long)blockIdx.x * 2 * N : dyn;
-  float2* B = A + N;
-
-  for (int e = threadIdx.x; e < N; e += blockDim.x) {
-    const int r = e / WX, c = e - (e / WX) * WX;
-    A[e] = gather_px(a, ix * a.sp_x + c, iy * a.sp_y + r);
-  }
-  __syncthreads();
-  float2* Z = fft_lines(A, B, WX, WY, 1, WX, plan.rx, plan.nrx, twx, false);
-  float2* O = (Z == A) ? B : A;
-  Z = fft_lines(Z, O, WY, WX, WX, 1, plan.ry, plan.nry, twy, false);
-  O = (Z == A) ? B : A;
-  for (int e = threadIdx.x; e < N; e += blockDim.x) {
-    const int r = e / WX, c = e - (e / WX) * WX;
-    const int rm = (WY - r) % WY, cm = (WX - c) % WX;
-    O[e] = xprod(Z[e], Z[rm * WX + cm]);
-  }
-  __syncthreads();
-  Z = fft_lines(O, Z, WX, WY, 1, WX, plan.rx, plan.nrx, twx, true);
-  O = (Z == A) ? B : A;
-  Z = fft_lines(Z, O, WY, WX, WX, 1, plan.ry, plan.nry, twy, true);
-  O = (Z == A) ? B : A;
-  float* cc = reinterpret_cast<float*>(O);
-  float* cand = cc + N;
-  for (int e = threadIdx.x; e < N; e += blockDim.x) {
-    const int r = e / WX, c = e - (e / WX) * WX;
-    const int rs = (r + WY / 2) % WY, cs = (c + WX / 2) % WX;
-    const float2 q = Z[e];
-    cc[rs * WX + cs] = (sqrtf(q.x * q.x + q.y * q.y) * 0.25f) / (float)N;
-  }
-  __syncthreads();
   int cnt = 0;
   for (int e = threadIdx.x; e < N; e += blockDim.x) {
     const int j = e / WX, i = e - (e / WX) * WX;
@@ -1409,7 +1405,9 @@ __global__ void __launch_bounds__(256) k_xcorr_generic(XcArgs a, int WX, int WY,
       const long long o = (long long)w * a.K + k;
       float dx = 0.f, dy = 0.f;
       bool ok = false;
-      if (bv > 0.f) ok = subpixel_fit(cc, WX, WX, WY, bi % WX, bi / WX, bv, px, py, a.lx1, a.lx2, a.ly1, a.ly2, dx, dy);
+      if (bv > 0.f)
+        ok = fit64 ? subpixel_fit_fp64near(cc, WX, WX, WY, bi % WX, bi / WX, bv, px, py, a.lx1, a.lx2, a.ly1, a.ly2, dx, dy)
+                   : subpixel_fit(cc, WX, WX, WY, bi % WX, bi / WX, bv, px, py, a.lx1, a.lx2, a.ly1, a.ly2, dx, dy);
       a.disp_x[o] = dx;
       a.disp_y[o] = dy;
       a.valid[o] = ok ? 1 : 0;
@@ -1417,6 +1415,232 @@ __global__ void __launch_bounds__(256) k_xcorr_generic(XcArgs a, int WX, int WY,
       a.pk_idx[o] = bv > 0.f ? bi : -1;
     }
   }
+}
+
+__global__ void __launch_bounds__(256) k_xcorr_generic(XcArgs a, int WX, int WY, GenPlan plan, const float2* twx,
+                                                       const float2* twy, float2* gscratch, int win0) {
+  if (a.gate && *a.gate == 0) return;  // learned pipeline: grid 1 runs only as the fallback
+  extern __shared__ float2 dyn[];
+  __shared__ float red_v[8];
+  __shared__ int red_i[8];
+  const int N = WX * WY;
+  const int w = win0 + blockIdx.x;
+  if (w >= a.nx_vec * a.ny_vec) return;
+  const int iy = w / a.nx_vec, ix = w - (w / a.nx_vec) * a.nx_vec;
+  float2* A = gscratch ? gscratch + (long long)blockIdx.x * 2 * N : dyn;
+  float2* B = A + N;
+
+  for (int e = threadIdx.x; e < N; e += blockDim.x) {
+    const int r = e / WX, c = e - (e / WX) * WX;
+    A[e] = gather_px(a, ix * a.sp_x + c, iy * a.sp_y + r);
+  }
+  __syncthreads();
+  float2* Z = fft_lines(A, B, WX, WY, 1, WX, plan.rx, plan.nrx, twx, false);
+  float2* O = (Z == A) ? B : A;
+  Z = fft_lines(Z, O, WY, WX, WX, 1, plan.ry, plan.nry, twy, false);
+  O = (Z == A) ? B : A;
+  for (int e = threadIdx.x; e < N; e += blockDim.x) {
+    const int r = e / WX, c = e - (e / WX) * WX;
+    const int rm = (WY - r) % WY, cm = (WX - c) % WX;
+    O[e] = xprod(Z[e], Z[rm * WX + cm]);
+  }
+  __syncthreads();
+  Z = fft_lines(O, Z, WX, WY, 1, WX, plan.rx, plan.nrx, twx, true);
+  O = (Z == A) ? B : A;
+  Z = fft_lines(Z, O, WY, WX, WX, 1, plan.ry, plan.nry, twy, true);
+  O = (Z == A) ? B : A;
+  float* cc = reinterpret_cast<float*>(O);
+  float* cand = cc + N;
+  for (int e = threadIdx.x; e < N; e += blockDim.x) {
+    const int r = e / WX, c = e - (e / WX) * WX;
+    const int rs = (r + WY / 2) % WY, cs = (c + WX / 2) % WX;
+    const float2 q = Z[e];
+    cc[rs * WX + cs] = (sqrtf(q.x * q.x + q.y * q.y) * 0.25f) / (float)N;
+  }
+  __syncthreads();
+  plane_peaks(a, w, cc, cand, WX, WY, red_v, red_i, false);
+}
+
+// ============================================================== precision form
+// Correlation as DFT-by-matrix contractions on the matrix cores (FP16 or
+// BF16 operands, FP32 accumulation; with SPLIT every operand x is carried as
+// hi = round(x) plus lo = round(x - hi) and each product as hi*hi + hi*lo +
+// lo*hi, which brings the products back to about FP32 accuracy), replacing
+// the batched cuFFT plans of cu:922-931, 1334-1350 for windows up to 64x64.
+// With F = C - iS (C = cos, S = sin of 2 pi jk / n) and z = f1 + i f2:
+//   T = F_H z        (Tr = [C  S][zr; zi],  Ti = [-S C][zr; zi])
+//   Z = T F_W        (Zr = [Tr Ti][C; S],   Zi = [Tr Ti][-S; C])
+//   P = conj(F1) F2  from Z and its mirror (Hermitian split), scaled by a
+//                    power of two so that FP16 operands cannot overflow
+//   U = F_H^H P      (Ur = [C -S][Pr; Pi],  Ui = [S C][Pr; Pi])
+//   cc = Re(U F_W^H) = [Ur Ui][C; -S]
+// Each step is a set of 16x16 output tiles of v_mfma_f32_16x16x32_{f16,bf16}
+// with K padded per half to a multiple of 16; the DFT matrices are constant
+// tables (DftTables), the data operands live in LDS in the layout the next
+// step reads (A row-major with K contiguous, B with K contiguous per
+// column).  The plane then goes through plane_peaks with the FP64 fit of
+// flat peaks.
+typedef _Float16 f16x8_t __attribute__((__vector_size__(16)));
+typedef __bf16 bf16x8_t __attribute__((__vector_size__(16)));
+typedef float f32x4_t __attribute__((__vector_size__(16)));
+typedef unsigned int u32x4_t __attribute__((__vector_size__(16)));
+
+template <bool BF>
+struct DftTy;
+template <>
+struct DftTy<false> {
+  static __device__ __forceinline__ uint16_t cvt(float f) { return __builtin_bit_cast(uint16_t, (_Float16)f); }
+  static __device__ __forceinline__ float back(uint16_t b) { return (float)__builtin_bit_cast(_Float16, b); }
+  static __device__ __forceinline__ f32x4_t mma(u32x4_t a, u32x4_t b, f32x4_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(f16x8_t, a), __builtin_bit_cast(f16x8_t, b), c,
+                                                  0, 0, 0);
+  }
+};
+template <>
+struct DftTy<true> {
+  static __device__ __forceinline__ uint16_t cvt(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
+  static __device__ __forceinline__ float back(uint16_t b) { return __builtin_bit_cast(float, (unsigned int)b << 16); }
+  static __device__ __forceinline__ f32x4_t mma(u32x4_t a, u32x4_t b, f32x4_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8_t, a), __builtin_bit_cast(bf16x8_t, b),
+                                                   c, 0, 0, 0);
+  }
+};
+
+__device__ __forceinline__ u32x4_t ld16(const uint16_t* p) { return *reinterpret_cast<const u32x4_t*>(p); }
+
+// one 16x16 output tile: D[m0 + .][n0 + .] = sum_k A[m][k] B[n][k]
+template <bool BF, bool SPLIT>
+__device__ __forceinline__ f32x4_t dft_tile(const uint16_t* Ah, const uint16_t* Al, int lda, const uint16_t* Bh,
+                                            const uint16_t* Bl, int ldb, int m0, int n0, int kdim, int lane) {
+  f32x4_t acc = {0.f, 0.f, 0.f, 0.f};
+  const int r = lane & 15, kg = lane >> 4;
+  const uint16_t* ah = Ah + (long long)(m0 + r) * lda + 8 * kg;
+  const uint16_t* bh = Bh + (long long)(n0 + r) * ldb + 8 * kg;
+  const uint16_t* al = Al + (long long)(m0 + r) * lda + 8 * kg;
+  const uint16_t* bl = Bl + (long long)(n0 + r) * ldb + 8 * kg;
+  for (int k0 = 0; k0 < kdim; k0 += 32) {
+    const u32x4_t a = ld16(ah + k0), b = ld16(bh + k0);
+    acc = DftTy<BF>::mma(a, b, acc);
+    if constexpr (SPLIT) {
+      acc = DftTy<BF>::mma(a, ld16(bl + k0), acc);
+      acc = DftTy<BF>::mma(ld16(al + k0), b, acc);
+    }
+  }
+  return acc;
+}
+
+template <bool BF>
+__device__ __forceinline__ void dft_put(uint16_t* hi, uint16_t* lo, int i, float v) {
+  const uint16_t h = DftTy<BF>::cvt(v);
+  hi[i] = h;
+  lo[i] = DftTy<BF>::cvt(v - DftTy<BF>::back(h));
+}
+
+// One workgroup (4 waves) per window; MAXW = the largest window side of the
+// size class (LDS sized for it), the window W x H (multiples of 8, <= MAXW)
+// at run time.
+template <int MAXW, bool BF, bool SPLIT>
+__global__ void __launch_bounds__(256) k_xcorr_dft(XcArgs a, DftTables tb, int W, int H) {
+  if (a.gate && *a.gate == 0) return;
+  constexpr int MK = 2 * MAXW + 8;  // largest padded row stride
+  __shared__ __align__(16) uint16_t Bh[MAXW * MK], Bl[MAXW * MK];  // B-layout data: z, then P
+  __shared__ __align__(16) uint16_t Ah[MAXW * MK], Al[MAXW * MK];  // A-layout data: T, then U
+  __shared__ __align__(16) float Zf[MAXW * MAXW * 2];              // Z (complex), then cc + cand
+  __shared__ float red_v[8];
+  __shared__ int red_i[8];
+  __shared__ float smax[4];
+  const int w = blockIdx.x;
+  if (w >= a.nx_vec * a.ny_vec) return;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int iy = w / a.nx_vec, ix = w - (w / a.nx_vec) * a.nx_vec;
+  const int PW = (W + 15) / 16 * 16, PH = (H + 15) / 16 * 16;
+  const int KH = 2 * PH, KW = 2 * PW, LB = KH + 8, LA = KW + 8;
+  const int N = W * H;
+  const int tiles = (PH / 16) * (PW / 16);
+  // 0. z into the B layout: column c holds [zr[0..H) | pad | zi[0..H) | pad]
+  for (int e = tid; e < PW * LB; e += 256) Bh[e] = Bl[e] = 0;
+  __syncthreads();
+  for (int e = tid; e < N; e += 256) {
+    const int r = e / W, c = e - (e / W) * W;
+    const float2 z = gather_px(a, ix * a.sp_x + c, iy * a.sp_y + r);
+    dft_put<BF>(Bh, Bl, c * LB + r, z.x);
+    dft_put<BF>(Bh, Bl, c * LB + PH + r, z.y);
+  }
+  __syncthreads();
+  // 1. [Tr | Ti] = A1[o] * z  -> A layout, row k = [Tr[k][0..PW) | Ti[k][0..PW)]
+  for (int t = wave; t < 2 * tiles; t += 4) {
+    const int o = t / tiles, q = t - o * tiles, mt = q / (PW / 16), nt = q - mt * (PW / 16);
+    const f32x4_t d = dft_tile<BF, SPLIT>(tb.a1h[o], tb.a1l[o], KH, Bh, Bl, LB, mt * 16, nt * 16, KH, lane);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int k = mt * 16 + 4 * (lane >> 4) + j, c = nt * 16 + (lane & 15);
+      dft_put<BF>(Ah, Al, k * LA + o * PW + c, d[j]);
+    }
+  }
+  __syncthreads();
+  // 2. [Zr, Zi] = T * B2[o] -> Zf[(k * W + l) * 2 + o]
+  for (int t = wave; t < 2 * tiles; t += 4) {
+    const int o = t / tiles, q = t - o * tiles, mt = q / (PW / 16), nt = q - mt * (PW / 16);
+    const f32x4_t d = dft_tile<BF, SPLIT>(Ah, Al, LA, tb.b2h[o], tb.b2l[o], KW, mt * 16, nt * 16, KW, lane);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int k = mt * 16 + 4 * (lane >> 4) + j, l = nt * 16 + (lane & 15);
+      if (k < H && l < W) Zf[(k * W + l) * 2 + o] = d[j];
+    }
+  }
+  __syncthreads();
+  // 3. P = conj(F1) F2 (4x, folded into the final scale), scaled by 2^-s so
+  //    that |P| <= 256: the next products stay inside the FP16 range
+  const float2* Z = reinterpret_cast<const float2*>(Zf);
+  float mx = 0.f;
+  for (int e = tid; e < N; e += 256) {
+    const int r = e / W, c = e - (e / W) * W;
+    const int rm = (H - r) % H, cm = (W - c) % W;
+    const float2 P = xprod(Z[e], Z[rm * W + cm]);
+    mx = fmaxf(mx, fmaxf(fabsf(P.x), fabsf(P.y)));
+  }
+  for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off, 64));
+  if (lane == 0) smax[wave] = mx;
+  __syncthreads();
+  mx = fmaxf(fmaxf(smax[0], smax[1]), fmaxf(smax[2], smax[3]));
+  int ex = 0;
+  if (mx > 0.f) (void)frexpf(mx, &ex);  // mx < 2^ex
+  const int sh = 8 - ex;                 // |P| * 2^sh < 256
+  for (int e = tid; e < N; e += 256) {
+    const int r = e / W, c = e - (e / W) * W;
+    const int rm = (H - r) % H, cm = (W - c) % W;
+    const float2 P = xprod(Z[e], Z[rm * W + cm]);
+    dft_put<BF>(Bh, Bl, c * LB + r, ldexpf(P.x, sh));
+    dft_put<BF>(Bh, Bl, c * LB + PH + r, ldexpf(P.y, sh));
+  }
+  __syncthreads();
+  // 4. [Ur | Ui] = A3[o] * P -> A layout
+  for (int t = wave; t < 2 * tiles; t += 4) {
+    const int o = t / tiles, q = t - o * tiles, mt = q / (PW / 16), nt = q - mt * (PW / 16);
+    const f32x4_t d = dft_tile<BF, SPLIT>(tb.a3h[o], tb.a3l[o], KH, Bh, Bl, LB, mt * 16, nt * 16, KH, lane);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int k = mt * 16 + 4 * (lane >> 4) + j, c = nt * 16 + (lane & 15);
+      dft_put<BF>(Ah, Al, k * LA + o * PW + c, d[j]);
+    }
+  }
+  __syncthreads();
+  // 5. cc = Re(U F_W^H), |.| / N (the exact correlation is real and >= 0),
+  //    written fftshifted
+  float* cc = Zf;
+  float* cand = Zf + N;
+  const float scale = ldexpf(0.25f / (float)N, -sh);
+  for (int t = wave; t < tiles; t += 4) {
+    const int mt = t / (PW / 16), nt = t - mt * (PW / 16);
+    const f32x4_t d = dft_tile<BF, SPLIT>(Ah, Al, LA, tb.b4h, tb.b4l, KW, mt * 16, nt * 16, KW, lane);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int r = mt * 16 + 4 * (lane >> 4) + j, c = nt * 16 + (lane & 15);
+      if (r < H && c < W) cc[((r + H / 2) % H) * W + (c + W / 2) % W] = fabsf(d[j]) * scale;
+    }
+  }
+  __syncthreads();
+  plane_peaks(a, w, cc, cand, W, H, red_v, red_i, true);
 }
 
 // ============================================================== deformation
@@ -1610,6 +1834,22 @@ __device__ __forceinline__ float median_of(float (&s)[NS], int n) {
   return select_sorted<NS>(s, (n - 1) / 2);
 }
 
+// The candidate test of median_filtering (cu:237-248): both normalised
+// residuals |d - median| / (residual median + eps) below the threshold.  With
+// a.fp64 (the precision form) the normalisation is evaluated in FP64.
+__device__ __forceinline__ bool median_accepts(const MedArgs& a, float cx, float cy, float mx, float my, float rx,
+                                               float ry) {
+#pragma clang fp contract(off)
+  if (a.fp64) {
+    const double dx = fabs((double)cx - (double)mx) / ((double)rx + (double)a.eps);
+    const double dy = fabs((double)cy - (double)my) / ((double)ry + (double)a.eps);
+    return dx < (double)a.thr && dy < (double)a.thr;
+  }
+  const float devix = fabsf(cx - mx) / (rx + a.eps);
+  const float deviy = fabsf(cy - my) / (ry + a.eps);
+  return devix < a.thr && deviy < a.thr;
+}
+
 __device__ __forceinline__ void median_decide(const MedArgs& a, int t, bool vmed, float mx, float my, float rx,
                                               float ry) {
 #pragma clang fp contract(off)
@@ -1620,9 +1860,8 @@ __device__ __forceinline__ void median_decide(const MedArgs& a, int t, bool vmed
     for (int k = 0; k < a.K; ++k) {
       const long long o = (long long)t * a.K + k;
       if (!a.cv[o]) continue;
-      const float devix = fabsf(a.cx[o] - mx) / (rx + a.eps);
-      const float deviy = fabsf(a.cy[o] - my) / (ry + a.eps);
-      if (devix < a.thr && deviy < a.thr) {
+      if (!median_accepts(a, a.cx[o], a.cy[o], mx, my, rx, ry)) continue;
+      {
         ok = true;
         ox = a.cx[o];
         oy = a.cy[o];
@@ -1784,9 +2023,7 @@ __global__ void __launch_bounds__(256) k_median2(MedArgs a) {
       const bool cvk = a.cv[o] != 0;
       const float cxk = a.cx[o], cyk = a.cy[o];
       if (!cvk) continue;
-      const float devix = fabsf(cxk - mx) / (rx + a.eps);
-      const float deviy = fabsf(cyk - my) / (ry + a.eps);
-      if (devix < a.thr && deviy < a.thr) {
+      if (median_accepts(a, cxk, cyk, mx, my, rx, ry)) {
         okc = true;
         ox = cxk;
         oy = cyk;
@@ -2293,6 +2530,26 @@ hipError_t launch_xcorr(const XcArgs& a, int ws_x, int ws_y, float2* gscratch, s
     if (e != hipSuccess) return e;
   }
   return hipSuccess;
+}
+
+hipError_t launch_xcorr_dft(const XcArgs& a, int ws_x, int ws_y, int mode, const DftTables& t, hipStream_t s) {
+  if (ws_x % 8 || ws_y % 8 || ws_x > 64 || ws_y > 64 || mode < 1 || mode > 4) return hipErrorInvalidValue;
+  const dim3 g((unsigned)(a.nx_vec * a.ny_vec)), b(256);
+  const bool small = ws_x <= 32 && ws_y <= 32;
+#define PIVX_DFT(MAXW)                                                                                 \
+  switch (mode) {                                                                                      \
+    case 1: hipLaunchKernelGGL((k_xcorr_dft<MAXW, false, false>), g, b, 0, s, a, t, ws_x, ws_y); break; \
+    case 2: hipLaunchKernelGGL((k_xcorr_dft<MAXW, true, false>), g, b, 0, s, a, t, ws_x, ws_y); break;  \
+    case 3: hipLaunchKernelGGL((k_xcorr_dft<MAXW, false, true>), g, b, 0, s, a, t, ws_x, ws_y); break;  \
+    default: hipLaunchKernelGGL((k_xcorr_dft<MAXW, true, true>), g, b, 0, s, a, t, ws_x, ws_y); break;  \
+  }
+  if (small) {
+    PIVX_DFT(32)
+  } else {
+    PIVX_DFT(64)
+  }
+#undef PIVX_DFT
+  return hipGetLastError();
 }
 
 hipError_t launch_deform(const DefArgs& a, hipStream_t s) {
