@@ -32,6 +32,7 @@ F_WIN = {(32, 32): 3 * 5 * 1024 * 10 + 10 * 1024, (64, 64): 3 * 5 * 4096 * 12 + 
 PEAK_FP32_TFLOPS = 157.3          # MI355X FP32 vector peak, which needs packed FP32 (v_pk_fma_f32)
 UNPACKED_FP32_TFLOPS = 78.6       # the same pipe issuing one FP32 FMA per lane per cycle (this build: no packed FP32)
 PEAK_HBM_GBS = 8000.0
+PEAK_BF16_TFLOPS = 2500.0         # dense FP16 / BF16 matrix-core peak
 
 
 def load_pivx():
@@ -151,6 +152,69 @@ def cli_leg(pivx, npairs, timeout_s=300):
         return out
     finally:
         shutil.rmtree(d, ignore_errors=True)
+
+
+def dft_mfma_flops(wx, wy, split):
+    """Matrix-core flops one window of the DFT-by-matrix form executes:
+    16x16x32 tiles over the padded matrices, three products per tile step
+    with hi+lo operands."""
+    pw, ph = (wx + 15) // 16 * 16, (wy + 15) // 16 * 16
+    tiles = (pw // 16) * (ph // 16)
+    steps = 2 * tiles * (2 * ph // 32) + 2 * tiles * (2 * pw // 32) + 2 * tiles * (2 * ph // 32) \
+        + tiles * (2 * pw // 32)
+    return steps * (3 if split else 1) * 16 * 16 * 32 * 2
+
+
+def precision_leg(pivx, a, b, reps):
+    """Config 3's precision form at config-2 scale: one undeformed pass of
+    113,300 32x32 windows (spacing 16) and one of 7,004 64x64 windows
+    (spacing 64) over the 6600x4400 pair, per correlation mode; kernel time
+    from HIP events, windows/s, matrix-core TFLOP/s executed, and agreement
+    with the FP32 FFT path (the FP64-oracle errors are stated by the parity
+    tests)."""
+    import numpy as np
+
+    names = {0: "fp32_fft", 1: "fp16", 2: "bf16", 3: "fp16_hilo", 4: "bf16_hilo"}
+    out = {}
+    for ws, sp in (((32, 32), (16, 16)), ((64, 64), (64, 64))):
+        ip = pivx.init_params(nx=6600, ny=4400, ws_x1=ws[0], ws_y1=ws[1], spacing_x1=sp[0], spacing_y1=sp[1],
+                              npass1=1, npass2=0, max_dispx_percent1=0.4, max_dispy_percent1=0.4)
+        key = f"{ws[0]}x{ws[1]}"
+        out[key] = {}
+        with pivx.Xcorr2dMulti(ip) as x:
+            x.prep_img_data(a, b)
+            nt = x.geometry(1).nt_vec
+            base = None
+            for mode in range(5):
+                x.set_correlation_precision(mode)
+                x.Calc_cc_map_p1(0)  # warm-up
+                x.enable_timing(True)
+                for _ in range(reps):
+                    x.Calc_cc_map_p1(0)
+                ms, _ = x.stage_ms(1, 1)
+                x.enable_timing(False)
+                val, idx, _ = x.peaks(1)
+                dx, dy, v = x.raw(1)
+                r = {"ms": round(ms, 4), "windows_per_s": round(nt / (ms * 1e-3), 1)}
+                if mode == 0:
+                    base = (idx, dx, dy, v)
+                    r["fp32_tflops_alg"] = round(F_WIN[ws] * nt / (ms * 1e-3) / 1e12, 2)
+                else:
+                    fl = dft_mfma_flops(ws[0], ws[1], mode >= 3)
+                    tf = fl * nt / (ms * 1e-3) / 1e12
+                    r["mfma_tflops_executed"] = round(tf, 1)
+                    r["mfma_frac_of_dense_peak"] = round(tf / PEAK_BF16_TFLOPS, 4)
+                    both = (v == 1) & (base[3] == 1) & (idx == base[0])
+                    r["vs_fp32_fft"] = {
+                        "first_peak_index_agree": round(float((idx[:, 0] == base[0][:, 0]).mean()), 5),
+                        "all_peaks_agree": round(float((idx == base[0]).all(axis=1).mean()), 5),
+                        "max_subpix_diff_px": round(float(np.maximum(np.abs(dx - base[1]),
+                                                                     np.abs(dy - base[2]))[both].max()), 6)}
+                out[key][names[mode]] = r
+            x.set_correlation_precision(0)
+        print(f"[bench] precision {key}: " + ", ".join(f"{k} {v['ms']} ms" for k, v in out[key].items()),
+              file=sys.stderr, flush=True)
+    return out
 
 
 def learned_leg(pivx, npairs, epochs, neval, nspeed, pipelines):
@@ -313,6 +377,7 @@ def main():
     ap.add_argument("--cli-pairs", type=int, default=16, help="pairs for the end-to-end CLI leg (0 = skip)")
     ap.add_argument("--pipelines", type=int, default=4,
                     help="engine contexts (each its own stream and pair graph) the pairs are spread over")
+    ap.add_argument("--precision-reps", type=int, default=10, help="timed passes per precision mode (0 = skip)")
     ap.add_argument("--learned-pairs", type=int, default=2000, help="training pairs of the learned leg (0 = skip)")
     ap.add_argument("--learned-epochs", type=int, default=16)
     ap.add_argument("--learned-eval", type=int, default=200, help="held-out pairs of the learned leg")
@@ -479,6 +544,8 @@ def main():
     }
     if args.cli_pairs > 0:
         out["end_to_end_cli"] = cli_leg(pivx, args.cli_pairs)
+    if args.precision_reps > 0:
+        out["precision_form"] = precision_leg(pivx, a, b, args.precision_reps)
     if args.learned_pairs > 0:
         out["learned"] = learned_leg(pivx, args.learned_pairs, args.learned_epochs, args.learned_eval,
                                      args.learned_speed_pairs, max(1, args.pipelines))

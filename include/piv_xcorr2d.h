@@ -254,8 +254,9 @@ int pivx_set_fused_deform(pivx_ctx* ctx, int mode);
  * (three products each, about FP32 accuracy); FP32 accumulation in all.
  * Modes 1-4 also evaluate the Gaussian fit of nearly flat peaks and the
  * median test's normalised residuals in FP64.  Windows of both grids must be
- * at most 64x64 with sides multiples of 8.  The fused grid-1 deformation is
- * not used while a mode other than 0 is set. */
+ * one of 16x16, 24x24, 32x32, 48x48, 64x64, 64x32, 32x64, 32x16, 16x32 or
+ * 48x24.  The fused grid-1 deformation is not used while a mode other than 0
+ * is set. */
 int pivx_set_correlation_precision(pivx_ctx* ctx, int mode);
 
 /* Wait for all work of the handle. */

@@ -1327,10 +1327,10 @@ int pivx_set_correlation_precision(pivx_ctx* c, int mode) {
   if (mode != 0) {
     for (int gi = 0; gi < 2; ++gi) {
       const pivx_geometry& g = c->grid[gi].g;
-      if (g.ws_x % 8 || g.ws_y % 8 || g.ws_x > 64 || g.ws_y > 64)
+      if (!xcorr_dft_supported(g.ws_x, g.ws_y))
         return fail(c, PIVX_ERR_PARAM,
-                    "the DFT-by-matrix precision form covers windows up to 64x64 with sides multiples of 8 "
-                    "(grid %d: %dx%d)", gi + 1, g.ws_x, g.ws_y);
+                    "the DFT-by-matrix precision form is compiled for windows 16x16, 24x24, 32x32, 48x48, 64x64, "
+                    "64x32, 32x64, 32x16, 16x32 and 48x24 (grid %d: %dx%d)", gi + 1, g.ws_x, g.ws_y);
     }
     for (int gi = 0; gi < 2; ++gi) {
       const int rc = build_dft_tables(c, gi, mode == 2 || mode == 4);

@@ -121,7 +121,8 @@ struct DftTables {
   const uint16_t* a3h[2]; const uint16_t* a3l[2];   // [C -S], [S C]   (step 3: Ur, Ui)
   const uint16_t* b4h; const uint16_t* b4l;         // [C; -S]         (step 4: cc)
 };
-// mode 1 FP16, 2 BF16, 3 FP16 hi+lo, 4 BF16 hi+lo
+// mode 1 FP16, 2 BF16, 3 FP16 hi+lo, 4 BF16 hi+lo; shapes: xcorr_dft_supported
+bool xcorr_dft_supported(int ws_x, int ws_y);
 hipError_t launch_xcorr_dft(const XcArgs& a, int ws_x, int ws_y, int mode, const DftTables& t, hipStream_t s);
 hipError_t launch_deform(const DefArgs& a, hipStream_t s);
 // The per-column and per-row interpolation terms of the deformation (as

@@ -1357,8 +1357,10 @@ __device__ void block_argmax(float& bv, int& bi, float* sv, int* si) {
 // (cu:585-606) into cand, the uncapped count, Npeak_save rounds of arg-max
 // (value desc, raster index asc) and the Gaussian fit of each peak.  fit64:
 // the FP64 fit of nearly flat peaks (subpixel_fit_fp64near).
-__device__ void plane_peaks(const XcArgs& a, int w, float* cc, float* cand, int WX, int WY, float* red_v, int* red_i,
-                            bool fit64) {
+template <int TWX = 0, int TWY = 0>  // compile-time window sides, or 0: the run-time rwx, rwy
+__device__ void plane_peaks(const XcArgs& a, int w, float* cc, float* cand, int rwx, int rwy, float* red_v,
+                            int* red_i, bool fit64) {
+  const int WX = TWX ? TWX : rwx, WY = TWY ? TWY : rwy;
   const int N = WX * WY;
   int cnt = 0;
   for (int e = threadIdx.x; e < N; e += blockDim.x) {
@@ -1536,16 +1538,18 @@ __device__ __forceinline__ void dft_put(uint16_t* hi, uint16_t* lo, int i, float
   lo[i] = DftTy<BF>::cvt(v - DftTy<BF>::back(h));
 }
 
-// One workgroup (4 waves) per window; MAXW = the largest window side of the
-// size class (LDS sized for it), the window W x H (multiples of 8, <= MAXW)
-// at run time.
-template <int MAXW, bool BF, bool SPLIT>
-__global__ void __launch_bounds__(256) k_xcorr_dft(XcArgs a, DftTables tb, int W, int H) {
+// One workgroup (4 waves) per window of W x H (multiples of 8, at most 64).
+template <int W, int H, bool BF>
+__global__ void __launch_bounds__(256) k_xcorr_dft(XcArgs a, DftTables tb, int split) {
   if (a.gate && *a.gate == 0) return;
-  constexpr int MK = 2 * MAXW + 8;  // largest padded row stride
-  __shared__ __align__(16) uint16_t Bh[MAXW * MK], Bl[MAXW * MK];  // B-layout data: z, then P
-  __shared__ __align__(16) uint16_t Ah[MAXW * MK], Al[MAXW * MK];  // A-layout data: T, then U
-  __shared__ __align__(16) float Zf[MAXW * MAXW * 2];              // Z (complex), then cc + cand
+  constexpr int PW = (W + 15) / 16 * 16, PH = (H + 15) / 16 * 16;
+  constexpr int KH = 2 * PH, KW = 2 * PW, LB = KH + 8, LA = KW + 8;
+  constexpr int N = W * H;
+  constexpr int tiles = (PH / 16) * (PW / 16);
+  constexpr int PER = (N + 255) / 256;  // plane elements per thread
+  __shared__ __align__(16) uint16_t Bh[PW * LB], Bl[PW * LB];  // B-layout data: z, then P
+  __shared__ __align__(16) uint16_t Ah[PH * LA], Al[PH * LA];  // A-layout data: T, then U
+  __shared__ __align__(16) float Zf[N * 2];                    // Z (complex), then cc + cand
   __shared__ float red_v[8];
   __shared__ int red_i[8];
   __shared__ float smax[4];
@@ -1553,13 +1557,11 @@ __global__ void __launch_bounds__(256) k_xcorr_dft(XcArgs a, DftTables tb, int W
   if (w >= a.nx_vec * a.ny_vec) return;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int iy = w / a.nx_vec, ix = w - (w / a.nx_vec) * a.nx_vec;
-  const int PW = (W + 15) / 16 * 16, PH = (H + 15) / 16 * 16;
-  const int KH = 2 * PH, KW = 2 * PW, LB = KH + 8, LA = KW + 8;
-  const int N = W * H;
-  const int tiles = (PH / 16) * (PW / 16);
   // 0. z into the B layout: column c holds [zr[0..H) | pad | zi[0..H) | pad]
-  for (int e = tid; e < PW * LB; e += 256) Bh[e] = Bl[e] = 0;
-  __syncthreads();
+  if constexpr (PW != W || PH != H) {
+    for (int e = tid; e < PW * LB; e += 256) Bh[e] = Bl[e] = 0;
+    __syncthreads();
+  }
   for (int e = tid; e < N; e += 256) {
     const int r = e / W, c = e - (e / W) * W;
     const float2 z = gather_px(a, ix * a.sp_x + c, iy * a.sp_y + r);
@@ -1570,7 +1572,8 @@ __global__ void __launch_bounds__(256) k_xcorr_dft(XcArgs a, DftTables tb, int W
   // 1. [Tr | Ti] = A1[o] * z  -> A layout, row k = [Tr[k][0..PW) | Ti[k][0..PW)]
   for (int t = wave; t < 2 * tiles; t += 4) {
     const int o = t / tiles, q = t - o * tiles, mt = q / (PW / 16), nt = q - mt * (PW / 16);
-    const f32x4_t d = dft_tile<BF, SPLIT>(tb.a1h[o], tb.a1l[o], KH, Bh, Bl, LB, mt * 16, nt * 16, KH, lane);
+    const f32x4_t d = split ? dft_tile<BF, true>(tb.a1h[o], tb.a1l[o], KH, Bh, Bl, LB, mt * 16, nt * 16, KH, lane)
+                            : dft_tile<BF, false>(tb.a1h[o], tb.a1l[o], KH, Bh, Bl, LB, mt * 16, nt * 16, KH, lane);
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const int k = mt * 16 + 4 * (lane >> 4) + j, c = nt * 16 + (lane & 15);
@@ -1581,7 +1584,8 @@ __global__ void __launch_bounds__(256) k_xcorr_dft(XcArgs a, DftTables tb, int W
   // 2. [Zr, Zi] = T * B2[o] -> Zf[(k * W + l) * 2 + o]
   for (int t = wave; t < 2 * tiles; t += 4) {
     const int o = t / tiles, q = t - o * tiles, mt = q / (PW / 16), nt = q - mt * (PW / 16);
-    const f32x4_t d = dft_tile<BF, SPLIT>(Ah, Al, LA, tb.b2h[o], tb.b2l[o], KW, mt * 16, nt * 16, KW, lane);
+    const f32x4_t d = split ? dft_tile<BF, true>(Ah, Al, LA, tb.b2h[o], tb.b2l[o], KW, mt * 16, nt * 16, KW, lane)
+                            : dft_tile<BF, false>(Ah, Al, LA, tb.b2h[o], tb.b2l[o], KW, mt * 16, nt * 16, KW, lane);
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const int k = mt * 16 + 4 * (lane >> 4) + j, l = nt * 16 + (lane & 15);
@@ -1593,11 +1597,16 @@ __global__ void __launch_bounds__(256) k_xcorr_dft(XcArgs a, DftTables tb, int W
   //    that |P| <= 256: the next products stay inside the FP16 range
   const float2* Z = reinterpret_cast<const float2*>(Zf);
   float mx = 0.f;
-  for (int e = tid; e < N; e += 256) {
-    const int r = e / W, c = e - (e / W) * W;
-    const int rm = (H - r) % H, cm = (W - c) % W;
-    const float2 P = xprod(Z[e], Z[rm * W + cm]);
-    mx = fmaxf(mx, fmaxf(fabsf(P.x), fabsf(P.y)));
+  float2 Pv[PER];
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const int e = tid + 256 * q;
+    if (e < N) {
+      const int r = e / W, c = e - (e / W) * W;
+      const int rm = r ? H - r : 0, cm = c ? W - c : 0;
+      Pv[q] = xprod(Z[e], Z[rm * W + cm]);
+      mx = fmaxf(mx, fmaxf(fabsf(Pv[q].x), fabsf(Pv[q].y)));
+    }
   }
   for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off, 64));
   if (lane == 0) smax[wave] = mx;
@@ -1606,18 +1615,22 @@ __global__ void __launch_bounds__(256) k_xcorr_dft(XcArgs a, DftTables tb, int W
   int ex = 0;
   if (mx > 0.f) (void)frexpf(mx, &ex);  // mx < 2^ex
   const int sh = 8 - ex;                 // |P| * 2^sh < 256
-  for (int e = tid; e < N; e += 256) {
-    const int r = e / W, c = e - (e / W) * W;
-    const int rm = (H - r) % H, cm = (W - c) % W;
-    const float2 P = xprod(Z[e], Z[rm * W + cm]);
-    dft_put<BF>(Bh, Bl, c * LB + r, ldexpf(P.x, sh));
-    dft_put<BF>(Bh, Bl, c * LB + PH + r, ldexpf(P.y, sh));
+  __syncthreads();  // every thread has read Z before the B layout is rewritten
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const int e = tid + 256 * q;
+    if (e < N) {
+      const int r = e / W, c = e - (e / W) * W;
+      dft_put<BF>(Bh, Bl, c * LB + r, ldexpf(Pv[q].x, sh));
+      dft_put<BF>(Bh, Bl, c * LB + PH + r, ldexpf(Pv[q].y, sh));
+    }
   }
   __syncthreads();
   // 4. [Ur | Ui] = A3[o] * P -> A layout
   for (int t = wave; t < 2 * tiles; t += 4) {
     const int o = t / tiles, q = t - o * tiles, mt = q / (PW / 16), nt = q - mt * (PW / 16);
-    const f32x4_t d = dft_tile<BF, SPLIT>(tb.a3h[o], tb.a3l[o], KH, Bh, Bl, LB, mt * 16, nt * 16, KH, lane);
+    const f32x4_t d = split ? dft_tile<BF, true>(tb.a3h[o], tb.a3l[o], KH, Bh, Bl, LB, mt * 16, nt * 16, KH, lane)
+                            : dft_tile<BF, false>(tb.a3h[o], tb.a3l[o], KH, Bh, Bl, LB, mt * 16, nt * 16, KH, lane);
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const int k = mt * 16 + 4 * (lane >> 4) + j, c = nt * 16 + (lane & 15);
@@ -1632,15 +1645,19 @@ __global__ void __launch_bounds__(256) k_xcorr_dft(XcArgs a, DftTables tb, int W
   const float scale = ldexpf(0.25f / (float)N, -sh);
   for (int t = wave; t < tiles; t += 4) {
     const int mt = t / (PW / 16), nt = t - mt * (PW / 16);
-    const f32x4_t d = dft_tile<BF, SPLIT>(Ah, Al, LA, tb.b4h, tb.b4l, KW, mt * 16, nt * 16, KW, lane);
+    const f32x4_t d = split ? dft_tile<BF, true>(Ah, Al, LA, tb.b4h, tb.b4l, KW, mt * 16, nt * 16, KW, lane)
+                            : dft_tile<BF, false>(Ah, Al, LA, tb.b4h, tb.b4l, KW, mt * 16, nt * 16, KW, lane);
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const int r = mt * 16 + 4 * (lane >> 4) + j, c = nt * 16 + (lane & 15);
-      if (r < H && c < W) cc[((r + H / 2) % H) * W + (c + W / 2) % W] = fabsf(d[j]) * scale;
+      if (r < H && c < W) {
+        const int rs = r < H / 2 ? r + H / 2 : r - H / 2, cs = c < W / 2 ? c + W / 2 : c - W / 2;
+        cc[rs * W + cs] = fabsf(d[j]) * scale;
+      }
     }
   }
   __syncthreads();
-  plane_peaks(a, w, cc, cand, W, H, red_v, red_i, true);
+  plane_peaks<W, H>(a, w, cc, cand, W, H, red_v, red_i, true);
 }
 
 // ============================================================== deformation
@@ -2532,24 +2549,33 @@ hipError_t launch_xcorr(const XcArgs& a, int ws_x, int ws_y, float2* gscratch, s
   return hipSuccess;
 }
 
-hipError_t launch_xcorr_dft(const XcArgs& a, int ws_x, int ws_y, int mode, const DftTables& t, hipStream_t s) {
-  if (ws_x % 8 || ws_y % 8 || ws_x > 64 || ws_y > 64 || mode < 1 || mode > 4) return hipErrorInvalidValue;
+template <int W, int H>
+static hipError_t launch_dft_shape(const XcArgs& a, int mode, const DftTables& t, hipStream_t s) {
   const dim3 g((unsigned)(a.nx_vec * a.ny_vec)), b(256);
-  const bool small = ws_x <= 32 && ws_y <= 32;
-#define PIVX_DFT(MAXW)                                                                                 \
-  switch (mode) {                                                                                      \
-    case 1: hipLaunchKernelGGL((k_xcorr_dft<MAXW, false, false>), g, b, 0, s, a, t, ws_x, ws_y); break; \
-    case 2: hipLaunchKernelGGL((k_xcorr_dft<MAXW, true, false>), g, b, 0, s, a, t, ws_x, ws_y); break;  \
-    case 3: hipLaunchKernelGGL((k_xcorr_dft<MAXW, false, true>), g, b, 0, s, a, t, ws_x, ws_y); break;  \
-    default: hipLaunchKernelGGL((k_xcorr_dft<MAXW, true, true>), g, b, 0, s, a, t, ws_x, ws_y); break;  \
-  }
-  if (small) {
-    PIVX_DFT(32)
-  } else {
-    PIVX_DFT(64)
-  }
-#undef PIVX_DFT
+  const int split = mode >= 3 ? 1 : 0;
+  if (mode == 2 || mode == 4) hipLaunchKernelGGL((k_xcorr_dft<W, H, true>), g, b, 0, s, a, t, split);
+  else hipLaunchKernelGGL((k_xcorr_dft<W, H, false>), g, b, 0, s, a, t, split);
   return hipGetLastError();
+}
+
+// window shapes of the precision form (sides multiples of 8 up to 64)
+#define PIVX_DFT_SIZES(X) X(32, 32) X(64, 64) X(16, 16) X(24, 24) X(48, 48) X(64, 32) X(32, 64) X(32, 16) X(16, 32) X(48, 24)
+
+bool xcorr_dft_supported(int ws_x, int ws_y) {
+#define PIVX_CASE(WX, WY) \
+  if (ws_x == WX && ws_y == WY) return true;
+  PIVX_DFT_SIZES(PIVX_CASE)
+#undef PIVX_CASE
+  return false;
+}
+
+hipError_t launch_xcorr_dft(const XcArgs& a, int ws_x, int ws_y, int mode, const DftTables& t, hipStream_t s) {
+  if (mode < 1 || mode > 4) return hipErrorInvalidValue;
+#define PIVX_CASE(WX, WY) \
+  if (ws_x == WX && ws_y == WY) return launch_dft_shape<WX, WY>(a, mode, t, s);
+  PIVX_DFT_SIZES(PIVX_CASE)
+#undef PIVX_CASE
+  return hipErrorInvalidValue;
 }
 
 hipError_t launch_deform(const DefArgs& a, hipStream_t s) {
