@@ -443,6 +443,34 @@ int pivx_run_pair_learned(pivx_ctx* ctx, const pivx_cnn* net, const pivx_filter_
  * median test, and 1 when the plain grid-1 passes ran instead. */
 int pivx_get_learned_stats(pivx_ctx* ctx, int* outliers, int* fallback);
 
+/* ------------------------------------------------------------------ batched passes
+ * No reference counterpart (the reference runs one pair at a time,
+ * main.cpp:248-380).  A batch runs the pivx_run_pair sequence of many handles
+ * (pairs of uneven frame size) at once: each correlation pass is one
+ * persistent work-queue launch over the windows of all pairs, on the batch's
+ * own stream, and the per-pair stages run on the handles' streams, joined by
+ * events.  Results are bit-identical to pivx_run_pair on each handle.  The
+ * handles keep their buffers and getters; all must share the device, window
+ * sizes of both grids (a shape with a batched kernel: the wave-pair shapes
+ * 32x32, 24x24, 32x16, 16x32 and the wave-per-window shapes 64x64, 48x48,
+ * 64x32, 32x64, 48x24), pass counts and Npeak_save, and use the FP32
+ * correlation.  While a batch exists its handles must not run other work. */
+typedef struct pivx_batch pivx_batch;
+int pivx_batch_create(pivx_ctx* const* ctxs, int n, pivx_batch** out);
+int pivx_batch_destroy(pivx_batch* b);
+/* Graph mode: the whole batch sequence captured once into one hipGraph and
+ * replayed (re-captured when the filter parameters change). */
+int pivx_batch_set_graph_mode(pivx_batch* b, int on);
+int pivx_batch_run(pivx_batch* b, const pivx_filter_params* grid1, const pivx_filter_params* grid2);
+int pivx_batch_synchronize(pivx_batch* b);
+/* Autotuner: times `reps` correlation passes of each grid for every
+ * persistent grid size (4, 8, 16, 32 waves per compute unit) and claim size
+ * (1, 4, 16 work items) and keeps the fastest.  Overwrites the handles' raw
+ * candidates; run it before the real passes. */
+int pivx_batch_autotune(pivx_batch* b, int reps);
+int pivx_batch_get_tuning(const pivx_batch* b, int grid, int* waves_per_cu, int* items_per_claim, double* ms,
+                          long long* items);
+
 #ifdef __cplusplus
 }
 #endif

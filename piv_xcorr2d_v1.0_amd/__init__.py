@@ -165,6 +165,14 @@ _SIGS = {
     "pivx_cnn_train": [_P, _P, C.POINTER(CnnTrainParams), _P, C.POINTER(CnnTrainReport)],
     "pivx_run_pair_learned": [_P, _P, C.POINTER(FilterParams), C.POINTER(FilterParams), C.c_float],
     "pivx_get_learned_stats": [_P, C.POINTER(C.c_int), C.POINTER(C.c_int)],
+    "pivx_batch_create": [_P, C.c_int, C.POINTER(_P)],
+    "pivx_batch_destroy": [_P],
+    "pivx_batch_set_graph_mode": [_P, C.c_int],
+    "pivx_batch_run": [_P, C.POINTER(FilterParams), C.POINTER(FilterParams)],
+    "pivx_batch_synchronize": [_P],
+    "pivx_batch_autotune": [_P, C.c_int],
+    "pivx_batch_get_tuning": [_P, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_double),
+                              C.POINTER(C.c_longlong)],
     "pivx_synth_flow": [C.POINTER(SynthParams), _P, _P, C.c_int, _P, _P],
 }
 _RESTYPE = {"pivx_last_error": C.c_char_p, "pivx_host_alloc": C.c_void_p, "pivx_host_free": None,
@@ -587,6 +595,52 @@ class Cnn:
         rep = CnnTrainReport()
         x._c(lib().pivx_cnn_train(x.h, self.h, C.byref(tp), _ptr(losses), C.byref(rep)))
         return losses, rep
+
+
+class Batch:
+    """Batched passes over many handles (pivx_batch_*): one persistent
+    correlation launch per pass over all pairs."""
+
+    def __init__(self, handles):
+        self.handles = list(handles)
+        arr = (C.c_void_p * len(self.handles))(*[x.h.value for x in self.handles])
+        h = C.c_void_p()
+        _check(lib().pivx_batch_create(arr, len(self.handles), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().pivx_batch_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_graph_mode(self, on=True):
+        _check(lib().pivx_batch_set_graph_mode(self.h, 1 if on else 0))
+
+    def run(self, f1: FilterParams, f2: FilterParams):
+        _check(lib().pivx_batch_run(self.h, C.byref(f1), C.byref(f2)))
+
+    def synchronize(self):
+        _check(lib().pivx_batch_synchronize(self.h))
+
+    def autotune(self, reps=3):
+        _check(lib().pivx_batch_autotune(self.h, reps))
+
+    def tuning(self, grid):
+        w, c, ms, n = C.c_int(), C.c_int(), C.c_double(), C.c_longlong()
+        _check(lib().pivx_batch_get_tuning(self.h, grid, C.byref(w), C.byref(c), C.byref(ms), C.byref(n)))
+        return dict(waves_per_cu=w.value, items_per_claim=c.value, ms=ms.value, items=n.value)
 
 
 class Results:

@@ -326,7 +326,9 @@ int run_deform(pivx_ctx* c, int gi) {
 // (cu:1287-1298).
 // With fused set, frame 2 is deformed inside the correlation kernel's gather
 // (FusedDef) and the deformed plane is not written.
-int run_xcorr(pivx_ctx* c, int gi, bool deformed, bool zero_pred = false, bool fused = false) {
+// Arguments of one correlation pass of grid gi (shared by the per-handle
+// launch and the batched launch over many handles).
+XcArgs make_xcargs(pivx_ctx* c, int gi, bool deformed, bool zero_pred, bool fused) {
   GridBuf& G = c->grid[gi];
   const pivx_geometry& g = G.g;
   XcArgs a{};
@@ -375,6 +377,13 @@ int run_xcorr(pivx_ctx* c, int gi, bool deformed, bool zero_pred = false, bool f
   a.pk_idx = G.pki;
   a.ncand = G.ncand;
   a.gate = c->gate;
+  return a;
+}
+
+int run_xcorr(pivx_ctx* c, int gi, bool deformed, bool zero_pred = false, bool fused = false) {
+  GridBuf& G = c->grid[gi];
+  const pivx_geometry& g = G.g;
+  const XcArgs a = make_xcargs(c, gi, deformed, zero_pred, fused);
   tic(c);
   hipError_t e;
   if (c->prec) e = launch_xcorr_dft(a, g.ws_x, g.ws_y, c->prec, c->dtab[gi], c->stream);
@@ -2328,6 +2337,362 @@ int pivx_get_learned_stats(pivx_ctx* c, int* outliers, int* fallback) {
   HIPCHK(c, hipStreamSynchronize(c->stream));
   if (outliers) *outliers = v[0];
   if (fallback) *fallback = v[1];
+  return PIVX_OK;
+}
+
+}  // extern "C"
+
+// ====================================================================== batched passes
+struct pivx_batch {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::vector<pivx_ctx*> ctx;
+  XcArgs* dargs[3] = {};  // pass kinds: grid-1 pass 0, grid-1 deformed pass, grid-2 pass
+  int* dstart[2] = {};    // work-item offsets per grid
+  int total[2] = {0, 0};
+  int* counter = nullptr;
+  int cus = 1;
+  int wgs[2] = {0, 0}, chunk[2] = {1, 1};
+  double tuned_ms[2] = {0.0, 0.0};
+  std::vector<hipEvent_t> ev;  // one per handle
+  hipEvent_t evb = nullptr;    // on the batch stream
+  bool use_graph = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  pivx_filter_params gf[2] = {};
+  std::vector<void*> allocs;
+};
+
+namespace {
+
+int batch_fail(const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  set_global_error(buf);
+  return PIVX_ERR_PARAM;
+}
+
+BatchQueue batch_queue(const pivx_batch* b, int kind) {
+  BatchQueue q{};
+  const int gi = kind == 2 ? 1 : 0;
+  q.args = b->dargs[kind];
+  q.item_start = b->dstart[gi];
+  q.npairs = (int)b->ctx.size();
+  q.total = b->total[gi];
+  q.counter = b->counter;
+  q.chunk = b->chunk[gi];
+  return q;
+}
+
+int batch_corr(pivx_batch* b, int kind) {
+  const int gi = kind == 2 ? 1 : 0;
+  const pivx_geometry& g = b->ctx[0]->grid[gi].g;
+  const hipError_t e = launch_xcorr_batch(g.ws_x, g.ws_y, batch_queue(b, kind), b->wgs[gi], b->stream);
+  if (e != hipSuccess) return batch_fail("batched correlation: %s", hipGetErrorString(e));
+  return PIVX_OK;
+}
+
+#define BCHK(expr)                                                                        \
+  do {                                                                                    \
+    const hipError_t e_ = (expr);                                                         \
+    if (e_ != hipSuccess) return batch_fail("%s: %s", #expr, hipGetErrorString(e_));      \
+  } while (0)
+
+// The per-pair sequence of pivx_run_pair for every handle, with each
+// correlation pass one persistent launch over all pairs on the batch stream;
+// the per-pair stages (deformation, validation, extension, upsampling) run on
+// the handles' own streams, joined to the batch stream by events.
+int batch_enqueue(pivx_batch* b, const pivx_filter_params* f1, const pivx_filter_params* f2) {
+  const size_t n = b->ctx.size();
+  const pivx_init_params& P = b->ctx[0]->p;
+  std::vector<char> ext_ok(n, 0);
+  int rc;
+  BCHK(hipEventRecord(b->evb, b->stream));
+  for (pivx_ctx* c : b->ctx) BCHK(hipStreamWaitEvent(c->stream, b->evb, 0));
+  auto join_to_batch = [&]() -> int {
+    for (size_t i = 0; i < n; ++i) {
+      BCHK(hipEventRecord(b->ev[i], b->ctx[i]->stream));
+      BCHK(hipStreamWaitEvent(b->stream, b->ev[i], 0));
+    }
+    return PIVX_OK;
+  };
+  auto fork_from_batch = [&]() -> int {
+    BCHK(hipEventRecord(b->evb, b->stream));
+    for (pivx_ctx* c : b->ctx) BCHK(hipStreamWaitEvent(c->stream, b->evb, 0));
+    return PIVX_OK;
+  };
+  for (int j = 0; j < P.npass1; ++j) {
+    for (size_t i = 0; i < n && j > 0; ++i) {
+      pivx_ctx* c = b->ctx[i];
+      if (!ext_ok[i] && (rc = extend(c, 0))) return rc;
+      if ((rc = run_deform(c, 0))) return rc;
+    }
+    if ((rc = join_to_batch())) return rc;
+    if ((rc = batch_corr(b, j == 0 ? 0 : 1))) return rc;
+    if ((rc = fork_from_batch())) return rc;
+    for (size_t i = 0; i < n; ++i) {
+      pivx_ctx* c = b->ctx[i];
+      GridBuf& G1 = c->grid[0];
+      bool ok = false;
+      if ((rc = run_filter(c, 0, f1->npass_median, f1->min_nneigh, f1->insert_thr, f1->npass_smooth, 1, G1.fx, G1.fy,
+                           G1.fv, G1.ex, G1.ey, &ok)))
+        return rc;
+      ext_ok[i] = ok;
+    }
+  }
+  if (P.npass2 > 0) {
+    for (size_t i = 0; i < n; ++i) {
+      pivx_ctx* c = b->ctx[i];
+      if (!ext_ok[i] && (rc = extend(c, 0))) return rc;
+      if ((rc = upsample(c, true))) return rc;
+      ext_ok[i] = 1;
+    }
+    for (int j = 0; j < P.npass2; ++j) {
+      for (size_t i = 0; i < n; ++i) {
+        pivx_ctx* c = b->ctx[i];
+        if (!ext_ok[i] && (rc = extend(c, 1))) return rc;
+        if ((rc = run_deform(c, 1))) return rc;
+      }
+      if ((rc = join_to_batch())) return rc;
+      if ((rc = batch_corr(b, 2))) return rc;
+      if ((rc = fork_from_batch())) return rc;
+      const bool last = j == P.npass2 - 1;
+      for (size_t i = 0; i < n; ++i) {
+        pivx_ctx* c = b->ctx[i];
+        GridBuf& G2 = c->grid[1];
+        bool ok = false;
+        if ((rc = run_filter(c, 1, f2->npass_median, f2->min_nneigh, f2->insert_thr, f2->npass_smooth, 1, G2.fx, G2.fy,
+                             G2.fv, G2.ex, G2.ey, &ok, last ? c->mo_x : nullptr, last ? c->mo_y : nullptr,
+                             last ? c->mo_v : nullptr)))
+          return rc;
+        ext_ok[i] = ok;
+      }
+    }
+  }
+  return join_to_batch();
+}
+
+void drop_batch_graph(pivx_batch* b) {
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->gexec = nullptr;
+  b->graph = nullptr;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pivx_batch_destroy(pivx_batch* b) {
+  if (!b) return PIVX_OK;
+  (void)hipSetDevice(b->device);
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  drop_batch_graph(b);
+  for (void* q : b->allocs) (void)hipFree(q);
+  for (hipEvent_t e : b->ev) (void)hipEventDestroy(e);
+  if (b->evb) (void)hipEventDestroy(b->evb);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+  return PIVX_OK;
+}
+
+int pivx_batch_create(pivx_ctx* const* ctxs, int n, pivx_batch** out) {
+  if (!ctxs || n < 1 || !out) return batch_fail("bad argument");
+  *out = nullptr;
+  const pivx_ctx* c0 = ctxs[0];
+  if (!c0) return batch_fail("null handle");
+  for (int gi = 0; gi < 2; ++gi) {
+    const pivx_geometry& g = c0->grid[gi].g;
+    if (!xcorr_batch_supported(g.ws_x, g.ws_y))
+      return batch_fail("no batched correlation kernel for %dx%d windows (grid %d)", g.ws_x, g.ws_y, gi + 1);
+  }
+  for (int i = 0; i < n; ++i) {
+    const pivx_ctx* c = ctxs[i];
+    if (!c) return batch_fail("null handle %d", i);
+    for (int j = 0; j < i; ++j)
+      if (ctxs[j] == c) return batch_fail("handle %d appears twice", i);
+    if (c->device != c0->device) return batch_fail("handles on different devices");
+    if (c->prec != 0) return batch_fail("handle %d uses the precision form; batched passes use the FP32 kernels", i);
+    const pivx_init_params &p = c->p, &p0 = c0->p;
+    if (p.ws_x1 != p0.ws_x1 || p.ws_y1 != p0.ws_y1 || p.ws_x2 != p0.ws_x2 || p.ws_y2 != p0.ws_y2 ||
+        p.npass1 != p0.npass1 || p.npass2 != p0.npass2 || p.npeak_save != p0.npeak_save)
+      return batch_fail("handle %d differs in window sizes, pass counts or Npeak_save", i);
+  }
+  if (hipSetDevice(c0->device) != hipSuccess) return batch_fail("hipSetDevice failed");
+  pivx_batch* b = new pivx_batch();
+  b->device = c0->device;
+  b->ctx.assign(ctxs, ctxs + n);
+  int rc = PIVX_OK;
+  auto hip = [&](hipError_t e) {
+    if (e != hipSuccess && rc == PIVX_OK) rc = batch_fail("batch set-up: %s", hipGetErrorString(e));
+  };
+  hip(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+  b->ev.assign((size_t)n, nullptr);
+  for (int i = 0; i < n && rc == PIVX_OK; ++i) hip(hipEventCreateWithFlags(&b->ev[i], hipEventDisableTiming));
+  if (rc == PIVX_OK) hip(hipEventCreateWithFlags(&b->evb, hipEventDisableTiming));
+  int cus = 0;
+  if (rc == PIVX_OK) hip(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device));
+  b->cus = std::max(1, cus);
+  std::vector<XcArgs> args[3];
+  std::vector<int> start[2];
+  for (int gi = 0; gi < 2 && rc == PIVX_OK; ++gi) {
+    start[gi].push_back(0);
+    long long tot = 0;
+    for (int i = 0; i < n; ++i) {
+      const pivx_geometry& g = ctxs[i]->grid[gi].g;
+      int chunks_x = 0;
+      tot += xcorr_batch_items(g.ws_x, g.ws_y, g.nx_vec, g.ny_vec, &chunks_x);
+      if (tot > 0x7fffffffLL) rc = batch_fail("batch holds too many work items");
+      start[gi].push_back((int)tot);
+      for (int kind = (gi == 0 ? 0 : 2); kind <= (gi == 0 ? 1 : 2); ++kind) {
+        XcArgs a = make_xcargs(ctxs[i], gi, kind != 0, kind == 0, false);
+        a.chunks_x = chunks_x;
+        args[kind].push_back(a);
+      }
+    }
+    b->total[gi] = (int)tot;
+  }
+  for (int kind = 0; kind < 3 && rc == PIVX_OK; ++kind) {
+    void* q = nullptr;
+    hip(hipMalloc(&q, args[kind].size() * sizeof(XcArgs)));
+    if (q) {
+      b->allocs.push_back(q);
+      b->dargs[kind] = static_cast<XcArgs*>(q);
+      hip(hipMemcpy(q, args[kind].data(), args[kind].size() * sizeof(XcArgs), hipMemcpyHostToDevice));
+    }
+  }
+  for (int gi = 0; gi < 2 && rc == PIVX_OK; ++gi) {
+    void* q = nullptr;
+    hip(hipMalloc(&q, start[gi].size() * sizeof(int)));
+    if (q) {
+      b->allocs.push_back(q);
+      b->dstart[gi] = static_cast<int*>(q);
+      hip(hipMemcpy(q, start[gi].data(), start[gi].size() * sizeof(int), hipMemcpyHostToDevice));
+    }
+  }
+  if (rc == PIVX_OK) {
+    void* q = nullptr;
+    hip(hipMalloc(&q, 256));
+    if (q) {
+      b->allocs.push_back(q);
+      b->counter = static_cast<int*>(q);
+    }
+  }
+  // untuned defaults: 8 persistent waves per CU, 4 items per claim
+  for (int gi = 0; gi < 2; ++gi) {
+    b->wgs[gi] = 8 * b->cus;
+    b->chunk[gi] = 4;
+  }
+  if (rc != PIVX_OK) {
+    pivx_batch_destroy(b);
+    return rc;
+  }
+  *out = b;
+  return PIVX_OK;
+}
+
+int pivx_batch_set_graph_mode(pivx_batch* b, int on) {
+  if (!b) return batch_fail("null batch");
+  b->use_graph = on != 0;
+  if (!b->use_graph) drop_batch_graph(b);
+  return PIVX_OK;
+}
+
+int pivx_batch_run(pivx_batch* b, const pivx_filter_params* f1, const pivx_filter_params* f2) {
+  if (!b || !f1 || !f2) return batch_fail("null argument");
+  for (pivx_ctx* c : b->ctx)
+    if (!c->have_frames) return batch_fail("a handle of the batch has no frames");
+  if (hipSetDevice(b->device) != hipSuccess) return batch_fail("hipSetDevice failed");
+  if (!b->use_graph) return batch_enqueue(b, f1, f2);
+  if (!b->gexec || !same_filter(b->gf[0], *f1) || !same_filter(b->gf[1], *f2)) {
+    drop_batch_graph(b);
+    BCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    int rc = batch_enqueue(b, f1, f2);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(b->stream, &g);
+    if (rc) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    if (e != hipSuccess) return batch_fail("hipStreamEndCapture: %s", hipGetErrorString(e));
+    b->graph = g;
+    const hipError_t e2 = hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0);
+    if (e2 != hipSuccess) {
+      drop_batch_graph(b);
+      return batch_fail("hipGraphInstantiate: %s", hipGetErrorString(e2));
+    }
+    b->gf[0] = *f1;
+    b->gf[1] = *f2;
+  }
+  BCHK(hipGraphLaunch(b->gexec, b->stream));
+  return PIVX_OK;
+}
+
+int pivx_batch_synchronize(pivx_batch* b) {
+  if (!b) return batch_fail("null batch");
+  BCHK(hipStreamSynchronize(b->stream));
+  for (pivx_ctx* c : b->ctx) {
+    const int rc = pivx_synchronize(c);
+    if (rc) return rc;
+  }
+  return PIVX_OK;
+}
+
+int pivx_batch_autotune(pivx_batch* b, int reps) {
+  if (!b || reps < 1) return batch_fail("bad argument");
+  for (pivx_ctx* c : b->ctx)
+    if (!c->have_frames) return batch_fail("a handle of the batch has no frames");
+  if (hipSetDevice(b->device) != hipSuccess) return batch_fail("hipSetDevice failed");
+  drop_batch_graph(b);
+  for (pivx_ctx* c : b->ctx) BCHK(hipStreamSynchronize(c->stream));
+  hipEvent_t t0 = nullptr, t1 = nullptr;
+  BCHK(hipEventCreate(&t0));
+  BCHK(hipEventCreate(&t1));
+  const int waves_per_cu[4] = {4, 8, 16, 32};
+  const int chunks[3] = {1, 4, 16};
+  int rc = PIVX_OK;
+  for (int gi = 0; gi < 2 && rc == PIVX_OK; ++gi) {
+    const int kind = gi == 0 ? 0 : 2;
+    double best = 1e30;
+    int bw = b->wgs[gi], bc = b->chunk[gi];
+    for (int wv : waves_per_cu)
+      for (int ch : chunks) {
+        b->wgs[gi] = wv * b->cus;
+        b->chunk[gi] = ch;
+        if ((rc = batch_corr(b, kind))) break;  // warm-up
+        if (hipEventRecord(t0, b->stream) != hipSuccess) rc = batch_fail("event record failed");
+        for (int r = 0; r < reps && rc == PIVX_OK; ++r) rc = batch_corr(b, kind);
+        float ms = 0.f;
+        if (rc == PIVX_OK && (hipEventRecord(t1, b->stream) != hipSuccess || hipEventSynchronize(t1) != hipSuccess ||
+                              hipEventElapsedTime(&ms, t0, t1) != hipSuccess))
+          rc = batch_fail("autotune timing failed");
+        if (rc) break;
+        if (ms / reps < best) {
+          best = ms / reps;
+          bw = b->wgs[gi];
+          bc = b->chunk[gi];
+        }
+      }
+    b->wgs[gi] = bw;
+    b->chunk[gi] = bc;
+    b->tuned_ms[gi] = best;
+  }
+  (void)hipEventDestroy(t0);
+  (void)hipEventDestroy(t1);
+  return rc;
+}
+
+int pivx_batch_get_tuning(const pivx_batch* b, int grid, int* waves_per_cu, int* items_per_claim, double* ms,
+                          long long* items) {
+  if (!b || (grid != 1 && grid != 2)) return batch_fail("bad argument");
+  const int gi = grid - 1;
+  if (waves_per_cu) *waves_per_cu = b->wgs[gi] / b->cus;
+  if (items_per_claim) *items_per_claim = b->chunk[gi];
+  if (ms) *ms = b->tuned_ms[gi];
+  if (items) *items = b->total[gi];
   return PIVX_OK;
 }
 

@@ -124,6 +124,18 @@ struct DftTables {
 // mode 1 FP16, 2 BF16, 3 FP16 hi+lo, 4 BF16 hi+lo; shapes: xcorr_dft_supported
 bool xcorr_dft_supported(int ws_x, int ws_y);
 hipError_t launch_xcorr_dft(const XcArgs& a, int ws_x, int ws_y, int mode, const DftTables& t, hipStream_t s);
+// Persistent work-queue correlation pass over many pairs (same window shape).
+struct BatchQueue {
+  const XcArgs* args;      // [npairs] per-pair arguments of the pass (device); chunks_x set for pair shapes
+  const int* item_start;   // [npairs + 1] first work item of every pair; item_start[npairs] = total
+  int npairs, total;
+  int* counter;            // claimed items (zeroed by the launcher's first kernel)
+  int chunk;               // consecutive items claimed per atomic
+};
+bool xcorr_batch_supported(int ws_x, int ws_y);
+// work items of one pair of the given grid size (chunks_x: window pairs per vector row, 0 for single-window shapes)
+long long xcorr_batch_items(int ws_x, int ws_y, int nx_vec, int ny_vec, int* chunks_x);
+hipError_t launch_xcorr_batch(int ws_x, int ws_y, const BatchQueue& q, int workgroups, hipStream_t s);
 hipError_t launch_deform(const DefArgs& a, hipStream_t s);
 // The per-column and per-row interpolation terms of the deformation (as
 // k_deform computes them) into col[nx_aug] and row[ny_aug], for FusedDef.

@@ -825,19 +825,22 @@ __device__ __forceinline__ void deform_window(const XcArgs& a, int ax0, int ay0,
   if (oob) atomicAdd(d.oob, (unsigned long long)oob);
 }
 
+// LDS of one wave-pair work item: one complex plane (= both windows' real
+// planes), 8.4 KB at 32x32.
 template <int WX, int WY>
-__global__ void __launch_bounds__(64) k_xcorr_pair(XcArgs a) {
-  if (a.gate && *a.gate == 0) return;  // learned pipeline: grid 1 runs only as the fallback
+constexpr int pair_lds_floats() { return 2 * WY * (WX + 1); }
+
+// Work item `bid` (window pair bid % chunks_x of vector row bid / chunks_x)
+// of the wave-pair kernel; lds holds pair_lds_floats<WX, WY>() floats.
+template <int WX, int WY>
+__device__ __forceinline__ void xcorr_pair_body(const XcArgs& a, int bid, float* lds) {
   static_assert(WX <= 32 && WY <= 32 && WX % 4 == 0 && WY % 2 == 0, "pair kernel covers windows up to 32x32");
   constexpr int SX = WX + 1;
   constexpr int NPIX = WX * WY;
   constexpr int HX = WX / 2, HY = WY / 2;
-  // one complex plane (= both windows' real planes): 8.4 KB at 32x32
-  __shared__ float lds[2 * WY * SX];
 
   const int lane = threadIdx.x;
   const int h = lane >> 5, l = lane & 31;
-  const int bid = xcd_remap(blockIdx.x, gridDim.x);
   const int iy = bid / a.chunks_x;
   const int ix = (bid - iy * a.chunks_x) * 2 + h;
   const bool win_ok = ix < a.nx_vec;
@@ -1020,6 +1023,13 @@ __global__ void __launch_bounds__(64) k_xcorr_pair(XcArgs a) {
   }
 }
 
+template <int WX, int WY>
+__global__ void __launch_bounds__(64) k_xcorr_pair(XcArgs a) {
+  if (a.gate && *a.gate == 0) return;  // learned pipeline: grid 1 runs only as the fallback
+  __shared__ float lds[pair_lds_floats<WX, WY>()];
+  xcorr_pair_body<WX, WY>(a, xcd_remap(blockIdx.x, gridDim.x), lds);
+}
+
 // ============================================================== wave-per-window kernel
 //
 // Windows up to 64x64: one 64-lane wave owns one window, no workgroup
@@ -1036,20 +1046,24 @@ __global__ void __launch_bounds__(64) k_xcorr_pair(XcArgs a) {
 //     transform) are packed into one: P[:, 0] + i P[:, WX/2];
 //   * real-output row IFFTs two rows at a time, H = G[y] + i G[y + WY/2] with
 //     G[y][-kx] = conj(G[y][kx]).
+// LDS of one wave-per-window work item (floats, then WY float2 of pnyq).
 template <int WX, int WY>
-__global__ void __launch_bounds__(64) k_xcorr_single(XcArgs a) {
-  if (a.gate && *a.gate == 0) return;  // learned pipeline: grid 1 runs only as the fallback
+constexpr int single_lds_floats() {
+  constexpr int SX = WX + 1, HX = WX / 2, HY = WY / 2, GS = HX + 1;
+  return std::max(std::max(WY * SX, 2 * WY * GS), 4 * HX * HY);
+}
+
+// Window w of the wave-per-window kernel; lds holds single_lds_floats floats,
+// pnyq WY float2.
+template <int WX, int WY>
+__device__ __forceinline__ void xcorr_single_body(const XcArgs& a, int w, float* lds, float2* pnyq) {
   static_assert(WX <= 64 && WY <= 64 && WX % 4 == 0 && WY % 4 == 0, "single-wave kernel covers windows up to 64x64");
   constexpr int SX = WX + 1;   // real plane row stride (floats)
   constexpr int NPIX = WX * WY;
   constexpr int HX = WX / 2, HY = WY / 2;
   constexpr int GS = HX + 1;   // G plane row stride (complex)
-  constexpr int NF = std::max(std::max(WY * SX, 2 * WY * GS), 4 * HX * HY);
-  __shared__ float lds[NF];
-  __shared__ float2 pnyq[WY];  // column WX/2 of P
 
   const int lane = threadIdx.x;
-  const int w = blockIdx.x;
   const int iy = w / a.nx_vec, ix = w - iy * a.nx_vec;
   float* const sp = lds;
 
@@ -1264,6 +1278,14 @@ __global__ void __launch_bounds__(64) k_xcorr_single(XcArgs a) {
   }
 }
 
+template <int WX, int WY>
+__global__ void __launch_bounds__(64) k_xcorr_single(XcArgs a) {
+  if (a.gate && *a.gate == 0) return;  // learned pipeline: grid 1 runs only as the fallback
+  __shared__ float lds[single_lds_floats<WX, WY>()];
+  __shared__ float2 pnyq[WY];  // column WX/2 of P
+  xcorr_single_body<WX, WY>(a, blockIdx.x, lds, pnyq);
+}
+
 // ============================================================== generic fused kernel
 //
 // Any legal window (4..1024, multiple of 4, radices from the host plan).  One
@@ -1461,6 +1483,36 @@ __global__ void __launch_bounds__(256) k_xcorr_generic(XcArgs a, int WX, int WY,
   }
   __syncthreads();
   plane_peaks(a, w, cc, cand, WX, WY, red_v, red_i, false);
+}
+
+// ============================================================== batched passes
+// One persistent launch runs a correlation pass over many pairs of uneven
+// size: every pair contributes its work items (window pairs of the wave-pair
+// kernel, windows of the wave-per-window kernel) to one queue, and each
+// workgroup claims `chunk` consecutive items at a time from an atomic counter
+// until the queue is empty.  Each item runs the same body as the per-pair
+// kernel with that pair's arguments, so results are bit-identical.
+template <int WX, int WY, bool PAIR>
+__global__ void __launch_bounds__(64) k_xcorr_batch(BatchQueue q) {
+  constexpr int NL = PAIR ? pair_lds_floats<WX, WY>() : single_lds_floats<WX, WY>();
+  __shared__ float lds[NL];
+  __shared__ float2 pnyq[WY];
+  int p = 0;
+  for (;;) {
+    int base = 0;
+    if (threadIdx.x == 0) base = atomicAdd(q.counter, q.chunk);
+    base = __shfl(base, 0, 64);
+    if (base >= q.total) break;  // every wave of the grid reaches this once the queue is empty
+    const int end = min(base + q.chunk, q.total);
+    for (int item = base; item < end; ++item) {
+      while (item >= q.item_start[p + 1]) ++p;  // claims of one workgroup only increase
+      const XcArgs a = q.args[p];
+      const int local = item - q.item_start[p];
+      if constexpr (PAIR) xcorr_pair_body<WX, WY>(a, local, lds);
+      else xcorr_single_body<WX, WY>(a, local, lds, pnyq);
+      wave_sync();  // the next item rewrites the LDS planes
+    }
+  }
 }
 
 // ============================================================== precision form
@@ -2574,6 +2626,50 @@ hipError_t launch_xcorr_dft(const XcArgs& a, int ws_x, int ws_y, int mode, const
 #define PIVX_CASE(WX, WY) \
   if (ws_x == WX && ws_y == WY) return launch_dft_shape<WX, WY>(a, mode, t, s);
   PIVX_DFT_SIZES(PIVX_CASE)
+#undef PIVX_CASE
+  return hipErrorInvalidValue;
+}
+
+// Shapes with a batched (persistent) correlation kernel: the wave-pair and
+// wave-per-window shapes of the register-FFT set.
+#define PIVX_BATCH_SIZES(X) \
+  X(32, 32, true) X(24, 24, true) X(32, 16, true) X(16, 32, true) X(64, 64, false) X(48, 48, false) \
+  X(64, 32, false) X(32, 64, false) X(48, 24, false)
+
+bool xcorr_batch_supported(int ws_x, int ws_y) {
+#define PIVX_CASE(WX, WY, PAIR) \
+  if (ws_x == WX && ws_y == WY) return true;
+  PIVX_BATCH_SIZES(PIVX_CASE)
+#undef PIVX_CASE
+  return false;
+}
+
+long long xcorr_batch_items(int ws_x, int ws_y, int nx_vec, int ny_vec, int* chunks_x) {
+  *chunks_x = 0;
+#define PIVX_CASE(WX, WY, PAIR)                                        \
+  if (ws_x == WX && ws_y == WY) {                                      \
+    if (PAIR) {                                                        \
+      *chunks_x = (nx_vec + 1) / 2;                                    \
+      return (long long)*chunks_x * ny_vec;                            \
+    }                                                                  \
+    return (long long)nx_vec * ny_vec;                                 \
+  }
+  PIVX_BATCH_SIZES(PIVX_CASE)
+#undef PIVX_CASE
+  return -1;
+}
+
+hipError_t launch_xcorr_batch(int ws_x, int ws_y, const BatchQueue& q, int workgroups, hipStream_t s) {
+  if (q.chunk < 1 || workgroups < 1) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, s, q.counter, 1, (const int*)nullptr);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+#define PIVX_CASE(WX, WY, PAIR)                                                                     \
+  if (ws_x == WX && ws_y == WY) {                                                                   \
+    hipLaunchKernelGGL((k_xcorr_batch<WX, WY, PAIR>), dim3(workgroups), dim3(64), 0, s, q);       \
+    return hipGetLastError();                                                                       \
+  }
+  PIVX_BATCH_SIZES(PIVX_CASE)
 #undef PIVX_CASE
   return hipErrorInvalidValue;
 }
