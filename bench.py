@@ -217,6 +217,69 @@ def precision_leg(pivx, a, b, reps):
     return out
 
 
+def batch_leg(pivx, reps):
+    """Config 4: 64 pairs of uneven size (16 of 1024x1024, 24 of 2048x1536,
+    16 of 4096x2048, 8 of 6600x4400; density U[0.02, 0.08] per pair; sums of 8
+    random Fourier modes, max 10 px), non-square windows 64x32 at 32x16 then
+    32x16 at 16x8 (50% overlap), 3 + 3 passes, 5x5 median x2, fill, 3x3
+    smoothing.  Frames rendered on the GPU and resident.  Timed: `reps` whole
+    batches (i) as one pivx_batch (one persistent correlation launch per pass
+    over all 64 pairs, the sequence replayed as one graph, launch shape from
+    the autotuner) and (ii) as 64 independent handles each replaying its own
+    pair graph on its own stream."""
+    import numpy as np
+
+    rng = np.random.default_rng(64)
+    sizes = [(1024, 1024)] * 16 + [(2048, 1536)] * 24 + [(4096, 2048)] * 16 + [(6600, 4400)] * 8
+    fp = pivx.FilterParams(npass_median=2, min_nneigh=5, residual_thr=2.0, insert_thr=2.0, npass_smooth=1)
+    xs = []
+    nwin = 0
+    try:
+        for i, (nx, ny) in enumerate(sizes):
+            ip = pivx.init_params(nx=nx, ny=ny, ws_x1=64, ws_y1=32, spacing_x1=32, spacing_y1=16, npass1=3, ws_x2=32,
+                                  ws_y2=16, spacing_x2=16, spacing_y2=8, npass2=3)
+            x = pivx.Xcorr2dMulti(ip)
+            xs.append(x)
+            dens = float(rng.uniform(0.02, 0.08))
+            x.synth_pair_device(pivx.synth_params(nx, ny, seed=64000 + i, density=dens, flow=2, a0=10.0))
+            nwin += 3 * x.geometry(1).nt_vec + 3 * x.geometry(2).nt_vec
+        out = {"workload": "64 pairs: 16x 1024x1024, 24x 2048x1536, 16x 4096x2048, 8x 6600x4400; grid1 64x32@32x16 x3, "
+                           "grid2 32x16@16x8 x3", "windows_per_batch": nwin}
+        with pivx.Batch(xs) as b:
+            b.autotune(3)
+            out["tuning"] = {"grid1": b.tuning(1), "grid2": b.tuning(2)}
+            b.set_graph_mode(True)
+            b.run(fp, fp)
+            b.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                b.run(fp, fp)
+            b.synchronize()
+            dt = (time.perf_counter() - t0) / reps
+        out["persistent"] = {"ms_per_batch": round(dt * 1e3, 3), "pairs_per_s": round(64 / dt, 1),
+                             "windows_per_s": round(nwin / dt, 1)}
+        for x in xs:
+            x.set_graph_mode(True)
+            x.run_pair(fp, fp)
+        for x in xs:
+            x.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            for x in xs:
+                x.run_pair(fp, fp)
+        for x in xs:
+            x.synchronize()
+        dt2 = (time.perf_counter() - t0) / reps
+        out["independent_handles"] = {"ms_per_batch": round(dt2 * 1e3, 3), "pairs_per_s": round(64 / dt2, 1),
+                                      "windows_per_s": round(nwin / dt2, 1)}
+        print(f"[bench] batch: persistent {dt * 1e3:.2f} ms, independent handles {dt2 * 1e3:.2f} ms per 64 pairs",
+              file=sys.stderr, flush=True)
+        return out
+    finally:
+        for x in xs:
+            x.close()
+
+
 def learned_leg(pivx, npairs, epochs, neval, nspeed, pipelines):
     """Config 5: the learned grid-1 predictor against the plain pipeline on
     2048x2048 pairs (random Fourier-mode flows, max 12 px, density 0.05).
@@ -378,6 +441,7 @@ def main():
     ap.add_argument("--pipelines", type=int, default=4,
                     help="engine contexts (each its own stream and pair graph) the pairs are spread over")
     ap.add_argument("--precision-reps", type=int, default=10, help="timed passes per precision mode (0 = skip)")
+    ap.add_argument("--batch-reps", type=int, default=5, help="timed 64-pair batches (0 = skip)")
     ap.add_argument("--learned-pairs", type=int, default=2000, help="training pairs of the learned leg (0 = skip)")
     ap.add_argument("--learned-epochs", type=int, default=16)
     ap.add_argument("--learned-eval", type=int, default=200, help="held-out pairs of the learned leg")
@@ -544,6 +608,8 @@ def main():
     }
     if args.cli_pairs > 0:
         out["end_to_end_cli"] = cli_leg(pivx, args.cli_pairs)
+    if args.batch_reps > 0:
+        out["batch"] = batch_leg(pivx, args.batch_reps)
     if args.precision_reps > 0:
         out["precision_form"] = precision_leg(pivx, a, b, args.precision_reps)
     if args.learned_pairs > 0:

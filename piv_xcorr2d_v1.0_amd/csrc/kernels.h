@@ -129,7 +129,7 @@ struct BatchQueue {
   const XcArgs* args;      // [npairs] per-pair arguments of the pass (device); chunks_x set for pair shapes
   const int* item_start;   // [npairs + 1] first work item of every pair; item_start[npairs] = total
   int npairs, total;
-  int* counter;            // claimed items (zeroed by the launcher's first kernel)
+  int* counter;            // [8] claimed items per XCD range (zeroed by the launcher's first kernel)
   int chunk;               // consecutive items claimed per atomic
 };
 bool xcorr_batch_supported(int ws_x, int ws_y);

@@ -1492,25 +1492,42 @@ __global__ void __launch_bounds__(256) k_xcorr_generic(XcArgs a, int WX, int WY,
 // workgroup claims `chunk` consecutive items at a time from an atomic counter
 // until the queue is empty.  Each item runs the same body as the per-pair
 // kernel with that pair's arguments, so results are bit-identical.
+// The queue is split into one contiguous range per XCD (workgroups are
+// dispatched to the 8 XCDs round-robin by workgroup index), so that
+// neighbouring windows, which share most of their pixels, are claimed by
+// workgroups of one XCD and meet in its L2; a workgroup whose range is empty
+// moves on to the next XCD's range, so every range drains.
+// waves per SIMD the per-pair kernels of a shape reach (their register use),
+// kept by the persistent form
 template <int WX, int WY, bool PAIR>
-__global__ void __launch_bounds__(64) k_xcorr_batch(BatchQueue q) {
+constexpr int batch_waves_per_eu() { return PAIR ? 4 : 2; }  // single: 3 would spill
+
+template <int WX, int WY, bool PAIR>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(batch_waves_per_eu<WX, WY, PAIR>())))
+k_xcorr_batch(BatchQueue q) {
   constexpr int NL = PAIR ? pair_lds_floats<WX, WY>() : single_lds_floats<WX, WY>();
+  constexpr int NXCD = 8;
   __shared__ float lds[NL];
   __shared__ float2 pnyq[WY];
-  int p = 0;
-  for (;;) {
-    int base = 0;
-    if (threadIdx.x == 0) base = atomicAdd(q.counter, q.chunk);
-    base = __shfl(base, 0, 64);
-    if (base >= q.total) break;  // every wave of the grid reaches this once the queue is empty
-    const int end = min(base + q.chunk, q.total);
-    for (int item = base; item < end; ++item) {
-      while (item >= q.item_start[p + 1]) ++p;  // claims of one workgroup only increase
-      const XcArgs a = q.args[p];
-      const int local = item - q.item_start[p];
-      if constexpr (PAIR) xcorr_pair_body<WX, WY>(a, local, lds);
-      else xcorr_single_body<WX, WY>(a, local, lds, pnyq);
-      wave_sync();  // the next item rewrites the LDS planes
+  const int home = blockIdx.x % NXCD;
+  for (int r = 0; r < NXCD; ++r) {
+    const int x = (home + r) % NXCD;
+    const int lo = (int)((long long)q.total * x / NXCD), hi = (int)((long long)q.total * (x + 1) / NXCD);
+    int p = 0;
+    for (;;) {
+      int base = 0;
+      if (threadIdx.x == 0) base = atomicAdd(q.counter + x, q.chunk);
+      base = __shfl(base, 0, 64) + lo;
+      if (base >= hi) break;  // this range is drained (every wave gets here for every range)
+      const int end = min(base + q.chunk, hi);
+      for (int item = base; item < end; ++item) {
+        while (item >= q.item_start[p + 1]) ++p;  // claims within one range only increase
+        const int local = item - q.item_start[p];
+        // the pair's arguments are read in place (scalar loads) rather than copied
+        if constexpr (PAIR) xcorr_pair_body<WX, WY>(q.args[p], local, lds);
+        else xcorr_single_body<WX, WY>(q.args[p], local, lds, pnyq);
+        wave_sync();  // the next item rewrites the LDS planes
+      }
     }
   }
 }
@@ -2661,7 +2678,7 @@ long long xcorr_batch_items(int ws_x, int ws_y, int nx_vec, int ny_vec, int* chu
 
 hipError_t launch_xcorr_batch(int ws_x, int ws_y, const BatchQueue& q, int workgroups, hipStream_t s) {
   if (q.chunk < 1 || workgroups < 1) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, s, q.counter, 1, (const int*)nullptr);
+  hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, s, q.counter, 8, (const int*)nullptr);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
 #define PIVX_CASE(WX, WY, PAIR)                                                                     \
