@@ -99,11 +99,90 @@ def cpu_baseline(pivx, ip, fp, a, b, threads, budget_s):
     per_win_1 = t_cal1 / n_cal1
     rows_1 = int(min(4400, max(128, (0.4 * budget_s / 6) / per_win_1 / n_cal1 * 128)) // 64 * 64)
     t_1, n_1 = timed(rows_1, 1)
+    stages = cpu_stages(pivx, pyoracle, ip, a, b)
+    cfg1 = cpu_config1(pivx, pyoracle, threads)
     return dict(value=round(n_mt / t_mt, 1), unit="windows/s", cores=threads, kind="port",
+                per_stage_single_thread=stages, config1=cfg1,
                 sample=(f"6600x{rows_mt} crop of the config-2 pair ({n_mt} windows over all 6 passes), FP32 port, "
                         f"{threads} threads, median of 5 after 1 warm-up"),
                 single_thread=dict(value=round(n_1 / t_1, 1), sample=f"6600x{rows_1} crop ({n_1} windows), 1 thread"),
                 seconds_per_run=round(t_mt, 3), hardware_concurrency=os.cpu_count(), cpu_model=_cpu_model())
+
+
+def cpu_stages(pivx, orc, ip, a, b, rows=512):
+    """Stage-by-stage single-threaded FP32 timings of the CPU port on a
+    6600 x `rows` crop of the config-2 pair (grid 2: 32x32 windows at 16):
+    gather, transforms (with product, magnitude, shift), peaks and fit per
+    window; deformation per aug pixel; upsampling, one median pass, fill and
+    smoothing per vector.  Median of 3 runs each."""
+    import numpy as np
+
+    a = np.ascontiguousarray(a[:rows])
+    b = np.ascontiguousarray(b[:rows])
+    g2 = orc.geometry(ip.nx, rows, 1, 32, 32, 16, 16, 0.4, 0.4)
+    g1 = orc.geometry(ip.nx, rows, 1, 64, 64, 64, 64, 0.4, 0.4)
+    nv = (g2["nx_vec"], g2["ny_vec"])
+    nt = nv[0] * nv[1]
+    rng = np.random.default_rng(3)
+    med = lambda f: sorted(f() for _ in range(3))[1]  # noqa: E731
+
+    def tpass():
+        return orc.xcorr_stage_seconds(a, f2u8=b, geom=g2, ws=(32, 32), sp=(16, 16), K=4)
+
+    runs = sorted((tpass() for _ in range(3)), key=sum)
+    tg, tc, tk = runs[1]
+    fx = rng.normal(2.0, 0.5, nv[::-1])
+    fy = rng.normal(-1.0, 0.5, nv[::-1])
+    ex, ey = orc.extend(fx, fy)
+
+    def timed(fn):
+        t0 = time.perf_counter()
+        fn()
+        return time.perf_counter() - t0
+
+    t_def = med(lambda: timed(lambda: orc.deform(b, g2, (32, 32), (16, 16), ex, ey, prec=32)))
+    e1x, e1y = orc.extend(rng.normal(2.0, 0.5, (g1["ny_vec"], g1["nx_vec"])),
+                          rng.normal(-1.0, 0.5, (g1["ny_vec"], g1["nx_vec"])))
+    t_up = med(lambda: timed(lambda: orc.upsample(e1x, e1y, (64, 64), (16, 16), nv, prec=32)))
+    cx = rng.normal(2.0, 0.3, (nt, 4))
+    cy = rng.normal(-1.0, 0.3, (nt, 4))
+    cv = (rng.random((nt, 4)) > 0.1).astype(np.uint8)
+    src = (cx[:, 0], cy[:, 0], cv[:, 0])
+    t_med = med(lambda: timed(lambda: orc.median_pass(src, (cx.ravel(), cy.ravel(), cv.ravel()), 4, nv, (5, 5), 5,
+                                                       2.0, prec=32)))
+    hv = (rng.random(nt) > 0.05).astype(np.uint8)
+    t_fill = med(lambda: timed(lambda: orc.fill(cx[:, 0], cy[:, 0], hv, nv, prec=32)))
+    t_sm = med(lambda: timed(lambda: orc.smooth(cx[:, 0], cy[:, 0], nv, (3, 3), prec=32)))
+    npix = g2["nx_aug"] * g2["ny_aug"]
+    return {"sample": f"6600x{rows} crop, grid 2 32x32@16 ({nt} windows), FP32, 1 thread, median of 3",
+            "gather_us_per_window": round(tg / nt * 1e6, 3), "fft_corr_us_per_window": round(tc / nt * 1e6, 3),
+            "peaks_fit_us_per_window": round(tk / nt * 1e6, 3),
+            "deform_ns_per_aug_pixel": round(t_def / npix * 1e9, 3),
+            "upsample_ns_per_vector": round(t_up / nt * 1e9, 3), "median_pass_ns_per_vector": round(t_med / nt * 1e9, 3),
+            "fill_ns_per_vector": round(t_fill / nt * 1e9, 3), "smooth_ns_per_vector": round(t_sm / nt * 1e9, 3)}
+
+
+def cpu_config1(pivx, orc, threads):
+    """Config 1 on the CPU port: one 512x512 pair, uniform (3.3, -1.7), 32x32
+    windows at spacing 16 (1,024 vectors, 961 interior), 1 pass, no
+    deformation, top-4, max_disp 0.5; median of 5 after 1 warm-up."""
+    sp = pivx.synth_params(512, 512, seed=1, flow=0, a0=3.3, a1=-1.7)
+    a, b = pivx.synth_pair(sp)
+    ip = pivx.init_params(nx=512, ny=512, ws_x1=32, ws_y1=32, spacing_x1=16, spacing_y1=16, npass1=1, npass2=0,
+                          ws_x2=32, ws_y2=32, spacing_x2=16, spacing_y2=16, max_dispx_percent1=0.5,
+                          max_dispy_percent1=0.5)
+    fp = pivx.FilterParams(npass_median=2, min_nneigh=5, residual_thr=2.0, insert_thr=2.0, npass_smooth=1)
+    out = {"windows": 1024}
+    for key, nthr in (("single_thread", 1), ("all_cores", threads)):
+        orc.run_pair(a, b, ip, fp, fp, prec=32, threads=nthr)
+        ts = []
+        for _ in range(5):
+            t0 = time.perf_counter()
+            orc.run_pair(a, b, ip, fp, fp, prec=32, threads=nthr)
+            ts.append(time.perf_counter() - t0)
+        t = sorted(ts)[2]
+        out[key] = {"threads": nthr, "ms_per_pair": round(t * 1e3, 3), "windows_per_s": round(1024 / t, 1)}
+    return out
 
 
 def cli_leg(pivx, npairs, timeout_s=300):

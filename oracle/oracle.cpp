@@ -42,6 +42,7 @@
 #include "oracle.h"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <complex>
 #include <deque>
@@ -825,6 +826,42 @@ int orc_xcorr_pass(int prec, const orc_pass* P, double* dx, double* dy, uint8_t*
                    uint8_t* tie) {
   if (!P || !P->f1 || (!P->f2u8 && !P->f2def)) return 1;
   return prec == 32 ? xcorr_pass<float>(P, dx, dy, vv, pv, pi, nc, tie) : xcorr_pass<double>(P, dx, dy, vv, pv, pi, nc, tie);
+}
+
+// One correlation pass split into its stages for the CPU baseline (one
+// thread, FP32): seconds spent in the window gather (cu:497-517, 654-673), in
+// the three transforms with the conjugate product, magnitude and shift
+// (cu:519-583, 1334-1350), and in the peak search, top-N ranking and fit
+// (cu:585-606, 4-172, 298-350); same arithmetic as orc_xcorr_pass.
+int orc_xcorr_stage_seconds(const orc_pass* P, double* t_gather, double* t_corr, double* t_peaks) {
+  if (!P || !P->f1 || (!P->f2u8 && !P->f2def) || !t_gather || !t_corr || !t_peaks) return 1;
+  using clk = std::chrono::steady_clock;
+  const int nt = P->nx_vec * P->ny_vec, N = P->wsx * P->wsy, K = P->K;
+  std::vector<float> f1w(N), f2w(N), cc(N);
+  std::vector<double> dx(K), dy(K), pv(K);
+  std::vector<uint8_t> vv(K);
+  std::vector<int> pi(K);
+  double tg = 0, tc = 0, tp = 0;
+  for (int w = 0; w < nt; ++w) {
+    const auto a = clk::now();
+    gather<float>(P, w, f1w.data(), f2w.data());
+    const auto b = clk::now();
+    correlate<float>(f1w.data(), f2w.data(), P->wsx, P->wsy, P->method, cc.data());
+    const auto c = clk::now();
+    const float px = P->pred_x ? (float)P->pred_x[w] : 0.f, py = P->pred_y ? (float)P->pred_y[w] : 0.f;
+    int ncand = 0;
+    peaks_and_fit<float>(cc.data(), P->wsx, P->wsy, P->bx, P->by, K, px, py, (float)P->lx1, (float)P->lx2,
+                         (float)P->ly1, (float)P->ly2, 0.0, 0.0, 0.0, dx.data(), dy.data(), vv.data(), pv.data(),
+                         pi.data(), &ncand, nullptr);
+    const auto d = clk::now();
+    tg += std::chrono::duration<double>(b - a).count();
+    tc += std::chrono::duration<double>(c - b).count();
+    tp += std::chrono::duration<double>(d - c).count();
+  }
+  *t_gather = tg;
+  *t_corr = tc;
+  *t_peaks = tp;
+  return 0;
 }
 
 int orc_deform(int prec, const uint8_t* f2, int nx, int ny, int left_x, int left_y, int nx_aug, int ny_aug,

@@ -69,6 +69,10 @@ int orc_xcorr_pass(int prec, const orc_pass* P, double* disp_x, double* disp_y, 
  * f2 (right) moves by at most delta; used for the ill-conditioned-fit flag. */
 double orc_fit_sensitivity(double f0, double f1, double f2, double delta);
 
+/* One FP32 correlation pass, single-threaded, timed per stage (seconds):
+ * gather, transforms with product/magnitude/shift, peaks with fit. */
+int orc_xcorr_stage_seconds(const orc_pass* P, double* t_gather, double* t_corr, double* t_peaks);
+
 /* Kernel_Window_Deformation_2d (cu:608-652); taps outside the image read 0. */
 int orc_deform(int prec, const uint8_t* f2, int nx, int ny, int left_x, int left_y, int nx_aug, int ny_aug,
                const double* ex_x, const double* ex_y, int nxe, int nye, double x_st, double y_st, double spx,

@@ -51,6 +51,8 @@ def lib():
         L.orc_correlate_window.argtypes = [C.c_int, _D, _D, C.c_int, C.c_int, C.c_int, _D]
         L.orc_fft1d.argtypes = [C.c_int, _D, C.c_int, C.c_int]
         L.orc_xcorr_pass.argtypes = [C.c_int, C.POINTER(OrcPass), _D, _D, _D, _D, _D, _D, _D]
+        L.orc_xcorr_stage_seconds.argtypes = [C.POINTER(OrcPass), C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                              C.POINTER(C.c_double)]
         L.orc_deform.argtypes = [C.c_int, _D] + [C.c_int] * 6 + [_D, _D, C.c_int, C.c_int] + [C.c_double] * 4 + [_D]
         L.orc_extend.argtypes = [_D, _D, C.c_int, C.c_int, C.c_int, _D, _D]
         L.orc_upsample.argtypes = [C.c_int, _D, _D, C.c_int, C.c_int] + [C.c_double] * 4 + [C.c_int] * 4 + [_D, _D]
@@ -168,6 +170,25 @@ def cnn_grad(params, x0, labels, bf16=False):
 def fit_sensitivity(f0, f1, f2, delta):
     """Bound on the change of one axis's 3-point fit when its values move by delta."""
     return lib().orc_fit_sensitivity(f0, f1, f2, delta)
+
+
+def xcorr_stage_seconds(f1, f2u8=None, f2def=None, *, geom, ws, sp, K, pred=None):
+    """(gather, transforms, peaks) seconds of one single-threaded FP32 pass."""
+    f1 = np.ascontiguousarray(f1, np.uint8)
+    ny, nx = f1.shape
+    f2u8 = None if f2u8 is None else np.ascontiguousarray(f2u8, np.uint8)
+    f2def = _f64(f2def)
+    px = py = None
+    if pred is not None:
+        px, py = _f64(pred[0]), _f64(pred[1])
+    P = OrcPass(f1=_p(f1), f2u8=_p(f2u8), f2def=_p(f2def), nx=nx, ny=ny, nx_aug=geom["nx_aug"],
+                ny_aug=geom["ny_aug"], wsx=ws[0], wsy=ws[1], spx=sp[0], spy=sp[1], left_x=geom["left_x"],
+                left_y=geom["left_y"], nx_vec=geom["nx_vec"], ny_vec=geom["ny_vec"], bx=geom["border_x"],
+                by=geom["border_y"], K=K, lx1=-100, lx2=100, ly1=-100, ly2=100, pred_x=_p(px), pred_y=_p(py),
+                method=0, tie_rel=0.0, threads=1, win_begin=0, win_end=0, fit_tol=0.0, fit_rel=0.0)
+    t = [C.c_double() for _ in range(3)]
+    assert lib().orc_xcorr_stage_seconds(C.byref(P), *[C.byref(v) for v in t]) == 0
+    return tuple(v.value for v in t)
 
 
 def deform(f2, geom, ws, sp, ex, ey, prec=64):
