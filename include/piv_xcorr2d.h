@@ -434,7 +434,8 @@ int pivx_cnn_train(pivx_ctx* ctx, pivx_cnn* net, const pivx_cnn_train_params* tp
  * minimum neighbour count from `grid1`) counts the vectors it rejects, and
  * when more than max_outlier_fraction of them are rejected the plain grid-1
  * passes run instead (decided on the device; the grid-1 kernels are launched
- * either way and return at once when not needed).  Then upsampling and the
+ * either way and return at once when not needed; with max_outlier_fraction
+ * >= 1 the fallback cannot trigger and is not launched).  Then upsampling and the
  * grid-2 passes as in pivx_run_pair.  Graph mode (pivx_set_graph_mode) applies.
  * The network's parameters must not change while pairs using it are queued. */
 int pivx_run_pair_learned(pivx_ctx* ctx, const pivx_cnn* net, const pivx_filter_params* grid1,

@@ -44,6 +44,10 @@ struct ConvFwdArgs {
   float* out_x;            // last layer: FP32 planes, sample b at out_x + b * out_bstride
   float* out_y;
   long long out_bstride;
+  uint8_t* out_v;          // last layer, batch 1 (optional): validity flags of the field, set to 1
+  float* ext_x;            // last layer, batch 1 (optional): the field's extension, border ext_b replicated
+  float* ext_y;
+  int ext_b;
 };
 
 // Gradient with respect to the layer input.

@@ -130,22 +130,29 @@ __device__ __forceinline__ bf16x8 gpatch8(const uint16_t* gout, const uint16_t* 
 }
 
 // ---------------------------------------------------------------- forward
-// One wave: MT output-channel tiles (of 16) of PT * 16 consecutive output
-// pixels of one row; the COUT / 16 / MT channel blocks of a pixel group go to
-// consecutive waves.  D[co][pixel] = W[co][k] * X[k][pixel], four waves per
-// workgroup.  The K loop is unrolled by three so that the loads of later
-// steps are in flight while the matrix cores work.
-template <int CIN, int COUT, int S, bool ACT, bool FINAL, int PT, int MT>
+// One output tile: MT output-channel tiles (of 16) of PT * 16 consecutive
+// output pixels of one row; the COUT / 16 / MT channel blocks of a pixel
+// group are consecutive tiles.  D[co][pixel] = W[co][k] * X[k][pixel].  With
+// KS = 4 the four waves of a workgroup split the K steps of one tile and sum
+// their partial tiles through LDS (short dependent chains for the small,
+// deep layers); with KS = 1 each wave owns a tile.  The K loop is unrolled
+// by three so that the loads of later steps are in flight while the matrix
+// cores work.
+template <int CIN, int COUT, int S, bool ACT, bool FINAL, int PT, int MT, int KS>
 __global__ void __launch_bounds__(256) k_conv_fwd(ConvFwdArgs a) {
   constexpr int KP = cnn_kpad(CIN);
   constexpr int NCB = cnn_pad16(COUT) / 16 / MT;  // channel blocks per pixel group
+  constexpr int KSTEPS = KP / 32;
+  constexpr int KPER = (KSTEPS + KS - 1) / KS;   // K steps per wave
+  __shared__ f32x4 part[KS > 1 ? KS - 1 : 1][MT * PT][64];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int col = lane & 15, kg = lane >> 4;
   const int gpr = a.wout / (16 * PT);
-  const long long gw = (long long)blockIdx.x * 4 + wave;
+  const long long gw = KS > 1 ? (long long)blockIdx.x : (long long)blockIdx.x * 4 + wave;
+  const int ks = KS > 1 ? wave : 0;
   const long long g = gw / NCB;
   const int cb = (int)(gw - g * NCB);
-  if (g >= (long long)a.batch * a.hout * gpr) return;  // whole wave; no barriers in this kernel
+  if (g >= (long long)a.batch * a.hout * gpr) return;  // whole workgroup (KS > 1) or whole wave
   const int gx = (int)(g % gpr);
   const long long r = g / gpr;
   const int oy = (int)(r % a.hout), b = (int)(r / a.hout);
@@ -157,8 +164,9 @@ __global__ void __launch_bounds__(256) k_conv_fwd(ConvFwdArgs a) {
   for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
     for (int pt = 0; pt < PT; ++pt) acc[mt][pt] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const int kbeg = ks * KPER * 32, kend = min(KP, (ks + 1) * KPER * 32);
 #pragma unroll 3
-  for (int k0 = 0; k0 < KP; k0 += 32) {
+  for (int k0 = kbeg; k0 < kend; k0 += 32) {
     const int k = k0 + 8 * kg;
     bf16x8 wa[MT], xb[PT];
 #pragma unroll
@@ -170,14 +178,42 @@ __global__ void __launch_bounds__(256) k_conv_fwd(ConvFwdArgs a) {
 #pragma unroll
       for (int pt = 0; pt < PT; ++pt) acc[mt][pt] = mfma(wa[mt], xb[pt], acc[mt][pt]);
   }
+  if constexpr (KS > 1) {
+    // partial tiles of waves 1..KS-1 to wave 0, summed in a fixed order
+    if (ks > 0)
+#pragma unroll
+      for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+        for (int pt = 0; pt < PT; ++pt) part[ks - 1][mt * PT + pt][lane] = acc[mt][pt];
+    __syncthreads();
+    if (ks > 0) return;
+#pragma unroll
+    for (int q = 0; q < KS - 1; ++q)
+#pragma unroll
+      for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+        for (int pt = 0; pt < PT; ++pt) acc[mt][pt] += part[q][mt * PT + pt][lane];
+  }
 #pragma unroll
   for (int pt = 0; pt < PT; ++pt) {
     const int ox = ox0 + pt * 16 + col;
     if constexpr (FINAL) {
       if (kg == 0) {
         const long long o = (long long)b * a.out_bstride + (long long)oy * a.wout + ox;
-        a.out_x[o] = acc[0][pt][0] + a.bias[0];
-        a.out_y[o] = acc[0][pt][1] + a.bias[1];
+        const float vx = acc[0][pt][0] + a.bias[0], vy = acc[0][pt][1] + a.bias[1];
+        a.out_x[o] = vx;
+        a.out_y[o] = vy;
+        if (a.out_v) a.out_v[o] = 1;
+        if (a.ext_x) {  // border-replicated extension of the field (the extend stage)
+          const int B = a.ext_b, nxe = a.wout + 2 * B;
+          const int i0 = ox == 0 ? 0 : ox + B, i1 = ox == a.wout - 1 ? nxe - 1 : ox + B;
+          const int j0 = oy == 0 ? 0 : oy + B, j1 = oy == a.hout - 1 ? a.hout + 2 * B - 1 : oy + B;
+          for (int j = j0; j <= j1; ++j)
+            for (int i = i0; i <= i1; ++i) {
+              a.ext_x[j * nxe + i] = vx;
+              a.ext_y[j * nxe + i] = vy;
+            }
+        }
       }
     } else {
 #pragma unroll
@@ -469,9 +505,11 @@ hipError_t fwd(const ConvFwdArgs& a, hipStream_t s) {
   constexpr int PT = 1;
   constexpr int MT = 1;  // one 16-channel tile per wave: the small layers need the waves
   constexpr int NCB = cnn_pad16(COUT) / 16 / MT;
+  constexpr int KS = cnn_kpad(CIN) / 32 >= 8 ? 4 : 1;  // split the K steps of the deep layers over 4 waves
   if (a.wout % (16 * PT) != 0) return hipErrorInvalidValue;
-  const long long waves = (long long)a.batch * a.hout * (a.wout / (16 * PT)) * NCB;
-  hipLaunchKernelGGL((k_conv_fwd<CIN, COUT, S, ACT, FINAL, PT, MT>), dim3(blocks(waves, 4)), dim3(256), 0, s, a);
+  const long long tiles = (long long)a.batch * a.hout * (a.wout / (16 * PT)) * NCB;
+  const long long wgs = KS > 1 ? tiles : blocks(tiles, 4);
+  hipLaunchKernelGGL((k_conv_fwd<CIN, COUT, S, ACT, FINAL, PT, MT, KS>), dim3((unsigned)wgs), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 

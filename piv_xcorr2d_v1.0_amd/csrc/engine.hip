@@ -1747,7 +1747,8 @@ bool cnn_shape_ok(int h, int w) { return h >= 8 && w >= 128 && h % 8 == 0 && w %
 // Forward pass over a batch: x0 sample b at x0 + (bidx ? bidx[b] : b) * xstride;
 // hidden activations into act[0..4]; output planes at out_x / out_y + b * ostride.
 int cnn_forward_enqueue(const pivx_cnn* net, const uint16_t* x0, long long xstride, const int* bidx, int B, int H,
-                        int W, uint16_t* const* act, float* out_x, float* out_y, long long ostride, hipStream_t s) {
+                        int W, uint16_t* const* act, float* out_x, float* out_y, long long ostride, hipStream_t s,
+                        uint8_t* out_v = nullptr, float* ext_x = nullptr, float* ext_y = nullptr, int ext_b = 0) {
   for (int l = 0; l < kCnnLayers; ++l) {
     const CnnLayer& L = kCnnArch[l];
     ConvFwdArgs a{};
@@ -1764,6 +1765,10 @@ int cnn_forward_enqueue(const pivx_cnn* net, const uint16_t* x0, long long xstri
       a.out_x = out_x;
       a.out_y = out_y;
       a.out_bstride = ostride;
+      a.out_v = out_v;
+      a.ext_x = ext_x;
+      a.ext_y = ext_y;
+      a.ext_b = ext_b;
     }
     const hipError_t e = launch_conv_fwd(l, a, s);
     if (e != hipSuccess) return cnn_fail("network layer %d: %s", l, hipGetErrorString(e));
@@ -1925,11 +1930,13 @@ int learned_enqueue(pivx_ctx* c, const pivx_cnn* net, const pivx_filter_params* 
   // network field -> grid-1 field
   hipError_t e = launch_downsample(c->f1, c->p.nx, c->f2, c->pitch2, c->p.nx, c->p.ny, c->cx0, c->stream);
   if (e != hipSuccess) return fail(c, PIVX_ERR_HIP, "network input: %s", hipGetErrorString(e));
-  if ((rc = cnn_forward_enqueue(net, c->cx0, 0, nullptr, 1, H, W, c->cact, G1.fx, G1.fy, 0, c->stream))) {
+  // the last layer also sets the field's flags and writes its extension
+  // (the fallback rewrites both)
+  if ((rc = cnn_forward_enqueue(net, c->cx0, 0, nullptr, 1, H, W, c->cact, G1.fx, G1.fy, 0, c->stream, G1.fv, G1.ex,
+                                G1.ey, kExBorder))) {
     c->err = global_error();
     return rc;
   }
-  if ((rc = check_launch(c, launch_fill_u8(G1.fv, nt1, 1, c->stream), "network field flags"))) return rc;
   // universal outlier test of the network field: the grid-1 median test with
   // the network vector as the only candidate; rejected vectors are counted
   MedArgs m{};
@@ -1959,13 +1966,15 @@ int learned_enqueue(pivx_ctx* c, const pivx_cnn* net, const pivx_filter_params* 
   const int max_out = (int)std::floor((double)max_frac * nt1);
   if ((rc = check_launch(c, launch_gate(G1.accepted, nt1, max_out, c->lgate, c->lcount, c->stream), "fallback gate")))
     return rc;
-  if ((rc = extend(c, 0))) return rc;  // the network field's extension (the fallback rewrites it)
-  // plain grid 1, run only when the gate says so
+  // plain grid 1, run only when the gate says so; with a fraction of 1 or
+  // more the gate can never open and the fallback is not enqueued at all
   bool ext_ok = false;
-  c->gate = c->lgate;
-  rc = enqueue_grid1(c, f1, &ext_ok);
-  c->gate = nullptr;
-  if (rc) return rc;
+  if (max_frac < 1.f) {
+    c->gate = c->lgate;
+    rc = enqueue_grid1(c, f1, &ext_ok);
+    c->gate = nullptr;
+    if (rc) return rc;
+  }
   // The extension now belongs to whichever field grid 1 holds: the network's
   // (written above) or the fallback's (written by its last smoothing pass, or
   // rewritten by the ungated extension in enqueue_grid2 when there is none).
