@@ -662,6 +662,20 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Workgroups are dealt to the 8 XCDs round-robin by index, and each XCD has
+// its own L2.  For a stencil kernel whose neighbouring blocks share input
+// rows, xcd_block maps launch index b to logical block (b % 8) * per + b / 8
+// (per = ceil(n / 8)), so each XCD runs one contiguous range of the n
+// logical blocks and the shared rows meet in its L2.  Launch xcd_grid(n)
+// workgroups; a result >= n is a padding workgroup with no work.
+constexpr int kXcds = 8;
+__host__ __device__ constexpr int xcd_grid(int n) { return (n + kXcds - 1) / kXcds * kXcds; }
+__device__ __forceinline__ int xcd_block(int n) {
+  const int per = (n + kXcds - 1) / kXcds;
+  const int b = blockIdx.x;
+  return (b % kXcds) * per + b / kXcds;
+}
+
 __device__ __forceinline__ int half_sum(int v) {
 #pragma unroll
   for (int off = 16; off > 0; off >>= 1) v += __shfl_xor(v, off, 32);
@@ -1755,6 +1769,9 @@ __global__ void __launch_bounds__(256) k_deform(DefArgs a) {
   // this block's output rows as a buffer resource: row r of pixel idx at
   // byte r * nx_aug * 4 (scalar offset) + idx * 4
   const unsigned row_bytes = (unsigned)a.nx_aug * 4u;
+  // stores carry the non-temporal (streaming) cache policy: the 117 MB plane
+  // of a grid-2 pass is written once and read once, by the next kernel
+  constexpr int kDefStoreNT = 2;
   const __amdgpu_buffer_rsrc_t outr = __builtin_amdgcn_make_buffer_rsrc(a.out + (long long)y0 * a.nx_aug, (short)0,
                                                                         (int)(row_bytes * (unsigned)nrow), 0x00020000);
   // row terms read at constant offsets from one base register
@@ -1770,7 +1787,7 @@ __global__ void __launch_bounds__(256) k_deform(DefArgs a) {
     }
     const float v = e.may_oob ? deform_px<true>(F, e, ct.x, ct.y, ct.w, rt.x, rt.y, rt.w, oob)
                               : deform_px<false>(F, e, ct.x, ct.y, ct.w, rt.x, rt.y, rt.w, oob);
-    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), outr, (int)(idx * 4), (int)(r * row_bytes), 0);
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), outr, (int)(idx * 4), (int)(r * row_bytes), kDefStoreNT);
   };
   // full blocks unrolled (row terms at constant LDS offsets), in groups of 8
   // rows: a group inside one cell whose samples stay below the guard has no
@@ -1794,7 +1811,7 @@ __global__ void __launch_bounds__(256) k_deform(DefArgs a) {
           static_for<8>([&](auto k) { v[k] = deform_px<false>(F, e, ct.x, ct.y, ct.w, rt[k].x, rt[k].y, rt[k].w, oob); });
         }
         static_for<8>([&](auto k) {
-          __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v[k]), outr, (int)(idx * 4), (int)((r0 + k) * row_bytes), 0);
+          __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v[k]), outr, (int)(idx * 4), (int)((r0 + k) * row_bytes), kDefStoreNT);
         });
       } else {
         static_for<8>([&](auto k) { row(r0 + k); });
@@ -2063,10 +2080,13 @@ __global__ void __launch_bounds__(256) k_median2(MedArgs a) {
   constexpr int SKIP = (WXM * WYM - 1) / 2;
   constexpr int HX = (WXM - 1) / 2, HY = (WYM - 1) / 2;
   const int nt = a.nx_vec * a.ny_vec;
-  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a.reset_count && blockIdx.x == 0 && threadIdx.x == 0) *a.reset_count = 0;
+  const int nb = (2 * nt + 255) / 256;
+  const int blk = xcd_block(nb);  // rows of neighbouring blocks share an XCD's L2
+  if (blk >= nb) return;
+  const int g = blk * 256 + (int)threadIdx.x;
   const int t = g >> 1, c = g & 1;
   const bool live = t < nt;
-  if (a.reset_count && g == 0) *a.reset_count = 0;
   const int tt = live ? t : nt - 1;  // both lanes of a pair stay active for the swap
   const int vy = tt / a.nx_vec, vx = tt - vy * a.nx_vec;
   const float* src = c ? a.sy : a.sx;
@@ -2361,7 +2381,7 @@ __device__ __forceinline__ int mhe_neigh_count(int tx, int ty, int mx, int my) {
 __global__ void __launch_bounds__(256) k_mhe_hist(MheArgs a) {
   __shared__ uint32_t hs[4][MHE_SUB][257];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int tile = blockIdx.x * 4 + wv;
+  const int tile = xcd_block((a.tiles_x * a.tiles_y + 3) / 4) * 4 + wv;  // tile rows banded per XCD
   if (tile >= a.tiles_x * a.tiles_y) return;
   uint32_t (*h)[257] = hs[wv];
   const int sub = lane & (MHE_SUB - 1);
@@ -2426,7 +2446,7 @@ __global__ void __launch_bounds__(256) k_mhe_hist(MheArgs a) {
 __global__ void __launch_bounds__(256) k_mhe_map(MheArgs a) {
   __shared__ uint8_t luts[4][256];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int tile = blockIdx.x * 4 + wv;
+  const int tile = xcd_block((a.tiles_x * a.tiles_y + 3) / 4) * 4 + wv;  // tile rows banded per XCD
   if (tile >= a.tiles_x * a.tiles_y) return;
   uint8_t* lut = luts[wv];
   const int ty = tile / a.tiles_x, tx = tile - ty * a.tiles_x;
@@ -2716,7 +2736,7 @@ hipError_t launch_median(const MedArgs& a, hipStream_t s) {
   const int nt = a.nx_vec * a.ny_vec;
   const int ntm = a.wx * a.wy;
   dim3 g(grid_for(nt, 128, 1 << 30)), b(128);
-  const dim3 g2(grid_for(2LL * nt, 256, 1 << 30)), b2(256);
+  const dim3 g2(xcd_grid((int)((2LL * nt + 255) / 256))), b2(256);
   if (a.wx == 5 && a.wy == 5) hipLaunchKernelGGL((k_median2<5, 5>), g2, b2, 0, s, a);
   else if (a.wx == 3 && a.wy == 3) hipLaunchKernelGGL((k_median2<3, 3>), g2, b2, 0, s, a);
   else if (a.wx == 7 && a.wy == 7) hipLaunchKernelGGL((k_median2<7, 7>), g2, b2, 0, s, a);
@@ -2766,7 +2786,7 @@ hipError_t launch_stream_copy(const void* src, void* dst, size_t bytes, hipStrea
 hipError_t launch_mhe(const MheArgs& a, hipStream_t s) {
   const int tiles = a.tiles_x * a.tiles_y;
   if (tiles < 1) return hipErrorInvalidValue;
-  const dim3 g((tiles + 3) / 4), b(256);
+  const dim3 g(xcd_grid((tiles + 3) / 4)), b(256);
   hipLaunchKernelGGL(k_mhe_hist, g, b, 0, s, a);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
