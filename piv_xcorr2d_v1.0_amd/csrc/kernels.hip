@@ -662,19 +662,6 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Workgroups are dealt to the 8 XCDs round-robin by index, and each XCD has
-// its own L2.  For a stencil kernel whose neighbouring blocks share input
-// rows, xcd_block maps launch index b to logical block (b % 8) * per + b / 8
-// (per = ceil(n / 8)), so each XCD runs one contiguous range of the n
-// logical blocks and the shared rows meet in its L2.  Launch xcd_grid(n)
-// workgroups; a result >= n is a padding workgroup with no work.
-constexpr int kXcds = 8;
-__host__ __device__ constexpr int xcd_grid(int n) { return (n + kXcds - 1) / kXcds * kXcds; }
-__device__ __forceinline__ int xcd_block(int n) {
-  const int per = (n + kXcds - 1) / kXcds;
-  const int b = blockIdx.x;
-  return (b % kXcds) * per + b / kXcds;
-}
 
 __device__ __forceinline__ int half_sum(int v) {
 #pragma unroll
@@ -781,11 +768,12 @@ __device__ __forceinline__ void load_row(const XcArgs& a, int ax0, int ay, float
 }
 
 // Frame 2 of the window at aug (ax0, ay0) deformed into the plane
-// p[r * SX + c] (FusedDef): lane = window column c, rows in order; the row
-// terms are wave-uniform and come through scalar loads, and the cell changes
-// with the row, as in k_deform.
+// p[r * SX + c] (FusedDef): lane = window column c, rows r_lo..r_hi - 1 (a
+// multiple of 8 each) in order; the row terms are wave-uniform and come
+// through scalar loads, and the cell changes with the row, as in k_deform.
 template <int WX, int WY, int SX>
-__device__ __forceinline__ void deform_window(const XcArgs& a, int ax0, int ay0, float* p, int lane) {
+__device__ __forceinline__ void deform_window(const XcArgs& a, int ax0, int ay0, float* p, int lane, int r_lo = 0,
+                                              int r_hi = WY) {
   const FusedDef& d = a.fdef;
   if (lane >= WX) return;
   const float4 ct = d.col[ax0 + lane];
@@ -802,7 +790,7 @@ __device__ __forceinline__ void deform_window(const XcArgs& a, int ax0, int ay0,
   // rows in groups of 8: when a group lies in one cell whose samples stay
   // below the out-of-buffer guard (the common case) the group has no branch
   // and its 32 frame taps are in flight together
-  for (int r0 = 0; r0 < WY; r0 += 8) {
+  for (int r0 = r_lo; r0 < r_hi; r0 += 8) {
     float ay[8], ay1[8], fidy[8];
     int iyf[8];
     static_for<8>([&](auto k) {
@@ -1067,6 +1055,73 @@ constexpr int single_lds_floats() {
   return std::max(std::max(WY * SX, 2 * WY * GS), 4 * HX * HY);
 }
 
+// Peak stage of a wave-per-window plane (fftshifted correlation cc, row
+// stride SX floats, in LDS): lane l walks column l of the window, keeping a
+// bit mask of the rows that are local maxima; Npeak_save rounds of 64-lane
+// arg-max; Gaussian fits of the winners.
+template <int WX, int WY, int SX>
+__device__ __forceinline__ void single_peaks(const XcArgs& a, int w, const float* cc, int lane) {
+  using Bits = typename std::conditional<(WY <= 32), uint32_t, uint64_t>::type;
+  const bool col_ok = lane < WX && lane >= a.bx && lane <= WX - 1 - a.bx;
+  const int cl = lane < WX ? lane : WX - 1;
+  const int lft = cl > 0 ? cl - 1 : 0, rgt = cl < WX - 1 ? cl + 1 : WX - 1;
+  // the search border (rows from a uniform mask, then the lane's column)
+  const Bits rows = search_rows<Bits, WY>(a.by);
+  const Bits keep = col_ok ? rows : (Bits)0;
+  Bits bits = column_maxima<WY, SX, Bits>(cc, cl, lft, rgt) & keep;
+  const int n = wave_sum(__popcll((unsigned long long)bits));
+  if (lane == 0) {
+    a.ncand[w] = n;
+    if (a.zero_x) a.zero_x[w] = a.zero_y[w] = 0.f;
+  }
+  const float px = a.pred_x ? a.pred_x[w] : 0.f;
+  const float py = a.pred_y ? a.pred_y[w] : 0.f;
+  // the lane's best two maxima of its column as peak keys (every maximum is
+  // > 0, so key 0 is "none"); the owner of a round's winner clears that row
+  // and promotes its second, and a lane that wins again rescans its column
+  uint64_t mk, mk2;
+  best_two<Bits, WX, SX>(cc, bits, cl, mk, mk2);
+  bool have2 = true;
+  // rounds in groups of 32: round k's winner is kept by lanes 2k and 2k + 1,
+  // which fit its x and its y
+  for (int k0 = 0; k0 < a.K; k0 += 32) {
+    const int kn = min(32, a.K - k0);
+    float my_v = 0.f;
+    int my_i = -1;
+    for (int k = 0; k < kn; ++k) {
+      const uint64_t best = group_max_key<64>(mk);
+      if (best != 0 && best == mk) {
+        bits &= ~((Bits)1 << (key_index(mk) / WX));
+        if (have2) {
+          mk = mk2;
+          have2 = false;
+        } else {
+          best_two<Bits, WX, SX>(cc, bits, cl, mk, mk2);
+          have2 = true;
+        }
+      }
+      if ((lane >> 1) == k && best != 0) {
+        my_v = key_value(best);
+        my_i = key_index(best);
+      }
+    }
+    if (lane < 2 * kn) {
+      const long long o = (long long)w * a.K + k0 + (lane >> 1);
+      const bool ya = (lane & 1) != 0;
+      float dx, dy;
+      const bool ok = subpixel_fit_pair<64>(cc, SX, WX, WY, my_i, my_v, px, py, a.lx1, a.lx2, a.ly1, a.ly2, ya, dx, dy);
+      if (ya) {
+        a.disp_y[o] = dy;
+        a.pk_idx[o] = my_i;
+      } else {
+        a.disp_x[o] = dx;
+        a.valid[o] = ok ? 1 : 0;
+        a.pk_val[o] = my_v;
+      }
+    }
+  }
+}
+
 // Window w of the wave-per-window kernel; lds holds single_lds_floats floats,
 // pnyq WY float2.
 template <int WX, int WY>
@@ -1228,68 +1283,8 @@ __device__ __forceinline__ void xcorr_single_body(const XcArgs& a, int w, float*
     });
   }
   wave_sync();
-  // 6. peaks: lane l walks column l of the window, keeping a bit mask of the
-  //    rows that are local maxima; Npeak_save rounds of 64-lane arg-max
-  const float* cc = ccp;
-  using Bits = typename std::conditional<(WY <= 32), uint32_t, uint64_t>::type;
-  const bool col_ok = lane < WX && lane >= a.bx && lane <= WX - 1 - a.bx;
-  const int cl = lane < WX ? lane : WX - 1;
-  const int lft = cl > 0 ? cl - 1 : 0, rgt = cl < WX - 1 ? cl + 1 : WX - 1;
-  // the search border (rows from a uniform mask, then the lane's column)
-  const Bits rows = search_rows<Bits, WY>(a.by);
-  const Bits keep = col_ok ? rows : (Bits)0;
-  Bits bits = column_maxima<WY, SX, Bits>(cc, cl, lft, rgt) & keep;
-  const int n = wave_sum(__popcll((unsigned long long)bits));
-  if (lane == 0) {
-    a.ncand[w] = n;
-    if (a.zero_x) a.zero_x[w] = a.zero_y[w] = 0.f;
-  }
-  const float px = a.pred_x ? a.pred_x[w] : 0.f;
-  const float py = a.pred_y ? a.pred_y[w] : 0.f;
-  // the lane's best two maxima of its column as peak keys (every maximum is
-  // > 0, so key 0 is "none"); the owner of a round's winner clears that row
-  // and promotes its second, and a lane that wins again rescans its column
-  uint64_t mk, mk2;
-  best_two<Bits, WX, SX>(cc, bits, cl, mk, mk2);
-  bool have2 = true;
-  // rounds in groups of 32: round k's winner is kept by lanes 2k and 2k + 1,
-  // which fit its x and its y
-  for (int k0 = 0; k0 < a.K; k0 += 32) {
-    const int kn = min(32, a.K - k0);
-    float my_v = 0.f;
-    int my_i = -1;
-    for (int k = 0; k < kn; ++k) {
-      const uint64_t best = group_max_key<64>(mk);
-      if (best != 0 && best == mk) {
-        bits &= ~((Bits)1 << (key_index(mk) / WX));
-        if (have2) {
-          mk = mk2;
-          have2 = false;
-        } else {
-          best_two<Bits, WX, SX>(cc, bits, cl, mk, mk2);
-          have2 = true;
-        }
-      }
-      if ((lane >> 1) == k && best != 0) {
-        my_v = key_value(best);
-        my_i = key_index(best);
-      }
-    }
-    if (lane < 2 * kn) {
-      const long long o = (long long)w * a.K + k0 + (lane >> 1);
-      const bool ya = (lane & 1) != 0;
-      float dx, dy;
-      const bool ok = subpixel_fit_pair<64>(cc, SX, WX, WY, my_i, my_v, px, py, a.lx1, a.lx2, a.ly1, a.ly2, ya, dx, dy);
-      if (ya) {
-        a.disp_y[o] = dy;
-        a.pk_idx[o] = my_i;
-      } else {
-        a.disp_x[o] = dx;
-        a.valid[o] = ok ? 1 : 0;
-        a.pk_val[o] = my_v;
-      }
-    }
-  }
+  // 6. peaks
+  single_peaks<WX, WY, SX>(a, w, ccp, lane);
 }
 
 template <int WX, int WY>
@@ -1298,6 +1293,335 @@ __global__ void __launch_bounds__(64) k_xcorr_single(XcArgs a) {
   __shared__ float lds[single_lds_floats<WX, WY>()];
   __shared__ float2 pnyq[WY];  // column WX/2 of P
   xcorr_single_body<WX, WY>(a, blockIdx.x, lds, pnyq);
+}
+
+// ============================================================== two-wave 64x64 kernel
+//
+// One 128-thread workgroup (two waves) per 64x64 window.  The window's
+// registers and LDS are shared by two waves (about half the VGPRs per lane
+// and half the LDS per wave of the wave-per-window kernel), so that twice as
+// many waves share every SIMD to hide the LDS and memory latency of the
+// transforms.  Every length-64 line is split in two by one radix-2
+// decimation-in-time step whose butterfly is left to the reader of the next
+// phase: with E and O the length-32 transforms of the even and odd samples
+// and T[k] = w^k O[k], X[k] = E[k] + T[k] and X[k + 32] = E[k] - T[k].
+//   1. row FFTs: wave h transforms the samples 2n + h of row `lane` (wave 1
+//      then twiddles); E and T of the row go to the plane as columns k and
+//      32 + k;
+//   2. column FFTs: wave h reads the rows 2n + h of column `lane`, forming
+//      X = E +- T as it reads, and transforms them (wave 1 twiddles); E' and
+//      T' go to the plane as rows k and 32 + k;
+//   3. the product P = 4 conj(F1) F2 on the half spectrum kx = 0..31, Z and
+//      its mirror Z[-ky][-kx] formed as E' +- T' from the plane, columns 0
+//      and 32 packed as P[:, 0] + i P[:, 32] (both transform to real
+//      columns); each column IFFT over ky is split four ways by a radix-4
+//      decimation-in-time step (thread q transforms the rows ky = 4m + q,
+//      length 16, and twiddles by w^(q k)) and combined in place in LDS, the
+//      four inputs of a radix-4 butterfly being exactly the slots of its
+//      four outputs;
+//   4. real-output row IFFTs of H = G[y] + i G[y + 32] (y < 32), G[y][-kx] =
+//      conj(G[y][kx]), split four ways the same way over kx = 4m + q; the
+//      outputs are scaled and written fftshifted (Re: row y + 32, Im: row y);
+//   5. peaks in wave 0 (single_peaks); wave 1 is done.
+// The planes move through LDS as real parts, then imaginary parts; the
+// phases are separated by workgroup barriers.
+#ifndef PIVX_SINGLE64_TWO_WAVES
+#define PIVX_SINGLE64_TWO_WAVES 1
+#endif
+constexpr bool kSingle64TwoWaves = PIVX_SINGLE64_TWO_WAVES != 0;  // 64x64 windows take this kernel
+constexpr int kW2SX = 65;  // real plane row stride (floats)
+constexpr int kW2GS = 33;  // complex row stride of the G / exchange plane
+constexpr int kW2LdsFloats = 64 * kW2GS * 2;
+
+// Pixels of parity P (columns 2n + P) of a row of u8 pixels, any alignment:
+// 16-byte chunk loads, bytes P and P + 2 of every word.
+template <int WX, int P>
+__device__ __forceinline__ void load_u8_row_par(const uint8_t* p, float (&f)[WX / 2]) {
+  static_assert(WX % 16 == 0, "16-byte chunks");
+  static_for<WX / 16>([&](auto qc) {
+    constexpr int q = qc;
+    uint4 w;
+    __builtin_memcpy(&w, p + 16 * q, 16);
+    const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
+    static_for<4>([&](auto jc) {
+      constexpr int j = jc;
+      f[8 * q + 2 * j + 0] = (float)((ws[j] >> (8 * P)) & 255u);
+      f[8 * q + 2 * j + 1] = (float)((ws[j] >> (8 * P + 16)) & 255u);
+    });
+  });
+}
+
+// The same for a row crossing the frame edge (0 outside the frame).
+template <int WX, int P>
+__device__ __forceinline__ void load_u8_row_edge_par(const uint8_t* img, long long pitch, int nx, int ny, int xi0,
+                                                     int yi, float (&f)[WX / 2]) {
+  const bool row_in = (unsigned)yi < (unsigned)ny;
+  const uint8_t* row = img + (long long)min(max(yi, 0), ny - 1) * pitch;
+  static_for<WX / 4>([&](auto qc) {
+    constexpr int q = qc;
+    const uint32_t w = row_in ? load_u8x4_edge(row, xi0 + 4 * q, nx) : 0u;
+    f[2 * q + 0] = (float)((w >> (8 * P)) & 255u);
+    f[2 * q + 1] = (float)((w >> (8 * P + 16)) & 255u);
+  });
+}
+
+// Samples 2n + P of one 64-pixel window row at aug (ax0, ay) as load_row
+// reads them: x[n] = (frame 1, frame 2); with the fused deformation frame 2
+// comes from the plane (the caller).
+template <int P>
+__device__ __forceinline__ void load_row_par64(const XcArgs& a, int ax0, int ay, float2 (&x)[32]) {
+  const int xi0 = ax0 - a.left_x, yi = ay - a.left_y;
+  const bool row_in = (unsigned)yi < (unsigned)a.ny;
+  const bool inside = row_in && xi0 >= 0 && xi0 + 64 <= a.nx;
+  float f1[32], f2[32];
+  if (inside) load_u8_row_par<64, P>(a.f1 + (long long)yi * a.pitch1 + xi0, f1);
+  else load_u8_row_edge_par<64, P>(a.f1, a.pitch1, a.nx, a.ny, xi0, yi, f1);
+  if (a.fdef.col) {
+    static_for<32>([&](auto c) { x[c].x = f1[c]; });
+    return;
+  }
+  if (a.f2def) {
+    const float* q = a.f2def + (long long)ay * a.def_pitch + ax0;
+    static_for<16>([&](auto qc) {
+      constexpr int k = qc;
+      float4 t;
+      __builtin_memcpy(&t, q + 4 * k, 16);
+      f2[2 * k + 0] = P ? t.y : t.x;
+      f2[2 * k + 1] = P ? t.w : t.z;
+    });
+  } else if (inside) {
+    load_u8_row_par<64, P>(a.f2u8 + (long long)yi * a.pitch2 + xi0, f2);
+  } else {
+    load_u8_row_edge_par<64, P>(a.f2u8, a.pitch2, a.nx, a.ny, xi0, yi, f2);
+  }
+  static_for<32>([&](auto c) { x[c] = make_float2(f1[c], f2[c]); });
+}
+
+// w^(q k) (w = e^(2 pi i / 64), inverse sign) on the 16 values of thread
+// quarter q, q in 0..3 (uniform within each half-wave).
+__device__ __forceinline__ void quarter_twiddle(float2 (&f)[16], int q) {
+  if (q == 1) static_for<16>([&](auto k) { f[k] = twiddle<k, 64, true>(f[k]); });
+  else if (q == 2) static_for<16>([&](auto k) { f[k] = twiddle<2 * k, 64, true>(f[k]); });
+  else if (q == 3) static_for<16>([&](auto k) { f[k] = twiddle<3 * k, 64, true>(f[k]); });
+}
+
+// In-place inverse radix-4 butterflies over the exchange plane x4[(16 q + k)
+// * kW2GS + c] (q = input quarter), for k = 4 kb .. 4 kb + 3 of column c:
+// output p lands in slot (16 p + k), the line's element k + 16 p.
+__device__ __forceinline__ void radix4_inplace(float2* x4, int kb, int c, float2 (&out)[16]) {
+  static_for<4>([&](auto jc) {
+    constexpr int j = jc;
+    const int k = 4 * kb + j;
+    const float2 t0 = x4[(k)*kW2GS + c], t1 = x4[(16 + k) * kW2GS + c];
+    const float2 t2 = x4[(32 + k) * kW2GS + c], t3 = x4[(48 + k) * kW2GS + c];
+    const float2 a0 = cadd(t0, t2), a1 = csub(t0, t2), b0 = cadd(t1, t3), b1 = csub(t1, t3);
+    out[4 * j + 0] = cadd(a0, b0);                                   // p = 0
+    out[4 * j + 1] = make_float2(a1.x - b1.y, a1.y + b1.x);          // p = 1: a1 + i b1
+    out[4 * j + 2] = csub(a0, b0);                                   // p = 2
+    out[4 * j + 3] = make_float2(a1.x + b1.y, a1.y - b1.x);          // p = 3: a1 - i b1
+  });
+}
+
+__device__ __forceinline__ void xcorr_single64x2_body(const XcArgs& a, int w, float* lds, float2* pnyq) {
+  constexpr int N = 64, H = 32, SX = kW2SX, GS = kW2GS;
+  const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+  const int iy = w / a.nx_vec, ix = w - iy * a.nx_vec;
+  const int ax0 = ix * a.sp_x, ay0 = iy * a.sp_y;
+  float* const sp = lds;
+
+  // 0. fused deformation: wave h deforms frame-2 rows 32h..32h+31 into the plane
+  if (a.fdef.col) {
+    deform_window<N, N, SX>(a, ax0, ay0, sp, lane, H * wv, H * wv + H);
+    __syncthreads();
+  }
+  // 1. row FFTs: wave = sample parity, lane = row
+  float2 v[H];
+  if (wv == 0) load_row_par64<0>(a, ax0, ay0 + lane, v);
+  else load_row_par64<1>(a, ax0, ay0 + lane, v);
+  if (a.fdef.col) {
+    static_for<H>([&](auto n) { v[n].y = sp[lane * SX + 2 * n + wv]; });
+    __syncthreads();  // the deformed plane is read before it is rewritten
+  }
+  fft_reg<H, false>(v);
+  if (wv) static_for<H>([&](auto k) { v[k] = twiddle<k, N, false>(v[k]); });
+  // E (wave 0) / T (wave 1) of row `lane` as plane columns 32 wv + k; the
+  // column phase reads X[r][c] = E[r][c % 32] +- T[r][c % 32] (+ for c < 32)
+  float2 cz[H];
+  const int cm = lane & (H - 1);
+  const float cs = lane < H ? 1.f : -1.f;
+  static_for<H>([&](auto k) { sp[lane * SX + H * wv + k] = v[k].x; });
+  __syncthreads();
+  static_for<H>([&](auto nc) {
+    const int r = 2 * nc + wv;
+    cz[nc].x = fmaf(cs, sp[r * SX + H + cm], sp[r * SX + cm]);
+  });
+  __syncthreads();
+  static_for<H>([&](auto k) { sp[lane * SX + H * wv + k] = v[k].y; });
+  __syncthreads();
+  static_for<H>([&](auto nc) {
+    const int r = 2 * nc + wv;
+    cz[nc].y = fmaf(cs, sp[r * SX + H + cm], sp[r * SX + cm]);
+  });
+  // 2. column FFTs: wave = row parity, lane = column
+  fft_reg<H, false>(cz);
+  if (wv) static_for<H>([&](auto k) { cz[k] = twiddle<k, N, false>(cz[k]); });
+  __syncthreads();  // every read of the row plane is done
+
+  // 3. product on the half spectrum and column IFFTs split four ways.
+  //    E' (wave 0) / T' (wave 1) of column `lane` as plane rows 32 wv + k:
+  //    Z[ky][c] = E'[ky % 32][c] +- T'[ky % 32][c] (+ for ky < 32).
+  //    Thread (q, t): column t < 32, rows ky = 4m + q.
+  const int q = tid >> 5, t = tid & 31;
+  const int tm = (N - t) & (N - 1);
+  float2 zo[16], zm[16];
+  float2 n0, n1;  // column 32 at row ky = tid and its mirror (wave 0)
+  const int kyn = lane, kynm = (N - lane) & (N - 1);
+  auto zread = [&](int ky, int c) {
+    const int kk = ky & (H - 1);
+    const float sg = ky < H ? 1.f : -1.f;
+    return fmaf(sg, sp[(H + kk) * SX + c], sp[kk * SX + c]);
+  };
+  static_for<H>([&](auto k) { sp[(H * wv + k) * SX + lane] = cz[k].x; });
+  __syncthreads();
+  static_for<16>([&](auto mc) {
+    constexpr int m = mc;
+    const int ky = 4 * m + q;
+    zo[m].x = zread(ky, t);
+    zm[m].x = zread((N - ky) & (N - 1), tm);
+  });
+  if (wv == 0) {
+    n0.x = zread(kyn, H);
+    n1.x = zread(kynm, H);
+  }
+  __syncthreads();
+  static_for<H>([&](auto k) { sp[(H * wv + k) * SX + lane] = cz[k].y; });
+  __syncthreads();
+  static_for<16>([&](auto mc) {
+    constexpr int m = mc;
+    const int ky = 4 * m + q;
+    zo[m].y = zread(ky, t);
+    zm[m].y = zread((N - ky) & (N - 1), tm);
+  });
+  if (wv == 0) {
+    n0.y = zread(kyn, H);
+    n1.y = zread(kynm, H);
+    pnyq[lane] = xprod(n0, n1);
+  }
+  __syncthreads();  // pnyq written; every read of the column plane is done
+  float2* const x4 = reinterpret_cast<float2*>(lds);
+  {
+    // threads t == 0 add i * P[ky][32] (the broadcast read is harmless elsewhere)
+    const float f = t == 0 ? 1.f : 0.f;
+    float2 qv[16];
+    static_for<16>([&](auto mc) {
+      constexpr int m = mc;
+      const float2 p = xprod(zo[m], zm[m]);
+      const float2 pn = pnyq[4 * m + q];
+      qv[m] = make_float2(fmaf(-f, pn.y, p.x), fmaf(f, pn.x, p.y));
+    });
+    fft_reg<16, true>(qv);
+    quarter_twiddle(qv, q);
+    static_for<16>([&](auto k) { x4[(16 * q + k) * GS + t] = qv[k]; });
+  }
+  __syncthreads();
+  {
+    float2 g[16];
+    radix4_inplace(x4, q, t, g);  // kb = q: k = 4q .. 4q + 3 of column t
+    static_for<4>([&](auto jc) {
+      static_for<4>([&](auto pc) {
+        constexpr int j = jc, pp = pc;
+        x4[(16 * pp + 4 * q + j) * GS + t] = g[4 * j + pp];
+      });
+    });
+  }
+  __syncthreads();
+
+  // 4. row IFFTs: thread (q, y), y < 32, samples kx = 4m + q of H_y
+  const int y = t;
+  {
+    const float2* const g0 = x4 + y * GS;
+    const float2* const g1 = x4 + (y + H) * GS;
+    float2 hv[16];
+    static_for<16>([&](auto mc) {
+      constexpr int m = mc;
+      if constexpr (m == 0) {
+        // kx = q: the packed real column 0 when q == 0
+        const float2 a0 = g0[q], a1 = g1[q];
+        hv[m] = q == 0 ? make_float2(a0.x, a1.x) : make_float2(a0.x - a1.y, a0.y + a1.x);
+      } else if constexpr (m < 8) {
+        const float2 a0 = g0[4 * m + q], a1 = g1[4 * m + q];
+        hv[m] = make_float2(a0.x - a1.y, a0.y + a1.x);
+      } else if constexpr (m == 8) {
+        // kx = 32 + q: the packed column's imaginary part when q == 0, else
+        // the conjugate of column 32 - q
+        const int c = q == 0 ? 0 : H - q;
+        const float2 a0 = g0[c], a1 = g1[c];
+        hv[m] = q == 0 ? make_float2(a0.y, a1.y) : make_float2(a0.x + a1.y, a1.x - a0.y);
+      } else {
+        const int c = N - 4 * m - q;
+        const float2 a0 = g0[c], a1 = g1[c];
+        hv[m] = make_float2(a0.x + a1.y, a1.x - a0.y);
+      }
+    });
+    fft_reg<16, true>(hv);
+    quarter_twiddle(hv, q);
+    __syncthreads();  // every read of G is done
+    static_for<16>([&](auto k) { x4[(16 * q + k) * GS + y] = hv[k]; });
+  }
+  __syncthreads();
+  {
+    float2 o[16];
+    radix4_inplace(x4, q, y, o);  // outputs x = 4q + j + 16 p of row pair y
+    __syncthreads();              // every read of the exchange plane is done
+    static_for<4>([&](auto jc) {
+      static_for<4>([&](auto pc) {
+        constexpr int j = jc, pp = pc;
+        const int xo = 4 * q + j + 16 * pp;
+        const int cx = (xo + H) & (N - 1);
+        const float2 r = o[4 * j + pp];
+        sp[(y + H) * SX + cx] = cc_scale<N * N>(r.x);
+        sp[y * SX + cx] = cc_scale<N * N>(r.y);
+      });
+    });
+  }
+  __syncthreads();
+  // 5. peaks (wave 0; wave 1 is done with this window)
+  if (wv == 0) single_peaks<N, N, SX>(a, w, sp, lane);
+}
+
+__global__ void __launch_bounds__(128) k_xcorr_single64x2(XcArgs a) {
+  if (a.gate && *a.gate == 0) return;  // learned pipeline: grid 1 runs only as the fallback
+  __shared__ __align__(16) float lds[kW2LdsFloats];
+  __shared__ float2 pnyq[64];
+  xcorr_single64x2_body(a, blockIdx.x, lds, pnyq);
+}
+
+// The persistent batched form of the two-wave 64x64 kernel (same queue and
+// claim order as k_xcorr_batch; `workgroups` counts two-wave workgroups).
+__global__ void __launch_bounds__(128) k_xcorr_batch64x2(BatchQueue q) {
+  constexpr int NXCD = 8;
+  __shared__ __align__(16) float lds[kW2LdsFloats];
+  __shared__ float2 pnyq[64];
+  __shared__ int claim;
+  const int home = blockIdx.x % NXCD;
+  for (int r = 0; r < NXCD; ++r) {
+    const int x = (home + r) % NXCD;
+    const int lo = (int)((long long)q.total * x / NXCD), hi = (int)((long long)q.total * (x + 1) / NXCD);
+    int p = 0;
+    for (;;) {
+      __syncthreads();  // the last item's planes and the last claim are read
+      if (threadIdx.x == 0) claim = atomicAdd(q.counter + x, q.chunk);
+      __syncthreads();
+      const int base = __builtin_amdgcn_readfirstlane(claim) + lo;  // uniform: arguments by scalar loads
+      if (base >= hi) break;  // this range is drained (both waves see the same claim)
+      const int end = min(base + q.chunk, hi);
+      for (int item = base; item < end; ++item) {
+        while (item >= q.item_start[p + 1]) ++p;  // claims within one range only increase
+        if (item > base) __syncthreads();         // the previous item's peak stage is done
+        xcorr_single64x2_body(q.args[p], item - q.item_start[p], lds, pnyq);
+      }
+    }
+  }
 }
 
 // ============================================================== generic fused kernel
@@ -2081,10 +2405,8 @@ __global__ void __launch_bounds__(256) k_median2(MedArgs a) {
   constexpr int HX = (WXM - 1) / 2, HY = (WYM - 1) / 2;
   const int nt = a.nx_vec * a.ny_vec;
   if (a.reset_count && blockIdx.x == 0 && threadIdx.x == 0) *a.reset_count = 0;
-  const int nb = (2 * nt + 255) / 256;
-  const int blk = xcd_block(nb);  // rows of neighbouring blocks share an XCD's L2
-  if (blk >= nb) return;
-  const int g = blk * 256 + (int)threadIdx.x;
+  // neighbouring blocks read mostly the same rows: keep them on one XCD's L2
+  const int g = xcd_remap(blockIdx.x, gridDim.x) * 256 + (int)threadIdx.x;
   const int t = g >> 1, c = g & 1;
   const bool live = t < nt;
   const int tt = live ? t : nt - 1;  // both lanes of a pair stay active for the swap
@@ -2381,7 +2703,7 @@ __device__ __forceinline__ int mhe_neigh_count(int tx, int ty, int mx, int my) {
 __global__ void __launch_bounds__(256) k_mhe_hist(MheArgs a) {
   __shared__ uint32_t hs[4][MHE_SUB][257];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int tile = xcd_block((a.tiles_x * a.tiles_y + 3) / 4) * 4 + wv;  // tile rows banded per XCD
+  const int tile = xcd_remap(blockIdx.x, gridDim.x) * 4 + wv;  // tile rows banded per XCD
   if (tile >= a.tiles_x * a.tiles_y) return;
   uint32_t (*h)[257] = hs[wv];
   const int sub = lane & (MHE_SUB - 1);
@@ -2446,7 +2768,7 @@ __global__ void __launch_bounds__(256) k_mhe_hist(MheArgs a) {
 __global__ void __launch_bounds__(256) k_mhe_map(MheArgs a) {
   __shared__ uint8_t luts[4][256];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int tile = xcd_block((a.tiles_x * a.tiles_y + 3) / 4) * 4 + wv;  // tile rows banded per XCD
+  const int tile = xcd_remap(blockIdx.x, gridDim.x) * 4 + wv;  // tile rows banded per XCD
   if (tile >= a.tiles_x * a.tiles_y) return;
   uint8_t* lut = luts[wv];
   const int ty = tile / a.tiles_x, tx = tile - ty * a.tiles_x;
@@ -2549,7 +2871,10 @@ static hipError_t launch_pair(const XcArgs& a0, hipStream_t s) {
 
 template <int WX, int WY>
 static hipError_t launch_single(const XcArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL((k_xcorr_single<WX, WY>), dim3(a.nx_vec * a.ny_vec), dim3(64), 0, s, a);
+  if constexpr (WX == 64 && WY == 64 && kSingle64TwoWaves)
+    hipLaunchKernelGGL(k_xcorr_single64x2, dim3(a.nx_vec * a.ny_vec), dim3(128), 0, s, a);
+  else
+    hipLaunchKernelGGL((k_xcorr_single<WX, WY>), dim3(a.nx_vec * a.ny_vec), dim3(64), 0, s, a);
   return hipGetLastError();
 }
 
@@ -2703,7 +3028,10 @@ hipError_t launch_xcorr_batch(int ws_x, int ws_y, const BatchQueue& q, int workg
   if (e != hipSuccess) return e;
 #define PIVX_CASE(WX, WY, PAIR)                                                                     \
   if (ws_x == WX && ws_y == WY) {                                                                   \
-    hipLaunchKernelGGL((k_xcorr_batch<WX, WY, PAIR>), dim3(workgroups), dim3(64), 0, s, q);       \
+    if (WX == 64 && WY == 64 && kSingle64TwoWaves) /* two waves per workgroup */                  \
+      hipLaunchKernelGGL(k_xcorr_batch64x2, dim3(std::max(1, workgroups / 2)), dim3(128), 0, s, q);  \
+    else                                                                                            \
+      hipLaunchKernelGGL((k_xcorr_batch<WX, WY, PAIR>), dim3(workgroups), dim3(64), 0, s, q);     \
     return hipGetLastError();                                                                       \
   }
   PIVX_BATCH_SIZES(PIVX_CASE)
@@ -2736,7 +3064,7 @@ hipError_t launch_median(const MedArgs& a, hipStream_t s) {
   const int nt = a.nx_vec * a.ny_vec;
   const int ntm = a.wx * a.wy;
   dim3 g(grid_for(nt, 128, 1 << 30)), b(128);
-  const dim3 g2(xcd_grid((int)((2LL * nt + 255) / 256))), b2(256);
+  const dim3 g2(grid_for(2LL * nt, 256, 1 << 30)), b2(256);
   if (a.wx == 5 && a.wy == 5) hipLaunchKernelGGL((k_median2<5, 5>), g2, b2, 0, s, a);
   else if (a.wx == 3 && a.wy == 3) hipLaunchKernelGGL((k_median2<3, 3>), g2, b2, 0, s, a);
   else if (a.wx == 7 && a.wy == 7) hipLaunchKernelGGL((k_median2<7, 7>), g2, b2, 0, s, a);
@@ -2786,7 +3114,7 @@ hipError_t launch_stream_copy(const void* src, void* dst, size_t bytes, hipStrea
 hipError_t launch_mhe(const MheArgs& a, hipStream_t s) {
   const int tiles = a.tiles_x * a.tiles_y;
   if (tiles < 1) return hipErrorInvalidValue;
-  const dim3 g(xcd_grid((tiles + 3) / 4)), b(256);
+  const dim3 g((tiles + 3) / 4), b(256);
   hipLaunchKernelGGL(k_mhe_hist, g, b, 0, s, a);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;

@@ -61,6 +61,39 @@ def test_batch_equals_single_pairs(pivx):
             x.close()
 
 
+# config-4 frame sizes with 64x64 / 32x32 windows (the two-wave 64x64
+# correlation kernel and its persistent form)
+SIZES_64 = [(4096, 2048, 0.05), (1024, 1024, 0.02), (6600, 4400, 0.05), (2048, 1536, 0.08)]
+
+
+def test_batch_equals_single_pairs_64x64_large_frames(pivx):
+    fp = pivx.FilterParams(2, 5, 2.0, 2.0, 1)
+    xs = []
+    try:
+        for i, (nx, ny, dens) in enumerate(SIZES_64):
+            p = pivx.init_params(nx=nx, ny=ny, ws_x1=64, ws_y1=64, spacing_x1=32, spacing_y1=32, npass1=2,
+                                 ws_x2=32, ws_y2=32, spacing_x2=16, spacing_y2=16, npass2=2)
+            x = pivx.Xcorr2dMulti(p)
+            xs.append(x)
+            x.synth_pair_device(pivx.synth_params(nx, ny, seed=5000 + i, density=dens, flow=2, a0=10.0))
+        alone = []
+        for x in xs:
+            x.run_pair(fp, fp)
+            alone.append(_fields(x))
+        with pivx.Batch(xs) as b:
+            b.autotune(1)
+            for graph in (False, True):
+                b.set_graph_mode(graph)
+                b.run(fp, fp)
+                b.synchronize()
+                for x, want in zip(xs, alone):
+                    _equal(_fields(x), want)
+            _report("batch 64x64", pairs=len(xs), grid1=b.tuning(1), grid2=b.tuning(2))
+    finally:
+        for x in xs:
+            x.close()
+
+
 def test_batch_rejects_mixed_windows(pivx):
     a = pivx.Xcorr2dMulti(_params(pivx, 1024, 1024))
     p = _params(pivx, 1024, 1024)
