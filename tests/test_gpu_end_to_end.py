@@ -244,3 +244,46 @@ def test_config2_full_size_vs_oracle(pivx, orc):
     check_pass("config2 grid-2 deformed pass", gpu_g2, ref2, (32, 32), 4)
     ref = orc.run_pair(a, b, p, fp, fp, prec=64, threads=THREADS)
     _compare_fields("end to end 6600x4400", gpu, ref, rms_tol=1e-3, far_frac=5e-3, valid_frac=5e-3)
+
+
+@pytest.mark.parametrize("ws", [(32, 32), (64, 64), (16, 16), (48, 24), (24, 24)])
+def test_delta_windows_zero_neighbours(pivx, orc, ws):
+    """Windows holding one bright pixel at their first pixel in both frames
+    (no window overlap): the correlation is a delta at zero shift with
+    neighbours exactly 0 in the register FFTs (radices 2, 3 and 4: every
+    nonzero value meets only trivial twiddles, every other sum is of zeros), so the
+    3-point fit takes log(0) = -inf, the estimate is NaN and the NaN removal
+    of kernel_remove_nan (cu:457-471) leaves an invalid vector with zero
+    displacement.  Every third window is left empty (all zero): no local
+    maximum.  GPU and FP64 oracle agree exactly on
+    candidate counts, indices, values and validity."""
+    nx, ny = 512, 384
+    p = pivx.init_params(nx=nx, ny=ny, ws_x1=ws[0], ws_y1=ws[1], spacing_x1=ws[0], spacing_y1=ws[1], npass1=1,
+                         npass2=0, max_dispx_percent1=0.4, max_dispy_percent1=0.4)
+    with pivx.Xcorr2dMulti(p) as x:
+        g = geom_dict(x.geometry(1))
+        a = np.zeros((ny, nx), np.uint8)
+        b = np.zeros((ny, nx), np.uint8)
+        for iy in range(g["ny_vec"]):
+            for ix in range(g["nx_vec"]):
+                xi, yi = ix * ws[0] - g["left_x"], iy * ws[1] - g["left_y"]
+                if 0 <= xi < nx and 0 <= yi < ny and (ix + iy) % 3:
+                    a[yi, xi] = 200
+                    b[yi, xi] = 100
+        x.prep_img_data(a, b)
+        x.Calc_cc_map_p1(0)
+        gpu = gpu_pass(x, 1)
+    ref = orc.xcorr_pass(a, f2u8=b, geom=g, ws=ws, sp=ws, K=4, tie_rel=tie_margin(*ws))
+    delta = ref["ncand"] == 1
+    _report(f"delta windows {ws[0]}x{ws[1]}", windows=len(delta), delta=int(delta.sum()),
+            empty=int((ref["ncand"] == 0).sum()))
+    assert delta.sum() > 0 and (ref["ncand"] == 0).sum() > 0
+    np.testing.assert_array_equal(gpu["nc"], ref["ncand"])
+    np.testing.assert_array_equal(gpu["idx"][delta, 0], ref["pk_idx"][delta, 0])
+    np.testing.assert_array_equal(gpu["idx"][delta, 0], (ws[1] // 2) * ws[0] + ws[0] // 2)
+    np.testing.assert_array_equal(gpu["val"][delta, 0], np.float32(200 * 100))
+    np.testing.assert_array_equal(gpu["val"][delta, 0], ref["pk_val"][delta, 0])
+    np.testing.assert_array_equal(gpu["v"], ref["valid"])
+    assert not gpu["v"].any()
+    np.testing.assert_array_equal(gpu["dx"][delta, 0], 0.0)
+    np.testing.assert_array_equal(gpu["dy"][delta, 0], 0.0)
