@@ -129,8 +129,8 @@ struct BatchQueue {
   const XcArgs* args;      // [npairs] per-pair arguments of the pass (device); chunks_x set for pair shapes
   const int* item_start;   // [npairs + 1] first work item of every pair; item_start[npairs] = total
   int npairs, total;
-  int* counter;            // [8] claimed items per XCD range (zeroed by the launcher's first kernel)
-  int chunk;               // consecutive items claimed per atomic
+  int* counter;            // [8] claims taken per XCD range (zeroed by the launcher's first kernel)
+  int chunk;               // items per claim (interleaved with other workgroups' claims)
 };
 bool xcorr_batch_supported(int ws_x, int ws_y);
 // work items of one pair of the given grid size (chunks_x: window pairs per vector row, 0 for single-window shapes)

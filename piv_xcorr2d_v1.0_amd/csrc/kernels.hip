@@ -732,8 +732,8 @@ __device__ __forceinline__ void load_u8_row_edge(const uint8_t* img, long long p
 // One window row of WX pixels starting at aug coordinates (ax0, ay):
 // x[c] = (frame 1, frame 2).  Rows fully inside the image use wide loads;
 // rows crossing the frame edge use masked 4-byte loads with 0 outside.
-template <int WX>
-__device__ __forceinline__ void load_row(const XcArgs& a, int ax0, int ay, float2 (&x)[WX]) {
+template <int WX, class A>
+__device__ __forceinline__ void load_row(const A& a, int ax0, int ay, float2 (&x)[WX]) {
   const int xi0 = ax0 - a.left_x, yi = ay - a.left_y;
   const bool row_in = (unsigned)yi < (unsigned)a.ny;
   const bool inside = row_in && xi0 >= 0 && xi0 + WX <= a.nx;
@@ -771,10 +771,10 @@ __device__ __forceinline__ void load_row(const XcArgs& a, int ax0, int ay, float
 // p[r * SX + c] (FusedDef): lane = window column c, rows r_lo..r_hi - 1 (a
 // multiple of 8 each) in order; the row terms are wave-uniform and come
 // through scalar loads, and the cell changes with the row, as in k_deform.
-template <int WX, int WY, int SX>
-__device__ __forceinline__ void deform_window(const XcArgs& a, int ax0, int ay0, float* p, int lane, int r_lo = 0,
+template <int WX, int WY, int SX, class A>
+__device__ __forceinline__ void deform_window(const A& a, int ax0, int ay0, float* p, int lane, int r_lo = 0,
                                               int r_hi = WY) {
-  const FusedDef& d = a.fdef;
+  const auto& d = a.fdef;
   if (lane >= WX) return;
   const float4 ct = d.col[ax0 + lane];
   const int ixf = __float_as_int(ct.z);
@@ -834,8 +834,8 @@ constexpr int pair_lds_floats() { return 2 * WY * (WX + 1); }
 
 // Work item `bid` (window pair bid % chunks_x of vector row bid / chunks_x)
 // of the wave-pair kernel; lds holds pair_lds_floats<WX, WY>() floats.
-template <int WX, int WY>
-__device__ __forceinline__ void xcorr_pair_body(const XcArgs& a, int bid, float* lds) {
+template <int WX, int WY, class A>
+__device__ __forceinline__ void xcorr_pair_body(const A& a, int bid, float* lds) {
   static_assert(WX <= 32 && WY <= 32 && WX % 4 == 0 && WY % 2 == 0, "pair kernel covers windows up to 32x32");
   constexpr int SX = WX + 1;
   constexpr int NPIX = WX * WY;
@@ -1059,8 +1059,8 @@ constexpr int single_lds_floats() {
 // stride SX floats, in LDS): lane l walks column l of the window, keeping a
 // bit mask of the rows that are local maxima; Npeak_save rounds of 64-lane
 // arg-max; Gaussian fits of the winners.
-template <int WX, int WY, int SX>
-__device__ __forceinline__ void single_peaks(const XcArgs& a, int w, const float* cc, int lane) {
+template <int WX, int WY, int SX, class A>
+__device__ __forceinline__ void single_peaks(const A& a, int w, const float* cc, int lane) {
   using Bits = typename std::conditional<(WY <= 32), uint32_t, uint64_t>::type;
   const bool col_ok = lane < WX && lane >= a.bx && lane <= WX - 1 - a.bx;
   const int cl = lane < WX ? lane : WX - 1;
@@ -1124,8 +1124,8 @@ __device__ __forceinline__ void single_peaks(const XcArgs& a, int w, const float
 
 // Window w of the wave-per-window kernel; lds holds single_lds_floats floats,
 // pnyq WY float2.
-template <int WX, int WY>
-__device__ __forceinline__ void xcorr_single_body(const XcArgs& a, int w, float* lds, float2* pnyq) {
+template <int WX, int WY, class A>
+__device__ __forceinline__ void xcorr_single_body(const A& a, int w, float* lds, float2* pnyq) {
   static_assert(WX <= 64 && WY <= 64 && WX % 4 == 0 && WY % 4 == 0, "single-wave kernel covers windows up to 64x64");
   constexpr int SX = WX + 1;   // real plane row stride (floats)
   constexpr int NPIX = WX * WY;
@@ -1295,6 +1295,31 @@ __global__ void __launch_bounds__(64) k_xcorr_single(XcArgs a) {
   xcorr_single_body<WX, WY>(a, blockIdx.x, lds, pnyq);
 }
 
+// ============================================================== batched work queue
+// The queue's tables are read through the constant address space: with a
+// uniform index these are scalar loads at the point of use, as for the
+// per-pair kernels' kernel arguments (read through a generic pointer they
+// would be flat loads into VGPRs, reloaded after every store; copied out
+// whole they would hold 60 SGPRs for the whole pass).
+__device__ __forceinline__ int batch_start(const BatchQueue& q, int p) {
+  return ((const __attribute__((address_space(4))) int*)q.item_start)[p];
+}
+typedef __attribute__((address_space(4))) XcArgs ConstXcArgs;
+// Claims of an XCD range [lo, hi): claim n covers `chunk` items interleaved
+// with the claims of the range's other g workgroups, items
+// lo + (n / g) g chunk + n % g + j g for j < chunk.  The workgroups of a
+// range then work on neighbouring items at any moment (windows sharing rows
+// meet in the XCD's L2) however many items a claim holds, and the first
+// items of successive claims only increase: a claim whose first item is past
+// the range ends the walk.  Every item of the range belongs to exactly one
+// claim.
+__device__ __forceinline__ long long batch_claim_first(int n, int lo, int g, int chunk) {
+  return (long long)lo + (long long)(n / g) * g * chunk + n % g;
+}
+__device__ __forceinline__ const ConstXcArgs& batch_args(const BatchQueue& q, int p) {
+  return ((const ConstXcArgs*)q.args)[p];
+}
+
 // ============================================================== two-wave 64x64 kernel
 //
 // One 128-thread workgroup (two waves) per 64x64 window.  The window's
@@ -1368,8 +1393,8 @@ __device__ __forceinline__ void load_u8_row_edge_par(const uint8_t* img, long lo
 // Samples 2n + P of one 64-pixel window row at aug (ax0, ay) as load_row
 // reads them: x[n] = (frame 1, frame 2); with the fused deformation frame 2
 // comes from the plane (the caller).
-template <int P>
-__device__ __forceinline__ void load_row_par64(const XcArgs& a, int ax0, int ay, float2 (&x)[32]) {
+template <int P, class A>
+__device__ __forceinline__ void load_row_par64(const A& a, int ax0, int ay, float2 (&x)[32]) {
   const int xi0 = ax0 - a.left_x, yi = ay - a.left_y;
   const bool row_in = (unsigned)yi < (unsigned)a.ny;
   const bool inside = row_in && xi0 >= 0 && xi0 + 64 <= a.nx;
@@ -1422,7 +1447,8 @@ __device__ __forceinline__ void radix4_inplace(float2* x4, int kb, int c, float2
   });
 }
 
-__device__ __forceinline__ void xcorr_single64x2_body(const XcArgs& a, int w, float* lds, float2* pnyq) {
+template <class A>
+__device__ __forceinline__ void xcorr_single64x2_body(const A& a, int w, float* lds, float2* pnyq) {
   constexpr int N = 64, H = 32, SX = kW2SX, GS = kW2GS;
   const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
   const int iy = w / a.nx_vec, ix = w - iy * a.nx_vec;
@@ -1604,21 +1630,27 @@ __global__ void __launch_bounds__(128) k_xcorr_batch64x2(BatchQueue q) {
   __shared__ float2 pnyq[64];
   __shared__ int claim;
   const int home = blockIdx.x % NXCD;
+  const int g = max(1, (int)gridDim.x / NXCD);  // workgroups per XCD range
   for (int r = 0; r < NXCD; ++r) {
     const int x = (home + r) % NXCD;
     const int lo = (int)((long long)q.total * x / NXCD), hi = (int)((long long)q.total * (x + 1) / NXCD);
-    int p = 0;
+    int p0 = 0;
     for (;;) {
       __syncthreads();  // the last item's planes and the last claim are read
-      if (threadIdx.x == 0) claim = atomicAdd(q.counter + x, q.chunk);
+      if (threadIdx.x == 0) claim = atomicAdd(q.counter + x, 1);
       __syncthreads();
-      const int base = __builtin_amdgcn_readfirstlane(claim) + lo;  // uniform: arguments by scalar loads
-      if (base >= hi) break;  // this range is drained (both waves see the same claim)
-      const int end = min(base + q.chunk, hi);
-      for (int item = base; item < end; ++item) {
-        while (item >= q.item_start[p + 1]) ++p;  // claims within one range only increase
-        if (item > base) __syncthreads();         // the previous item's peak stage is done
-        xcorr_single64x2_body(q.args[p], item - q.item_start[p], lds, pnyq);
+      const long long first = batch_claim_first(__builtin_amdgcn_readfirstlane(claim), lo, g, q.chunk);
+      if (first >= hi) break;  // this range is drained (both waves see the same claim)
+      while (first >= batch_start(q, p0 + 1)) ++p0;
+      int p = p0;
+      for (int j = 0; j < q.chunk; ++j) {
+        const long long it = first + (long long)j * g;
+        if (it >= hi) break;
+        const int item = (int)it;
+        while (item >= batch_start(q, p + 1)) ++p;  // the items of a claim increase
+        if (j > 0) __syncthreads();                 // the previous item's peak stage is done
+        const ConstXcArgs& a = batch_args(q, p);
+        xcorr_single64x2_body(a, item - batch_start(q, p), lds, pnyq);
       }
     }
   }
@@ -1827,8 +1859,9 @@ __global__ void __launch_bounds__(256) k_xcorr_generic(XcArgs a, int WX, int WY,
 // One persistent launch runs a correlation pass over many pairs of uneven
 // size: every pair contributes its work items (window pairs of the wave-pair
 // kernel, windows of the wave-per-window kernel) to one queue, and each
-// workgroup claims `chunk` consecutive items at a time from an atomic counter
-// until the queue is empty.  Each item runs the same body as the per-pair
+// workgroup claims `chunk` items at a time (interleaved with the other
+// workgroups' claims, batch_claim_first) from an atomic counter until the
+// queue is empty.  Each item runs the same body as the per-pair
 // kernel with that pair's arguments, so results are bit-identical.
 // The queue is split into one contiguous range per XCD (workgroups are
 // dispatched to the 8 XCDs round-robin by workgroup index), so that
@@ -1838,7 +1871,7 @@ __global__ void __launch_bounds__(256) k_xcorr_generic(XcArgs a, int WX, int WY,
 // waves per SIMD the per-pair kernels of a shape reach (their register use),
 // kept by the persistent form
 template <int WX, int WY, bool PAIR>
-constexpr int batch_waves_per_eu() { return PAIR ? 4 : 2; }  // single: 3 would spill
+constexpr int batch_waves_per_eu() { return PAIR ? 4 : (WX * WY <= 48 * 24 ? 3 : 2); }  // larger: 3 spills
 
 template <int WX, int WY, bool PAIR>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(batch_waves_per_eu<WX, WY, PAIR>())))
@@ -1848,22 +1881,26 @@ k_xcorr_batch(BatchQueue q) {
   __shared__ float lds[NL];
   __shared__ float2 pnyq[WY];
   const int home = blockIdx.x % NXCD;
+  const int g = max(1, (int)gridDim.x / NXCD);  // workgroups per XCD range
   for (int r = 0; r < NXCD; ++r) {
     const int x = (home + r) % NXCD;
     const int lo = (int)((long long)q.total * x / NXCD), hi = (int)((long long)q.total * (x + 1) / NXCD);
-    int p = 0;
+    int p0 = 0;
     for (;;) {
-      int base = 0;
-      if (threadIdx.x == 0) base = atomicAdd(q.counter + x, q.chunk);
-      base = __shfl(base, 0, 64) + lo;
-      if (base >= hi) break;  // this range is drained (every wave gets here for every range)
-      const int end = min(base + q.chunk, hi);
-      for (int item = base; item < end; ++item) {
-        while (item >= q.item_start[p + 1]) ++p;  // claims within one range only increase
-        const int local = item - q.item_start[p];
-        // the pair's arguments are read in place (scalar loads) rather than copied
-        if constexpr (PAIR) xcorr_pair_body<WX, WY>(q.args[p], local, lds);
-        else xcorr_single_body<WX, WY>(q.args[p], local, lds, pnyq);
+      int n = 0;
+      if (threadIdx.x == 0) n = atomicAdd(q.counter + x, 1);
+      const long long first = batch_claim_first(__builtin_amdgcn_readfirstlane(__shfl(n, 0, 64)), lo, g, q.chunk);
+      if (first >= hi) break;  // this range is drained (every wave gets here for every range)
+      while (first >= batch_start(q, p0 + 1)) ++p0;
+      int p = p0;
+      for (int j = 0; j < q.chunk; ++j) {
+        const long long it = first + (long long)j * g;
+        if (it >= hi) break;
+        const int item = (int)it;
+        while (item >= batch_start(q, p + 1)) ++p;  // the items of a claim increase
+        const ConstXcArgs& a = batch_args(q, p);
+        if constexpr (PAIR) xcorr_pair_body<WX, WY>(a, item - batch_start(q, p), lds);
+        else xcorr_single_body<WX, WY>(a, item - batch_start(q, p), lds, pnyq);
         wave_sync();  // the next item rewrites the LDS planes
       }
     }
