@@ -1616,10 +1616,23 @@ __device__ __forceinline__ void xcorr_single64x2_body(const A& a, int w, float* 
 }
 
 __global__ void __launch_bounds__(128) k_xcorr_single64x2(XcArgs a) {
-  if (a.gate && *a.gate == 0) return;  // learned pipeline: grid 1 runs only as the fallback
   __shared__ __align__(16) float lds[kW2LdsFloats];
   __shared__ float2 pnyq[64];
   xcorr_single64x2_body(a, blockIdx.x, lds, pnyq);
+}
+
+// The same behind the learned pipeline's fallback gate: launched with fewer
+// workgroups, which walk the windows, so that the launch costs little when
+// the gate is closed (the usual case).
+__global__ void __launch_bounds__(128) k_xcorr_single64x2_gated(XcArgs a) {
+  if (*a.gate == 0) return;
+  __shared__ __align__(16) float lds[kW2LdsFloats];
+  __shared__ float2 pnyq[64];
+  const int nt = a.nx_vec * a.ny_vec;
+  for (int w = blockIdx.x; w < nt; w += gridDim.x) {
+    if (w != (int)blockIdx.x) __syncthreads();  // the last window's peak stage is done
+    xcorr_single64x2_body(a, w, lds, pnyq);
+  }
 }
 
 // The persistent batched form of the two-wave 64x64 kernel (same queue and
@@ -2110,15 +2123,18 @@ __global__ void __launch_bounds__(256) k_xcorr_dft(XcArgs a, DftTables tb, int s
 // (deform_terms); the four field samples of a cell are reloaded only when
 // the row crosses into the next cell; each pixel is deform_px.
 constexpr int DEF_ROWS = 16;
+// Grid caps of the kernels the learned pipeline launches behind its fallback
+// gate: an empty launch then dispatches a few hundred workgroups instead of
+// one per window or band.
+constexpr int kGatedBands = 32;
+constexpr int kGatedWindows = 1024;
 
-__global__ void __launch_bounds__(256) k_deform(DefArgs a) {
-  if (a.gate && *a.gate == 0) return;  // learned pipeline: grid 1 runs only as the fallback
-  __shared__ float4 rowt[DEF_ROWS];
-  const int y0 = blockIdx.y * DEF_ROWS;
+// One block: aug rows y0 .. y0 + DEF_ROWS - 1, columns 256 bx .. 256 bx + 255.
+__device__ __forceinline__ void deform_block(const DefArgs& a, int y0, int bx, float4* rowt) {
   const int nrow = min(DEF_ROWS, a.ny_aug - y0);
   if (threadIdx.x < nrow) rowt[threadIdx.x] = deform_terms(y0 + (int)threadIdx.x, a.y_st, a.spy, a.nye);
   __syncthreads();
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  const int idx = bx * 256 + (int)threadIdx.x;
   if (idx >= a.nx_aug) return;
   const float4 ct = deform_terms(idx, a.x_st, a.spx, a.nxe);
   const int ixf = __float_as_int(ct.z);
@@ -2182,6 +2198,19 @@ __global__ void __launch_bounds__(256) k_deform(DefArgs a) {
     for (int r = 0; r < nrow; ++r) row(r);
   }
   if (oob) atomicAdd(a.oob, (unsigned long long)oob);
+}
+
+// Blocks in row bands: gridDim.y is the band count, or fewer when the launch
+// is gated (the learned pipeline's fallback, which mostly does nothing: a
+// small grid keeps its empty launches cheap) and the bands are walked.
+__global__ void __launch_bounds__(256) k_deform(DefArgs a) {
+  if (a.gate && *a.gate == 0) return;  // learned pipeline: grid 1 runs only as the fallback
+  __shared__ float4 rowt[DEF_ROWS];
+  const int nby = (a.ny_aug + DEF_ROWS - 1) / DEF_ROWS;
+  for (int by = blockIdx.y; by < nby; by += gridDim.y) {
+    if (by != (int)blockIdx.y) __syncthreads();  // the last band's row terms are read
+    deform_block(a, by * DEF_ROWS, blockIdx.x, rowt);
+  }
 }
 
 // The per-column and per-row terms of k_deform as tables (FusedDef).
@@ -2908,8 +2937,13 @@ static hipError_t launch_pair(const XcArgs& a0, hipStream_t s) {
 
 template <int WX, int WY>
 static hipError_t launch_single(const XcArgs& a, hipStream_t s) {
-  if constexpr (WX == 64 && WY == 64 && kSingle64TwoWaves)
-    hipLaunchKernelGGL(k_xcorr_single64x2, dim3(a.nx_vec * a.ny_vec), dim3(128), 0, s, a);
+  if constexpr (WX == 64 && WY == 64 && kSingle64TwoWaves) {
+    if (a.gate)
+      hipLaunchKernelGGL(k_xcorr_single64x2_gated, dim3(std::min(a.nx_vec * a.ny_vec, kGatedWindows)), dim3(128), 0,
+                         s, a);
+    else
+      hipLaunchKernelGGL(k_xcorr_single64x2, dim3(a.nx_vec * a.ny_vec), dim3(128), 0, s, a);
+  }
   else
     hipLaunchKernelGGL((k_xcorr_single<WX, WY>), dim3(a.nx_vec * a.ny_vec), dim3(64), 0, s, a);
   return hipGetLastError();
@@ -3077,7 +3111,8 @@ hipError_t launch_xcorr_batch(int ws_x, int ws_y, const BatchQueue& q, int workg
 }
 
 hipError_t launch_deform(const DefArgs& a, hipStream_t s) {
-  const dim3 grid((a.nx_aug + 255) / 256, (a.ny_aug + DEF_ROWS - 1) / DEF_ROWS);
+  const int nby = (a.ny_aug + DEF_ROWS - 1) / DEF_ROWS;
+  const dim3 grid((a.nx_aug + 255) / 256, a.gate ? std::min(nby, kGatedBands) : nby);
   hipLaunchKernelGGL(k_deform, grid, dim3(256), 0, s, a);
   return hipGetLastError();
 }
