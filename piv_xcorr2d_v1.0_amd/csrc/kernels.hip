@@ -2623,7 +2623,10 @@ __global__ void __launch_bounds__(256) k_fill_list(FillArgs a) {
     }
 }
 
-__global__ void __launch_bounds__(1024) k_fill(FillArgs a) {
+// 256 threads (four waves): a single larger workgroup waited up to 175 us for
+// a CU with room beside concurrent correlation kernels of other handles; at
+// four waves it is placed at once (longest of 750 launches 8 us).
+__global__ void __launch_bounds__(256) k_fill(FillArgs a) {
 #pragma clang fp contract(off)
   if (a.gate && *a.gate == 0) return;  // learned pipeline: grid 1 runs only as the fallback
   __shared__ int n_holes, n_next, n_filled;
@@ -3157,7 +3160,7 @@ hipError_t launch_fill(const FillArgs& a, hipStream_t s) {
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
-  hipLaunchKernelGGL(k_fill, dim3(1), dim3(1024), 0, s, a);
+  hipLaunchKernelGGL(k_fill, dim3(1), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 
