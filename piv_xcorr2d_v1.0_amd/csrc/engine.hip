@@ -1315,6 +1315,35 @@ static int build_dft_tables(pivx_ctx* c, int gi, bool bf) {
   }
   HIPCHK(c, hipMemcpy(dev, buf.data(), total * sizeof(uint16_t), hipMemcpyHostToDevice));
   DftTables& T = c->dtab[gi];
+  T.lane32 = nullptr;
+  if (W == 32 && H == 32) {
+    // C and S in the MFMA lane layout: [m][t][hl][lane][j] = M[16t + lane % 16][8 (lane / 16) + j]
+    std::vector<uint16_t> lt(2 * 2 * 2 * 64 * 8);
+    for (int m = 0; m < 2; ++m)
+      for (int t = 0; t < 2; ++t)
+        for (int lane = 0; lane < 64; ++lane)
+          for (int j = 0; j < 8; ++j) {
+            const int row = 16 * t + (lane & 15), k = 8 * (lane >> 4) + j;
+            const float f = (float)cs(32, row, k, m == 1);
+            uint16_t h, l;
+            if (bf) {
+              h = host_bf16(f);
+              float hb;
+              const uint32_t hu = (uint32_t)h << 16;
+              std::memcpy(&hb, &hu, 4);
+              l = host_bf16(f - hb);
+            } else {
+              h = host_f16(f);
+              l = host_f16(f - host_f16_back(h));
+            }
+            lt[(((size_t)(m * 2 + t) * 2 + 0) * 64 + lane) * 8 + j] = h;
+            lt[(((size_t)(m * 2 + t) * 2 + 1) * 64 + lane) * 8 + j] = l;
+          }
+    uint16_t* ldev = nullptr;
+    if (dalloc(c, &ldev, lt.size())) return PIVX_ERR_HIP;
+    HIPCHK(c, hipMemcpy(ldev, lt.data(), lt.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
+    T.lane32 = ldev;
+  }
   for (int o = 0; o < 2; ++o) {
     T.a1h[o] = ptr[o][0];
     T.a1l[o] = ptr[o][1];

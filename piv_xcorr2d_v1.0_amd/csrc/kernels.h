@@ -120,6 +120,9 @@ struct DftTables {
   const uint16_t* b2h[2]; const uint16_t* b2l[2];   // [C; S], [-S; C] (step 2: Zr, Zi)
   const uint16_t* a3h[2]; const uint16_t* a3l[2];   // [C -S], [S C]   (step 3: Ur, Ui)
   const uint16_t* b4h; const uint16_t* b4l;         // [C; -S]         (step 4: cc)
+  // 32x32 windows: C and S in the MFMA lane layout of the wave-per-window
+  // kernel, [C, S][tile t][hi, lo][lane][8]: M[16t + lane % 16][8 (lane / 16) + j]
+  const uint16_t* lane32;
 };
 // mode 1 FP16, 2 BF16, 3 FP16 hi+lo, 4 BF16 hi+lo; shapes: xcorr_dft_supported
 bool xcorr_dft_supported(int ws_x, int ws_y);

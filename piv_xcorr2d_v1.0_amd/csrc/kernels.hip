@@ -239,7 +239,7 @@ __device__ __forceinline__ bool subpixel_fit_fp64near(const float* cc, int sx, i
 // coordinates and both apply the tests.  Lanes 2k and 2k + 1 of a W-lane
 // group hold the same peak (v at pixel idx, -1 / 0 for none); returns its
 // validity and writes the displacement on both lanes.
-template <int W>
+template <int W, bool FIT64 = false>
 __device__ __forceinline__ bool subpixel_fit_pair(const float* cc, int sx, int wsx, int wsy, int idx, float v,
                                                   float px, float py, float lx1, float lx2, float ly1, float ly2,
                                                   bool yaxis, float& dx, float& dy) {
@@ -249,10 +249,16 @@ __device__ __forceinline__ bool subpixel_fit_pair(const float* cc, int sx, int w
   float d = 0.f;
   if (fit) {
     const int c = j * sx + i, step = yaxis ? sx : 1;
-    const float f1 = logf(v);
-    const float f0 = logf(cc[c - step]);
-    const float f2 = logf(cc[c + step]);
-    d = (float)(yaxis ? j : i) + (f0 - f2) / (f0 + f2 - 2 * f1) / 2 - (float)(yaxis ? wsy : wsx) / 2 + (yaxis ? py : px);
+    float r;
+    if constexpr (FIT64) {
+      r = fit_axis_fp64near(v, cc[c - step], cc[c + step]);  // FP64 near flat peaks (precision form)
+    } else {
+      const float f1 = logf(v);
+      const float f0 = logf(cc[c - step]);
+      const float f2 = logf(cc[c + step]);
+      r = (f0 - f2) / (f0 + f2 - 2 * f1) / 2;
+    }
+    d = (float)(yaxis ? j : i) + r - (float)(yaxis ? wsy : wsx) / 2 + (yaxis ? py : px);
   }
   const float od = __shfl_xor(d, 1, W);
   const float x = yaxis ? od : d, y = yaxis ? d : od;
@@ -1059,7 +1065,7 @@ constexpr int single_lds_floats() {
 // stride SX floats, in LDS): lane l walks column l of the window, keeping a
 // bit mask of the rows that are local maxima; Npeak_save rounds of 64-lane
 // arg-max; Gaussian fits of the winners.
-template <int WX, int WY, int SX, class A>
+template <int WX, int WY, int SX, bool FIT64 = false, class A>
 __device__ __forceinline__ void single_peaks(const A& a, int w, const float* cc, int lane) {
   using Bits = typename std::conditional<(WY <= 32), uint32_t, uint64_t>::type;
   const bool col_ok = lane < WX && lane >= a.bx && lane <= WX - 1 - a.bx;
@@ -1109,7 +1115,8 @@ __device__ __forceinline__ void single_peaks(const A& a, int w, const float* cc,
       const long long o = (long long)w * a.K + k0 + (lane >> 1);
       const bool ya = (lane & 1) != 0;
       float dx, dy;
-      const bool ok = subpixel_fit_pair<64>(cc, SX, WX, WY, my_i, my_v, px, py, a.lx1, a.lx2, a.ly1, a.ly2, ya, dx, dy);
+      const bool ok =
+          subpixel_fit_pair<64, FIT64>(cc, SX, WX, WY, my_i, my_v, px, py, a.lx1, a.lx2, a.ly1, a.ly2, ya, dx, dy);
       if (ya) {
         a.disp_y[o] = dy;
         a.pk_idx[o] = my_i;
@@ -2117,6 +2124,251 @@ __global__ void __launch_bounds__(256) k_xcorr_dft(XcArgs a, DftTables tb, int s
   plane_peaks<W, H>(a, w, cc, cand, W, H, red_v, red_i, true);
 }
 
+// ============================================================== precision form, 32x32, one wave per window
+// The precision form of 32x32 windows with one wave per window and the DFT
+// matrices in registers.  F = C - iS; every product is one 16x16x32 MFMA over
+// the whole K = 32 of a matrix (2 x 2 output tiles).  Lane l = 16 g + r
+// supplies row r of an A tile (K values 8g..8g+7) and column r of a B tile
+// (the same K values) and receives rows 4g..4g+3 of column r of the output
+// tile.  C and S are symmetric, so one register copy of each (hi, lo) serves
+// as A operand (rows) and as B operand (columns).
+//   T  = z F:         Tr = zr C + zi S          Ti = zi C - zr S
+//   Z  = F T:         Zr = C Tr + S Ti          Zi = C Ti - S Tr
+//   P  = 4 conj(F1) F2 from Z and its mirror, scaled by 2^sh (|P| < 256)
+//   U  = F^H P:       Ur = C Pr - S Pi          Ui = C Pi + S Pr
+//   cc = Re(U F^H) =  Ur C - Ui S
+// Between products the FP32 results pass through an LDS plane pair (real and
+// imaginary parts, 32 x 33 floats each) and are read back in the next
+// operand layout, then rounded to FP16 or BF16 (hi, and lo = x - hi when
+// split).  The plane then goes through the peak stage of the wave-per-window
+// kernel, with the FP64 fit of flat peaks.
+constexpr int kD32S = 33;  // plane row stride (floats)
+
+template <bool BF>
+__device__ __forceinline__ uint32_t pack2(float x, float y) {
+  return (uint32_t)DftTy<BF>::cvt(x) | ((uint32_t)DftTy<BF>::cvt(y) << 16);
+}
+// eight values as an MFMA operand: hi, and lo = v - hi (split)
+template <bool BF, bool SPLIT>
+__device__ __forceinline__ void operand8(const float (&v)[8], u32x4_t& hi, u32x4_t& lo) {
+#pragma clang fp contract(off)
+  float r[8];
+  static_for<4>([&](auto kc) {
+    constexpr int k = kc;
+    hi[k] = pack2<BF>(v[2 * k], v[2 * k + 1]);
+    if constexpr (SPLIT) {
+      r[2 * k] = v[2 * k] - DftTy<BF>::back((uint16_t)(hi[k] & 0xffffu));
+      r[2 * k + 1] = v[2 * k + 1] - DftTy<BF>::back((uint16_t)(hi[k] >> 16));
+      lo[k] = pack2<BF>(r[2 * k], r[2 * k + 1]);
+    }
+  });
+}
+__device__ __forceinline__ u32x4_t neg8(u32x4_t v) {  // sign of every half
+  const uint32_t m = 0x80008000u;
+  return u32x4_t{v[0] ^ m, v[1] ^ m, v[2] ^ m, v[3] ^ m};
+}
+// acc + A B with the split products hi*hi + hi*lo + lo*hi
+template <bool BF, bool SPLIT>
+__device__ __forceinline__ f32x4_t mm(f32x4_t acc, u32x4_t ah, u32x4_t al, u32x4_t bh, u32x4_t bl) {
+  acc = DftTy<BF>::mma(ah, bh, acc);
+  if constexpr (SPLIT) {
+    acc = DftTy<BF>::mma(ah, bl, acc);
+    acc = DftTy<BF>::mma(al, bh, acc);
+  }
+  return acc;
+}
+
+template <bool BF, bool SPLIT>
+__global__ void __launch_bounds__(64) k_xcorr_dft32w(XcArgs a, const uint16_t* lanetab) {
+#pragma clang fp contract(off)
+  if (a.gate && *a.gate == 0) return;
+  constexpr int N = 32, SX = kD32S;
+  __shared__ __align__(16) float pr[N * SX], pi[N * SX];
+  const int w = blockIdx.x;
+  const int lane = threadIdx.x, r = lane & 15, g = lane >> 4;
+  const int iy = w / a.nx_vec, ix = w - iy * a.nx_vec;
+  const int ax0 = ix * a.sp_x, ay0 = iy * a.sp_y;
+  // C (m = 0) and S (m = 1) in the lane layout: [m][t][hl] = M[16t + r][8g .. 8g+7]
+  u32x4_t cst[2][2][2];
+  {
+    const u32x4_t* tab = reinterpret_cast<const u32x4_t*>(lanetab);
+    static_for<8>([&](auto qc) {
+      constexpr int q = qc;
+      cst[q >> 2][(q >> 1) & 1][q & 1] = tab[q * 64 + lane];
+    });
+  }
+  const u32x4_t z4 = {0u, 0u, 0u, 0u};
+  auto C = [&](int t, int hl) { return cst[0][t][hl]; };
+  auto S = [&](int t, int hl) { return cst[1][t][hl]; };
+  // ---- 1. T = z F.  A operands: z rows 16 mt + r, columns 8g .. 8g+7
+  u32x4_t zr[2][2], zi[2][2], nzr[2][2];
+  static_for<2>([&](auto mc) {
+    constexpr int mt = mc;
+    float2 x[8];
+    load_row<8>(a, ax0 + 8 * g, ay0 + 16 * mt + r, x);
+    float vr[8], vi[8];
+    static_for<8>([&](auto jc) { vr[jc] = x[jc].x; vi[jc] = x[jc].y; });
+    operand8<BF, SPLIT>(vr, zr[mt][0], zr[mt][1]);
+    operand8<BF, SPLIT>(vi, zi[mt][0], zi[mt][1]);
+    nzr[mt][0] = neg8(zr[mt][0]);
+    nzr[mt][1] = SPLIT ? neg8(zr[mt][1]) : z4;
+  });
+  // tiles (mt, nt) of a product, written to the planes in the output layout
+  auto store = [&](const f32x4_t (&re)[2][2], const f32x4_t (&im)[2][2]) {
+    static_for<2>([&](auto mc) {
+      static_for<2>([&](auto nc) {
+        constexpr int mt = mc, nt = nc;
+        static_for<4>([&](auto ic) {
+          constexpr int i = ic;
+          const int e = (16 * mt + 4 * g + i) * SX + 16 * nt + r;
+          pr[e] = re[mt][nt][i];
+          pi[e] = im[mt][nt][i];
+        });
+      });
+    });
+  };
+  {
+    f32x4_t tr[2][2], ti[2][2];
+    static_for<2>([&](auto mc) {
+      static_for<2>([&](auto nc) {
+        constexpr int mt = mc, nt = nc;
+        f32x4_t acc = {0.f, 0.f, 0.f, 0.f};
+        acc = mm<BF, SPLIT>(acc, zr[mt][0], zr[mt][1], C(nt, 0), C(nt, 1));
+        tr[mt][nt] = mm<BF, SPLIT>(acc, zi[mt][0], zi[mt][1], S(nt, 0), S(nt, 1));
+        acc = f32x4_t{0.f, 0.f, 0.f, 0.f};
+        acc = mm<BF, SPLIT>(acc, zi[mt][0], zi[mt][1], C(nt, 0), C(nt, 1));
+        ti[mt][nt] = mm<BF, SPLIT>(acc, nzr[mt][0], nzr[mt][1], S(nt, 0), S(nt, 1));
+      });
+    });
+    store(tr, ti);
+  }
+  wave_sync();
+  // ---- 2. Z = F T.  B operands: T rows 8g .. 8g+7 of column 16 nt + r
+  {
+    u32x4_t br[2][2], bi[2][2], nbr[2][2];
+    static_for<2>([&](auto nc) {
+      constexpr int nt = nc;
+      float vr[8], vi[8];
+      static_for<8>([&](auto jc) {
+        constexpr int j = jc;
+        vr[j] = pr[(8 * g + j) * SX + 16 * nt + r];
+        vi[j] = pi[(8 * g + j) * SX + 16 * nt + r];
+      });
+      operand8<BF, SPLIT>(vr, br[nt][0], br[nt][1]);
+      operand8<BF, SPLIT>(vi, bi[nt][0], bi[nt][1]);
+      nbr[nt][0] = neg8(br[nt][0]);
+      nbr[nt][1] = SPLIT ? neg8(br[nt][1]) : z4;
+    });
+    wave_sync();  // the planes are read before they are rewritten
+    f32x4_t zre[2][2], zim[2][2];
+    static_for<2>([&](auto mc) {
+      static_for<2>([&](auto nc) {
+        constexpr int mt = mc, nt = nc;
+        f32x4_t acc = {0.f, 0.f, 0.f, 0.f};
+        acc = mm<BF, SPLIT>(acc, C(mt, 0), C(mt, 1), br[nt][0], br[nt][1]);
+        zre[mt][nt] = mm<BF, SPLIT>(acc, S(mt, 0), S(mt, 1), bi[nt][0], bi[nt][1]);
+        acc = f32x4_t{0.f, 0.f, 0.f, 0.f};
+        acc = mm<BF, SPLIT>(acc, C(mt, 0), C(mt, 1), bi[nt][0], bi[nt][1]);
+        zim[mt][nt] = mm<BF, SPLIT>(acc, S(mt, 0), S(mt, 1), nbr[nt][0], nbr[nt][1]);
+      });
+    });
+    store(zre, zim);
+  }
+  wave_sync();
+  // ---- 3. P = 4 conj(F1) F2 at the B positions of step 4 (rows 8g + j,
+  //      columns 16 nt + r) from Z and its mirror Z[-ky][-kx]; scaled by
+  //      2^sh so that |P| < 256 (FP16 range for the next products)
+  float2 P[2][8];
+  float mx = 0.f;
+  static_for<2>([&](auto nc) {
+    constexpr int nt = nc;
+    const int kx = 16 * nt + r, kxm = (N - kx) & (N - 1);
+    static_for<8>([&](auto jc) {
+      constexpr int j = jc;
+      const int ky = 8 * g + j, kym = (N - ky) & (N - 1);
+      const float2 zo = make_float2(pr[ky * SX + kx], pi[ky * SX + kx]);
+      const float2 zm = make_float2(pr[kym * SX + kxm], pi[kym * SX + kxm]);
+      P[nt][j] = xprod(zo, zm);
+      mx = fmaxf(mx, fmaxf(fabsf(P[nt][j].x), fabsf(P[nt][j].y)));
+    });
+  });
+  for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off, 64));
+  int ex = 0;
+  if (mx > 0.f) (void)frexpf(mx, &ex);  // mx < 2^ex
+  const int sh = 8 - ex;
+  u32x4_t ppr[2][2], ppi[2][2], npi[2][2];
+  static_for<2>([&](auto nc) {
+    constexpr int nt = nc;
+    float vr[8], vi[8];
+    static_for<8>([&](auto jc) {
+      vr[jc] = ldexpf(P[nt][jc].x, sh);
+      vi[jc] = ldexpf(P[nt][jc].y, sh);
+    });
+    operand8<BF, SPLIT>(vr, ppr[nt][0], ppr[nt][1]);
+    operand8<BF, SPLIT>(vi, ppi[nt][0], ppi[nt][1]);
+    npi[nt][0] = neg8(ppi[nt][0]);
+    npi[nt][1] = SPLIT ? neg8(ppi[nt][1]) : z4;
+  });
+  wave_sync();  // every read of Z is done before U is stored
+  // ---- 4. U = F^H P
+  {
+    f32x4_t ure[2][2], uim[2][2];
+    static_for<2>([&](auto mc) {
+      static_for<2>([&](auto nc) {
+        constexpr int mt = mc, nt = nc;
+        f32x4_t acc = {0.f, 0.f, 0.f, 0.f};
+        acc = mm<BF, SPLIT>(acc, C(mt, 0), C(mt, 1), ppr[nt][0], ppr[nt][1]);
+        ure[mt][nt] = mm<BF, SPLIT>(acc, S(mt, 0), S(mt, 1), npi[nt][0], npi[nt][1]);
+        acc = f32x4_t{0.f, 0.f, 0.f, 0.f};
+        acc = mm<BF, SPLIT>(acc, C(mt, 0), C(mt, 1), ppi[nt][0], ppi[nt][1]);
+        uim[mt][nt] = mm<BF, SPLIT>(acc, S(mt, 0), S(mt, 1), ppr[nt][0], ppr[nt][1]);
+      });
+    });
+    store(ure, uim);
+  }
+  wave_sync();
+  // ---- 5. cc = Re(U F^H) = Ur C - Ui S.  A operands: U rows 16 mt + r,
+  //      columns 8g .. 8g+7
+  f32x4_t cc4[2][2];
+  {
+    u32x4_t ar[2][2], nai[2][2];
+    static_for<2>([&](auto mc) {
+      constexpr int mt = mc;
+      float vr[8], vi[8];
+      static_for<8>([&](auto jc) {
+        constexpr int j = jc;
+        vr[j] = pr[(16 * mt + r) * SX + 8 * g + j];
+        vi[j] = -pi[(16 * mt + r) * SX + 8 * g + j];
+      });
+      operand8<BF, SPLIT>(vr, ar[mt][0], ar[mt][1]);
+      operand8<BF, SPLIT>(vi, nai[mt][0], nai[mt][1]);
+    });
+    wave_sync();  // the planes are read before the correlation plane is written
+    static_for<2>([&](auto mc) {
+      static_for<2>([&](auto nc) {
+        constexpr int mt = mc, nt = nc;
+        f32x4_t acc = {0.f, 0.f, 0.f, 0.f};
+        acc = mm<BF, SPLIT>(acc, ar[mt][0], ar[mt][1], C(nt, 0), C(nt, 1));
+        cc4[mt][nt] = mm<BF, SPLIT>(acc, nai[mt][0], nai[mt][1], S(nt, 0), S(nt, 1));
+      });
+    });
+  }
+  // |.| / N (the exact correlation is real and >= 0), written fftshifted
+  const float scale = ldexpf(0.25f / (float)(N * N), -sh);
+  static_for<2>([&](auto mc) {
+    static_for<2>([&](auto nc) {
+      constexpr int mt = mc, nt = nc;
+      static_for<4>([&](auto ic) {
+        constexpr int i = ic;
+        const int row = (16 * mt + 4 * g + i + N / 2) & (N - 1), col = (16 * nt + r + N / 2) & (N - 1);
+        pr[row * SX + col] = fabsf(cc4[mt][nt][i]) * scale;
+      });
+    });
+  });
+  wave_sync();
+  single_peaks<N, N, SX, true>(a, w, pr, lane);
+}
+
 // ============================================================== deformation
 // One thread per aug column, DEF_ROWS consecutive rows.  The row terms are
 // computed once per workgroup into LDS and the column terms once per thread
@@ -3041,6 +3293,16 @@ template <int W, int H>
 static hipError_t launch_dft_shape(const XcArgs& a, int mode, const DftTables& t, hipStream_t s) {
   const dim3 g((unsigned)(a.nx_vec * a.ny_vec)), b(256);
   const int split = mode >= 3 ? 1 : 0;
+  if constexpr (W == 32 && H == 32) {
+    if (t.lane32) {  // one wave per window, matrices in registers
+      const dim3 b64(64);
+      if (mode == 1) hipLaunchKernelGGL((k_xcorr_dft32w<false, false>), g, b64, 0, s, a, t.lane32);
+      else if (mode == 2) hipLaunchKernelGGL((k_xcorr_dft32w<true, false>), g, b64, 0, s, a, t.lane32);
+      else if (mode == 3) hipLaunchKernelGGL((k_xcorr_dft32w<false, true>), g, b64, 0, s, a, t.lane32);
+      else hipLaunchKernelGGL((k_xcorr_dft32w<true, true>), g, b64, 0, s, a, t.lane32);
+      return hipGetLastError();
+    }
+  }
   if (mode == 2 || mode == 4) hipLaunchKernelGGL((k_xcorr_dft<W, H, true>), g, b, 0, s, a, t, split);
   else hipLaunchKernelGGL((k_xcorr_dft<W, H, false>), g, b, 0, s, a, t, split);
   return hipGetLastError();
