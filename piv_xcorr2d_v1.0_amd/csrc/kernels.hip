@@ -2179,7 +2179,7 @@ __device__ __forceinline__ f32x4_t mm(f32x4_t acc, u32x4_t ah, u32x4_t al, u32x4
 }
 
 template <bool BF, bool SPLIT>
-__global__ void __launch_bounds__(64) k_xcorr_dft32w(XcArgs a, const uint16_t* lanetab) {
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_xcorr_dft32w(XcArgs a, const uint16_t* lanetab) {
 #pragma clang fp contract(off)
   if (a.gate && *a.gate == 0) return;
   constexpr int N = 32, SX = kD32S;
