@@ -2144,22 +2144,31 @@ __global__ void __launch_bounds__(256) k_xcorr_dft(XcArgs a, DftTables tb, int s
 // kernel, with the FP64 fit of flat peaks.
 constexpr int kD32S = 33;  // plane row stride (floats)
 
+typedef float f32x2_t __attribute__((ext_vector_type(2)));
+typedef _Float16 f16x2_t __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+// two values rounded to nearest (one packed conversion), and back to FP32
 template <bool BF>
 __device__ __forceinline__ uint32_t pack2(float x, float y) {
-  return (uint32_t)DftTy<BF>::cvt(x) | ((uint32_t)DftTy<BF>::cvt(y) << 16);
+  const f32x2_t v = {x, y};
+  if constexpr (BF) return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf16x2_t));
+  else return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, f16x2_t));
+}
+template <bool BF>
+__device__ __forceinline__ f32x2_t unpack2(uint32_t u) {
+  if constexpr (BF) return f32x2_t{__uint_as_float(u << 16), __uint_as_float(u & 0xffff0000u)};
+  else return __builtin_convertvector(__builtin_bit_cast(f16x2_t, u), f32x2_t);
 }
 // eight values as an MFMA operand: hi, and lo = v - hi (split)
 template <bool BF, bool SPLIT>
 __device__ __forceinline__ void operand8(const float (&v)[8], u32x4_t& hi, u32x4_t& lo) {
 #pragma clang fp contract(off)
-  float r[8];
   static_for<4>([&](auto kc) {
     constexpr int k = kc;
     hi[k] = pack2<BF>(v[2 * k], v[2 * k + 1]);
     if constexpr (SPLIT) {
-      r[2 * k] = v[2 * k] - DftTy<BF>::back((uint16_t)(hi[k] & 0xffffu));
-      r[2 * k + 1] = v[2 * k + 1] - DftTy<BF>::back((uint16_t)(hi[k] >> 16));
-      lo[k] = pack2<BF>(r[2 * k], r[2 * k + 1]);
+      const f32x2_t h = unpack2<BF>(hi[k]);
+      lo[k] = pack2<BF>(v[2 * k] - h.x, v[2 * k + 1] - h.y);
     }
   });
 }
