@@ -1344,6 +1344,35 @@ static int build_dft_tables(pivx_ctx* c, int gi, bool bf) {
     HIPCHK(c, hipMemcpy(ldev, lt.data(), lt.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
     T.lane32 = ldev;
   }
+  T.lane64 = nullptr;
+  if (W == 64 && H == 64) {
+    std::vector<uint16_t> lt(2 * 2 * 4 * 2 * 64 * 8);
+    for (int m = 0; m < 2; ++m)
+      for (int t = 0; t < 4; ++t)
+        for (int ks = 0; ks < 2; ++ks)
+          for (int lane = 0; lane < 64; ++lane)
+            for (int j = 0; j < 8; ++j) {
+              const int row = 16 * t + (lane & 15), k = 32 * ks + 8 * (lane >> 4) + j;
+              const float f = (float)cs(64, row, k, m == 1);
+              uint16_t h, l;
+              if (bf) {
+                h = host_bf16(f);
+                float hb;
+                const uint32_t hu = (uint32_t)h << 16;
+                std::memcpy(&hb, &hu, 4);
+                l = host_bf16(f - hb);
+              } else {
+                h = host_f16(f);
+                l = host_f16(f - host_f16_back(h));
+              }
+              for (int hl = 0; hl < 2; ++hl)
+                lt[((((size_t)(m * 2 + hl) * 4 + t) * 2 + ks) * 64 + lane) * 8 + j] = hl ? l : h;
+            }
+    uint16_t* ldev = nullptr;
+    if (dalloc(c, &ldev, lt.size())) return PIVX_ERR_HIP;
+    HIPCHK(c, hipMemcpy(ldev, lt.data(), lt.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
+    T.lane64 = ldev;
+  }
   for (int o = 0; o < 2; ++o) {
     T.a1h[o] = ptr[o][0];
     T.a1l[o] = ptr[o][1];

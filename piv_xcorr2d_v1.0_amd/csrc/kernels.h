@@ -123,6 +123,8 @@ struct DftTables {
   // 32x32 windows: C and S in the MFMA lane layout of the wave-per-window
   // kernel, [C, S][tile t][hi, lo][lane][8]: M[16t + lane % 16][8 (lane / 16) + j]
   const uint16_t* lane32;
+  // 64x64 windows: [C, S][hi, lo][tile t][k-step ks][lane][8]: M[16t + lane % 16][32 ks + 8 (lane / 16) + j]
+  const uint16_t* lane64;
 };
 // mode 1 FP16, 2 BF16, 3 FP16 hi+lo, 4 BF16 hi+lo; shapes: xcorr_dft_supported
 bool xcorr_dft_supported(int ws_x, int ws_y);

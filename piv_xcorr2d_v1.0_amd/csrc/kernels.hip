@@ -2378,6 +2378,188 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
   single_peaks<N, N, SX, true>(a, w, pr, lane);
 }
 
+// The same for 64x64 windows with one 256-thread workgroup (four waves) per
+// window: K = 64 takes two 16x16x32 MFMAs per tile product, the output has
+// 4 x 4 tiles, and the waves split each product by tile rows (steps with a
+// constant B operand: wave w computes rows 16w..16w+15) or by tile columns
+// (steps with a constant A operand: columns 16w..16w+15), so that every wave
+// reads or converts only its own band of the data operand.  The DFT matrices
+// are read in their lane layout straight from the table (L1/L2-resident):
+// [C, S][hi, lo][tile t][k-step ks][lane][8] = M[16t + lane % 16][32 ks + 8 (lane / 16) + j].
+template <bool BF, bool SPLIT>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k_xcorr_dft64w(XcArgs a, const uint16_t* lanetab) {
+#pragma clang fp contract(off)
+  if (a.gate && *a.gate == 0) return;
+  constexpr int N = 64, SX = N + 1;
+  __shared__ __align__(16) float pr[N * SX], pi[N * SX];
+  __shared__ float smax[4];
+  const int w = blockIdx.x;
+  const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63, r = lane & 15, g = lane >> 4;
+  const int iy = w / a.nx_vec, ix = w - iy * a.nx_vec;
+  const int ax0 = ix * a.sp_x, ay0 = iy * a.sp_y;
+  const u32x4_t* const tab = reinterpret_cast<const u32x4_t*>(lanetab);
+  auto cst = [&](int m, int hl, int t, int ks) { return tab[(((m * 2 + hl) * 4 + t) * 2 + ks) * 64 + lane]; };
+  const u32x4_t z4 = {0u, 0u, 0u, 0u};
+  auto lo_or0 = [&](u32x4_t v) { return SPLIT ? v : z4; };
+  auto put = [&](int mt, int nt, f32x4_t re, f32x4_t im) {
+    static_for<4>([&](auto ic) {
+      constexpr int i = ic;
+      const int e = (16 * mt + 4 * g + i) * SX + 16 * nt + r;
+      pr[e] = re[i];
+      pi[e] = im[i];
+    });
+  };
+  // ---- 1. T = z F, tile rows 16 wv + r: A = z (two k-steps), B = F columns
+  {
+    u32x4_t zr[2][2], zi[2][2], nzr[2][2];
+    static_for<2>([&](auto kc) {
+      constexpr int ks = kc;
+      float2 x[8];
+      load_row<8>(a, ax0 + 32 * ks + 8 * g, ay0 + 16 * wv + r, x);
+      float vr[8], vi[8];
+      static_for<8>([&](auto jc) { vr[jc] = x[jc].x; vi[jc] = x[jc].y; });
+      operand8<BF, SPLIT>(vr, zr[ks][0], zr[ks][1]);
+      operand8<BF, SPLIT>(vi, zi[ks][0], zi[ks][1]);
+      nzr[ks][0] = neg8(zr[ks][0]);
+      nzr[ks][1] = lo_or0(neg8(zr[ks][1]));
+    });
+    for (int nt = 0; nt < 4; ++nt) {
+      f32x4_t tr = {0.f, 0.f, 0.f, 0.f}, ti = {0.f, 0.f, 0.f, 0.f};
+      static_for<2>([&](auto kc) {
+        constexpr int ks = kc;
+        const u32x4_t ch = cst(0, 0, nt, ks), cl = lo_or0(cst(0, 1, nt, ks));
+        const u32x4_t sh_ = cst(1, 0, nt, ks), sl = lo_or0(cst(1, 1, nt, ks));
+        tr = mm<BF, SPLIT>(tr, zr[ks][0], zr[ks][1], ch, cl);
+        tr = mm<BF, SPLIT>(tr, zi[ks][0], zi[ks][1], sh_, sl);
+        ti = mm<BF, SPLIT>(ti, zi[ks][0], zi[ks][1], ch, cl);
+        ti = mm<BF, SPLIT>(ti, nzr[ks][0], nzr[ks][1], sh_, sl);
+      });
+      put(wv, nt, tr, ti);
+    }
+  }
+  __syncthreads();
+  // ---- 2. Z = F T, tile columns 16 wv + r: A = F rows, B = T (rows 32 ks + 8g + j)
+  {
+    u32x4_t br[2][2], bi[2][2], nbr[2][2];
+    static_for<2>([&](auto kc) {
+      constexpr int ks = kc;
+      float vr[8], vi[8];
+      static_for<8>([&](auto jc) {
+        constexpr int j = jc;
+        vr[j] = pr[(32 * ks + 8 * g + j) * SX + 16 * wv + r];
+        vi[j] = pi[(32 * ks + 8 * g + j) * SX + 16 * wv + r];
+      });
+      operand8<BF, SPLIT>(vr, br[ks][0], br[ks][1]);
+      operand8<BF, SPLIT>(vi, bi[ks][0], bi[ks][1]);
+      nbr[ks][0] = neg8(br[ks][0]);
+      nbr[ks][1] = lo_or0(neg8(br[ks][1]));
+    });
+    __syncthreads();  // the T planes are read before Z is stored
+    for (int mt = 0; mt < 4; ++mt) {
+      f32x4_t zre = {0.f, 0.f, 0.f, 0.f}, zim = {0.f, 0.f, 0.f, 0.f};
+      static_for<2>([&](auto kc) {
+        constexpr int ks = kc;
+        const u32x4_t ch = cst(0, 0, mt, ks), cl = lo_or0(cst(0, 1, mt, ks));
+        const u32x4_t sh_ = cst(1, 0, mt, ks), sl = lo_or0(cst(1, 1, mt, ks));
+        zre = mm<BF, SPLIT>(zre, ch, cl, br[ks][0], br[ks][1]);
+        zre = mm<BF, SPLIT>(zre, sh_, sl, bi[ks][0], bi[ks][1]);
+        zim = mm<BF, SPLIT>(zim, ch, cl, bi[ks][0], bi[ks][1]);
+        zim = mm<BF, SPLIT>(zim, sh_, sl, nbr[ks][0], nbr[ks][1]);
+      });
+      put(mt, wv, zre, zim);
+    }
+  }
+  __syncthreads();
+  // ---- 3. P = 4 conj(F1) F2 at rows 32 ks + 8g + j of column 16 wv + r,
+  //      scaled by 2^sh so that |P| < 256 over the whole window
+  float2 P[2][8];
+  float mx = 0.f;
+  {
+    const int kx = 16 * wv + r, kxm = (N - kx) & (N - 1);
+    static_for<2>([&](auto kc) {
+      constexpr int ks = kc;
+      static_for<8>([&](auto jc) {
+        constexpr int j = jc;
+        const int ky = 32 * ks + 8 * g + j, kym = (N - ky) & (N - 1);
+        const float2 zo = make_float2(pr[ky * SX + kx], pi[ky * SX + kx]);
+        const float2 zm = make_float2(pr[kym * SX + kxm], pi[kym * SX + kxm]);
+        P[ks][j] = xprod(zo, zm);
+        mx = fmaxf(mx, fmaxf(fabsf(P[ks][j].x), fabsf(P[ks][j].y)));
+      });
+    });
+  }
+  for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off, 64));
+  if (lane == 0) smax[wv] = mx;
+  __syncthreads();  // also: every read of Z is done before U is stored
+  mx = fmaxf(fmaxf(smax[0], smax[1]), fmaxf(smax[2], smax[3]));
+  int ex = 0;
+  if (mx > 0.f) (void)frexpf(mx, &ex);  // mx < 2^ex
+  const int sh = 8 - ex;
+  // ---- 4. U = F^H P, tile columns 16 wv + r
+  {
+    u32x4_t ppr[2][2], ppi[2][2], npi[2][2];
+    static_for<2>([&](auto kc) {
+      constexpr int ks = kc;
+      float vr[8], vi[8];
+      static_for<8>([&](auto jc) {
+        vr[jc] = ldexpf(P[ks][jc].x, sh);
+        vi[jc] = ldexpf(P[ks][jc].y, sh);
+      });
+      operand8<BF, SPLIT>(vr, ppr[ks][0], ppr[ks][1]);
+      operand8<BF, SPLIT>(vi, ppi[ks][0], ppi[ks][1]);
+      npi[ks][0] = neg8(ppi[ks][0]);
+      npi[ks][1] = lo_or0(neg8(ppi[ks][1]));
+    });
+    for (int mt = 0; mt < 4; ++mt) {
+      f32x4_t ure = {0.f, 0.f, 0.f, 0.f}, uim = {0.f, 0.f, 0.f, 0.f};
+      static_for<2>([&](auto kc) {
+        constexpr int ks = kc;
+        const u32x4_t ch = cst(0, 0, mt, ks), cl = lo_or0(cst(0, 1, mt, ks));
+        const u32x4_t sh_ = cst(1, 0, mt, ks), sl = lo_or0(cst(1, 1, mt, ks));
+        ure = mm<BF, SPLIT>(ure, ch, cl, ppr[ks][0], ppr[ks][1]);
+        ure = mm<BF, SPLIT>(ure, sh_, sl, npi[ks][0], npi[ks][1]);
+        uim = mm<BF, SPLIT>(uim, ch, cl, ppi[ks][0], ppi[ks][1]);
+        uim = mm<BF, SPLIT>(uim, sh_, sl, ppr[ks][0], ppr[ks][1]);
+      });
+      put(mt, wv, ure, uim);
+    }
+  }
+  __syncthreads();
+  // ---- 5. cc = Re(U F^H) = Ur C - Ui S, tile rows 16 wv + r: A = U rows
+  {
+    u32x4_t ar[2][2], nai[2][2];
+    static_for<2>([&](auto kc) {
+      constexpr int ks = kc;
+      float vr[8], vi[8];
+      static_for<8>([&](auto jc) {
+        constexpr int j = jc;
+        vr[j] = pr[(16 * wv + r) * SX + 32 * ks + 8 * g + j];
+        vi[j] = -pi[(16 * wv + r) * SX + 32 * ks + 8 * g + j];
+      });
+      operand8<BF, SPLIT>(vr, ar[ks][0], ar[ks][1]);
+      operand8<BF, SPLIT>(vi, nai[ks][0], nai[ks][1]);
+    });
+    __syncthreads();  // the U planes are read before the correlation plane is written
+    const float scale = ldexpf(0.25f / (float)(N * N), -sh);
+    for (int nt = 0; nt < 4; ++nt) {
+      f32x4_t cc = {0.f, 0.f, 0.f, 0.f};
+      static_for<2>([&](auto kc) {
+        constexpr int ks = kc;
+        cc = mm<BF, SPLIT>(cc, ar[ks][0], ar[ks][1], cst(0, 0, nt, ks), lo_or0(cst(0, 1, nt, ks)));
+        cc = mm<BF, SPLIT>(cc, nai[ks][0], nai[ks][1], cst(1, 0, nt, ks), lo_or0(cst(1, 1, nt, ks)));
+      });
+      // |.| / N^2 (the exact correlation is real and >= 0), written fftshifted
+      static_for<4>([&](auto ic) {
+        constexpr int i = ic;
+        const int row = (16 * wv + 4 * g + i + N / 2) & (N - 1), col = (16 * nt + r + N / 2) & (N - 1);
+        pr[row * SX + col] = fabsf(cc[i]) * scale;
+      });
+    }
+  }
+  __syncthreads();
+  if (wv == 0) single_peaks<N, N, SX, true>(a, w, pr, lane);
+}
+
 // ============================================================== deformation
 // One thread per aug column, DEF_ROWS consecutive rows.  The row terms are
 // computed once per workgroup into LDS and the column terms once per thread
@@ -3302,6 +3484,15 @@ template <int W, int H>
 static hipError_t launch_dft_shape(const XcArgs& a, int mode, const DftTables& t, hipStream_t s) {
   const dim3 g((unsigned)(a.nx_vec * a.ny_vec)), b(256);
   const int split = mode >= 3 ? 1 : 0;
+  if constexpr (W == 64 && H == 64) {
+    if (t.lane64) {  // four waves per window, matrices from the lane-layout table
+      if (mode == 1) hipLaunchKernelGGL((k_xcorr_dft64w<false, false>), g, b, 0, s, a, t.lane64);
+      else if (mode == 2) hipLaunchKernelGGL((k_xcorr_dft64w<true, false>), g, b, 0, s, a, t.lane64);
+      else if (mode == 3) hipLaunchKernelGGL((k_xcorr_dft64w<false, true>), g, b, 0, s, a, t.lane64);
+      else hipLaunchKernelGGL((k_xcorr_dft64w<true, true>), g, b, 0, s, a, t.lane64);
+      return hipGetLastError();
+    }
+  }
   if constexpr (W == 32 && H == 32) {
     if (t.lane32) {  // one wave per window, matrices in registers
       const dim3 b64(64);
