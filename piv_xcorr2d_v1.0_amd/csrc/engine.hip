@@ -437,6 +437,48 @@ int enqueue_filter(pivx_ctx* c, int gi, const FilterJob& J, int npass_median, in
   const bool fill = iffill == 1;
   const bool listed = fill && npass_median > 0;  // the last median pass lists the holes
   int rc;
+  // The gated grid-1 fallback of the learned pipeline mostly does nothing, yet
+  // every launch of it costs its stream a dispatch: it runs the whole sequence
+  // as one launch of k_filter_seq (one workgroup, the same per-vector code)
+  // when the median window has the register path and the steps fit.
+  const int nsteps = (npass_median > 0 ? 1 + npass_median : 2) + 2 + std::max(0, npass_smooth);
+  const bool seq = J.gate != nullptr && (npass_median <= 0 || filter_seq_supported(wxm, wym)) &&
+                   nsteps <= kFilterSeqSteps;
+  FilterSeq q{};
+  q.wxm = wxm;
+  q.wym = wym;
+  q.gate = J.gate;
+  auto add_step = [&](int kind) -> FilterStep& {
+    FilterStep& st = q.step[q.nsteps++];
+    st.kind = kind;
+    return st;
+  };
+  auto prep = [&](float* dx, float* dy, uint8_t* dv, int* reset) -> hipError_t {
+    if (!seq) return launch_prep(J.cx, J.cy, J.cv, K, dx, dy, dv, nt, reset, stream, J.gate);
+    add_step(FS_PREP).prep = PrepArgs{J.cx, J.cy, J.cv, K, dx, dy, dv, nt, reset};
+    return hipSuccess;
+  };
+  auto zero = [&](int* p) -> hipError_t {
+    if (!seq) return launch_zero_ints(p, 1, stream, J.gate);
+    add_step(FS_ZERO).zero = p;
+    return hipSuccess;
+  };
+  auto median = [&](const MedArgs& m) -> hipError_t {
+    if (!seq) return launch_median(m, stream);
+    add_step(FS_MEDIAN).med = m;
+    return hipSuccess;
+  };
+  auto fill_holes = [&](const FillArgs& f) -> hipError_t {
+    if (!seq) return launch_fill(f, stream);
+    if (!f.prelisted) add_step(FS_FILL_LIST).fill = f;
+    add_step(FS_FILL).fill = f;
+    return hipSuccess;
+  };
+  auto smooth = [&](const SmoothArgs& sa) -> hipError_t {
+    if (!seq) return launch_smooth(sa, stream);
+    add_step(FS_SMOOTH).smooth = sa;
+    return hipSuccess;
+  };
   tic(c);
   // The hole counter must be 0 when the holes are listed (by the last median
   // pass, or by k_fill_list without median passes).  k_fill zeroes it after
@@ -444,16 +486,11 @@ int enqueue_filter(pivx_ctx* c, int gi, const FilterJob& J, int npass_median, in
   // earlier sequence (an interrupted pair, a caller's partial sequence) can
   // leave a stale count behind.
   if (npass_median <= 0) {
-    rc = check_launch(c, launch_prep(J.cx, J.cy, J.cv, K, t0x, t0y, ov, nt, fill ? J.nholes : nullptr, stream, J.gate),
-                      "median_filter_data_prep");
-    if (!rc && mx)
-      rc = check_launch(c, launch_prep(J.cx, J.cy, J.cv, K, mx, my, mv, nt, nullptr, stream, J.gate),
-                        "median_filter_data_prep");
+    rc = check_launch(c, prep(t0x, t0y, ov, fill ? J.nholes : nullptr), "median_filter_data_prep");
+    if (!rc && mx) rc = check_launch(c, prep(mx, my, mv, nullptr), "median_filter_data_prep");
     if (rc) return rc;
   } else {
-    if (listed && npass_median == 1 &&
-        (rc = check_launch(c, launch_zero_ints(J.nholes, 1, stream, J.gate), "hole counter")))
-      return rc;
+    if (listed && npass_median == 1 && (rc = check_launch(c, zero(J.nholes), "hole counter"))) return rc;
     // intermediate sets: I1 = (ux, uy, uv), I2 = whichever of (ox, oy) / (sx, sy) is not T0, with sv
     float* i2x = smooth_odd ? ox : J.sx;
     float* i2y = smooth_odd ? oy : J.sy;
@@ -500,7 +537,7 @@ int enqueue_filter(pivx_ctx* c, int gi, const FilterJob& J, int npass_median, in
       m.reset_count = (listed && p == 0 && back > 0) ? J.nholes : nullptr;
       m.gate = J.gate;
       m.fp64 = J.fp64;
-      rc = check_launch(c, launch_median(m, stream), "median_filtering");
+      rc = check_launch(c, median(m), "median_filtering");
       if (rc) return rc;
       srcx = tx;
       srcy = ty;
@@ -508,7 +545,7 @@ int enqueue_filter(pivx_ctx* c, int gi, const FilterJob& J, int npass_median, in
       sstride = 1;
     }
   }
-  toc(c, gi, 2);
+  if (!seq) toc(c, gi, 2);
   if (fill) {
     FillArgs f{};
     f.dx = t0x;
@@ -527,13 +564,13 @@ int enqueue_filter(pivx_ctx* c, int gi, const FilterJob& J, int npass_median, in
     f.gate = J.gate;
     f.sweeps = J.sweeps;
     f.prelisted = listed ? 1 : 0;
-    tic(c);
-    rc = check_launch(c, launch_fill(f, stream), "fill_blank_vecfield");
-    toc(c, gi, 3);
+    if (!seq) tic(c);
+    rc = check_launch(c, fill_holes(f), "fill_blank_vecfield");
+    if (!seq) toc(c, gi, 3);
     if (rc) return rc;
   }
   if (npass_smooth > 0) {
-    tic(c);
+    if (!seq) tic(c);
     float *cx = t0x, *cy = t0y;
     for (int s = 0; s < npass_smooth; ++s) {
       float* nx_ = (cx == ox) ? J.sx : ox;
@@ -556,12 +593,17 @@ int enqueue_filter(pivx_ctx* c, int gi, const FilterJob& J, int npass_median, in
         sa.border = kExBorder;
         if (ext_done) *ext_done = true;
       }
-      rc = check_launch(c, launch_smooth(sa, stream), "smoothing_tophat");
+      rc = check_launch(c, smooth(sa), "smoothing_tophat");
       if (rc) return rc;
       cx = nx_;
       cy = ny_;
     }
-    toc(c, gi, 4);
+    if (!seq) toc(c, gi, 4);
+  }
+  if (seq) {
+    rc = check_launch(c, launch_filter_seq(q, stream), "filter sequence");
+    toc(c, gi, 2);
+    if (rc) return rc;
   }
   return PIVX_OK;
 }

@@ -94,6 +94,37 @@ struct SmoothArgs {
   const int* gate;         // if set: the launch does nothing unless *gate != 0
 };
 
+struct PrepArgs {
+  const float* cx; const float* cy; const uint8_t* cv; int K;
+  float* dx; float* dy; uint8_t* dv; int nt;
+  int* reset_count;        // if set: 0 is written here
+};
+
+// A whole filter sequence (median passes, hole listing and filling,
+// smoothing) run by one workgroup in one launch: the steps of the launch
+// sequence, in order, with a workgroup barrier between them.  Used for the
+// learned pipeline's gated grid-1 fallback, whose launches mostly do nothing:
+// one launch instead of one per step.  Each step runs the per-vector code of
+// its own kernel, so the results are those of the launch sequence.
+enum FilterStepKind : int { FS_PREP = 0, FS_ZERO = 1, FS_MEDIAN = 2, FS_FILL_LIST = 3, FS_FILL = 4, FS_SMOOTH = 5 };
+struct FilterStep {
+  int kind;
+  union {
+    PrepArgs prep;
+    int* zero;             // FS_ZERO: one int set to 0
+    MedArgs med;           // k_median2 window shapes only (3x3, 5x5, 7x7)
+    FillArgs fill;
+    SmoothArgs smooth;
+  };
+};
+constexpr int kFilterSeqSteps = 12;
+struct FilterSeq {
+  FilterStep step[kFilterSeqSteps];
+  int nsteps;
+  int wxm, wym;            // median window of the FS_MEDIAN steps
+  const int* gate;         // if set: the launch does nothing unless *gate != 0
+};
+
 // MHE contrast enhancement of one u8 frame (MHE7_4_gpu.cu).  The central
 // tiles_x*ws by tiles_y*ws region (origin xs, ys) is split into ws x ws tiles;
 // every pixel there is mapped through its tile's transfer table, every pixel
@@ -162,6 +193,9 @@ hipError_t launch_prep(const float* cx, const float* cy, const uint8_t* cv, int 
                        int nt, int* reset_count, hipStream_t s, const int* gate = nullptr);
 // p[0..n) = 0 by a kernel (no memset node in a captured pair graph).
 hipError_t launch_zero_ints(int* p, int n, hipStream_t s, const int* gate = nullptr);
+// One workgroup runs the steps of q (median windows 3x3, 5x5 or 7x7 only).
+bool filter_seq_supported(int wxm, int wym);
+hipError_t launch_filter_seq(const FilterSeq& q, hipStream_t s);
 hipError_t launch_stream_copy(const void* src, void* dst, size_t bytes, hipStream_t s);
 hipError_t launch_mhe(const MheArgs& a, hipStream_t s);
 // Device rendering of one synthetic pair (synth.hip) into frames f1, f2 (row

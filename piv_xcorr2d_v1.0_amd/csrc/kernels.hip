@@ -2904,18 +2904,16 @@ __device__ __forceinline__ float median_low(const float (&v)[MedianNet<M>::NS], 
   return at((n - 1) / 2);
 }
 
+// Lane g of the pass: vector g / 2, component g % 2 (both lanes of a vector
+// sit in one wave, as neighbours).
 template <int WXM, int WYM>
-__global__ void __launch_bounds__(256) k_median2(MedArgs a) {
+__device__ __forceinline__ void median2_lane(const MedArgs& a, int g) {
 #pragma clang fp contract(off)
-  if (a.gate && *a.gate == 0) return;  // learned pipeline: grid 1 runs only as the fallback
   constexpr int M = WXM * WYM - 1;
   constexpr int NS = MedianNet<M>::NS;
   constexpr int SKIP = (WXM * WYM - 1) / 2;
   constexpr int HX = (WXM - 1) / 2, HY = (WYM - 1) / 2;
   const int nt = a.nx_vec * a.ny_vec;
-  if (a.reset_count && blockIdx.x == 0 && threadIdx.x == 0) *a.reset_count = 0;
-  // neighbouring blocks read mostly the same rows: keep them on one XCD's L2
-  const int g = xcd_remap(blockIdx.x, gridDim.x) * 256 + (int)threadIdx.x;
   const int t = g >> 1, c = g & 1;
   const bool live = t < nt;
   const int tt = live ? t : nt - 1;  // both lanes of a pair stay active for the swap
@@ -2987,6 +2985,14 @@ __global__ void __launch_bounds__(256) k_median2(MedArgs a) {
   }
 }
 
+template <int WXM, int WYM>
+__global__ void __launch_bounds__(256) k_median2(MedArgs a) {
+  if (a.gate && *a.gate == 0) return;  // learned pipeline: grid 1 runs only as the fallback
+  if (a.reset_count && blockIdx.x == 0 && threadIdx.x == 0) *a.reset_count = 0;
+  // neighbouring blocks read mostly the same rows: keep them on one XCD's L2
+  median2_lane<WXM, WYM>(a, xcd_remap(blockIdx.x, gridDim.x) * 256 + (int)threadIdx.x);
+}
+
 // Windows larger than the register path: per-vector lists in global scratch,
 // insertion sort (any wx*wy the reference accepts).
 __device__ float median_scratch(float* s, int n) {
@@ -3056,23 +3062,32 @@ __global__ void __launch_bounds__(128) k_median_generic(MedArgs a) {
 // list_cap cannot arise from a pass that starts at 0 (list_cap >= every
 // grid's vector count); k_fill adds any excess to *drops, which the host
 // reports as an error.
-__global__ void __launch_bounds__(256) k_fill_list(FillArgs a) {
-  if (a.gate && *a.gate == 0) return;  // learned pipeline: grid 1 runs only as the fallback
+__device__ __forceinline__ void fill_list_range(const FillArgs& a, int first, int stride) {
   const int nt = a.nx_vec * a.ny_vec;
-  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nt; t += gridDim.x * blockDim.x)
+  for (int t = first; t < nt; t += stride)
     if (!a.dv[t]) {
       const int slot = atomicAdd(a.nholes, 1);
       if (slot < a.list_cap) a.list_a[slot] = t;
     }
 }
 
+__global__ void __launch_bounds__(256) k_fill_list(FillArgs a) {
+  if (a.gate && *a.gate == 0) return;  // learned pipeline: grid 1 runs only as the fallback
+  fill_list_range(a, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
+}
+
 // 256 threads (four waves): a single larger workgroup waited up to 175 us for
 // a CU with room beside concurrent correlation kernels of other handles; at
 // four waves it is placed at once (longest of 750 launches 8 us).
-__global__ void __launch_bounds__(256) k_fill(FillArgs a) {
+// All sweeps, by every thread of the workgroup; sh is workgroup-shared.
+struct FillShared {
+  int n_holes, n_next, n_filled;
+};
+__device__ void fill_sweeps(const FillArgs& a, FillShared& sh) {
 #pragma clang fp contract(off)
-  if (a.gate && *a.gate == 0) return;  // learned pipeline: grid 1 runs only as the fallback
-  __shared__ int n_holes, n_next, n_filled;
+  int& n_holes = sh.n_holes;
+  int& n_next = sh.n_next;
+  int& n_filled = sh.n_filled;
   if (threadIdx.x == 0) {
     const int counted = *a.nholes;
     n_holes = min(counted, a.list_cap);
@@ -3142,14 +3157,19 @@ __global__ void __launch_bounds__(256) k_fill(FillArgs a) {
   }
 }
 
+__global__ void __launch_bounds__(256) k_fill(FillArgs a) {
+  if (a.gate && *a.gate == 0) return;  // learned pipeline: grid 1 runs only as the fallback
+  __shared__ FillShared sh;
+  fill_sweeps(a, sh);
+}
+
 // ============================================================== smoothing
 // organize_smooth_filter_data + smoothing_tophat (cu:399-443): mean over the
 // in-bounds wx*wy window including the centre, ignoring validity; raster sum.
-__global__ void k_smooth(SmoothArgs a) {
+__device__ __forceinline__ void smooth_range(const SmoothArgs& a, int first, int stride) {
 #pragma clang fp contract(off)
-  if (a.gate && *a.gate == 0) return;  // learned pipeline: grid 1 runs only as the fallback
   const int nt = a.nx_vec * a.ny_vec;
-  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nt; t += gridDim.x * blockDim.x) {
+  for (int t = first; t < nt; t += stride) {
     const int vy = t / a.nx_vec, vx = t - (t / a.nx_vec) * a.nx_vec;
     float sx = 0.f, sy = 0.f;
     int n = 0;
@@ -3171,15 +3191,59 @@ __global__ void k_smooth(SmoothArgs a) {
   }
 }
 
-__global__ void k_prep(const float* cx, const float* cy, const uint8_t* cv, int K, float* dx, float* dy, uint8_t* dv,
-                       int nt, int* reset_count, const int* gate) {
+__global__ void k_smooth(SmoothArgs a) {
+  if (a.gate && *a.gate == 0) return;  // learned pipeline: grid 1 runs only as the fallback
+  smooth_range(a, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
+}
+
+__device__ __forceinline__ void prep_range(const PrepArgs& p, int first, int stride) {
+  for (int t = first; t < p.nt; t += stride) {
+    const long long o = (long long)t * p.K;
+    p.dv[t] = p.cv[o];
+    p.dx[t] = p.cx[o];
+    p.dy[t] = p.cy[o];
+  }
+}
+
+__global__ void k_prep(PrepArgs p, const int* gate) {
   if (gate && *gate == 0) return;
-  if (reset_count && blockIdx.x == 0 && threadIdx.x == 0) *reset_count = 0;
-  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nt; t += gridDim.x * blockDim.x) {
-    const long long o = (long long)t * K;
-    dv[t] = cv[o];
-    dx[t] = cx[o];
-    dy[t] = cy[o];
+  if (p.reset_count && blockIdx.x == 0 && threadIdx.x == 0) *p.reset_count = 0;
+  prep_range(p, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
+}
+
+// The steps of a filter sequence in one workgroup (FilterSeq), a barrier
+// after each; __syncthreads makes each step's global writes visible to the
+// whole workgroup before the next step reads them.
+constexpr int kFilterSeqThreads = 1024;
+template <int WXM, int WYM>
+__global__ void __launch_bounds__(kFilterSeqThreads) k_filter_seq(FilterSeq q) {
+  if (q.gate && *q.gate == 0) return;  // learned pipeline: grid 1 runs only as the fallback
+  __shared__ FillShared sh;
+  const int tid = (int)threadIdx.x;
+  for (int i = 0; i < q.nsteps; ++i) {
+    const int kind = q.step[i].kind;
+    if (kind == FS_PREP) {
+      const PrepArgs p = q.step[i].prep;
+      if (p.reset_count && tid == 0) *p.reset_count = 0;
+      prep_range(p, tid, kFilterSeqThreads);
+    } else if (kind == FS_ZERO) {
+      if (tid == 0) *q.step[i].zero = 0;
+    } else if (kind == FS_MEDIAN) {
+      const MedArgs m = q.step[i].med;
+      if (m.reset_count && tid == 0) *m.reset_count = 0;
+      __syncthreads();  // the reset lands before any append of this pass
+      const int lanes = 2 * m.nx_vec * m.ny_vec;
+      // every lane of a wave takes part in each round (the two lanes of a
+      // vector exchange their medians); lanes past the end write nothing
+      for (int g0 = 0; g0 < lanes; g0 += kFilterSeqThreads) median2_lane<WXM, WYM>(m, g0 + tid);
+    } else if (kind == FS_FILL_LIST) {
+      fill_list_range(q.step[i].fill, tid, kFilterSeqThreads);
+    } else if (kind == FS_FILL) {
+      fill_sweeps(q.step[i].fill, sh);
+    } else if (kind == FS_SMOOTH) {
+      smooth_range(q.step[i].smooth, tid, kFilterSeqThreads);
+    }
+    __syncthreads();
   }
 }
 __global__ void k_zero_ints(int* p, int n, const int* gate) {
@@ -3633,8 +3697,22 @@ hipError_t launch_smooth(const SmoothArgs& a, hipStream_t s) {
 
 hipError_t launch_prep(const float* cx, const float* cy, const uint8_t* cv, int K, float* dx, float* dy, uint8_t* dv,
                        int nt, int* reset_count, hipStream_t s, const int* gate) {
-  hipLaunchKernelGGL(k_prep, dim3(grid_for(nt, 256)), dim3(256), 0, s, cx, cy, cv, K, dx, dy, dv, nt, reset_count,
-                     gate);
+  const PrepArgs p{cx, cy, cv, K, dx, dy, dv, nt, reset_count};
+  hipLaunchKernelGGL(k_prep, dim3(grid_for(nt, 256)), dim3(256), 0, s, p, gate);
+  return hipGetLastError();
+}
+
+bool filter_seq_supported(int wxm, int wym) {
+  return (wxm == 5 && wym == 5) || (wxm == 3 && wym == 3) || (wxm == 7 && wym == 7);
+}
+
+hipError_t launch_filter_seq(const FilterSeq& q, hipStream_t s) {
+  if (q.nsteps < 0 || q.nsteps > kFilterSeqSteps) return hipErrorInvalidValue;
+  const dim3 g(1), b(kFilterSeqThreads);
+  if (q.wxm == 5 && q.wym == 5) hipLaunchKernelGGL((k_filter_seq<5, 5>), g, b, 0, s, q);
+  else if (q.wxm == 3 && q.wym == 3) hipLaunchKernelGGL((k_filter_seq<3, 3>), g, b, 0, s, q);
+  else if (q.wxm == 7 && q.wym == 7) hipLaunchKernelGGL((k_filter_seq<7, 7>), g, b, 0, s, q);
+  else return hipErrorInvalidValue;
   return hipGetLastError();
 }
 
