@@ -129,6 +129,51 @@ __device__ __forceinline__ bf16x8 gpatch8(const uint16_t* gout, const uint16_t* 
   }
 }
 
+// Network input value at downsampled pixel (y, x): the normalised mean of a
+// 4x4 block of frame 1 (rows 4-byte aligned) or frame 2 (any alignment).
+__device__ __forceinline__ uint16_t down_f1(const uint8_t* f1, long long p1, int y, int x) {
+  unsigned int s1 = 0;
+#pragma unroll
+  for (int r = 0; r < kCnnDown; ++r) {
+    const uchar4 a = *reinterpret_cast<const uchar4*>(f1 + (long long)(y * kCnnDown + r) * p1 + x * kCnnDown);
+    s1 += a.x + a.y + a.z + a.w;
+  }
+  const float inv = 1.0f / (kCnnDown * kCnnDown);
+  return f2bf(((float)s1 * inv - kCnnMu) * kCnnInvSigma);
+}
+__device__ __forceinline__ uint16_t down_f2(const uint8_t* f2, long long p2, int y, int x) {
+  unsigned int s2 = 0;
+#pragma unroll
+  for (int r = 0; r < kCnnDown; ++r) {
+    const uint8_t* q = f2 + (long long)(y * kCnnDown + r) * p2 + x * kCnnDown;
+    s2 += q[0] + q[1] + q[2] + q[3];
+  }
+  const float inv = 1.0f / (kCnnDown * kCnnDown);
+  return f2bf(((float)s2 * inv - kCnnMu) * kCnnInvSigma);
+}
+
+// Eight K values (k = tap * 2 + frame) of the first layer's input patch of
+// output pixel (oy, ox), formed from the frames (the values of patch8 over
+// the k_downsample plane).
+template <int S>
+__device__ __forceinline__ bf16x8 patch8_frames(const ConvFwdArgs& a, int oy, int ox, int k) {
+  uint16_t v[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const int kk = k + i;
+    uint16_t x = 0;
+    if (kk < 18) {
+      const int tap = kk >> 1, ci = kk & 1;
+      const int ky = tap / 3, kx = tap - 3 * ky;
+      const int iy = oy * S + ky - 1, ix = ox * S + kx - 1;
+      if ((unsigned)iy < (unsigned)a.hin && (unsigned)ix < (unsigned)a.win)
+        x = ci == 0 ? down_f1(a.f1, a.p1, iy, ix) : down_f2(a.f2, a.p2, iy, ix);
+    }
+    v[i] = x;
+  }
+  return pack8(v);
+}
+
 // ---------------------------------------------------------------- forward
 // One output tile: MT output-channel tiles (of 16) of PT * 16 consecutive
 // output pixels of one row; the COUT / 16 / MT channel blocks of a pixel
@@ -138,7 +183,7 @@ __device__ __forceinline__ bf16x8 gpatch8(const uint16_t* gout, const uint16_t* 
 // deep layers); with KS = 1 each wave owns a tile.  The K loop is unrolled
 // by three so that the loads of later steps are in flight while the matrix
 // cores work.
-template <int CIN, int COUT, int S, bool ACT, bool FINAL, int PT, int MT, int KS>
+template <int CIN, int COUT, int S, bool ACT, bool FINAL, int PT, int MT, int KS, bool FRAMES = false>
 __global__ void __launch_bounds__(256) k_conv_fwd(ConvFwdArgs a) {
   constexpr int KP = cnn_kpad(CIN);
   constexpr int NCB = cnn_pad16(COUT) / 16 / MT;  // channel blocks per pixel group
@@ -172,7 +217,10 @@ __global__ void __launch_bounds__(256) k_conv_fwd(ConvFwdArgs a) {
 #pragma unroll
     for (int mt = 0; mt < MT; ++mt) wa[mt] = ld8(wbase + (long long)(mt * 16 + col) * KP + k);
 #pragma unroll
-    for (int pt = 0; pt < PT; ++pt) xb[pt] = patch8<CIN, S>(in, a.hin, a.win, oy, ox0 + pt * 16 + col, k);
+    for (int pt = 0; pt < PT; ++pt) {
+      if constexpr (FRAMES) xb[pt] = patch8_frames<S>(a, oy, ox0 + pt * 16 + col, k);
+      else xb[pt] = patch8<CIN, S>(in, a.hin, a.win, oy, ox0 + pt * 16 + col, k);
+    }
 #pragma unroll
     for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
@@ -457,23 +505,14 @@ __global__ void k_pack_layer(const float* w, int cin, int cout, uint16_t* wf, ui
   wt[(long long)ci * cnn_kpad(cout) + tap * cout + co] = h;
 }
 
+
 // ---------------------------------------------------------------- input, gate
 __global__ void k_downsample(const uint8_t* f1, long long p1, const uint8_t* f2, long long p2, int w, int h,
                              uint16_t* out) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= w * h) return;
   const int y = t / w, x = t - (t / w) * w;
-  unsigned int s1 = 0, s2 = 0;
-#pragma unroll
-  for (int r = 0; r < kCnnDown; ++r) {
-    const uchar4 a = *reinterpret_cast<const uchar4*>(f1 + (long long)(y * kCnnDown + r) * p1 + x * kCnnDown);
-    s1 += a.x + a.y + a.z + a.w;
-    const uint8_t* q = f2 + (long long)(y * kCnnDown + r) * p2 + x * kCnnDown;
-    s2 += q[0] + q[1] + q[2] + q[3];
-  }
-  const float inv = 1.0f / (kCnnDown * kCnnDown);
-  const uint16_t a = f2bf(((float)s1 * inv - kCnnMu) * kCnnInvSigma);
-  const uint16_t b = f2bf(((float)s2 * inv - kCnnMu) * kCnnInvSigma);
+  const uint16_t a = down_f1(f1, p1, y, x), b = down_f2(f2, p2, y, x);
   reinterpret_cast<unsigned int*>(out)[t] = a | ((unsigned int)b << 16);
 }
 
@@ -503,6 +542,20 @@ inline int blocks(long long n, int b) { return (int)((n + b - 1) / b); }
 template <int CIN, int COUT, int S, bool ACT, bool FINAL>
 hipError_t fwd(const ConvFwdArgs& a, hipStream_t s) {
   constexpr int PT = 1;
+  if constexpr (CIN == 2) {
+    if (a.f1) {
+      // input from the frames: both 16-channel blocks of a pixel group in
+      // one wave, so that each input patch is formed once
+      if (a.batch != 1 || !a.f2 || (a.p1 & 3) != 0 || (reinterpret_cast<uintptr_t>(a.f1) & 3) != 0 ||
+          a.wout % 16 != 0)
+        return hipErrorInvalidValue;
+      constexpr int MT2 = cnn_pad16(COUT) / 16;
+      const long long tiles = (long long)a.hout * (a.wout / 16);
+      hipLaunchKernelGGL((k_conv_fwd<CIN, COUT, S, ACT, FINAL, 1, MT2, 1, true>), dim3((unsigned)blocks(tiles, 4)),
+                         dim3(256), 0, s, a);
+      return hipGetLastError();
+    }
+  }
   constexpr int MT = 1;  // one 16-channel tile per wave: the small layers need the waves
   constexpr int NCB = cnn_pad16(COUT) / 16 / MT;
   constexpr int KS = cnn_kpad(CIN) / 32 >= 8 ? 4 : 1;  // split the K steps of the deep layers over 4 waves

@@ -94,7 +94,7 @@ struct pivx_ctx {
   // of the network field, network input and hidden activations (batch 1)
   int* lgate = nullptr;
   int* lcount = nullptr;
-  uint16_t* cx0 = nullptr;
+  bool learned_ready = false;  // lgate, lcount and cact are allocated
   uint16_t* cact[5] = {};
   hipGraph_t lgraph = nullptr;
   hipGraphExec_t lgexec = nullptr;
@@ -1844,16 +1844,30 @@ void layer_dims(int h, int w, int l, int& hin, int& win, int& hout, int& wout) {
 
 bool cnn_shape_ok(int h, int w) { return h >= 8 && w >= 128 && h % 8 == 0 && w % 128 == 0; }
 
+// The frames of a pair as the network input (batch 1): layer 0 averages them
+// over 4x4 blocks as it reads them, as k_downsample would.
+struct FrameInput {
+  const uint8_t* f1; long long p1;
+  const uint8_t* f2; long long p2;
+};
+
 // Forward pass over a batch: x0 sample b at x0 + (bidx ? bidx[b] : b) * xstride;
 // hidden activations into act[0..4]; output planes at out_x / out_y + b * ostride.
 int cnn_forward_enqueue(const pivx_cnn* net, const uint16_t* x0, long long xstride, const int* bidx, int B, int H,
                         int W, uint16_t* const* act, float* out_x, float* out_y, long long ostride, hipStream_t s,
-                        uint8_t* out_v = nullptr, float* ext_x = nullptr, float* ext_y = nullptr, int ext_b = 0) {
+                        uint8_t* out_v = nullptr, float* ext_x = nullptr, float* ext_y = nullptr, int ext_b = 0,
+                        const FrameInput* frames = nullptr) {
   for (int l = 0; l < kCnnLayers; ++l) {
     const CnnLayer& L = kCnnArch[l];
     ConvFwdArgs a{};
     layer_dims(H, W, l, a.hin, a.win, a.hout, a.wout);
     a.in = l == 0 ? x0 : act[l - 1];
+    if (l == 0 && frames) {  // the first layer forms its input from the frames
+      a.f1 = frames->f1;
+      a.p1 = frames->p1;
+      a.f2 = frames->f2;
+      a.p2 = frames->p2;
+    }
     a.in_bstride = l == 0 ? xstride : (long long)a.hin * a.win * L.cin;
     a.batch_index = l == 0 ? bidx : nullptr;
     a.batch = B;
@@ -2001,21 +2015,20 @@ int learned_shape_check(const pivx_ctx* c) {
 
 // per-handle buffers of the learned pipeline (allocated once, outside any capture)
 int learned_alloc(pivx_ctx* c) {
-  if (c->cx0) return PIVX_OK;
+  if (c->learned_ready) return PIVX_OK;
   const int H = c->p.ny / kCnnDown, W = c->p.nx / kCnnDown;
   int rc = 0;
-  rc |= dalloc(c, &c->lgate, 1);
-  rc |= dalloc(c, &c->lcount, 1);
+  if (!c->lgate) rc |= dalloc(c, &c->lgate, 1);
+  if (!c->lcount) rc |= dalloc(c, &c->lcount, 1);
   for (int l = 0; l < kCnnLayers - 1; ++l) {
     int hin, win, hout, wout;
     layer_dims(H, W, l, hin, win, hout, wout);
-    rc |= dalloc(c, &c->cact[l], (size_t)hout * wout * kCnnArch[l].cout);
+    if (!c->cact[l]) rc |= dalloc(c, &c->cact[l], (size_t)hout * wout * kCnnArch[l].cout);
   }
   if (rc) return PIVX_ERR_HIP;
-  rc = dalloc(c, &c->cx0, (size_t)H * W * 2);
-  if (rc) return rc;
   HIPCHK(c, hipMemsetAsync(c->lgate, 0, sizeof(int), c->stream));
   HIPCHK(c, hipMemsetAsync(c->lcount, 0, sizeof(int), c->stream));
+  c->learned_ready = true;
   return PIVX_OK;
 }
 
@@ -2027,13 +2040,12 @@ int learned_enqueue(pivx_ctx* c, const pivx_cnn* net, const pivx_filter_params* 
   const int nt1 = g1.nt_vec;
   const int H = c->p.ny / kCnnDown, W = c->p.nx / kCnnDown;
   int rc;
-  // network field -> grid-1 field
-  hipError_t e = launch_downsample(c->f1, c->p.nx, c->f2, c->pitch2, c->p.nx, c->p.ny, c->cx0, c->stream);
-  if (e != hipSuccess) return fail(c, PIVX_ERR_HIP, "network input: %s", hipGetErrorString(e));
-  // the last layer also sets the field's flags and writes its extension
-  // (the fallback rewrites both)
-  if ((rc = cnn_forward_enqueue(net, c->cx0, 0, nullptr, 1, H, W, c->cact, G1.fx, G1.fy, 0, c->stream, G1.fv, G1.ex,
-                                G1.ey, kExBorder))) {
+  // network field -> grid-1 field; the first layer reads the frames
+  // directly (4x4 block means formed on the fly), the last also sets the
+  // field's flags and writes its extension (the fallback rewrites both)
+  const FrameInput fin{c->f1, c->p.nx, c->f2, c->pitch2};
+  if ((rc = cnn_forward_enqueue(net, nullptr, 0, nullptr, 1, H, W, c->cact, G1.fx, G1.fy, 0, c->stream, G1.fv, G1.ex,
+                                G1.ey, kExBorder, &fin))) {
     c->err = global_error();
     return rc;
   }

@@ -217,3 +217,34 @@ def test_fallback_runs_plain_grid1(pivx, orc):
             _, fb = x.learned_stats()
             assert fb == 0
             assert not np.array_equal(x.filtered(1)[0], want[1][0])  # the network field was kept
+
+
+def _net_input(img):
+    """The network input plane of one frame on the host: 4x4 block means,
+    normalised as (mean - 30) / 40 in FP32, rounded to bfloat16."""
+    ny, nx = img.shape
+    s = img.reshape(ny // 4, 4, nx // 4, 4).astype(np.int64).sum(axis=(1, 3))
+    f = (s.astype(np.float32) * np.float32(1.0 / 16.0) - np.float32(30.0)) * np.float32(1.0 / 40.0)
+    return _bf16_round(f)
+
+
+def test_network_input_formed_from_frames(pivx):
+    """The learned pipeline's first layer forms the network input from the
+    frames as it reads them (no downsampled plane): the grid-1 field it
+    produces equals pivx_cnn_forward on the same input formed on the host
+    (4x4 block means of both frames, normalised, rounded to bfloat16), bit for
+    bit."""
+    p, fp = _learned_setup(pivx, 2048, 1024)
+    sp = pivx.synth_params(2048, 1024, seed=4242, flow=2, a0=8.0)
+    a, b = pivx.synth_pair(sp)
+    with pivx.Xcorr2dMulti(p) as x, pivx.Cnn(0, seed=3) as net:
+        x.prep_img_data(a, b)
+        x.run_pair_learned(net, fp, fp, max_outlier_fraction=1.0)
+        fx, fy, fv = x.filtered(1)
+        x0 = np.stack([_net_input(a), _net_input(b)], axis=-1)[None]
+        out = net.forward(x0)
+    _report("network input from frames", max_abs_dx=float(np.abs(fx - out[0, 0]).max()),
+            max_abs_dy=float(np.abs(fy - out[0, 1]).max()))
+    np.testing.assert_array_equal(fx, out[0, 0])
+    np.testing.assert_array_equal(fy, out[0, 1])
+    assert fv.all()
