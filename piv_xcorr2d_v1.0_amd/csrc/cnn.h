@@ -55,6 +55,18 @@ struct ConvFwdArgs {
   const uint8_t* f2; long long p2;
 };
 
+// Layers 3..5 of one pair (batch 1) in one launch (k_conv_tail): the two
+// 64 -> 64 stride-1 layers and the 64 -> 2 field layer.
+struct ConvTailArgs {
+  const uint16_t* in;        // layer-2 output, bf16 NHWC [h][w][64]
+  int h, w;                  // field size
+  const uint16_t* w3; const uint16_t* w4; const uint16_t* w5;  // packed weights [pad16(cout)][576]
+  const float* b3; const float* b4; const float* b5;           // biases
+  float* out_x; float* out_y;  // the field planes [h][w]
+  uint8_t* out_v;            // optional: validity flags, set to 1
+  float* ext_x; float* ext_y; int ext_b;  // optional: the field's border-replicated extension
+};
+
 // Gradient with respect to the layer input.
 struct ConvBwdDataArgs {
   const uint16_t* gout;    // bf16 dL/d(layer output) NHWC [batch][hout][wout][cout]
@@ -79,6 +91,7 @@ int cnn_param_count();
 int cnn_weight_offset(int layer);  // master layout per layer: W [cout][9][cin], then b [cout]
 
 hipError_t launch_conv_fwd(int layer, const ConvFwdArgs& a, hipStream_t s);
+hipError_t launch_conv_tail(const ConvTailArgs& a, hipStream_t s);
 hipError_t launch_conv_bwd_data(int layer, const ConvBwdDataArgs& a, hipStream_t s);
 hipError_t launch_conv_bwd_w(int layer, const ConvBwdWArgs& a, int nchunks, hipStream_t s);
 // grad_w [cout][9][cin] and grad_b [cout] of one layer from the partial sums.

@@ -174,6 +174,28 @@ __device__ __forceinline__ bf16x8 patch8_frames(const ConvFwdArgs& a, int oy, in
   return pack8(v);
 }
 
+// Field vector (vx, vy) of output cell (oy, ox) of the last layer: the FP32
+// planes at base + oy * wout + ox, the validity flag (optional) and the
+// border-replicated extension of the field (optional, the extend stage).
+__device__ __forceinline__ void store_field(float* out_x, float* out_y, uint8_t* out_v, float* ext_x, float* ext_y,
+                                            int ext_b, int hout, int wout, long long base, int oy, int ox, float vx,
+                                            float vy) {
+  const long long o = base + (long long)oy * wout + ox;
+  out_x[o] = vx;
+  out_y[o] = vy;
+  if (out_v) out_v[o] = 1;
+  if (ext_x) {
+    const int B = ext_b, nxe = wout + 2 * B;
+    const int i0 = ox == 0 ? 0 : ox + B, i1 = ox == wout - 1 ? nxe - 1 : ox + B;
+    const int j0 = oy == 0 ? 0 : oy + B, j1 = oy == hout - 1 ? hout + 2 * B - 1 : oy + B;
+    for (int j = j0; j <= j1; ++j)
+      for (int i = i0; i <= i1; ++i) {
+        ext_x[j * nxe + i] = vx;
+        ext_y[j * nxe + i] = vy;
+      }
+  }
+}
+
 // ---------------------------------------------------------------- forward
 // One output tile: MT output-channel tiles (of 16) of PT * 16 consecutive
 // output pixels of one row; the COUT / 16 / MT channel blocks of a pixel
@@ -246,23 +268,9 @@ __global__ void __launch_bounds__(256) k_conv_fwd(ConvFwdArgs a) {
   for (int pt = 0; pt < PT; ++pt) {
     const int ox = ox0 + pt * 16 + col;
     if constexpr (FINAL) {
-      if (kg == 0) {
-        const long long o = (long long)b * a.out_bstride + (long long)oy * a.wout + ox;
-        const float vx = acc[0][pt][0] + a.bias[0], vy = acc[0][pt][1] + a.bias[1];
-        a.out_x[o] = vx;
-        a.out_y[o] = vy;
-        if (a.out_v) a.out_v[o] = 1;
-        if (a.ext_x) {  // border-replicated extension of the field (the extend stage)
-          const int B = a.ext_b, nxe = a.wout + 2 * B;
-          const int i0 = ox == 0 ? 0 : ox + B, i1 = ox == a.wout - 1 ? nxe - 1 : ox + B;
-          const int j0 = oy == 0 ? 0 : oy + B, j1 = oy == a.hout - 1 ? a.hout + 2 * B - 1 : oy + B;
-          for (int j = j0; j <= j1; ++j)
-            for (int i = i0; i <= i1; ++i) {
-              a.ext_x[j * nxe + i] = vx;
-              a.ext_y[j * nxe + i] = vy;
-            }
-        }
-      }
+      if (kg == 0)
+        store_field(a.out_x, a.out_y, a.out_v, a.ext_x, a.ext_y, a.ext_b, a.hout, a.wout,
+                    (long long)b * a.out_bstride, oy, ox, acc[0][pt][0] + a.bias[0], acc[0][pt][1] + a.bias[1]);
     } else {
 #pragma unroll
       for (int mt = 0; mt < MT; ++mt) {
@@ -282,6 +290,133 @@ __global__ void __launch_bounds__(256) k_conv_fwd(ConvFwdArgs a) {
       }
     }
   }
+}
+
+
+// ---------------------------------------------------------------- forward, fused tail
+// The three stride-1 layers at the field resolution (64 -> 64, 64 -> 64,
+// 64 -> 2) of one pair in one launch.  A workgroup computes a TT x TT tile of
+// the field: it stages the layer-2 output on the tile and a 3-cell halo in
+// LDS, computes the first hidden layer on the tile plus 2 cells, the second
+// on the tile plus 1 (cells outside the field are set to 0, the zero padding
+// of the next layer), then the field itself.  Wave w owns output channels
+// 16w .. 16w + 15 of a hidden layer and holds that channel block's weights
+// in registers for the whole layer; the K steps of every product run in the
+// same four ranges, summed in the same order, as in k_conv_fwd with KS = 4,
+// so the field is bit-identical to the layer-by-layer forward pass.
+constexpr int kTailTile = 8;
+constexpr int kTailC = 64;              // channels of the hidden layers
+constexpr int kTailPix = kTailC + 8;    // LDS elements per cell (16 bytes of padding)
+constexpr int kTailKSteps = 9 * kTailC / 32;
+constexpr int kTailKPer = (kTailKSteps + 3) / 4;  // K steps per range, as KS = 4 splits them
+
+
+// One 16-channel x 16-cell product of a 3x3 layer from region src (side RS,
+// cells of kTailPix elements) to cell indices p0 .. p0 + 15 of a region of
+// side RD = RS - 2; wreg: this lane's weight slices (row = 16 m + col).
+template <int RS>
+__device__ __forceinline__ f32x4 tail_product(const uint16_t* src, const bf16x8 (&wreg)[kTailKSteps], int p0,
+                                              int col, int kg) {
+  constexpr int RD = RS - 2;
+  int p = p0 + col;
+  p = p < RD * RD ? p : RD * RD - 1;  // cells past the region: any valid cell, not stored
+  const int py = p / RD, px = p - (p / RD) * RD;
+  f32x4 part[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) part[q] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int j = 0; j < kTailKPer; ++j)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int st = q * kTailKPer + j;
+      if (st < kTailKSteps) {
+        const int k = st * 32 + 8 * kg;
+        const int tap = k / kTailC, ci = k - tap * kTailC;
+        const int ky = tap / 3, kx = tap - 3 * ky;
+        const bf16x8 xb = __builtin_bit_cast(
+            bf16x8, *reinterpret_cast<const u32x4*>(src + ((py + ky) * RS + (px + kx)) * kTailPix + ci));
+        part[q] = mfma(wreg[st], xb, part[q]);
+      }
+    }
+  f32x4 acc = part[0];
+#pragma unroll
+  for (int q = 1; q < 4; ++q) acc += part[q];
+  return acc;
+}
+
+__device__ __forceinline__ void tail_weights(const uint16_t* w, int row, int kg, bf16x8 (&wreg)[kTailKSteps]) {
+#pragma unroll
+  for (int st = 0; st < kTailKSteps; ++st) wreg[st] = ld8(w + (long long)row * (9 * kTailC) + st * 32 + 8 * kg);
+}
+
+// Hidden layer from src (side RS) into dst (side RS - 2, origin (y0, x0) in
+// field cells): bias, leaky ReLU, bf16; cells outside the field hold 0.
+template <int RS>
+__device__ __forceinline__ void tail_hidden(const uint16_t* src, uint16_t* dst, const uint16_t* w, const float* bias,
+                                            int y0, int x0, int h, int wd, int wave, int col, int kg) {
+  constexpr int RD = RS - 2;
+  bf16x8 wreg[kTailKSteps];
+  tail_weights(w, wave * 16 + col, kg, wreg);
+  const int co0 = wave * 16 + 4 * kg;
+  float bb[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) bb[j] = bias[co0 + j];
+  for (int p0 = 0; p0 < RD * RD; p0 += 16) {
+    const f32x4 acc = tail_product<RS>(src, wreg, p0, col, kg);
+    const int p = p0 + col;
+    if (p < RD * RD) {
+      const int py = p / RD, px = p - (p / RD) * RD;
+      const bool inside = (unsigned)(y0 + py) < (unsigned)h && (unsigned)(x0 + px) < (unsigned)wd;
+      uint16_t hv[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        float v = acc[j] + bb[j];
+        v = v > 0.f ? v : v * kCnnLeak;
+        hv[j] = inside ? f2bf(v) : (uint16_t)0;
+      }
+      uint2 u;
+      u.x = hv[0] | ((unsigned int)hv[1] << 16);
+      u.y = hv[2] | ((unsigned int)hv[3] << 16);
+      *reinterpret_cast<uint2*>(dst + p * kTailPix + co0) = u;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256) k_conv_tail(ConvTailArgs a) {
+  constexpr int T = kTailTile, RI = T + 6, RA = T + 4, RB = T + 2;
+  // region I (layer-2 output, side RI) and later region B (side RB) share
+  // the first buffer; region A (side RA) has its own
+  __shared__ __attribute__((aligned(16))) uint16_t bufI[RI * RI * kTailPix];
+  __shared__ __attribute__((aligned(16))) uint16_t bufA[RA * RA * kTailPix];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int col = lane & 15, kg = lane >> 4;
+  const int tiles_x = (a.w + T - 1) / T;
+  const int ty = blockIdx.x / tiles_x, tx = blockIdx.x - (blockIdx.x / tiles_x) * tiles_x;
+  const int oy0 = ty * T, ox0 = tx * T;
+  // stage region I: cells (oy0 - 3 + r, ox0 - 3 + c), 8 channels per thread and step
+  for (int i = threadIdx.x; i < RI * RI * (kTailC / 8); i += 256) {
+    const int cell = i / (kTailC / 8), c8 = i - cell * (kTailC / 8);
+    const int r = cell / RI, c = cell - (cell / RI) * RI;
+    const int y = oy0 - 3 + r, x = ox0 - 3 + c;
+    u32x4 v = u32x4{0u, 0u, 0u, 0u};
+    if ((unsigned)y < (unsigned)a.h && (unsigned)x < (unsigned)a.w)
+      v = *reinterpret_cast<const u32x4*>(a.in + ((long long)y * a.w + x) * kTailC + c8 * 8);
+    *reinterpret_cast<u32x4*>(bufI + cell * kTailPix + c8 * 8) = v;
+  }
+  __syncthreads();
+  tail_hidden<RI>(bufI, bufA, a.w3, a.b3, oy0 - 2, ox0 - 2, a.h, a.w, wave, col, kg);
+  __syncthreads();  // region I is read: its buffer takes region B
+  tail_hidden<RA>(bufA, bufI, a.w4, a.b4, oy0 - 1, ox0 - 1, a.h, a.w, wave, col, kg);
+  __syncthreads();
+  // the field: one 16-cell product per wave (T * T = 64 cells)
+  bf16x8 wreg[kTailKSteps];
+  tail_weights(a.w5, col, kg, wreg);
+  const f32x4 acc = tail_product<RB>(bufI, wreg, wave * 16, col, kg);
+  const int p = wave * 16 + col;
+  const int oy = oy0 + p / T, ox = ox0 + (p - (p / T) * T);
+  if (kg == 0 && oy < a.h && ox < a.w)
+    store_field(a.out_x, a.out_y, a.out_v, a.ext_x, a.ext_y, a.ext_b, a.h, a.w, 0, oy, ox, acc[0] + a.b5[0],
+                acc[1] + a.b5[1]);
 }
 
 // ---------------------------------------------------------------- backward, data
@@ -609,6 +744,13 @@ hipError_t launch_conv_fwd(int layer, const ConvFwdArgs& a, hipStream_t s) {
     case 5: return fwd<64, 2, 1, false, true>(a, s);
     default: return hipErrorInvalidValue;
   }
+}
+
+hipError_t launch_conv_tail(const ConvTailArgs& a, hipStream_t s) {
+  if (a.h < 1 || a.w < 1) return hipErrorInvalidValue;
+  const int tiles = ((a.h + kTailTile - 1) / kTailTile) * ((a.w + kTailTile - 1) / kTailTile);
+  hipLaunchKernelGGL(k_conv_tail, dim3(tiles), dim3(256), 0, s, a);
+  return hipGetLastError();
 }
 
 hipError_t launch_conv_bwd_data(int layer, const ConvBwdDataArgs& a, hipStream_t s) {

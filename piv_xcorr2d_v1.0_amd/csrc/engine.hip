@@ -1857,7 +1857,13 @@ int cnn_forward_enqueue(const pivx_cnn* net, const uint16_t* x0, long long xstri
                         int W, uint16_t* const* act, float* out_x, float* out_y, long long ostride, hipStream_t s,
                         uint8_t* out_v = nullptr, float* ext_x = nullptr, float* ext_y = nullptr, int ext_b = 0,
                         const FrameInput* frames = nullptr) {
-  for (int l = 0; l < kCnnLayers; ++l) {
+  // one pair: layers 3..5 run as one launch (k_conv_tail, bit-identical)
+  const int nlayers = B == 1 ? 3 : kCnnLayers;
+  static_assert(kCnnLayers == 6 && kCnnArch[3].cin == 64 && kCnnArch[3].cout == 64 && kCnnArch[4].cout == 64 &&
+                    kCnnArch[5].cout == 2 && kCnnArch[3].stride == 1 && kCnnArch[4].stride == 1 &&
+                    kCnnArch[5].stride == 1,
+                "k_conv_tail computes layers 3..5 of this architecture");
+  for (int l = 0; l < nlayers; ++l) {
     const CnnLayer& L = kCnnArch[l];
     ConvFwdArgs a{};
     layer_dims(H, W, l, a.hin, a.win, a.hout, a.wout);
@@ -1886,6 +1892,26 @@ int cnn_forward_enqueue(const pivx_cnn* net, const uint16_t* x0, long long xstri
     }
     const hipError_t e = launch_conv_fwd(l, a, s);
     if (e != hipSuccess) return cnn_fail("network layer %d: %s", l, hipGetErrorString(e));
+  }
+  if (nlayers < kCnnLayers) {
+    ConvTailArgs t{};
+    int hin, win;
+    layer_dims(H, W, 3, hin, win, t.h, t.w);
+    t.in = act[2];
+    t.w3 = net->wf[3];
+    t.w4 = net->wf[4];
+    t.w5 = net->wf[5];
+    t.b3 = net->master + cnn_weight_offset(3) + kCnnArch[3].cout * 9 * kCnnArch[3].cin;
+    t.b4 = net->master + cnn_weight_offset(4) + kCnnArch[4].cout * 9 * kCnnArch[4].cin;
+    t.b5 = net->master + cnn_weight_offset(5) + kCnnArch[5].cout * 9 * kCnnArch[5].cin;
+    t.out_x = out_x;
+    t.out_y = out_y;
+    t.out_v = out_v;
+    t.ext_x = ext_x;
+    t.ext_y = ext_y;
+    t.ext_b = ext_b;
+    const hipError_t e = launch_conv_tail(t, s);
+    if (e != hipSuccess) return cnn_fail("network layers 3-5: %s", hipGetErrorString(e));
   }
   return PIVX_OK;
 }

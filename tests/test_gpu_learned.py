@@ -230,10 +230,12 @@ def _net_input(img):
 
 def test_network_input_formed_from_frames(pivx):
     """The learned pipeline's first layer forms the network input from the
-    frames as it reads them (no downsampled plane): the grid-1 field it
-    produces equals pivx_cnn_forward on the same input formed on the host
-    (4x4 block means of both frames, normalised, rounded to bfloat16), bit for
-    bit."""
+    frames as it reads them (no downsampled plane), and for one pair the last
+    three layers run as one launch (k_conv_tail): the grid-1 field it produces
+    equals pivx_cnn_forward on the same input formed on the host (4x4 block
+    means of both frames, normalised, rounded to bfloat16), bit for bit, both
+    for that one input (the same fused path) and for a batch of two copies of
+    it (every layer its own launch)."""
     p, fp = _learned_setup(pivx, 2048, 1024)
     sp = pivx.synth_params(2048, 1024, seed=4242, flow=2, a0=8.0)
     a, b = pivx.synth_pair(sp)
@@ -243,8 +245,12 @@ def test_network_input_formed_from_frames(pivx):
         fx, fy, fv = x.filtered(1)
         x0 = np.stack([_net_input(a), _net_input(b)], axis=-1)[None]
         out = net.forward(x0)
+        out2 = net.forward(np.concatenate([x0, x0]))
     _report("network input from frames", max_abs_dx=float(np.abs(fx - out[0, 0]).max()),
-            max_abs_dy=float(np.abs(fy - out[0, 1]).max()))
+            max_abs_dy=float(np.abs(fy - out[0, 1]).max()),
+            layerwise_max_abs=float(np.abs(out2[0] - out[0]).max()))
     np.testing.assert_array_equal(fx, out[0, 0])
     np.testing.assert_array_equal(fy, out[0, 1])
+    np.testing.assert_array_equal(out2[0], out[0])
+    np.testing.assert_array_equal(out2[1], out[0])
     assert fv.all()
