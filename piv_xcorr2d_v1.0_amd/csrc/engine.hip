@@ -94,6 +94,7 @@ struct pivx_ctx {
   // of the network field, network input and hidden activations (batch 1)
   int* lgate = nullptr;
   int* lcount = nullptr;
+  int* lgate_work = nullptr;  // [2]: count accumulator and block ticket of the gated median, 0 between launches
   bool learned_ready = false;  // lgate, lcount and cact are allocated
   uint16_t* cact[5] = {};
   hipGraph_t lgraph = nullptr;
@@ -2046,6 +2047,7 @@ int learned_alloc(pivx_ctx* c) {
   int rc = 0;
   if (!c->lgate) rc |= dalloc(c, &c->lgate, 1);
   if (!c->lcount) rc |= dalloc(c, &c->lcount, 1);
+  if (!c->lgate_work) rc |= dalloc(c, &c->lgate_work, 2);
   for (int l = 0; l < kCnnLayers - 1; ++l) {
     int hin, win, hout, wout;
     layer_dims(H, W, l, hin, win, hout, wout);
@@ -2054,6 +2056,7 @@ int learned_alloc(pivx_ctx* c) {
   if (rc) return PIVX_ERR_HIP;
   HIPCHK(c, hipMemsetAsync(c->lgate, 0, sizeof(int), c->stream));
   HIPCHK(c, hipMemsetAsync(c->lcount, 0, sizeof(int), c->stream));
+  HIPCHK(c, hipMemsetAsync(c->lgate_work, 0, 2 * sizeof(int), c->stream));
   c->learned_ready = true;
   return PIVX_OK;
 }
@@ -2100,9 +2103,20 @@ int learned_enqueue(pivx_ctx* c, const pivx_cnn* net, const pivx_filter_params* 
   m.eps = kEpsilon;
   m.accepted = G1.accepted;
   m.scratch = G1.med_scratch;
-  if ((rc = check_launch(c, launch_median(m, c->stream), "network field median test"))) return rc;
   const int max_out = (int)std::floor((double)max_frac * nt1);
-  if ((rc = check_launch(c, launch_gate(G1.accepted, nt1, max_out, c->lgate, c->lcount, c->stream), "fallback gate")))
+  // the median pass sets the gate itself where it can (its last block to
+  // finish counts the rejected vectors), else k_gate counts them
+  const bool gate_in_median = median_gate_supported(m.wx, m.wy);
+  if (gate_in_median) {
+    m.gate_out = c->lgate;
+    m.gate_count = c->lcount;
+    m.gate_max = max_out;
+    m.gate_acc = c->lgate_work;
+    m.gate_ticket = reinterpret_cast<unsigned int*>(c->lgate_work + 1);
+  }
+  if ((rc = check_launch(c, launch_median(m, c->stream), "network field median test"))) return rc;
+  if (!gate_in_median &&
+      (rc = check_launch(c, launch_gate(G1.accepted, nt1, max_out, c->lgate, c->lcount, c->stream), "fallback gate")))
     return rc;
   // plain grid 1, run only when the gate says so; with a fraction of 1 or
   // more the gate can never open and the fallback is not enqueued at all

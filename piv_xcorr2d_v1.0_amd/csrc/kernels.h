@@ -73,6 +73,10 @@ struct MedArgs {
   int* reset_count;        // if set: the launch writes 0 here (the hole counter, ahead of a later listing pass)
   int fp64;                // normalised residuals in FP64 (precision form)
   const int* gate;         // if set: the launch does nothing unless *gate != 0
+  // optional outlier gate (k_median2 shapes): the pass counts the vectors it
+  // leaves invalid into *gate_count and sets *gate_out = (count > gate_max);
+  // gate_acc and gate_ticket are work counters, 0 between launches
+  int* gate_out; int* gate_count; int gate_max; int* gate_acc; unsigned int* gate_ticket;
 };
 
 struct FillArgs {
@@ -124,6 +128,7 @@ struct FilterSeq {
   int wxm, wym;            // median window of the FS_MEDIAN steps
   const int* gate;         // if set: the launch does nothing unless *gate != 0
 };
+static_assert(sizeof(FilterSeq) <= 4096, "kernel arguments are limited to 4 KB");
 
 // MHE contrast enhancement of one u8 frame (MHE7_4_gpu.cu).  The central
 // tiles_x*ws by tiles_y*ws region (origin xs, ys) is split into ws x ws tiles;
@@ -195,6 +200,8 @@ hipError_t launch_prep(const float* cx, const float* cy, const uint8_t* cv, int 
 hipError_t launch_zero_ints(int* p, int n, hipStream_t s, const int* gate = nullptr);
 // One workgroup runs the steps of q (median windows 3x3, 5x5 or 7x7 only).
 bool filter_seq_supported(int wxm, int wym);
+// Whether launch_median honours MedArgs::gate_out (the k_median2 shapes).
+bool median_gate_supported(int wxm, int wym);
 hipError_t launch_filter_seq(const FilterSeq& q, hipStream_t s);
 hipError_t launch_stream_copy(const void* src, void* dst, size_t bytes, hipStream_t s);
 hipError_t launch_mhe(const MheArgs& a, hipStream_t s);

@@ -2907,7 +2907,7 @@ __device__ __forceinline__ float median_low(const float (&v)[MedianNet<M>::NS], 
 // Lane g of the pass: vector g / 2, component g % 2 (both lanes of a vector
 // sit in one wave, as neighbours).
 template <int WXM, int WYM>
-__device__ __forceinline__ void median2_lane(const MedArgs& a, int g) {
+__device__ __forceinline__ bool median2_lane(const MedArgs& a, int g) {
 #pragma clang fp contract(off)
   constexpr int M = WXM * WYM - 1;
   constexpr int NS = MedianNet<M>::NS;
@@ -2946,7 +2946,7 @@ __device__ __forceinline__ void median2_lane(const MedArgs& a, int g) {
     dev = median_low<M>(v, n);
   }
   const float omed = __shfl_xor(med, 1), odev = __shfl_xor(dev, 1);
-  if (!live) return;
+  if (!live) return false;
   const float mx = c ? omed : med, my = c ? med : omed;
   const float rx = c ? odev : dev, ry = c ? dev : odev;
   float ox = 0.f, oy = 0.f;
@@ -2983,6 +2983,7 @@ __device__ __forceinline__ void median2_lane(const MedArgs& a, int g) {
     a.dy[t] = oy;
     if (a.dy2) a.dy2[t] = oy;
   }
+  return c == 0 && !okc;  // this lane's vector is left invalid
 }
 
 template <int WXM, int WYM>
@@ -2990,7 +2991,22 @@ __global__ void __launch_bounds__(256) k_median2(MedArgs a) {
   if (a.gate && *a.gate == 0) return;  // learned pipeline: grid 1 runs only as the fallback
   if (a.reset_count && blockIdx.x == 0 && threadIdx.x == 0) *a.reset_count = 0;
   // neighbouring blocks read mostly the same rows: keep them on one XCD's L2
-  median2_lane<WXM, WYM>(a, xcd_remap(blockIdx.x, gridDim.x) * 256 + (int)threadIdx.x);
+  const bool rejected = median2_lane<WXM, WYM>(a, xcd_remap(blockIdx.x, gridDim.x) * 256 + (int)threadIdx.x);
+  if (a.gate_out) {
+    // outlier gate: block counts into *gate_acc; the last block to finish
+    // (ticket) reads the total, sets the gate and clears both counters
+    const int n = __syncthreads_count(rejected);
+    if (threadIdx.x == 0) {
+      atomicAdd(a.gate_acc, n);
+      __threadfence();
+      if (atomicAdd(a.gate_ticket, 1u) == gridDim.x - 1) {
+        const int total = atomicExch(a.gate_acc, 0);
+        *a.gate_count = total;
+        *a.gate_out = total > a.gate_max ? 1 : 0;
+        atomicExch(a.gate_ticket, 0u);
+      }
+    }
+  }
 }
 
 // Windows larger than the register path: per-vector lists in global scratch,
@@ -3701,6 +3717,8 @@ hipError_t launch_prep(const float* cx, const float* cy, const uint8_t* cv, int 
   hipLaunchKernelGGL(k_prep, dim3(grid_for(nt, 256)), dim3(256), 0, s, p, gate);
   return hipGetLastError();
 }
+
+bool median_gate_supported(int wxm, int wym) { return filter_seq_supported(wxm, wym); }
 
 bool filter_seq_supported(int wxm, int wym) {
   return (wxm == 5 && wym == 5) || (wxm == 3 && wym == 3) || (wxm == 7 && wym == 7);

@@ -214,9 +214,16 @@ def test_fallback_runs_plain_grid1(pivx, orc):
                     for a, b in zip(A, B):
                         np.testing.assert_array_equal(a, b)
             x.run_pair_learned(net, fp, fp, max_outlier_fraction=1.0)
-            _, fb = x.learned_stats()
+            outl, fb = x.learned_stats()
             assert fb == 0
             assert not np.array_equal(x.filtered(1)[0], want[1][0])  # the network field was kept
+            # the gate's count (set by the median launch's last block) equals the
+            # vectors the FP32 median test rejects in the network field
+            fx, fy, fv = x.filtered(1)
+            _, _, dv, _ = orc.median_pass((fx, fy, fv), (fx, fy, fv), 1, (64, 32), (p.wx1_median, p.wy1_median),
+                                          fp.min_nneigh, fp.insert_thr, prec=32)
+            _report("gate count", graph=graph, gpu=outl, oracle=int((dv == 0).sum()))
+            assert outl == int((dv == 0).sum())
 
 
 def _net_input(img):
