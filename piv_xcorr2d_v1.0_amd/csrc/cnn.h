@@ -55,6 +55,17 @@ struct ConvFwdArgs {
   const uint8_t* f2; long long p2;
 };
 
+// Layers 0..2 of one pair (batch 1) in one launch (k_conv_head): from the
+// network input plane, or formed from the frames, to the layer-2 plane.
+struct ConvHeadArgs {
+  const uint16_t* in;        // network input, bf16 [8 ph][8 pw][2], or null with f1 set
+  const uint8_t* f1; long long p1; const uint8_t* f2; long long p2;  // the frames (optional)
+  int ph, pw;                // layer-2 plane (input / 8)
+  const uint16_t* w0; const uint16_t* w1; const uint16_t* w2;  // packed weights
+  const float* b0; const float* b1; const float* b2;           // biases
+  uint16_t* out;             // layer-2 output, bf16 NHWC [ph][pw][64]
+};
+
 // Layers 3..5 of one pair (batch 1) in one launch (k_conv_tail): the two
 // 64 -> 64 stride-1 layers and the 64 -> 2 field layer.
 struct ConvTailArgs {
@@ -91,6 +102,7 @@ int cnn_param_count();
 int cnn_weight_offset(int layer);  // master layout per layer: W [cout][9][cin], then b [cout]
 
 hipError_t launch_conv_fwd(int layer, const ConvFwdArgs& a, hipStream_t s);
+hipError_t launch_conv_head(const ConvHeadArgs& a, hipStream_t s);
 hipError_t launch_conv_tail(const ConvTailArgs& a, hipStream_t s);
 hipError_t launch_conv_bwd_data(int layer, const ConvBwdDataArgs& a, hipStream_t s);
 hipError_t launch_conv_bwd_w(int layer, const ConvBwdWArgs& a, int nchunks, hipStream_t s);

@@ -293,130 +293,237 @@ __global__ void __launch_bounds__(256) k_conv_fwd(ConvFwdArgs a) {
 }
 
 
-// ---------------------------------------------------------------- forward, fused tail
-// The three stride-1 layers at the field resolution (64 -> 64, 64 -> 64,
-// 64 -> 2) of one pair in one launch.  A workgroup computes a TT x TT tile of
-// the field: it stages the layer-2 output on the tile and a 3-cell halo in
-// LDS, computes the first hidden layer on the tile plus 2 cells, the second
-// on the tile plus 1 (cells outside the field are set to 0, the zero padding
-// of the next layer), then the field itself.  Wave w owns output channels
-// 16w .. 16w + 15 of a hidden layer and holds that channel block's weights
-// in registers for the whole layer; the K steps of every product run in the
-// same four ranges, summed in the same order, as in k_conv_fwd with KS = 4,
+// ---------------------------------------------------------------- forward, fused layers
+// For one pair (batch 1) the network runs as two launches: k_conv_head
+// (layers 0-2, from the frames or the input plane) and k_conv_tail (layers
+// 3-5, to the field).  A workgroup computes a tile of its last layer and
+// recomputes the layers before it on the tile plus the halo each 3x3 layer
+// needs (cells outside a layer's plane are set to 0, the zero padding of the
+// next layer), with every intermediate region in LDS.  Each product runs its
+// K steps in the ranges k_conv_fwd splits them into (KS = 4 for the deep
+// layers), summed in the same order, and rounds to bf16 at the same points,
 // so the field is bit-identical to the layer-by-layer forward pass.
-constexpr int kTailTile = 8;
-constexpr int kTailC = 64;              // channels of the hidden layers
-constexpr int kTailPix = kTailC + 8;    // LDS elements per cell (16 bytes of padding)
-constexpr int kTailKSteps = 9 * kTailC / 32;
-constexpr int kTailKPer = (kTailKSteps + 3) / 4;  // K steps per range, as KS = 4 splits them
 
+// K steps of a 3x3 layer with CIN input channels and k_conv_fwd's split of
+// them over KS waves (KPER steps each, the last range possibly empty).
+template <int CIN>
+struct KPlan {
+  static constexpr int steps = cnn_kpad(CIN) / 32;
+  static constexpr int ks = steps >= 8 ? 4 : 1;
+  static constexpr int per = (steps + ks - 1) / ks;
+};
 
-// One 16-channel x 16-cell product of a 3x3 layer from region src (side RS,
-// cells of kTailPix elements) to cell indices p0 .. p0 + 15 of a region of
-// side RD = RS - 2; wreg: this lane's weight slices (row = 16 m + col).
-template <int RS>
-__device__ __forceinline__ f32x4 tail_product(const uint16_t* src, const bf16x8 (&wreg)[kTailKSteps], int p0,
-                                              int col, int kg) {
-  constexpr int RD = RS - 2;
+// LDS elements per cell of a region of C channels (16 bytes of padding).
+template <int C>
+constexpr int region_pix() { return C + 8; }
+
+// This lane's weight slices of output channel `row` (all K steps).
+template <int CIN>
+__device__ __forceinline__ void region_weights(const uint16_t* w, int row, int kg,
+                                               bf16x8 (&wreg)[KPlan<CIN>::steps]) {
+#pragma unroll
+  for (int st = 0; st < KPlan<CIN>::steps; ++st)
+    wreg[st] = ld8(w + (long long)row * cnn_kpad(CIN) + st * 32 + 8 * kg);
+}
+
+// One 16-channel x 16-cell product of a 3x3 layer (CIN a multiple of 8) from
+// LDS region src (side RS) to cells p0 .. p0 + 15 of a region of side RD at
+// stride S: cell (r, c), tap (ky, kx) reads src cell (S r + ky, S c + kx).
+template <int CIN, int RS, int RD, int S>
+__device__ __forceinline__ f32x4 region_product(const uint16_t* src, const bf16x8 (&wreg)[KPlan<CIN>::steps], int p0,
+                                                int col, int kg) {
+  using P = KPlan<CIN>;
+  constexpr int PIX = region_pix<CIN>();
   int p = p0 + col;
   p = p < RD * RD ? p : RD * RD - 1;  // cells past the region: any valid cell, not stored
-  const int py = p / RD, px = p - (p / RD) * RD;
-  f32x4 part[4];
+  const int r = p / RD, c = p - (p / RD) * RD;
+  f32x4 part[P::ks];
 #pragma unroll
-  for (int q = 0; q < 4; ++q) part[q] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int q = 0; q < P::ks; ++q) part[q] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-  for (int j = 0; j < kTailKPer; ++j)
+  for (int j = 0; j < P::per; ++j)
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int st = q * kTailKPer + j;
-      if (st < kTailKSteps) {
+    for (int q = 0; q < P::ks; ++q) {
+      const int st = q * P::per + j;
+      if (st < P::steps) {
         const int k = st * 32 + 8 * kg;
-        const int tap = k / kTailC, ci = k - tap * kTailC;
+        const int tap = k / CIN, ci = k - tap * CIN;
         const int ky = tap / 3, kx = tap - 3 * ky;
         const bf16x8 xb = __builtin_bit_cast(
-            bf16x8, *reinterpret_cast<const u32x4*>(src + ((py + ky) * RS + (px + kx)) * kTailPix + ci));
+            bf16x8, *reinterpret_cast<const u32x4*>(src + ((S * r + ky) * RS + (S * c + kx)) * PIX + ci));
         part[q] = mfma(wreg[st], xb, part[q]);
       }
     }
   f32x4 acc = part[0];
 #pragma unroll
-  for (int q = 1; q < 4; ++q) acc += part[q];
+  for (int q = 1; q < P::ks; ++q) acc += part[q];
   return acc;
 }
 
-__device__ __forceinline__ void tail_weights(const uint16_t* w, int row, int kg, bf16x8 (&wreg)[kTailKSteps]) {
+// Bias, leaky ReLU and bf16 of the four channels co0 .. co0 + 3 of region
+// cell p (side RD, origin (y0, x0) in a plane of h x w cells; 0 outside it).
+template <int COUT, int RD>
+__device__ __forceinline__ void region_store(uint16_t* dst, const f32x4& acc, const float (&bb)[4], int p, int co0,
+                                             int y0, int x0, int h, int w) {
+  if (p >= RD * RD) return;
+  const int r = p / RD, c = p - (p / RD) * RD;
+  const bool inside = (unsigned)(y0 + r) < (unsigned)h && (unsigned)(x0 + c) < (unsigned)w;
+  uint16_t hv[4];
 #pragma unroll
-  for (int st = 0; st < kTailKSteps; ++st) wreg[st] = ld8(w + (long long)row * (9 * kTailC) + st * 32 + 8 * kg);
+  for (int j = 0; j < 4; ++j) {
+    float v = acc[j] + bb[j];
+    v = v > 0.f ? v : v * kCnnLeak;
+    hv[j] = inside ? f2bf(v) : (uint16_t)0;
+  }
+  uint2 u;
+  u.x = hv[0] | ((unsigned int)hv[1] << 16);
+  u.y = hv[2] | ((unsigned int)hv[3] << 16);
+  *reinterpret_cast<uint2*>(dst + p * region_pix<COUT>() + co0) = u;
 }
 
-// Hidden layer from src (side RS) into dst (side RS - 2, origin (y0, x0) in
-// field cells): bias, leaky ReLU, bf16; cells outside the field hold 0.
-template <int RS>
-__device__ __forceinline__ void tail_hidden(const uint16_t* src, uint16_t* dst, const uint16_t* w, const float* bias,
-                                            int y0, int x0, int h, int wd, int wave, int col, int kg) {
-  constexpr int RD = RS - 2;
-  bf16x8 wreg[kTailKSteps];
-  tail_weights(w, wave * 16 + col, kg, wreg);
+// Hidden layer CIN -> 64 from region src (side RS) into region dst (side
+// RD, stride S, origin (y0, x0) in its plane of h x w cells): wave w computes
+// output channels 16 w .. 16 w + 15, with their weights held in registers.
+template <int CIN, int RS, int RD, int S>
+__device__ __forceinline__ void region_hidden64(const uint16_t* src, uint16_t* dst, const uint16_t* w,
+                                                const float* bias, int y0, int x0, int h, int wd, int wave, int col,
+                                                int kg) {
+  bf16x8 wreg[KPlan<CIN>::steps];
+  region_weights<CIN>(w, wave * 16 + col, kg, wreg);
   const int co0 = wave * 16 + 4 * kg;
   float bb[4];
 #pragma unroll
   for (int j = 0; j < 4; ++j) bb[j] = bias[co0 + j];
   for (int p0 = 0; p0 < RD * RD; p0 += 16) {
-    const f32x4 acc = tail_product<RS>(src, wreg, p0, col, kg);
-    const int p = p0 + col;
-    if (p < RD * RD) {
-      const int py = p / RD, px = p - (p / RD) * RD;
-      const bool inside = (unsigned)(y0 + py) < (unsigned)h && (unsigned)(x0 + px) < (unsigned)wd;
-      uint16_t hv[4];
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        float v = acc[j] + bb[j];
-        v = v > 0.f ? v : v * kCnnLeak;
-        hv[j] = inside ? f2bf(v) : (uint16_t)0;
-      }
-      uint2 u;
-      u.x = hv[0] | ((unsigned int)hv[1] << 16);
-      u.y = hv[2] | ((unsigned int)hv[3] << 16);
-      *reinterpret_cast<uint2*>(dst + p * kTailPix + co0) = u;
-    }
+    const f32x4 acc = region_product<CIN, RS, RD, S>(src, wreg, p0, col, kg);
+    region_store<64, RD>(dst, acc, bb, p0 + col, co0, y0, x0, h, wd);
   }
 }
 
+constexpr int kTailTile = 8;  // field cells per tile side
+constexpr int kHeadTile = 4;  // layer-2 cells per tile side
+
 __global__ void __launch_bounds__(256) k_conv_tail(ConvTailArgs a) {
   constexpr int T = kTailTile, RI = T + 6, RA = T + 4, RB = T + 2;
+  constexpr int PIX = region_pix<64>();
   // region I (layer-2 output, side RI) and later region B (side RB) share
   // the first buffer; region A (side RA) has its own
-  __shared__ __attribute__((aligned(16))) uint16_t bufI[RI * RI * kTailPix];
-  __shared__ __attribute__((aligned(16))) uint16_t bufA[RA * RA * kTailPix];
+  __shared__ __attribute__((aligned(16))) uint16_t bufI[RI * RI * PIX];
+  __shared__ __attribute__((aligned(16))) uint16_t bufA[RA * RA * PIX];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int col = lane & 15, kg = lane >> 4;
   const int tiles_x = (a.w + T - 1) / T;
   const int ty = blockIdx.x / tiles_x, tx = blockIdx.x - (blockIdx.x / tiles_x) * tiles_x;
   const int oy0 = ty * T, ox0 = tx * T;
   // stage region I: cells (oy0 - 3 + r, ox0 - 3 + c), 8 channels per thread and step
-  for (int i = threadIdx.x; i < RI * RI * (kTailC / 8); i += 256) {
-    const int cell = i / (kTailC / 8), c8 = i - cell * (kTailC / 8);
+  for (int i = threadIdx.x; i < RI * RI * 8; i += 256) {
+    const int cell = i >> 3, c8 = i & 7;
     const int r = cell / RI, c = cell - (cell / RI) * RI;
     const int y = oy0 - 3 + r, x = ox0 - 3 + c;
     u32x4 v = u32x4{0u, 0u, 0u, 0u};
     if ((unsigned)y < (unsigned)a.h && (unsigned)x < (unsigned)a.w)
-      v = *reinterpret_cast<const u32x4*>(a.in + ((long long)y * a.w + x) * kTailC + c8 * 8);
-    *reinterpret_cast<u32x4*>(bufI + cell * kTailPix + c8 * 8) = v;
+      v = *reinterpret_cast<const u32x4*>(a.in + ((long long)y * a.w + x) * 64 + c8 * 8);
+    *reinterpret_cast<u32x4*>(bufI + cell * PIX + c8 * 8) = v;
   }
   __syncthreads();
-  tail_hidden<RI>(bufI, bufA, a.w3, a.b3, oy0 - 2, ox0 - 2, a.h, a.w, wave, col, kg);
+  region_hidden64<64, RI, RA, 1>(bufI, bufA, a.w3, a.b3, oy0 - 2, ox0 - 2, a.h, a.w, wave, col, kg);
   __syncthreads();  // region I is read: its buffer takes region B
-  tail_hidden<RA>(bufA, bufI, a.w4, a.b4, oy0 - 1, ox0 - 1, a.h, a.w, wave, col, kg);
+  region_hidden64<64, RA, RB, 1>(bufA, bufI, a.w4, a.b4, oy0 - 1, ox0 - 1, a.h, a.w, wave, col, kg);
   __syncthreads();
   // the field: one 16-cell product per wave (T * T = 64 cells)
-  bf16x8 wreg[kTailKSteps];
-  tail_weights(a.w5, col, kg, wreg);
-  const f32x4 acc = tail_product<RB>(bufI, wreg, wave * 16, col, kg);
+  bf16x8 wreg[KPlan<64>::steps];
+  region_weights<64>(a.w5, col, kg, wreg);
+  const f32x4 acc = region_product<64, RB, T, 1>(bufI, wreg, wave * 16, col, kg);
   const int p = wave * 16 + col;
   const int oy = oy0 + p / T, ox = ox0 + (p - (p / T) * T);
   if (kg == 0 && oy < a.h && ox < a.w)
     store_field(a.out_x, a.out_y, a.out_v, a.ext_x, a.ext_y, a.ext_b, a.h, a.w, 0, oy, ox, acc[0] + a.b5[0],
                 acc[1] + a.b5[1]);
+}
+
+// Layers 0-2 of one pair: a kHeadTile^2 tile of the layer-2 plane (ph x pw
+// cells), layer 1 on 9 x 9 cells, layer 0 on 19 x 19, from the network input
+// on 39 x 39 cells (read from the plane `in`, or formed from the frames).
+__global__ void __launch_bounds__(256) k_conv_head(ConvHeadArgs a) {
+  constexpr int T = kHeadTile, R1 = 2 * T + 1, R0 = 2 * R1 + 1, RX = 2 * R0 + 1;
+  constexpr int PIX0 = region_pix<32>(), PIX1 = region_pix<64>();
+  __shared__ __attribute__((aligned(16))) uint16_t bufX[RX * RX * 2];  // input cells, 2 channels
+  __shared__ __attribute__((aligned(16))) uint16_t buf0[R0 * R0 * PIX0];
+  __shared__ __attribute__((aligned(16))) uint16_t buf1[R1 * R1 * PIX1];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int col = lane & 15, kg = lane >> 4;
+  const int h0 = 2 * a.ph * 2, w0 = 2 * a.pw * 2;  // layer-0 plane (input / 2)
+  const int h1 = 2 * a.ph, w1 = 2 * a.pw;          // layer-1 plane
+  const int hx = 2 * h0, wx = 2 * w0;              // network input plane
+  const int tiles_x = (a.pw + T - 1) / T;
+  const int ty = blockIdx.x / tiles_x, tx = blockIdx.x - (blockIdx.x / tiles_x) * tiles_x;
+  const int oy0 = ty * T, ox0 = tx * T;                  // layer-2 origin
+  const int y1 = 2 * oy0 - 1, x1 = 2 * ox0 - 1;          // layer-1 region origin
+  const int y0 = 2 * y1 - 1, x0 = 2 * x1 - 1;            // layer-0 region origin
+  const int yx = 2 * y0 - 1, xx = 2 * x0 - 1;            // input region origin
+  // stage the input region (0 outside the plane)
+  for (int cell = threadIdx.x; cell < RX * RX; cell += 256) {
+    const int r = cell / RX, c = cell - (cell / RX) * RX;
+    const int y = yx + r, x = xx + c;
+    unsigned int v = 0u;
+    if ((unsigned)y < (unsigned)hx && (unsigned)x < (unsigned)wx) {
+      if (a.f1) v = down_f1(a.f1, a.p1, y, x) | ((unsigned int)down_f2(a.f2, a.p2, y, x) << 16);
+      else v = *reinterpret_cast<const unsigned int*>(a.in + ((long long)y * wx + x) * 2);
+    }
+    *reinterpret_cast<unsigned int*>(bufX + cell * 2) = v;
+  }
+  __syncthreads();
+  // layer 0 (2 -> 32, stride 2, one K step): tasks (channel block, cell block)
+  {
+    constexpr int NB = (R0 * R0 + 15) / 16;
+    for (int t = wave; t < 2 * NB; t += 4) {
+      const int m = t / NB, nb = t - (t / NB) * NB;
+      const bf16x8 wa = ld8(a.w0 + (long long)(m * 16 + col) * cnn_kpad(2) + 8 * kg);
+      int p = nb * 16 + col;
+      p = p < R0 * R0 ? p : R0 * R0 - 1;
+      const int r = p / R0, c = p - (p / R0) * R0;
+      uint16_t v[8];
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        const int k = 8 * kg + i;
+        uint16_t xv = 0;
+        if (k < 18) {
+          const int tap = k >> 1, ci = k & 1;
+          const int ky = tap / 3, kx = tap - 3 * ky;
+          xv = bufX[((2 * r + ky) * RX + (2 * c + kx)) * 2 + ci];
+        }
+        v[i] = xv;
+      }
+      const f32x4 acc = mfma(wa, pack8(v), f32x4{0.f, 0.f, 0.f, 0.f});
+      const int co0 = m * 16 + 4 * kg;
+      float bb[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) bb[j] = a.b0[co0 + j];
+      region_store<32, R0>(buf0, acc, bb, nb * 16 + col, co0, y0, x0, h0, w0);
+    }
+  }
+  __syncthreads();
+  region_hidden64<32, R0, R1, 2>(buf0, buf1, a.w1, a.b1, y1, x1, h1, w1, wave, col, kg);
+  __syncthreads();
+  // layer 2 (64 -> 64, stride 2): one 16-cell product per wave, to the plane
+  bf16x8 wreg[KPlan<64>::steps];
+  region_weights<64>(a.w2, wave * 16 + col, kg, wreg);
+  const f32x4 acc = region_product<64, R1, T, 2>(buf1, wreg, 0, col, kg);
+  const int co0 = wave * 16 + 4 * kg;
+  const int oy = oy0 + col / T, ox = ox0 + (col - (col / T) * T);
+  if (oy < a.ph && ox < a.pw) {
+    uint16_t hv[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      float v = acc[j] + a.b2[co0 + j];
+      v = v > 0.f ? v : v * kCnnLeak;
+      hv[j] = f2bf(v);
+    }
+    uint2 u;
+    u.x = hv[0] | ((unsigned int)hv[1] << 16);
+    u.y = hv[2] | ((unsigned int)hv[3] << 16);
+    *reinterpret_cast<uint2*>(a.out + ((long long)oy * a.pw + ox) * 64 + co0) = u;
+  }
 }
 
 // ---------------------------------------------------------------- backward, data
@@ -744,6 +851,15 @@ hipError_t launch_conv_fwd(int layer, const ConvFwdArgs& a, hipStream_t s) {
     case 5: return fwd<64, 2, 1, false, true>(a, s);
     default: return hipErrorInvalidValue;
   }
+}
+
+hipError_t launch_conv_head(const ConvHeadArgs& a, hipStream_t s) {
+  if (a.ph < 1 || a.pw < 1 || (!a.f1 && !a.in) || (a.f1 && (!a.f2 || (a.p1 & 3) != 0 ||
+                                                          (reinterpret_cast<uintptr_t>(a.f1) & 3) != 0)))
+    return hipErrorInvalidValue;
+  const int tiles = ((a.ph + kHeadTile - 1) / kHeadTile) * ((a.pw + kHeadTile - 1) / kHeadTile);
+  hipLaunchKernelGGL(k_conv_head, dim3(tiles), dim3(256), 0, s, a);
+  return hipGetLastError();
 }
 
 hipError_t launch_conv_tail(const ConvTailArgs& a, hipStream_t s) {

@@ -1857,29 +1857,68 @@ struct FrameInput {
 int cnn_forward_enqueue(const pivx_cnn* net, const uint16_t* x0, long long xstride, const int* bidx, int B, int H,
                         int W, uint16_t* const* act, float* out_x, float* out_y, long long ostride, hipStream_t s,
                         uint8_t* out_v = nullptr, float* ext_x = nullptr, float* ext_y = nullptr, int ext_b = 0,
-                        const FrameInput* frames = nullptr) {
-  // one pair: layers 3..5 run as one launch (k_conv_tail, bit-identical)
-  const int nlayers = B == 1 ? 3 : kCnnLayers;
-  static_assert(kCnnLayers == 6 && kCnnArch[3].cin == 64 && kCnnArch[3].cout == 64 && kCnnArch[4].cout == 64 &&
-                    kCnnArch[5].cout == 2 && kCnnArch[3].stride == 1 && kCnnArch[4].stride == 1 &&
-                    kCnnArch[5].stride == 1,
-                "k_conv_tail computes layers 3..5 of this architecture");
-  for (int l = 0; l < nlayers; ++l) {
+                        const FrameInput* frames = nullptr, bool inference = false) {
+  auto bias = [&](int l) { return net->master + cnn_weight_offset(l) + kCnnArch[l].cout * 9 * kCnnArch[l].cin; };
+  if (inference && B == 1 && !bidx) {
+    // one pair, no backward pass to follow: two launches, layers 0-2
+    // (k_conv_head, from the input plane or the frames) and layers 3-5
+    // (k_conv_tail), bit-identical to the layer-by-layer pass below, which
+    // also keeps every layer's activations for the backward pass
+    static_assert(kCnnLayers == 6 && kCnnArch[0].cin == 2 && kCnnArch[0].cout == 32 && kCnnArch[0].stride == 2 &&
+                      kCnnArch[1].cout == 64 && kCnnArch[1].stride == 2 && kCnnArch[2].cout == 64 &&
+                      kCnnArch[2].stride == 2 && kCnnArch[3].cout == 64 && kCnnArch[3].stride == 1 &&
+                      kCnnArch[4].cout == 64 && kCnnArch[4].stride == 1 && kCnnArch[5].cout == 2 &&
+                      kCnnArch[5].stride == 1,
+                  "k_conv_head and k_conv_tail compute the layers of this architecture");
+    ConvHeadArgs hd{};
+    int hin, win;
+    layer_dims(H, W, 2, hin, win, hd.ph, hd.pw);
+    if (frames) {
+      hd.f1 = frames->f1;
+      hd.p1 = frames->p1;
+      hd.f2 = frames->f2;
+      hd.p2 = frames->p2;
+    } else {
+      hd.in = x0;
+    }
+    hd.w0 = net->wf[0];
+    hd.w1 = net->wf[1];
+    hd.w2 = net->wf[2];
+    hd.b0 = bias(0);
+    hd.b1 = bias(1);
+    hd.b2 = bias(2);
+    hd.out = act[2];
+    hipError_t e = launch_conv_head(hd, s);
+    if (e != hipSuccess) return cnn_fail("network layers 0-2: %s", hipGetErrorString(e));
+    ConvTailArgs t{};
+    layer_dims(H, W, 3, hin, win, t.h, t.w);
+    t.in = act[2];
+    t.w3 = net->wf[3];
+    t.w4 = net->wf[4];
+    t.w5 = net->wf[5];
+    t.b3 = bias(3);
+    t.b4 = bias(4);
+    t.b5 = bias(5);
+    t.out_x = out_x;
+    t.out_y = out_y;
+    t.out_v = out_v;
+    t.ext_x = ext_x;
+    t.ext_y = ext_y;
+    t.ext_b = ext_b;
+    e = launch_conv_tail(t, s);
+    if (e != hipSuccess) return cnn_fail("network layers 3-5: %s", hipGetErrorString(e));
+    return PIVX_OK;
+  }
+  for (int l = 0; l < kCnnLayers; ++l) {
     const CnnLayer& L = kCnnArch[l];
     ConvFwdArgs a{};
     layer_dims(H, W, l, a.hin, a.win, a.hout, a.wout);
     a.in = l == 0 ? x0 : act[l - 1];
-    if (l == 0 && frames) {  // the first layer forms its input from the frames
-      a.f1 = frames->f1;
-      a.p1 = frames->p1;
-      a.f2 = frames->f2;
-      a.p2 = frames->p2;
-    }
     a.in_bstride = l == 0 ? xstride : (long long)a.hin * a.win * L.cin;
     a.batch_index = l == 0 ? bidx : nullptr;
     a.batch = B;
     a.w = net->wf[l];
-    a.bias = net->master + cnn_weight_offset(l) + L.cout * 9 * L.cin;
+    a.bias = bias(l);
     if (l < kCnnLayers - 1) {
       a.out = act[l];
     } else {
@@ -1893,26 +1932,6 @@ int cnn_forward_enqueue(const pivx_cnn* net, const uint16_t* x0, long long xstri
     }
     const hipError_t e = launch_conv_fwd(l, a, s);
     if (e != hipSuccess) return cnn_fail("network layer %d: %s", l, hipGetErrorString(e));
-  }
-  if (nlayers < kCnnLayers) {
-    ConvTailArgs t{};
-    int hin, win;
-    layer_dims(H, W, 3, hin, win, t.h, t.w);
-    t.in = act[2];
-    t.w3 = net->wf[3];
-    t.w4 = net->wf[4];
-    t.w5 = net->wf[5];
-    t.b3 = net->master + cnn_weight_offset(3) + kCnnArch[3].cout * 9 * kCnnArch[3].cin;
-    t.b4 = net->master + cnn_weight_offset(4) + kCnnArch[4].cout * 9 * kCnnArch[4].cin;
-    t.b5 = net->master + cnn_weight_offset(5) + kCnnArch[5].cout * 9 * kCnnArch[5].cin;
-    t.out_x = out_x;
-    t.out_y = out_y;
-    t.out_v = out_v;
-    t.ext_x = ext_x;
-    t.ext_y = ext_y;
-    t.ext_b = ext_b;
-    const hipError_t e = launch_conv_tail(t, s);
-    if (e != hipSuccess) return cnn_fail("network layers 3-5: %s", hipGetErrorString(e));
   }
   return PIVX_OK;
 }
@@ -2074,7 +2093,7 @@ int learned_enqueue(pivx_ctx* c, const pivx_cnn* net, const pivx_filter_params* 
   // field's flags and writes its extension (the fallback rewrites both)
   const FrameInput fin{c->f1, c->p.nx, c->f2, c->pitch2};
   if ((rc = cnn_forward_enqueue(net, nullptr, 0, nullptr, 1, H, W, c->cact, G1.fx, G1.fy, 0, c->stream, G1.fv, G1.ex,
-                                G1.ey, kExBorder, &fin))) {
+                                G1.ey, kExBorder, &fin, true))) {
     c->err = global_error();
     return rc;
   }
@@ -2297,7 +2316,7 @@ int pivx_cnn_forward(pivx_cnn* net, const float* x0, int batch, int h, int w, fl
   if (rc) return rc;
   const int hw = (h / 8) * (w / 8);
   if ((rc = cnn_forward_enqueue(net, net->xin, (long long)h * w * 2, nullptr, batch, h, w, net->act, net->out,
-                                net->out + hw, 2LL * hw, net->stream)))
+                                net->out + hw, 2LL * hw, net->stream, nullptr, nullptr, nullptr, 0, nullptr, true)))
     return rc;
   if (hipMemcpyAsync(out, net->out, (size_t)batch * 2 * hw * 4, hipMemcpyDeviceToHost, net->stream) != hipSuccess ||
       hipStreamSynchronize(net->stream) != hipSuccess)
