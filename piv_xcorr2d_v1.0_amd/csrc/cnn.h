@@ -48,11 +48,6 @@ struct ConvFwdArgs {
   float* ext_x;            // last layer, batch 1 (optional): the field's extension, border ext_b replicated
   float* ext_y;
   int ext_b;
-  // first layer, batch 1 (optional): the network input is formed on the fly
-  // from the frames (4x4 block means, normalised, as k_downsample writes it)
-  // instead of being read from `in`
-  const uint8_t* f1; long long p1;
-  const uint8_t* f2; long long p2;
 };
 
 // Layers 0..2 of one pair (batch 1) in one launch (k_conv_head): from the

@@ -152,28 +152,6 @@ __device__ __forceinline__ uint16_t down_f2(const uint8_t* f2, long long p2, int
   return f2bf(((float)s2 * inv - kCnnMu) * kCnnInvSigma);
 }
 
-// Eight K values (k = tap * 2 + frame) of the first layer's input patch of
-// output pixel (oy, ox), formed from the frames (the values of patch8 over
-// the k_downsample plane).
-template <int S>
-__device__ __forceinline__ bf16x8 patch8_frames(const ConvFwdArgs& a, int oy, int ox, int k) {
-  uint16_t v[8];
-#pragma unroll
-  for (int i = 0; i < 8; ++i) {
-    const int kk = k + i;
-    uint16_t x = 0;
-    if (kk < 18) {
-      const int tap = kk >> 1, ci = kk & 1;
-      const int ky = tap / 3, kx = tap - 3 * ky;
-      const int iy = oy * S + ky - 1, ix = ox * S + kx - 1;
-      if ((unsigned)iy < (unsigned)a.hin && (unsigned)ix < (unsigned)a.win)
-        x = ci == 0 ? down_f1(a.f1, a.p1, iy, ix) : down_f2(a.f2, a.p2, iy, ix);
-    }
-    v[i] = x;
-  }
-  return pack8(v);
-}
-
 // Field vector (vx, vy) of output cell (oy, ox) of the last layer: the FP32
 // planes at base + oy * wout + ox, the validity flag (optional) and the
 // border-replicated extension of the field (optional, the extend stage).
@@ -205,7 +183,7 @@ __device__ __forceinline__ void store_field(float* out_x, float* out_y, uint8_t*
 // deep layers); with KS = 1 each wave owns a tile.  The K loop is unrolled
 // by three so that the loads of later steps are in flight while the matrix
 // cores work.
-template <int CIN, int COUT, int S, bool ACT, bool FINAL, int PT, int MT, int KS, bool FRAMES = false>
+template <int CIN, int COUT, int S, bool ACT, bool FINAL, int PT, int MT, int KS>
 __global__ void __launch_bounds__(256) k_conv_fwd(ConvFwdArgs a) {
   constexpr int KP = cnn_kpad(CIN);
   constexpr int NCB = cnn_pad16(COUT) / 16 / MT;  // channel blocks per pixel group
@@ -239,10 +217,7 @@ __global__ void __launch_bounds__(256) k_conv_fwd(ConvFwdArgs a) {
 #pragma unroll
     for (int mt = 0; mt < MT; ++mt) wa[mt] = ld8(wbase + (long long)(mt * 16 + col) * KP + k);
 #pragma unroll
-    for (int pt = 0; pt < PT; ++pt) {
-      if constexpr (FRAMES) xb[pt] = patch8_frames<S>(a, oy, ox0 + pt * 16 + col, k);
-      else xb[pt] = patch8<CIN, S>(in, a.hin, a.win, oy, ox0 + pt * 16 + col, k);
-    }
+    for (int pt = 0; pt < PT; ++pt) xb[pt] = patch8<CIN, S>(in, a.hin, a.win, oy, ox0 + pt * 16 + col, k);
 #pragma unroll
     for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
@@ -461,14 +436,20 @@ __global__ void __launch_bounds__(256) k_conv_head(ConvHeadArgs a) {
   const int y1 = 2 * oy0 - 1, x1 = 2 * ox0 - 1;          // layer-1 region origin
   const int y0 = 2 * y1 - 1, x0 = 2 * x1 - 1;            // layer-0 region origin
   const int yx = 2 * y0 - 1, xx = 2 * x0 - 1;            // input region origin
+  const bool f2_aligned = ((a.p2 & 3) | (reinterpret_cast<uintptr_t>(a.f2) & 3)) == 0;
   // stage the input region (0 outside the plane)
   for (int cell = threadIdx.x; cell < RX * RX; cell += 256) {
     const int r = cell / RX, c = cell - (cell / RX) * RX;
     const int y = yx + r, x = xx + c;
     unsigned int v = 0u;
     if ((unsigned)y < (unsigned)hx && (unsigned)x < (unsigned)wx) {
-      if (a.f1) v = down_f1(a.f1, a.p1, y, x) | ((unsigned int)down_f2(a.f2, a.p2, y, x) << 16);
-      else v = *reinterpret_cast<const unsigned int*>(a.in + ((long long)y * wx + x) * 2);
+      if (a.f1) {
+        // frame 2 rows 4-byte aligned (the engine's bordered frame): 4-byte loads as for frame 1
+        const uint16_t b = f2_aligned ? down_f1(a.f2, a.p2, y, x) : down_f2(a.f2, a.p2, y, x);
+        v = down_f1(a.f1, a.p1, y, x) | ((unsigned int)b << 16);
+      } else {
+        v = *reinterpret_cast<const unsigned int*>(a.in + ((long long)y * wx + x) * 2);
+      }
     }
     *reinterpret_cast<unsigned int*>(bufX + cell * 2) = v;
   }
@@ -784,20 +765,6 @@ inline int blocks(long long n, int b) { return (int)((n + b - 1) / b); }
 template <int CIN, int COUT, int S, bool ACT, bool FINAL>
 hipError_t fwd(const ConvFwdArgs& a, hipStream_t s) {
   constexpr int PT = 1;
-  if constexpr (CIN == 2) {
-    if (a.f1) {
-      // input from the frames: both 16-channel blocks of a pixel group in
-      // one wave, so that each input patch is formed once
-      if (a.batch != 1 || !a.f2 || (a.p1 & 3) != 0 || (reinterpret_cast<uintptr_t>(a.f1) & 3) != 0 ||
-          a.wout % 16 != 0)
-        return hipErrorInvalidValue;
-      constexpr int MT2 = cnn_pad16(COUT) / 16;
-      const long long tiles = (long long)a.hout * (a.wout / 16);
-      hipLaunchKernelGGL((k_conv_fwd<CIN, COUT, S, ACT, FINAL, 1, MT2, 1, true>), dim3((unsigned)blocks(tiles, 4)),
-                         dim3(256), 0, s, a);
-      return hipGetLastError();
-    }
-  }
   constexpr int MT = 1;  // one 16-channel tile per wave: the small layers need the waves
   constexpr int NCB = cnn_pad16(COUT) / 16 / MT;
   constexpr int KS = cnn_kpad(CIN) / 32 >= 8 ? 4 : 1;  // split the K steps of the deep layers over 4 waves

@@ -346,7 +346,9 @@ XcArgs make_xcargs(pivx_ctx* c, int gi, bool deformed, bool zero_pred, bool fuse
     a.fdef.hfws_y = g.ws_y / 2;
     a.fdef.nx_aug = g.nx_aug;
     a.fdef.ny_aug = g.ny_aug;
-    a.fdef.oob = c->counters + gi;
+    // with overlapping windows (the learned pipeline's fallback) a pixel is
+    // deformed once per window that holds it: its taps are not counted
+    a.fdef.oob = G.fuse ? c->counters + gi : nullptr;
   }
   a.pitch1 = c->p.nx;
   a.pitch2 = c->pitch2;
@@ -712,9 +714,10 @@ int alloc_grid(pivx_ctx* c, int gi) {
   if ((long long)wxm * wym > 64) {
     if (dalloc(c, &G.med_scratch, nt * 2 * (size_t)wxm * wym)) return PIVX_ERR_HIP;
   }
-  // fused deformation: each aug pixel in at most one window
+  // fused deformation: each aug pixel in at most one window (the learned
+  // pipeline's gated fallback fuses it with overlapping windows as well)
   G.fuse = xcorr_fuses_deform(g.ws_x, g.ws_y) && g.spacing_x >= g.ws_x && g.spacing_y >= g.ws_y;
-  if (G.fuse) {
+  if (xcorr_fuses_deform(g.ws_x, g.ws_y)) {
     if (dalloc(c, &G.dcol, (size_t)g.nx_aug) || dalloc(c, &G.drow, (size_t)g.ny_aug)) return PIVX_ERR_HIP;
     const int rc2 = check_launch(c, launch_deform_terms(def_args(c, gi), G.dcol, G.drow, c->stream), "deformation terms");
     if (rc2) return rc2;
@@ -1006,8 +1009,12 @@ int pivx_calc_cc_map_p1(pivx_ctx* c, int idx_pass) {
   int rc;
   if (idx_pass == 0) return run_xcorr(c, 0, false, true);
   // the deformation of non-overlapping windows runs in the correlation
-  // kernel (within pivx_run_pair by default, pivx_set_fused_deform)
-  if (c->grid[0].fuse && c->prec == 0 && (c->fuse_mode == 2 || (c->fuse_mode == 1 && c->in_pair)))
+  // kernel (within pivx_run_pair by default, pivx_set_fused_deform); so does
+  // that of the learned pipeline's gated fallback, whatever the overlap: it
+  // saves the fallback a launch per pass that mostly does nothing
+  const GridBuf& G1 = c->grid[0];
+  if (c->prec == 0 && ((G1.fuse && (c->fuse_mode == 2 || (c->fuse_mode == 1 && c->in_pair))) ||
+                       (c->gate && c->in_pair && xcorr_fuses_deform(G1.g.ws_x, G1.g.ws_y))))
     return run_xcorr(c, 0, true, false, true);
   rc = run_deform(c, 0);
   if (rc) return rc;

@@ -15,7 +15,7 @@ struct FusedDef {
   const float4* row;      // per aug row: (ay, 1 - ay, iyf as int bits, (float)idy)
   const float* ex_x; const float* ex_y; int nxe;  // extended field of the grid
   int hfws_x, hfws_y, nx_aug, ny_aug;             // for the out-of-buffer tap count
-  unsigned long long* oob;                        // cumulative out-of-buffer tap count
+  unsigned long long* oob;                        // cumulative out-of-buffer tap count (null: not counted)
 };
 
 // Fused correlate-and-peak pass over all windows of one grid.

@@ -12,8 +12,9 @@
 //        window per wave, up to 64x64), k_xcorr_fast<WX,WY,WPB> (16x16), and
 //        k_xcorr_generic (any legal size, LDS or global planes).
 //   Kernel_Window_Deformation_2d cu:608-652           -> k_deform, or inside
-//     k_xcorr_single (deform_window) for windows that do not overlap, with
-//     the term tables of k_deform_terms
+//     k_xcorr_single (deform_window) for windows that do not overlap (and in
+//     the learned pipeline's gated grid-1 fallback), with the term tables of
+//     k_deform_terms
 //   extend_filtered_vecfield_p1/p2 cu:1547-1618       -> k_extend
 //   Upsampling_vecfield_kernel_2d cu:675-704          -> k_upsample
 //   median_filter_data_prep, organize_median_filter_data, median_validity,
@@ -830,7 +831,7 @@ __device__ __forceinline__ void deform_window(const A& a, int ax0, int ay0, floa
       });
     }
   }
-  if (oob) atomicAdd(d.oob, (unsigned long long)oob);
+  if (oob && d.oob) atomicAdd(d.oob, (unsigned long long)oob);
 }
 
 // LDS of one wave-pair work item: one complex plane (= both windows' real
