@@ -94,7 +94,8 @@ struct pivx_ctx {
   // of the network field, network input and hidden activations (batch 1)
   int* lgate = nullptr;
   int* lcount = nullptr;
-  int* lgate_work = nullptr;  // [2]: count accumulator and block ticket of the gated median, 0 between launches
+  int* lgate_work = nullptr;  // [3]: count accumulator and block ticket of the gated median, ticket of the
+                              // fallback's correlation-and-filter launches; 0 between launches
   bool learned_ready = false;  // lgate, lcount and cact are allocated
   uint16_t* cact[5] = {};
   hipGraph_t lgraph = nullptr;
@@ -426,9 +427,17 @@ struct FilterJob {
 // and *ext_done is set.  With (mx, my, mv) set, the field after the median
 // stage (before filling and smoothing) is also written there.  c may be null
 // (no stage timing; errors go to the context-free error text).
+// Whether a gated filter sequence runs as one FilterSeq (k_filter_seq).
+bool filter_fits_seq(int npass_median, int npass_smooth, int wxm, int wym) {
+  const int nsteps = (npass_median > 0 ? 1 + npass_median : 2) + 2 + std::max(0, npass_smooth);
+  return (npass_median <= 0 || filter_seq_supported(wxm, wym)) && nsteps <= kFilterSeqSteps;
+}
+
+// With defer set (a gated sequence that fits one FilterSeq), the sequence is
+// returned in *defer instead of launched.
 int enqueue_filter(pivx_ctx* c, int gi, const FilterJob& J, int npass_median, int min_nneigh, float insert_thr,
                    int npass_smooth, int iffill, float* ox, float* oy, uint8_t* ov, float* ext_x, float* ext_y,
-                   bool* ext_done, float* mx, float* my, uint8_t* mv) {
+                   bool* ext_done, float* mx, float* my, uint8_t* mv, FilterSeq* defer = nullptr) {
   const int nt = J.nx_vec * J.ny_vec;
   const int K = J.K;
   const int wxm = J.wxm, wym = J.wym, wxs = J.wxs, wys = J.wys;
@@ -444,9 +453,8 @@ int enqueue_filter(pivx_ctx* c, int gi, const FilterJob& J, int npass_median, in
   // every launch of it costs its stream a dispatch: it runs the whole sequence
   // as one launch of k_filter_seq (one workgroup, the same per-vector code)
   // when the median window has the register path and the steps fit.
-  const int nsteps = (npass_median > 0 ? 1 + npass_median : 2) + 2 + std::max(0, npass_smooth);
-  const bool seq = J.gate != nullptr && (npass_median <= 0 || filter_seq_supported(wxm, wym)) &&
-                   nsteps <= kFilterSeqSteps;
+  const bool seq = J.gate != nullptr && filter_fits_seq(npass_median, npass_smooth, wxm, wym);
+  if (defer && !seq) return fail(c, PIVX_ERR_STATE, "filter sequence cannot be deferred");
   FilterSeq q{};
   q.wxm = wxm;
   q.wym = wym;
@@ -603,7 +611,9 @@ int enqueue_filter(pivx_ctx* c, int gi, const FilterJob& J, int npass_median, in
     }
     if (!seq) toc(c, gi, 4);
   }
-  if (seq) {
+  if (seq && defer) {
+    *defer = q;
+  } else if (seq) {
     rc = check_launch(c, launch_filter_seq(q, stream), "filter sequence");
     toc(c, gi, 2);
     if (rc) return rc;
@@ -613,7 +623,8 @@ int enqueue_filter(pivx_ctx* c, int gi, const FilterJob& J, int npass_median, in
 
 int run_filter(pivx_ctx* c, int gi, int npass_median, int min_nneigh, float insert_thr, int npass_smooth, int iffill,
                float* ox, float* oy, uint8_t* ov, float* ext_x = nullptr, float* ext_y = nullptr,
-               bool* ext_done = nullptr, float* mx = nullptr, float* my = nullptr, uint8_t* mv = nullptr) {
+               bool* ext_done = nullptr, float* mx = nullptr, float* my = nullptr, uint8_t* mv = nullptr,
+               FilterSeq* defer = nullptr) {
   GridBuf& G = c->grid[gi];
   FilterJob J{};
   J.cx = G.dx;
@@ -647,7 +658,7 @@ int run_filter(pivx_ctx* c, int gi, int npass_median, int min_nneigh, float inse
   J.gate = c->gate;
   J.fp64 = c->prec != 0 ? 1 : 0;
   return enqueue_filter(c, gi, J, npass_median, min_nneigh, insert_thr, npass_smooth, iffill, ox, oy, ov, ext_x, ext_y,
-                        ext_done, mx, my, mv);
+                        ext_done, mx, my, mv, defer);
 }
 
 int extend(pivx_ctx* c, int gi) {
@@ -1149,9 +1160,29 @@ struct InPair {
 static int enqueue_grid1(pivx_ctx* c, const pivx_filter_params* f1, bool* ext_ok) {
   int rc;
   GridBuf& G1 = c->grid[0];
+  const pivx_geometry& g1 = G1.g;
   *ext_ok = false;
+  // The learned pipeline's gated fallback: each pass is one launch, the
+  // correlation whose last workgroup runs the pass's filter sequence (the
+  // deformation inside the correlation), where the shapes allow it.
+  const bool one_launch = c->gate && c->in_pair && c->prec == 0 && c->lgate_work &&
+                          xcorr_gated_filter_supported(g1.ws_x, g1.ws_y, c->p.wx1_median, c->p.wy1_median) &&
+                          xcorr_fuses_deform(g1.ws_x, g1.ws_y) &&
+                          filter_fits_seq(f1->npass_median, f1->npass_smooth, c->p.wx1_median, c->p.wy1_median);
   for (int j = 0; j < c->p.npass1; ++j) {
     if (j > 0 && !*ext_ok && (rc = extend(c, 0))) return rc;
+    if (one_launch) {
+      FilterSeq q{};
+      if ((rc = run_filter(c, 0, f1->npass_median, f1->min_nneigh, f1->insert_thr, f1->npass_smooth, 1, G1.fx, G1.fy,
+                           G1.fv, G1.ex, G1.ey, ext_ok, nullptr, nullptr, nullptr, &q)))
+        return rc;
+      const XcArgs a = make_xcargs(c, 0, j > 0, j == 0, j > 0);
+      rc = check_launch(c, launch_xcorr_gated_filter(a, g1.ws_x, g1.ws_y, q,
+                                                     reinterpret_cast<unsigned int*>(c->lgate_work + 2), c->stream),
+                        "gated correlation and filter");
+      if (rc) return rc;
+      continue;
+    }
     if ((rc = pivx_calc_cc_map_p1(c, j))) return rc;
     if ((rc = run_filter(c, 0, f1->npass_median, f1->min_nneigh, f1->insert_thr, f1->npass_smooth, 1, G1.fx, G1.fy,
                          G1.fv, G1.ex, G1.ey, ext_ok)))
@@ -2073,7 +2104,7 @@ int learned_alloc(pivx_ctx* c) {
   int rc = 0;
   if (!c->lgate) rc |= dalloc(c, &c->lgate, 1);
   if (!c->lcount) rc |= dalloc(c, &c->lcount, 1);
-  if (!c->lgate_work) rc |= dalloc(c, &c->lgate_work, 2);
+  if (!c->lgate_work) rc |= dalloc(c, &c->lgate_work, 3);
   for (int l = 0; l < kCnnLayers - 1; ++l) {
     int hin, win, hout, wout;
     layer_dims(H, W, l, hin, win, hout, wout);
@@ -2082,7 +2113,7 @@ int learned_alloc(pivx_ctx* c) {
   if (rc) return PIVX_ERR_HIP;
   HIPCHK(c, hipMemsetAsync(c->lgate, 0, sizeof(int), c->stream));
   HIPCHK(c, hipMemsetAsync(c->lcount, 0, sizeof(int), c->stream));
-  HIPCHK(c, hipMemsetAsync(c->lgate_work, 0, 2 * sizeof(int), c->stream));
+  HIPCHK(c, hipMemsetAsync(c->lgate_work, 0, 3 * sizeof(int), c->stream));
   c->learned_ready = true;
   return PIVX_OK;
 }

@@ -200,6 +200,12 @@ hipError_t launch_prep(const float* cx, const float* cy, const uint8_t* cv, int 
 hipError_t launch_zero_ints(int* p, int n, hipStream_t s, const int* gate = nullptr);
 // One workgroup runs the steps of q (median windows 3x3, 5x5 or 7x7 only).
 bool filter_seq_supported(int wxm, int wym);
+// A gated correlation pass whose last workgroup then runs the filter
+// sequence q (the learned pipeline's fallback, 64x64 windows); ticket: one
+// counter, 0 between launches.
+bool xcorr_gated_filter_supported(int ws_x, int ws_y, int wxm, int wym);
+hipError_t launch_xcorr_gated_filter(const XcArgs& a, int ws_x, int ws_y, const FilterSeq& q, unsigned int* ticket,
+                                     hipStream_t s);
 // Whether launch_median honours MedArgs::gate_out (the k_median2 shapes).
 bool median_gate_supported(int wxm, int wym);
 hipError_t launch_filter_seq(const FilterSeq& q, hipStream_t s);

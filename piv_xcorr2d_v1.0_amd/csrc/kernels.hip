@@ -3233,9 +3233,7 @@ __global__ void k_prep(PrepArgs p, const int* gate) {
 // whole workgroup before the next step reads them.
 constexpr int kFilterSeqThreads = 1024;
 template <int WXM, int WYM>
-__global__ void __launch_bounds__(kFilterSeqThreads) k_filter_seq(FilterSeq q) {
-  if (q.gate && *q.gate == 0) return;  // learned pipeline: grid 1 runs only as the fallback
-  __shared__ FillShared sh;
+__device__ void filter_seq_run(const FilterSeq& q, FillShared& sh, int nthreads) {
   const int tid = (int)threadIdx.x;
   for (int i = 0; i < q.nsteps; ++i) {
     const int kind = q.step[i].kind;
@@ -3252,16 +3250,52 @@ __global__ void __launch_bounds__(kFilterSeqThreads) k_filter_seq(FilterSeq q) {
       const int lanes = 2 * m.nx_vec * m.ny_vec;
       // every lane of a wave takes part in each round (the two lanes of a
       // vector exchange their medians); lanes past the end write nothing
-      for (int g0 = 0; g0 < lanes; g0 += kFilterSeqThreads) median2_lane<WXM, WYM>(m, g0 + tid);
+      for (int g0 = 0; g0 < lanes; g0 += nthreads) median2_lane<WXM, WYM>(m, g0 + tid);
     } else if (kind == FS_FILL_LIST) {
-      fill_list_range(q.step[i].fill, tid, kFilterSeqThreads);
+      fill_list_range(q.step[i].fill, tid, nthreads);
     } else if (kind == FS_FILL) {
       fill_sweeps(q.step[i].fill, sh);
     } else if (kind == FS_SMOOTH) {
-      smooth_range(q.step[i].smooth, tid, kFilterSeqThreads);
+      smooth_range(q.step[i].smooth, tid, nthreads);
     }
     __syncthreads();
   }
+}
+
+template <int WXM, int WYM>
+__global__ void __launch_bounds__(kFilterSeqThreads) k_filter_seq(FilterSeq q) {
+  if (q.gate && *q.gate == 0) return;  // learned pipeline: grid 1 runs only as the fallback
+  __shared__ FillShared sh;
+  filter_seq_run<WXM, WYM>(q, sh, kFilterSeqThreads);
+}
+
+// The learned pipeline's gated grid-1 pass in one launch: the two-wave 64x64
+// correlation walking the windows (k_xcorr_single64x2_gated), then, in the
+// workgroup that finishes last, the pass's filter sequence.  Every thread
+// releases its results before the workgroup takes its ticket, and the last
+// workgroup acquires before it reads the others'; it clears the ticket for
+// the next launch (all tickets are taken by then).  With the gate closed no
+// workgroup touches the ticket.
+template <int WXM, int WYM>
+__global__ void __launch_bounds__(128) k_xcorr_single64x2_gated_filter(XcArgs a, FilterSeq q, unsigned int* ticket) {
+  if (*a.gate == 0) return;
+  __shared__ __align__(16) float lds[kW2LdsFloats];
+  __shared__ float2 pnyq[64];
+  __shared__ FillShared sh;
+  __shared__ int last;
+  const int nt = a.nx_vec * a.ny_vec;
+  for (int w = blockIdx.x; w < nt; w += gridDim.x) {
+    if (w != (int)blockIdx.x) __syncthreads();  // the last window's peak stage is done
+    xcorr_single64x2_body(a, w, lds, pnyq);
+  }
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) last = atomicAdd(ticket, 1u) == gridDim.x - 1 ? 1 : 0;
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  if (threadIdx.x == 0) atomicExch(ticket, 0u);
+  filter_seq_run<WXM, WYM>(q, sh, 128);
 }
 __global__ void k_zero_ints(int* p, int n, const int* gate) {
   if (gate && *gate == 0) return;
@@ -3720,6 +3754,22 @@ hipError_t launch_prep(const float* cx, const float* cy, const uint8_t* cv, int 
 }
 
 bool median_gate_supported(int wxm, int wym) { return filter_seq_supported(wxm, wym); }
+
+bool xcorr_gated_filter_supported(int ws_x, int ws_y, int wxm, int wym) {
+  return ws_x == 64 && ws_y == 64 && kSingle64TwoWaves && filter_seq_supported(wxm, wym);
+}
+
+hipError_t launch_xcorr_gated_filter(const XcArgs& a, int ws_x, int ws_y, const FilterSeq& q, unsigned int* ticket,
+                                     hipStream_t s) {
+  if (!a.gate || !ticket || !xcorr_gated_filter_supported(ws_x, ws_y, q.wxm, q.wym) || q.nsteps < 0 ||
+      q.nsteps > kFilterSeqSteps)
+    return hipErrorInvalidValue;
+  const dim3 g(std::min(a.nx_vec * a.ny_vec, kGatedWindows)), b(128);
+  if (q.wxm == 5) hipLaunchKernelGGL((k_xcorr_single64x2_gated_filter<5, 5>), g, b, 0, s, a, q, ticket);
+  else if (q.wxm == 3) hipLaunchKernelGGL((k_xcorr_single64x2_gated_filter<3, 3>), g, b, 0, s, a, q, ticket);
+  else hipLaunchKernelGGL((k_xcorr_single64x2_gated_filter<7, 7>), g, b, 0, s, a, q, ticket);
+  return hipGetLastError();
+}
 
 bool filter_seq_supported(int wxm, int wym) {
   return (wxm == 5 && wym == 5) || (wxm == 3 && wym == 3) || (wxm == 7 && wym == 7);
