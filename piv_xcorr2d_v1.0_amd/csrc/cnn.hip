@@ -356,20 +356,27 @@ __device__ __forceinline__ void region_store(uint16_t* dst, const f32x4& acc, co
   *reinterpret_cast<uint2*>(dst + p * region_pix<COUT>() + co0) = u;
 }
 
+// Workgroups of the fused network kernels: eight waves, two per SIMD (a
+// handful of workgroups per pair, so the waves of one workgroup are what
+// hides the LDS and matrix-core latency).
+constexpr int kNetThreads = 512;
+
 // Hidden layer CIN -> 64 from region src (side RS) into region dst (side
 // RD, stride S, origin (y0, x0) in its plane of h x w cells): wave w computes
-// output channels 16 w .. 16 w + 15, with their weights held in registers.
+// output channels 16 (w mod 4) .. + 15, with their weights held in registers,
+// for every other block of 16 cells (w / 4 picks which).
 template <int CIN, int RS, int RD, int S>
 __device__ __forceinline__ void region_hidden64(const uint16_t* src, uint16_t* dst, const uint16_t* w,
                                                 const float* bias, int y0, int x0, int h, int wd, int wave, int col,
                                                 int kg) {
+  const int m = wave & 3;
   bf16x8 wreg[KPlan<CIN>::steps];
-  region_weights<CIN>(w, wave * 16 + col, kg, wreg);
-  const int co0 = wave * 16 + 4 * kg;
+  region_weights<CIN>(w, m * 16 + col, kg, wreg);
+  const int co0 = m * 16 + 4 * kg;
   float bb[4];
 #pragma unroll
   for (int j = 0; j < 4; ++j) bb[j] = bias[co0 + j];
-  for (int p0 = 0; p0 < RD * RD; p0 += 16) {
+  for (int p0 = 16 * (wave >> 2); p0 < RD * RD; p0 += 32) {
     const f32x4 acc = region_product<CIN, RS, RD, S>(src, wreg, p0, col, kg);
     region_store<64, RD>(dst, acc, bb, p0 + col, co0, y0, x0, h, wd);
   }
@@ -378,7 +385,7 @@ __device__ __forceinline__ void region_hidden64(const uint16_t* src, uint16_t* d
 constexpr int kTailTile = 8;  // field cells per tile side
 constexpr int kHeadTile = 4;  // layer-2 cells per tile side
 
-__global__ void __launch_bounds__(256) k_conv_tail(ConvTailArgs a) {
+__global__ void __launch_bounds__(kNetThreads) k_conv_tail(ConvTailArgs a) {
   constexpr int T = kTailTile, RI = T + 6, RA = T + 4, RB = T + 2;
   constexpr int PIX = region_pix<64>();
   // region I (layer-2 output, side RI) and later region B (side RB) share
@@ -390,22 +397,34 @@ __global__ void __launch_bounds__(256) k_conv_tail(ConvTailArgs a) {
   const int tiles_x = (a.w + T - 1) / T;
   const int ty = blockIdx.x / tiles_x, tx = blockIdx.x - (blockIdx.x / tiles_x) * tiles_x;
   const int oy0 = ty * T, ox0 = tx * T;
-  // stage region I: cells (oy0 - 3 + r, ox0 - 3 + c), 8 channels per thread and step
-  for (int i = threadIdx.x; i < RI * RI * 8; i += 256) {
-    const int cell = i >> 3, c8 = i & 7;
-    const int r = cell / RI, c = cell - (cell / RI) * RI;
-    const int y = oy0 - 3 + r, x = ox0 - 3 + c;
-    u32x4 v = u32x4{0u, 0u, 0u, 0u};
-    if ((unsigned)y < (unsigned)a.h && (unsigned)x < (unsigned)a.w)
-      v = *reinterpret_cast<const u32x4*>(a.in + ((long long)y * a.w + x) * 64 + c8 * 8);
-    *reinterpret_cast<u32x4*>(bufI + cell * PIX + c8 * 8) = v;
+  // stage region I: cells (oy0 - 3 + r, ox0 - 3 + c), 8 channels per thread
+  // and step; all of a thread's loads are issued before its first LDS store
+  {
+    constexpr int N = (RI * RI * 8 + kNetThreads - 1) / kNetThreads;
+    u32x4 v[N];
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      const int i = (int)threadIdx.x + kNetThreads * j;
+      const int cell = i >> 3, c8 = i & 7;
+      const int r = cell / RI, c = cell - (cell / RI) * RI;
+      const int y = oy0 - 3 + r, x = ox0 - 3 + c;
+      v[j] = u32x4{0u, 0u, 0u, 0u};
+      if (i < RI * RI * 8 && (unsigned)y < (unsigned)a.h && (unsigned)x < (unsigned)a.w)
+        v[j] = *reinterpret_cast<const u32x4*>(a.in + ((long long)y * a.w + x) * 64 + c8 * 8);
+    }
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      const int i = (int)threadIdx.x + kNetThreads * j;
+      if (i < RI * RI * 8) *reinterpret_cast<u32x4*>(bufI + (i >> 3) * PIX + (i & 7) * 8) = v[j];
+    }
   }
   __syncthreads();
   region_hidden64<64, RI, RA, 1>(bufI, bufA, a.w3, a.b3, oy0 - 2, ox0 - 2, a.h, a.w, wave, col, kg);
   __syncthreads();  // region I is read: its buffer takes region B
   region_hidden64<64, RA, RB, 1>(bufA, bufI, a.w4, a.b4, oy0 - 1, ox0 - 1, a.h, a.w, wave, col, kg);
   __syncthreads();
-  // the field: one 16-cell product per wave (T * T = 64 cells)
+  // the field: one 16-cell product for each of waves 0-3 (T * T = 64 cells)
+  if (wave >= 4) return;
   bf16x8 wreg[KPlan<64>::steps];
   region_weights<64>(a.w5, col, kg, wreg);
   const f32x4 acc = region_product<64, RB, T, 1>(bufI, wreg, wave * 16, col, kg);
@@ -419,7 +438,7 @@ __global__ void __launch_bounds__(256) k_conv_tail(ConvTailArgs a) {
 // Layers 0-2 of one pair: a kHeadTile^2 tile of the layer-2 plane (ph x pw
 // cells), layer 1 on 9 x 9 cells, layer 0 on 19 x 19, from the network input
 // on 39 x 39 cells (read from the plane `in`, or formed from the frames).
-__global__ void __launch_bounds__(256) k_conv_head(ConvHeadArgs a) {
+__global__ void __launch_bounds__(kNetThreads) k_conv_head(ConvHeadArgs a) {
   constexpr int T = kHeadTile, R1 = 2 * T + 1, R0 = 2 * R1 + 1, RX = 2 * R0 + 1;
   constexpr int PIX0 = region_pix<32>(), PIX1 = region_pix<64>();
   __shared__ __attribute__((aligned(16))) uint16_t bufX[RX * RX * 2];  // input cells, 2 channels
@@ -437,29 +456,49 @@ __global__ void __launch_bounds__(256) k_conv_head(ConvHeadArgs a) {
   const int y0 = 2 * y1 - 1, x0 = 2 * x1 - 1;            // layer-0 region origin
   const int yx = 2 * y0 - 1, xx = 2 * x0 - 1;            // input region origin
   const bool f2_aligned = ((a.p2 & 3) | (reinterpret_cast<uintptr_t>(a.f2) & 3)) == 0;
-  // stage the input region (0 outside the plane)
-  for (int cell = threadIdx.x; cell < RX * RX; cell += 256) {
-    const int r = cell / RX, c = cell - (cell / RX) * RX;
-    const int y = yx + r, x = xx + c;
-    unsigned int v = 0u;
-    if ((unsigned)y < (unsigned)hx && (unsigned)x < (unsigned)wx) {
-      if (a.f1) {
-        // frame 2 rows 4-byte aligned (the engine's bordered frame): 4-byte loads as for frame 1
-        const uint16_t b = f2_aligned ? down_f1(a.f2, a.p2, y, x) : down_f2(a.f2, a.p2, y, x);
-        v = down_f1(a.f1, a.p1, y, x) | ((unsigned int)b << 16);
-      } else {
-        v = *reinterpret_cast<const unsigned int*>(a.in + ((long long)y * wx + x) * 2);
+  // stage the input region (0 outside the plane); all of a thread's loads
+  // are issued before its first LDS store
+  {
+    constexpr int N = (RX * RX + kNetThreads - 1) / kNetThreads;
+    unsigned int v[N];
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      const int cell = (int)threadIdx.x + kNetThreads * j;
+      const int r = cell / RX, c = cell - (cell / RX) * RX;
+      const int y = yx + r, x = xx + c;
+      v[j] = 0u;
+      if (cell < RX * RX && (unsigned)y < (unsigned)hx && (unsigned)x < (unsigned)wx) {
+        if (a.f1) {
+          // frame 2 rows 4-byte aligned (the engine's bordered frame): 4-byte loads as for frame 1
+          const uint16_t b = f2_aligned ? down_f1(a.f2, a.p2, y, x) : down_f2(a.f2, a.p2, y, x);
+          v[j] = down_f1(a.f1, a.p1, y, x) | ((unsigned int)b << 16);
+        } else {
+          v[j] = *reinterpret_cast<const unsigned int*>(a.in + ((long long)y * wx + x) * 2);
+        }
       }
     }
-    *reinterpret_cast<unsigned int*>(bufX + cell * 2) = v;
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      const int cell = (int)threadIdx.x + kNetThreads * j;
+      if (cell < RX * RX) *reinterpret_cast<unsigned int*>(bufX + cell * 2) = v[j];
+    }
   }
   __syncthreads();
   // layer 0 (2 -> 32, stride 2, one K step): tasks (channel block, cell block)
   {
     constexpr int NB = (R0 * R0 + 15) / 16;
-    for (int t = wave; t < 2 * NB; t += 4) {
+    // weights and biases of both channel blocks, loaded once
+    const bf16x8 wa0 = ld8(a.w0 + (long long)col * cnn_kpad(2) + 8 * kg);
+    const bf16x8 wa1 = ld8(a.w0 + (long long)(16 + col) * cnn_kpad(2) + 8 * kg);
+    float bb0[4], bb1[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      bb0[j] = a.b0[4 * kg + j];
+      bb1[j] = a.b0[16 + 4 * kg + j];
+    }
+    for (int t = wave; t < 2 * NB; t += kNetThreads / 64) {
       const int m = t / NB, nb = t - (t / NB) * NB;
-      const bf16x8 wa = ld8(a.w0 + (long long)(m * 16 + col) * cnn_kpad(2) + 8 * kg);
+      const bf16x8 wa = m ? wa1 : wa0;
       int p = nb * 16 + col;
       p = p < R0 * R0 ? p : R0 * R0 - 1;
       const int r = p / R0, c = p - (p / R0) * R0;
@@ -477,16 +516,15 @@ __global__ void __launch_bounds__(256) k_conv_head(ConvHeadArgs a) {
       }
       const f32x4 acc = mfma(wa, pack8(v), f32x4{0.f, 0.f, 0.f, 0.f});
       const int co0 = m * 16 + 4 * kg;
-      float bb[4];
-#pragma unroll
-      for (int j = 0; j < 4; ++j) bb[j] = a.b0[co0 + j];
-      region_store<32, R0>(buf0, acc, bb, nb * 16 + col, co0, y0, x0, h0, w0);
+      region_store<32, R0>(buf0, acc, m ? bb1 : bb0, nb * 16 + col, co0, y0, x0, h0, w0);
     }
   }
   __syncthreads();
   region_hidden64<32, R0, R1, 2>(buf0, buf1, a.w1, a.b1, y1, x1, h1, w1, wave, col, kg);
   __syncthreads();
-  // layer 2 (64 -> 64, stride 2): one 16-cell product per wave, to the plane
+  // layer 2 (64 -> 64, stride 2): one 16-cell product for each of waves 0-3
+  // (channel block = wave), to the plane
+  if (wave >= 4) return;
   bf16x8 wreg[KPlan<64>::steps];
   region_weights<64>(a.w2, wave * 16 + col, kg, wreg);
   const f32x4 acc = region_product<64, R1, T, 2>(buf1, wreg, 0, col, kg);
@@ -825,14 +863,14 @@ hipError_t launch_conv_head(const ConvHeadArgs& a, hipStream_t s) {
                                                           (reinterpret_cast<uintptr_t>(a.f1) & 3) != 0)))
     return hipErrorInvalidValue;
   const int tiles = ((a.ph + kHeadTile - 1) / kHeadTile) * ((a.pw + kHeadTile - 1) / kHeadTile);
-  hipLaunchKernelGGL(k_conv_head, dim3(tiles), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(k_conv_head, dim3(tiles), dim3(kNetThreads), 0, s, a);
   return hipGetLastError();
 }
 
 hipError_t launch_conv_tail(const ConvTailArgs& a, hipStream_t s) {
   if (a.h < 1 || a.w < 1) return hipErrorInvalidValue;
   const int tiles = ((a.h + kTailTile - 1) / kTailTile) * ((a.w + kTailTile - 1) / kTailTile);
-  hipLaunchKernelGGL(k_conv_tail, dim3(tiles), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(k_conv_tail, dim3(tiles), dim3(kNetThreads), 0, s, a);
   return hipGetLastError();
 }
 
