@@ -423,14 +423,14 @@ __global__ void __launch_bounds__(kNetThreads) k_conv_tail(ConvTailArgs a) {
   __syncthreads();  // region I is read: its buffer takes region B
   region_hidden64<64, RA, RB, 1>(bufA, bufI, a.w4, a.b4, oy0 - 1, ox0 - 1, a.h, a.w, wave, col, kg);
   __syncthreads();
-  // the field: one 16-cell product for each of waves 0-3 (T * T = 64 cells)
-  if (wave >= 4) return;
+  // the field: one 16-cell product for each of the first T * T / 16 waves
+  if (wave * 16 >= T * T) return;
   bf16x8 wreg[KPlan<64>::steps];
   region_weights<64>(a.w5, col, kg, wreg);
   const f32x4 acc = region_product<64, RB, T, 1>(bufI, wreg, wave * 16, col, kg);
   const int p = wave * 16 + col;
   const int oy = oy0 + p / T, ox = ox0 + (p - (p / T) * T);
-  if (kg == 0 && oy < a.h && ox < a.w)
+  if (kg == 0 && p < T * T && oy < a.h && ox < a.w)
     store_field(a.out_x, a.out_y, a.out_v, a.ext_x, a.ext_y, a.ext_b, a.h, a.w, 0, oy, ox, acc[0] + a.b5[0],
                 acc[1] + a.b5[1]);
 }
