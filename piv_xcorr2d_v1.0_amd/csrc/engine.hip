@@ -490,7 +490,7 @@ int enqueue_filter(pivx_ctx* c, int gi, const FilterJob& J, int npass_median, in
     add_step(FS_SMOOTH).smooth = sa;
     return hipSuccess;
   };
-  tic(c);
+  if (!defer) tic(c);  // a deferred sequence is launched by the caller, untimed
   // The hole counter must be 0 when the holes are listed (by the last median
   // pass, or by k_fill_list without median passes).  k_fill zeroes it after
   // use; the launch ahead of the listing one zeroes it as well, so that no
