@@ -32,6 +32,15 @@ constexpr float kCnnInvSigma = 1.0f / 40.0f;
 constexpr int cnn_kpad(int c) { return (9 * c + 31) / 32 * 32; }
 constexpr int cnn_pad16(int c) { return (c + 15) / 16 * 16; }
 
+// Lane-ordered weight layout of the fused inference kernels: for channel
+// block m (16 output channels) and K step st (32 of k = tap * cin + ci), the
+// 64 lanes' MFMA operands are one contiguous 1 KiB block, lane
+// kg * 16 + col holding channel 16 m + col, k = 32 st + 8 kg .. + 7.  Same
+// size as the forward layout (pad16(cout) x kpad(cin)), zero padded.
+__host__ __device__ constexpr long long cnn_lane_index(int co, int k, int cin) {
+  return (((long long)(co / 16) * (cnn_kpad(cin) / 32) + k / 32) * 64 + ((k % 32) / 8) * 16 + co % 16) * 8 + k % 8;
+}
+
 // Forward convolution of one layer over a batch.
 struct ConvFwdArgs {
   const uint16_t* in;      // bf16 NHWC [batch][hin][win][cin]
@@ -56,7 +65,7 @@ struct ConvHeadArgs {
   const uint16_t* in;        // network input, bf16 [8 ph][8 pw][2], or null with f1 set
   const uint8_t* f1; long long p1; const uint8_t* f2; long long p2;  // the frames (optional)
   int ph, pw;                // layer-2 plane (input / 8)
-  const uint16_t* w0; const uint16_t* w1; const uint16_t* w2;  // packed weights
+  const uint16_t* w0; const uint16_t* w1; const uint16_t* w2;  // lane-ordered weights (cnn_lane_index)
   const float* b0; const float* b1; const float* b2;           // biases
   uint16_t* out;             // layer-2 output, bf16 NHWC [ph][pw][64]
 };
@@ -66,7 +75,7 @@ struct ConvHeadArgs {
 struct ConvTailArgs {
   const uint16_t* in;        // layer-2 output, bf16 NHWC [h][w][64]
   int h, w;                  // field size
-  const uint16_t* w3; const uint16_t* w4; const uint16_t* w5;  // packed weights [pad16(cout)][576]
+  const uint16_t* w3; const uint16_t* w4; const uint16_t* w5;  // lane-ordered weights (cnn_lane_index)
   const float* b3; const float* b4; const float* b5;           // biases
   float* out_x; float* out_y;  // the field planes [h][w]
   uint8_t* out_v;            // optional: validity flags, set to 1
@@ -115,8 +124,10 @@ hipError_t launch_loss_final(const float* loss_part, int n, double count, float*
 // One Adam step on the FP32 master parameters (t = step number, from 1).
 hipError_t launch_adam(float* master, const float* grad, float* m, float* v, int n, float lr, float b1, float b2,
                        float eps, long long t, hipStream_t s);
-// bf16 GEMM layouts of every layer's weights from the FP32 master.
-hipError_t launch_pack(const float* master, uint16_t* const* wf, uint16_t* const* wt, hipStream_t s);
+// bf16 GEMM layouts of every layer's weights from the FP32 master: forward
+// (wf), transposed for the data gradient (wt) and lane-ordered (wl).
+hipError_t launch_pack(const float* master, uint16_t* const* wf, uint16_t* const* wt, uint16_t* const* wl,
+                       hipStream_t s);
 // Frames (u8, row pitches p1, p2) -> bf16 NHWC [ny/4][nx/4][2] network input.
 hipError_t launch_downsample(const uint8_t* f1, long long p1, const uint8_t* f2, long long p2, int nx, int ny,
                              uint16_t* out, hipStream_t s);

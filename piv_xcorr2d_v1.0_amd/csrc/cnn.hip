@@ -292,13 +292,14 @@ struct KPlan {
 template <int C>
 constexpr int region_pix() { return C + 8; }
 
-// This lane's weight slices of output channel `row` (all K steps).
+// This lane's weight slices of output channels 16 m .. 16 m + 15 (all K
+// steps) from the lane-ordered layout: one contiguous 1 KiB load per step.
 template <int CIN>
-__device__ __forceinline__ void region_weights(const uint16_t* w, int row, int kg,
+__device__ __forceinline__ void region_weights(const uint16_t* wl, int m, int lane,
                                                bf16x8 (&wreg)[KPlan<CIN>::steps]) {
+  const uint16_t* p = wl + ((long long)m * KPlan<CIN>::steps * 64 + lane) * 8;
 #pragma unroll
-  for (int st = 0; st < KPlan<CIN>::steps; ++st)
-    wreg[st] = ld8(w + (long long)row * cnn_kpad(CIN) + st * 32 + 8 * kg);
+  for (int st = 0; st < KPlan<CIN>::steps; ++st) wreg[st] = ld8(p + st * 64 * 8);
 }
 
 // One 16-channel x 16-cell product of a 3x3 layer (CIN a multiple of 8) from
@@ -371,7 +372,7 @@ __device__ __forceinline__ void region_hidden64(const uint16_t* src, uint16_t* d
                                                 int kg) {
   const int m = wave & 3;
   bf16x8 wreg[KPlan<CIN>::steps];
-  region_weights<CIN>(w, m * 16 + col, kg, wreg);
+  region_weights<CIN>(w, m, kg * 16 + col, wreg);
   const int co0 = m * 16 + 4 * kg;
   float bb[4];
 #pragma unroll
@@ -426,7 +427,7 @@ __global__ void __launch_bounds__(kNetThreads) k_conv_tail(ConvTailArgs a) {
   // the field: one 16-cell product for each of the first T * T / 16 waves
   if (wave * 16 >= T * T) return;
   bf16x8 wreg[KPlan<64>::steps];
-  region_weights<64>(a.w5, col, kg, wreg);
+  region_weights<64>(a.w5, 0, lane, wreg);
   const f32x4 acc = region_product<64, RB, T, 1>(bufI, wreg, wave * 16, col, kg);
   const int p = wave * 16 + col;
   const int oy = oy0 + p / T, ox = ox0 + (p - (p / T) * T);
@@ -488,8 +489,8 @@ __global__ void __launch_bounds__(kNetThreads) k_conv_head(ConvHeadArgs a) {
   {
     constexpr int NB = (R0 * R0 + 15) / 16;
     // weights and biases of both channel blocks, loaded once
-    const bf16x8 wa0 = ld8(a.w0 + (long long)col * cnn_kpad(2) + 8 * kg);
-    const bf16x8 wa1 = ld8(a.w0 + (long long)(16 + col) * cnn_kpad(2) + 8 * kg);
+    const bf16x8 wa0 = ld8(a.w0 + lane * 8);
+    const bf16x8 wa1 = ld8(a.w0 + (64 + lane) * 8);
     float bb0[4], bb1[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
@@ -526,7 +527,7 @@ __global__ void __launch_bounds__(kNetThreads) k_conv_head(ConvHeadArgs a) {
   // (channel block = wave), to the plane
   if (wave >= 4) return;
   bf16x8 wreg[KPlan<64>::steps];
-  region_weights<64>(a.w2, wave * 16 + col, kg, wreg);
+  region_weights<64>(a.w2, wave, lane, wreg);
   const f32x4 acc = region_product<64, R1, T, 2>(buf1, wreg, 0, col, kg);
   const int co0 = wave * 16 + 4 * kg;
   const int oy = oy0 + col / T, ox = ox0 + (col - (col / T) * T);
@@ -757,13 +758,14 @@ __global__ void k_adam(float* p, const float* g, float* m, float* v, int n, floa
   p[t] -= lr * (mt / c1) / (sqrtf(vt / c2) + eps);
 }
 
-__global__ void k_pack_layer(const float* w, int cin, int cout, uint16_t* wf, uint16_t* wt) {
+__global__ void k_pack_layer(const float* w, int cin, int cout, uint16_t* wf, uint16_t* wt, uint16_t* wl) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= cout * 9 * cin) return;
   const int co = t / (9 * cin), r = t - co * 9 * cin, tap = r / cin, ci = r - tap * cin;
   const uint16_t h = f2bf(w[t]);
   wf[(long long)co * cnn_kpad(cin) + tap * cin + ci] = h;
   wt[(long long)ci * cnn_kpad(cout) + tap * cout + co] = h;
+  wl[cnn_lane_index(co, r, cin)] = h;
 }
 
 
@@ -928,12 +930,13 @@ hipError_t launch_adam(float* master, const float* grad, float* m, float* v, int
   return hipGetLastError();
 }
 
-hipError_t launch_pack(const float* master, uint16_t* const* wf, uint16_t* const* wt, hipStream_t s) {
+hipError_t launch_pack(const float* master, uint16_t* const* wf, uint16_t* const* wt, uint16_t* const* wl,
+                       hipStream_t s) {
   for (int l = 0; l < kCnnLayers; ++l) {
     const CnnLayer& L = kCnnArch[l];
     const int n = L.cout * 9 * L.cin;
     hipLaunchKernelGGL(k_pack_layer, dim3(blocks(n, 256)), dim3(256), 0, s, master + cnn_weight_offset(l), L.cin,
-                       L.cout, wf[l], wt[l]);
+                       L.cout, wf[l], wt[l], wl[l]);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }

@@ -1827,6 +1827,7 @@ struct pivx_cnn {
   float *master = nullptr, *m = nullptr, *v = nullptr, *grad = nullptr;
   uint16_t* wf[kCnnLayers] = {};
   uint16_t* wt[kCnnLayers] = {};
+  uint16_t* wl[kCnnLayers] = {};  // lane-ordered copy for the fused inference kernels
   long long step = 0;
   // batch workspace (training and the host test entry points)
   int wb = 0, wh = 0, ww = 0;
@@ -1919,9 +1920,9 @@ int cnn_forward_enqueue(const pivx_cnn* net, const uint16_t* x0, long long xstri
     } else {
       hd.in = x0;
     }
-    hd.w0 = net->wf[0];
-    hd.w1 = net->wf[1];
-    hd.w2 = net->wf[2];
+    hd.w0 = net->wl[0];
+    hd.w1 = net->wl[1];
+    hd.w2 = net->wl[2];
     hd.b0 = bias(0);
     hd.b1 = bias(1);
     hd.b2 = bias(2);
@@ -1931,9 +1932,9 @@ int cnn_forward_enqueue(const pivx_cnn* net, const uint16_t* x0, long long xstri
     ConvTailArgs t{};
     layer_dims(H, W, 3, hin, win, t.h, t.w);
     t.in = act[2];
-    t.w3 = net->wf[3];
-    t.w4 = net->wf[4];
-    t.w5 = net->wf[5];
+    t.w3 = net->wl[3];
+    t.w4 = net->wl[4];
+    t.w5 = net->wl[5];
     t.b3 = bias(3);
     t.b4 = bias(4);
     t.b5 = bias(5);
@@ -2056,7 +2057,7 @@ int cnn_workspace(pivx_cnn* net, int B, int H, int W) {
 }
 
 int cnn_pack(pivx_cnn* net) {
-  const hipError_t e = launch_pack(net->master, net->wf, net->wt, net->stream);
+  const hipError_t e = launch_pack(net->master, net->wf, net->wt, net->wl, net->stream);
   if (e != hipSuccess) return cnn_fail("weight packing: %s", hipGetErrorString(e));
   return PIVX_OK;
 }
@@ -2075,7 +2076,7 @@ int cnn_step_enqueue(pivx_cnn* net, const uint16_t* x0, long long xstride, const
   if (lr > 0.f) {
     net->step += 1;
     e = launch_adam(net->master, net->grad, net->m, net->v, net->np, lr, 0.9f, 0.999f, 1e-8f, net->step, s);
-    if (e == hipSuccess) e = launch_pack(net->master, net->wf, net->wt, s);
+    if (e == hipSuccess) e = launch_pack(net->master, net->wf, net->wt, net->wl, s);
     if (e != hipSuccess) return cnn_fail("Adam step: %s", hipGetErrorString(e));
   }
   return PIVX_OK;
@@ -2269,8 +2270,10 @@ int pivx_cnn_create(int device_id, pivx_cnn** out) {
     const size_t nf = (size_t)cnn_pad16(L.cout) * cnn_kpad(L.cin), nt = (size_t)cnn_pad16(L.cin) * cnn_kpad(L.cout);
     rc |= nalloc(net->allocs, &net->wf[l], nf);
     rc |= nalloc(net->allocs, &net->wt[l], nt);
+    rc |= nalloc(net->allocs, &net->wl[l], nf);
     if (!rc && (hipMemsetAsync(net->wf[l], 0, nf * 2, net->stream) != hipSuccess ||
-                hipMemsetAsync(net->wt[l], 0, nt * 2, net->stream) != hipSuccess))
+                hipMemsetAsync(net->wt[l], 0, nt * 2, net->stream) != hipSuccess ||
+                hipMemsetAsync(net->wl[l], 0, nf * 2, net->stream) != hipSuccess))
       rc = PIVX_ERR_HIP;
   }
   if (!rc && (hipMemsetAsync(net->master, 0, (size_t)net->np * 4, net->stream) != hipSuccess ||
@@ -2386,7 +2389,7 @@ int pivx_cnn_adam_step(pivx_cnn* net, float lr) {
   net->step += 1;
   hipError_t e = launch_adam(net->master, net->grad, net->m, net->v, net->np, lr, 0.9f, 0.999f, 1e-8f, net->step,
                              net->stream);
-  if (e == hipSuccess) e = launch_pack(net->master, net->wf, net->wt, net->stream);
+  if (e == hipSuccess) e = launch_pack(net->master, net->wf, net->wt, net->wl, net->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(net->stream);
   if (e != hipSuccess) return cnn_fail("Adam step: %s", hipGetErrorString(e));
   return PIVX_OK;
