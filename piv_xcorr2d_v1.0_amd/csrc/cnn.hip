@@ -133,9 +133,10 @@ __device__ __forceinline__ bf16x8 gpatch8(const uint16_t* gout, const uint16_t* 
 // 4x4 block of frame 1 (rows 4-byte aligned) or frame 2 (any alignment).
 __device__ __forceinline__ uint16_t down_f1(const uint8_t* f1, long long p1, int y, int x) {
   unsigned int s1 = 0;
+  const uint8_t* q = f1 + (long long)y * kCnnDown * p1 + x * kCnnDown;
 #pragma unroll
-  for (int r = 0; r < kCnnDown; ++r) {
-    const uchar4 a = *reinterpret_cast<const uchar4*>(f1 + (long long)(y * kCnnDown + r) * p1 + x * kCnnDown);
+  for (int r = 0; r < kCnnDown; ++r, q += p1) {
+    const uchar4 a = *reinterpret_cast<const uchar4*>(q);
     s1 += a.x + a.y + a.z + a.w;
   }
   const float inv = 1.0f / (kCnnDown * kCnnDown);
@@ -143,11 +144,9 @@ __device__ __forceinline__ uint16_t down_f1(const uint8_t* f1, long long p1, int
 }
 __device__ __forceinline__ uint16_t down_f2(const uint8_t* f2, long long p2, int y, int x) {
   unsigned int s2 = 0;
+  const uint8_t* q = f2 + (long long)y * kCnnDown * p2 + x * kCnnDown;
 #pragma unroll
-  for (int r = 0; r < kCnnDown; ++r) {
-    const uint8_t* q = f2 + (long long)(y * kCnnDown + r) * p2 + x * kCnnDown;
-    s2 += q[0] + q[1] + q[2] + q[3];
-  }
+  for (int r = 0; r < kCnnDown; ++r, q += p2) s2 += q[0] + q[1] + q[2] + q[3];
   const float inv = 1.0f / (kCnnDown * kCnnDown);
   return f2bf(((float)s2 * inv - kCnnMu) * kCnnInvSigma);
 }
@@ -485,7 +484,10 @@ __global__ void __launch_bounds__(kNetThreads) k_conv_head(ConvHeadArgs a) {
     }
   }
   __syncthreads();
-  // layer 0 (2 -> 32, stride 2, one K step): tasks (channel block, cell block)
+  // layer 0 (2 -> 32, stride 2, one K step): one cell block per task, both
+  // channel blocks from one operand.  A lane's 8 K values are both channels
+  // of the input cells at taps 4 kg .. 4 kg + 3, i.e. four 32-bit words of
+  // bufX (taps past 8 are the K padding, 0).
   {
     constexpr int NB = (R0 * R0 + 15) / 16;
     // weights and biases of both channel blocks, loaded once
@@ -497,27 +499,31 @@ __global__ void __launch_bounds__(kNetThreads) k_conv_head(ConvHeadArgs a) {
       bb0[j] = a.b0[4 * kg + j];
       bb1[j] = a.b0[16 + 4 * kg + j];
     }
-    for (int t = wave; t < 2 * NB; t += kNetThreads / 64) {
-      const int m = t / NB, nb = t - (t / NB) * NB;
-      const bf16x8 wa = m ? wa1 : wa0;
+    int toff[4];
+    bool tok[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int tap = 4 * kg + j;
+      tok[j] = tap < 9;
+      toff[j] = tok[j] ? (tap / 3) * RX + tap % 3 : 0;
+    }
+    const uint32_t* const bx = reinterpret_cast<const uint32_t*>(bufX);
+    for (int nb = wave; nb < NB; nb += kNetThreads / 64) {
       int p = nb * 16 + col;
       p = p < R0 * R0 ? p : R0 * R0 - 1;
       const int r = p / R0, c = p - (p / R0) * R0;
-      uint16_t v[8];
+      const int base = 2 * r * RX + 2 * c;
+      u32x4 xw;
 #pragma unroll
-      for (int i = 0; i < 8; ++i) {
-        const int k = 8 * kg + i;
-        uint16_t xv = 0;
-        if (k < 18) {
-          const int tap = k >> 1, ci = k & 1;
-          const int ky = tap / 3, kx = tap - 3 * ky;
-          xv = bufX[((2 * r + ky) * RX + (2 * c + kx)) * 2 + ci];
-        }
-        v[i] = xv;
+      for (int j = 0; j < 4; ++j) {
+        const uint32_t v = bx[base + toff[j]];
+        xw[j] = tok[j] ? v : 0u;
       }
-      const f32x4 acc = mfma(wa, pack8(v), f32x4{0.f, 0.f, 0.f, 0.f});
-      const int co0 = m * 16 + 4 * kg;
-      region_store<32, R0>(buf0, acc, m ? bb1 : bb0, nb * 16 + col, co0, y0, x0, h0, w0);
+      const bf16x8 xb = __builtin_bit_cast(bf16x8, xw);
+      const f32x4 acc0 = mfma(wa0, xb, f32x4{0.f, 0.f, 0.f, 0.f});
+      const f32x4 acc1 = mfma(wa1, xb, f32x4{0.f, 0.f, 0.f, 0.f});
+      region_store<32, R0>(buf0, acc0, bb0, nb * 16 + col, 4 * kg, y0, x0, h0, w0);
+      region_store<32, R0>(buf0, acc1, bb1, nb * 16 + col, 16 + 4 * kg, y0, x0, h0, w0);
     }
   }
   __syncthreads();
