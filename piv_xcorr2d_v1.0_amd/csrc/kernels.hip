@@ -3327,6 +3327,8 @@ __device__ __forceinline__ int mhe_neigh_count(int tx, int ty, int mx, int my) {
 
 // MakeHist_kernel (mhe:66-83): the 256-bin histogram of the wave's tile,
 // written into the tile's slot of the bordered histogram array.
+// (16 sub-histograms of packed 16-bit counts, so that fewer lanes share one,
+// were measured slower: 25.6 against 21.9 us per 6600x4400 frame.)
 __global__ void __launch_bounds__(256) k_mhe_hist(MheArgs a) {
   __shared__ uint32_t hs[4][MHE_SUB][257];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -3339,6 +3341,7 @@ __global__ void __launch_bounds__(256) k_mhe_hist(MheArgs a) {
 #pragma unroll
     for (int j = 0; j < 4; ++j) h[q][lane + 64 * j] = 0;
   wave_sync();
+  auto bump = [&](uint32_t v) { atomicAdd(&h[sub][v], 1u); };
   const int ty = tile / a.tiles_x, tx = tile - ty * a.tiles_x;
   const uint8_t* base = a.in + (long long)(a.ys + ty * a.ws) * a.nx + a.xs + tx * a.ws;
   if ((a.ws & 3) == 0) {
@@ -3350,38 +3353,44 @@ __global__ void __launch_bounds__(256) k_mhe_hist(MheArgs a) {
       return w;
     };
     auto count = [&](uint32_t w) {
-      atomicAdd(&h[sub][w & 255u], 1u);
-      atomicAdd(&h[sub][(w >> 8) & 255u], 1u);
-      atomicAdd(&h[sub][(w >> 16) & 255u], 1u);
-      atomicAdd(&h[sub][w >> 24], 1u);
+      bump(w & 255u);
+      bump((w >> 8) & 255u);
+      bump((w >> 16) & 255u);
+      bump(w >> 24);
     };
-    int d0 = lane;
-    for (; d0 + 64 * (MHE_U - 1) < nd; d0 += 64 * MHE_U) {
+    // batches of MHE_U words per lane, every load of a batch issued before
+    // the first count (a 40x40 tile is one batch of at most 7 words)
+    for (int d0 = lane; d0 < nd; d0 += 64 * MHE_U) {
       uint32_t w[MHE_U];
 #pragma unroll
-      for (int j = 0; j < MHE_U; ++j) w[j] = word(d0 + 64 * j);
+      for (int j = 0; j < MHE_U; ++j) w[j] = d0 + 64 * j < nd ? word(d0 + 64 * j) : 0u;
 #pragma unroll
-      for (int j = 0; j < MHE_U; ++j) count(w[j]);
+      for (int j = 0; j < MHE_U; ++j)
+        if (d0 + 64 * j < nd) count(w[j]);
     }
-    for (; d0 < nd; d0 += 64) count(word(d0));
   } else {
     const int npix = a.ws * a.ws;
-    for (int p = lane; p < npix; p += 64) {
-      const int r = p / a.ws, c = p - r * a.ws;
-      atomicAdd(&h[sub][base[(long long)r * a.nx + c]], 1u);
+    for (int p0 = lane; p0 < npix; p0 += 64 * MHE_U) {
+      uint32_t v[MHE_U];
+#pragma unroll
+      for (int j = 0; j < MHE_U; ++j) {
+        const int p = p0 + 64 * j, r = p / a.ws, c = p - r * a.ws;
+        v[j] = p < npix ? base[(long long)r * a.nx + c] : 0u;
+      }
+#pragma unroll
+      for (int j = 0; j < MHE_U; ++j)
+        if (p0 + 64 * j < npix) bump(v[j]);
     }
   }
   wave_sync();
-  uint4 out;
-  uint32_t* o = reinterpret_cast<uint32_t*>(&out);
+  // lane l: levels 4l .. 4l + 3, summed over the sub-histograms
+  uint32_t o[4] = {0u, 0u, 0u, 0u};
 #pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    uint32_t sum = 0;
+  for (int q = 0; q < MHE_SUB; ++q)
 #pragma unroll
-    for (int q = 0; q < MHE_SUB; ++q) sum += h[q][4 * lane + j];
-    o[j] = sum;
-  }
-  reinterpret_cast<uint4*>(a.hist + ((long long)(ty + 1) * (a.tiles_x + 2) + tx + 1) * 256)[lane] = out;
+    for (int j = 0; j < 4; ++j) o[j] += h[q][4 * lane + j];
+  reinterpret_cast<uint4*>(a.hist + ((long long)(ty + 1) * (a.tiles_x + 2) + tx + 1) * 256)[lane] =
+      make_uint4(o[0], o[1], o[2], o[3]);
 }
 
 // CreateTransfer_kernel + prescan_kernel + Calc_Enhanced_value_kernel +

@@ -719,6 +719,24 @@ def test_mhe_bitexact(pivx, orc, nx, ny, ws, thr):
     _report(f"mhe {nx}x{ny} ws={ws} thr={thr}", pixels=nx * ny, changed=int((got != a).sum()))
 
 
+@pytest.mark.parametrize("nx,ny,ws,const", [(1000, 1000, 1000, 77), (2048, 1100, 1024, None)])
+def test_mhe_large_windows(pivx, orc, nx, ny, ws, const):
+    """Windows of 1000 and 1024 px (several load batches per lane in the
+    histogram kernel): a constant frame, whose every pixel falls on one grey
+    level, and a particle frame with two tiles; bit-exact against the
+    oracle."""
+    if const is None:
+        a, _ = pivx.synth_pair(pivx.synth_params(nx, ny, seed=5, flow=0, a0=1.0, a1=0.5))
+    else:
+        a = np.full((ny, nx), const, np.uint8)
+    p = config1_params(pivx, nx=nx, ny=ny, ws_x1=32, ws_y1=32, spacing_x1=32, spacing_y1=32, npass2=0)
+    with pivx.Xcorr2dMulti(p) as x:
+        x.set_mhe(True, ws, 0.5)
+        got = x.mhe_enhance(a)
+    np.testing.assert_array_equal(got, orc.mhe(a, ws, 0.5))
+    _report(f"mhe limits {nx}x{ny} ws={ws}", changed=int((got != a).sum()))
+
+
 @pytest.mark.parametrize("sp1", [32, 64])
 def test_mhe_run_pair_equals_enhanced_frames(pivx, orc, sp1):
     """With MHE switched on, run_pair equals run_pair on frames the oracle
