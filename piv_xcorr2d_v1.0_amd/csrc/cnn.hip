@@ -607,105 +607,161 @@ __global__ void __launch_bounds__(256) k_conv_bwd_data(ConvBwdDataArgs a) {
 // D[co][n] = Gpre[co][o] * X[o][n], n = tap * CIN + ci, K = the output pixels
 // o of one chunk (linear over [batch][hout][wout]).  Both operands are
 // gathered one pixel at a time (a lane's eight K values are eight pixels).
-// Grid: (chunks, groups of NTG column tiles), one wave per workgroup.
-template <int CIN, int COUT, int S, bool ACT, int NTG>
+// Grid: (chunks, groups of NTG column tiles, groups of MTG row tiles + 1),
+// one wave per workgroup; the last z slice sums the bias gradient.
+template <int CIN, int COUT, int S, bool ACT, int NTG, int MTG>
 __global__ void __launch_bounds__(64) k_conv_bwd_w(ConvBwdWArgs a) {
   constexpr int KPI = cnn_kpad(CIN);
-  constexpr int MT = cnn_pad16(COUT) / 16;
-  const int lane = threadIdx.x, col = lane & 15, kg = lane >> 4;
+  const int mt0 = blockIdx.z * MTG;  // this workgroup's row tiles mt0 .. mt0 + MTG - 1
   const long long npx = (long long)a.batch * a.hout * a.wout;
+  if (blockIdx.z == gridDim.z - 1) {
+    // the last z slice: the bias partial sums of the chunk (the same bf16
+    // gradient values), one lane per channel, in pixel order; the loads of a
+    // batch of pixels are issued before its sums
+    if (blockIdx.y != 0 || threadIdx.x >= COUT) return;
+    const long long ob = (long long)blockIdx.x * a.chunk_px;
+    const long long oe = ob + a.chunk_px < npx ? ob + a.chunk_px : npx;
+    constexpr int B = 64;
+    float s = 0.f;
+    for (long long o0 = ob; o0 < oe; o0 += B) {
+      float v[B];
+#pragma unroll
+      for (int j = 0; j < B; ++j) {
+        const long long q = ((o0 + j < oe) ? o0 + j : ob) * COUT + threadIdx.x;
+        v[j] = bf2f(ACT ? act_grad(a.gout[q], a.post[q]) : a.gout[q]);
+      }
+#pragma unroll
+      for (int j = 0; j < B; ++j)
+        if (o0 + j < oe) s += v[j];
+    }
+    a.partial_b[(long long)blockIdx.x * COUT + threadIdx.x] = s;
+    return;
+  }
+  const int lane = threadIdx.x, col = lane & 15, kg = lane >> 4;
   const long long ob = (long long)blockIdx.x * a.chunk_px;
   const long long oe = ob + a.chunk_px < npx ? ob + a.chunk_px : npx;
   const int nt0 = blockIdx.y * NTG;
-  f32x4 acc[MT][NTG];
+  f32x4 acc[MTG][NTG];
 #pragma unroll
-  for (int mt = 0; mt < MT; ++mt)
+  for (int mt = 0; mt < MTG; ++mt)
 #pragma unroll
     for (int nt = 0; nt < NTG; ++nt) acc[mt][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
   const int hw = a.hout * a.wout;
+  // this lane's columns n = tap * CIN + ci of the NTG tiles
+  int ntap_ky[NTG], ntap_kx[NTG], nci[NTG];
+  bool nok[NTG];
+#pragma unroll
+  for (int nt = 0; nt < NTG; ++nt) {
+    const int n = (nt0 + nt) * 16 + col;
+    const int tap = n / CIN;
+    nci[nt] = n - tap * CIN;
+    ntap_ky[nt] = tap / 3;
+    ntap_kx[nt] = tap - 3 * (tap / 3);
+    nok[nt] = n < 9 * CIN;
+  }
+  // one 32-pixel step per iteration (two or four steps gathered ahead of
+  // their products were measured slower)
   for (long long o0 = ob; o0 < oe; o0 += 32) {
     const long long ok = o0 + 8 * kg;
-    bf16x8 ga[MT], xb[NTG];
+    // (sample, row, column) of the lane's 8 consecutive output pixels: one
+    // division for the first, then steps along the row
+    int pb[8], py[8], px[8];
+    {
+      const int o = (int)ok;  // the host keeps batch x hout x wout below 2^31
+      int b = o / hw;
+      const int rem = o - b * hw;
+      int oy = rem / a.wout, ox = rem - (rem / a.wout) * a.wout;
 #pragma unroll
-    for (int mt = 0; mt < MT; ++mt) {
-      const int co = mt * 16 + col;
+      for (int i = 0; i < 8; ++i) {
+        pb[i] = b;
+        py[i] = oy;
+        px[i] = ox;
+        if (++ox == a.wout) {
+          ox = 0;
+          if (++oy == a.hout) {
+            oy = 0;
+            ++b;
+          }
+        }
+      }
+    }
+    // every gather load is unconditional (from a valid address), the value
+    // selected afterwards, so that all of a step's loads are in flight at once
+    bf16x8 ga[MTG], xb[NTG];
+#pragma unroll
+    for (int mt = 0; mt < MTG; ++mt) {
+      const int co = (mt0 + mt) * 16 + col;
       uint16_t v[8];
 #pragma unroll
       for (int i = 0; i < 8; ++i) {
         const long long o = ok + i;
-        uint16_t x = 0;
-        if (o < oe && co < COUT) {
-          const long long q = o * COUT + co;
-          x = ACT ? act_grad(a.gout[q], a.post[q]) : a.gout[q];
-        }
-        v[i] = x;
+        const bool ov = o < oe && co < COUT;
+        const long long q = (ov ? o : ob) * COUT + (co < COUT ? co : 0);
+        const uint16_t g = a.gout[q];
+        const uint16_t x = ACT ? act_grad(g, a.post[q]) : g;
+        v[i] = ov ? x : (uint16_t)0;
       }
       ga[mt] = pack8(v);
     }
+    long long sb[8];  // sample offsets of the 8 pixels
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int b = pb[i] < a.batch ? pb[i] : a.batch - 1;
+      sb[i] = (long long)(a.batch_index ? a.batch_index[b] : b) * a.in_bstride;
+    }
 #pragma unroll
     for (int nt = 0; nt < NTG; ++nt) {
-      const int n = (nt0 + nt) * 16 + col;
       uint16_t v[8];
-      const int tap = n / CIN, ci = n - (n / CIN) * CIN;
-      const int ky = tap / 3, kx = tap - 3 * (tap / 3);
 #pragma unroll
       for (int i = 0; i < 8; ++i) {
-        const long long o = ok + i;
-        uint16_t x = 0;
-        if (o < oe && n < 9 * CIN) {
-          const int b = (int)(o / hw);
-          const int rem = (int)(o - (long long)b * hw);
-          const int oy = rem / a.wout, ox = rem - (rem / a.wout) * a.wout;
-          const int iy = oy * S + ky - 1, ix = ox * S + kx - 1;
-          if ((unsigned)iy < (unsigned)a.hin && (unsigned)ix < (unsigned)a.win) {
-            const uint16_t* in = a.in + (long long)(a.batch_index ? a.batch_index[b] : b) * a.in_bstride;
-            x = in[((long long)iy * a.win + ix) * CIN + ci];
-          }
-        }
-        v[i] = x;
+        const int iy = py[i] * S + ntap_ky[nt] - 1, ix = px[i] * S + ntap_kx[nt] - 1;
+        const bool iv = ok + i < oe && nok[nt] && (unsigned)iy < (unsigned)a.hin && (unsigned)ix < (unsigned)a.win;
+        const uint16_t x = a.in[sb[i] + ((long long)(iv ? iy : 0) * a.win + (iv ? ix : 0)) * CIN + (iv ? nci[nt] : 0)];
+        v[i] = iv ? x : (uint16_t)0;
       }
       xb[nt] = pack8(v);
     }
 #pragma unroll
-    for (int mt = 0; mt < MT; ++mt)
+    for (int mt = 0; mt < MTG; ++mt)
 #pragma unroll
       for (int nt = 0; nt < NTG; ++nt) acc[mt][nt] = mfma(ga[mt], xb[nt], acc[mt][nt]);
   }
-  float* part = a.partial + (long long)blockIdx.x * (MT * 16) * KPI;
+  float* part = a.partial + (long long)blockIdx.x * cnn_pad16(COUT) * KPI;
 #pragma unroll
-  for (int mt = 0; mt < MT; ++mt)
+  for (int mt = 0; mt < MTG; ++mt)
 #pragma unroll
     for (int nt = 0; nt < NTG; ++nt)
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        const int co = mt * 16 + 4 * kg + j, n = (nt0 + nt) * 16 + col;
+        const int co = (mt0 + mt) * 16 + 4 * kg + j, n = (nt0 + nt) * 16 + col;
         part[(long long)co * KPI + n] = acc[mt][nt][j];
       }
-  // bias partial sums of the chunk (the same bf16 gradient values), one lane per channel
-  if (blockIdx.y == 0 && lane < COUT) {
-    float s = 0.f;
-    for (long long o = ob; o < oe; ++o) {
-      const long long q = o * COUT + lane;
-      s += bf2f(ACT ? act_grad(a.gout[q], a.post[q]) : a.gout[q]);
-    }
-    a.partial_b[(long long)blockIdx.x * COUT + lane] = s;
-  }
 }
 
 __global__ void k_wgrad_reduce(const float* partial, const float* partial_b, int nchunks, int cin, int cout, int kpi,
                                int mrows, float* grad) {
   const int nw = cout * 9 * cin;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  // sums in chunk order; the loads of a batch of chunks are issued first
+  auto chunk_sum = [&](const float* p, long long stride) {
+    constexpr int B = 16;
+    float s = 0.f;
+    for (int c0 = 0; c0 < nchunks; c0 += B) {
+      float v[B];
+#pragma unroll
+      for (int j = 0; j < B; ++j) v[j] = c0 + j < nchunks ? p[(c0 + j) * stride] : 0.f;
+#pragma unroll
+      for (int j = 0; j < B; ++j)
+        if (c0 + j < nchunks) s += v[j];
+    }
+    return s;
+  };
   if (t < nw) {
     const int co = t / (9 * cin), n = t - co * 9 * cin;  // n = tap * cin + ci
-    float s = 0.f;
-    for (int c = 0; c < nchunks; ++c) s += partial[((long long)c * mrows + co) * kpi + n];
-    grad[t] = s;
+    grad[t] = chunk_sum(partial + (long long)co * kpi + n, (long long)mrows * kpi);
   } else if (t < nw + cout) {
     const int co = t - nw;
-    float s = 0.f;
-    for (int c = 0; c < nchunks; ++c) s += partial_b[(long long)c * cout + co];
-    grad[t] = s;
+    grad[t] = chunk_sum(partial_b + co, cout);
   }
 }
 
@@ -832,11 +888,17 @@ hipError_t bwd_data(const ConvBwdDataArgs& a, hipStream_t s) {
 
 template <int CIN, int COUT, int S, bool ACT>
 hipError_t bwd_w(const ConvBwdWArgs& a, int nchunks, hipStream_t s) {
-  constexpr int NTG = 2;
+  constexpr int MT = cnn_pad16(COUT) / 16;
   constexpr int ntiles = cnn_kpad(CIN) / 16;
+  // a small layer (layer 0: 2 x 2 output tiles) takes one tile per
+  // workgroup, so that its launch has enough waves to hide the gathers'
+  // latency; every tile sums its pixels in the same order either way
+  constexpr bool split = MT * ntiles <= 4;
+  constexpr int NTG = split ? 1 : 2, MTG = split ? 1 : MT;
   static_assert(ntiles % NTG == 0, "column tiles per group");
-  if (a.chunk_px % 32 != 0) return hipErrorInvalidValue;
-  hipLaunchKernelGGL((k_conv_bwd_w<CIN, COUT, S, ACT, NTG>), dim3(nchunks, ntiles / NTG), dim3(64), 0, s, a);
+  if (a.chunk_px % 32 != 0 || (long long)a.batch * a.hout * a.wout >= (1LL << 31) - 64) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((k_conv_bwd_w<CIN, COUT, S, ACT, NTG, MTG>), dim3(nchunks, ntiles / NTG, MT / MTG + 1), dim3(64), 0,
+                     s, a);
   return hipGetLastError();
 }
 
