@@ -220,13 +220,27 @@ Flow make_flow(const pivx_synth_params* sp) {
       f.fpsi[m] = 2 * M_PI * r.uni();
     }
     f.scale = 1;
+    // largest displacement over an 8-px grid, rows split over host threads
+    // (a maximum: the same value whatever the split)
+    const int rows = (sp->ny + 7) / 8;
+    const int nt = (int)std::max(1u, std::min({16u, std::thread::hardware_concurrency(), (unsigned)rows}));
+    std::vector<double> part((size_t)nt, 1e-12);
+    auto scan = [&](int t) {
+      double m = 1e-12;
+      for (int j = t; j < rows; j += nt)
+        for (int x = 0; x < sp->nx; x += 8) {
+          double u, v;
+          f.eval(x, 8 * j, u, v);
+          m = std::max(m, std::sqrt(u * u + v * v));
+        }
+      part[(size_t)t] = m;
+    };
+    std::vector<std::thread> workers;
+    for (int t = 1; t < nt; ++t) workers.emplace_back(scan, t);
+    scan(0);
+    for (auto& w : workers) w.join();
     double mx = 1e-12;
-    for (int y = 0; y < sp->ny; y += 8)
-      for (int x = 0; x < sp->nx; x += 8) {
-        double u, v;
-        f.eval(x, y, u, v);
-        mx = std::max(mx, std::sqrt(u * u + v * v));
-      }
+    for (double m : part) mx = std::max(mx, m);
     f.scale = sp->a0 / mx;
   }
   return f;
