@@ -738,6 +738,105 @@ __global__ void __launch_bounds__(64) k_conv_bwd_w(ConvBwdWArgs a) {
       }
 }
 
+// Layer 0 (2 -> 32, stride 2) of k_conv_bwd_w, the same products and bias
+// sums in the same order from operands staged in LDS: each block of 256
+// output pixels of a chunk is read once with row-contiguous loads, the
+// activation gradient as [channel][pixel] and the input patches as
+// [k][pixel] (k = tap * 2 + ci, rows 18-31 the zero padding), and every MFMA
+// operand is one 16-byte LDS read.  Waves 0-3 compute output tile
+// (w / 2, w mod 2); wave 4 adds the staged gradients of each channel to the
+// bias sum in pixel order.  (The gathering form issued about 24,000 VALU
+// instructions per wave for 43 MFMAs; its bias sums, read pixel by pixel from
+// global memory, took 386 of its 437 us.)
+template <bool ACT>
+__global__ void __launch_bounds__(320) k_conv_bwd_w0(ConvBwdWArgs a) {
+  constexpr int COUT = 32, KPI = cnn_kpad(2), PB = 256, LP = PB + 8;
+  __shared__ __attribute__((aligned(16))) uint16_t gA[COUT][LP];  // activation gradient [co][pixel]
+  __shared__ __attribute__((aligned(16))) uint16_t gB[32][LP];    // input patches [k][pixel]
+  const long long npx = (long long)a.batch * a.hout * a.wout;
+  const long long ob = (long long)blockIdx.x * a.chunk_px;
+  const long long oe = ob + a.chunk_px < npx ? ob + a.chunk_px : npx;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int col = lane & 15, kg = lane >> 4, mt = (wave >> 1) & 1, nt = wave & 1;
+  if (t < PB)
+    for (int k = 18; k < 32; ++k) gB[k][t] = 0;
+  const int hw = a.hout * a.wout;
+  f32x4 acc = f32x4{0.f, 0.f, 0.f, 0.f};
+  float bsum = 0.f;
+  for (long long p0 = ob; p0 < oe; p0 += PB) {
+    __syncthreads();  // the previous block's operands are read
+    const long long o = p0 + t;
+    const bool ov = o < oe;
+    if (t < PB) {
+      // pixel t: 32 channels of the gradient, four 16-byte loads of each plane
+      const long long q = (ov ? o : ob) * COUT;
+      u32x4 g[4], pp[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        g[j] = *reinterpret_cast<const u32x4*>(a.gout + q + 8 * j);
+        if (ACT) pp[j] = *reinterpret_cast<const u32x4*>(a.post + q + 8 * j);
+      }
+      // pixel t: its 3x3 patch of both input channels (0 outside the plane)
+      const int oi = (int)(ov ? o : ob);
+      const int b = oi / hw, rem = oi - b * hw;
+      const int oy = rem / a.wout, ox = rem - (rem / a.wout) * a.wout;
+      const uint32_t* in = reinterpret_cast<const uint32_t*>(
+          a.in + (long long)(a.batch_index ? a.batch_index[b] : b) * a.in_bstride);
+      uint32_t w[9];
+#pragma unroll
+      for (int ky = 0; ky < 3; ++ky)
+#pragma unroll
+        for (int kx = 0; kx < 3; ++kx) {
+          const int iy = 2 * oy + ky - 1, ix = 2 * ox + kx - 1;
+          const bool iv = ov && (unsigned)iy < (unsigned)a.hin && (unsigned)ix < (unsigned)a.win;
+          const uint32_t v = in[(long long)(iv ? iy : 0) * a.win + (iv ? ix : 0)];  // both channels of the cell
+          w[3 * ky + kx] = iv ? v : 0u;
+        }
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+          const uint16_t gv = (uint16_t)(g[j][e / 2] >> (16 * (e & 1)));
+          const uint16_t x = ACT ? act_grad(gv, (uint16_t)(pp[j][e / 2] >> (16 * (e & 1)))) : gv;
+          gA[8 * j + e][t] = ov ? x : (uint16_t)0;
+        }
+#pragma unroll
+      for (int tap = 0; tap < 9; ++tap) {
+        gB[2 * tap][t] = (uint16_t)(w[tap] & 0xffffu);
+        gB[2 * tap + 1][t] = (uint16_t)(w[tap] >> 16);
+      }
+    }
+    __syncthreads();
+    if (wave < 4) {
+#pragma unroll
+      for (int st = 0; st < PB / 32; ++st) {
+        if (p0 + 32 * st >= oe) break;
+        const bf16x8 ga =
+            __builtin_bit_cast(bf16x8, *reinterpret_cast<const u32x4*>(&gA[mt * 16 + col][32 * st + 8 * kg]));
+        const bf16x8 xb =
+            __builtin_bit_cast(bf16x8, *reinterpret_cast<const u32x4*>(&gB[nt * 16 + col][32 * st + 8 * kg]));
+        acc = mfma(ga, xb, acc);
+      }
+    } else if (lane < COUT) {
+      // bias: channel `lane`, the block's pixels in order
+      const int n = oe - p0 < PB ? (int)(oe - p0) : PB;
+      for (int p8 = 0; p8 < n; p8 += 8) {
+        const u32x4 v = *reinterpret_cast<const u32x4*>(&gA[lane][p8]);
+#pragma unroll
+        for (int e = 0; e < 8; ++e)
+          if (p8 + e < n) bsum += bf2f((uint16_t)(v[e / 2] >> (16 * (e & 1))));
+      }
+    }
+  }
+  if (wave < 4) {
+    float* part = a.partial + (long long)blockIdx.x * COUT * KPI;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) part[(long long)(mt * 16 + 4 * kg + j) * KPI + nt * 16 + col] = acc[j];
+  } else if (lane < COUT) {
+    a.partial_b[(long long)blockIdx.x * COUT + lane] = bsum;
+  }
+}
+
 __global__ void k_wgrad_reduce(const float* partial, const float* partial_b, int nchunks, int cin, int cout, int kpi,
                                int mrows, float* grad) {
   const int nw = cout * 9 * cin;
@@ -957,7 +1056,10 @@ hipError_t launch_conv_bwd_data(int layer, const ConvBwdDataArgs& a, hipStream_t
 
 hipError_t launch_conv_bwd_w(int layer, const ConvBwdWArgs& a, int nchunks, hipStream_t s) {
   switch (layer) {
-    case 0: return bwd_w<2, 32, 2, true>(a, nchunks, s);
+    case 0:
+      if (a.chunk_px % 32 != 0 || (long long)a.batch * a.hout * a.wout >= (1LL << 31) - 256) return hipErrorInvalidValue;
+      hipLaunchKernelGGL(k_conv_bwd_w0<true>, dim3(nchunks), dim3(320), 0, s, a);
+      return hipGetLastError();
     case 1: return bwd_w<32, 64, 2, true>(a, nchunks, s);
     case 2: return bwd_w<64, 64, 2, true>(a, nchunks, s);
     case 3: return bwd_w<64, 64, 1, true>(a, nchunks, s);
