@@ -236,7 +236,12 @@ Flow make_flow(const pivx_synth_params* sp) {
       part[(size_t)t] = m;
     };
     std::vector<std::thread> workers;
-    for (int t = 1; t < nt; ++t) workers.emplace_back(scan, t);
+    int spawned = 1;
+    try {
+      for (; spawned < nt; ++spawned) workers.emplace_back(scan, spawned);
+    } catch (...) {  // no thread to be had: the rest of the rows here
+    }
+    for (int t = spawned; t < nt; ++t) scan(t);
     scan(0);
     for (auto& w : workers) w.join();
     double mx = 1e-12;
