@@ -666,6 +666,22 @@ def main():
     b_win = 1024 + 1024 * 4 + 17 * 4
     hbm_alg = b_win * g2.nt_vec / (corr2_ms * 1e-3) / 1e9
     def_bytes = g2.nx_aug * g2.ny_aug * 5
+    # the whole pair against the same algorithmic counts: F_win per window,
+    # B_win per window (frame-1 u8, frame 2 u8 on an undeformed pass or FP32
+    # deformed, 17 B per candidate), plus the grid-2 deformations (grid 1
+    # deforms inside its correlation, counted in its B_win)
+    K = 4
+    nw1, nw2 = g1.nt_vec, g2.nt_vec
+    flops_pair = ip.npass1 * nw1 * F_WIN[(64, 64)] + ip.npass2 * nw2 * F_WIN[(32, 32)]
+    b1 = nw1 * ((4096 + 4096 + 17 * K) + (ip.npass1 - 1) * (4096 + 4 * 4096 + 17 * K))
+    b2 = ip.npass2 * (nw2 * (1024 + 4 * 1024 + 17 * K) + def_bytes)
+    pps = args.steps / dt
+    whole_pair = {"flops_per_pair": flops_pair, "tflops_alg": round(flops_pair * pps / 1e12, 2),
+                  "frac_of_fp32_peak": round(flops_pair * pps / 1e12 / PEAK_FP32_TFLOPS, 4),
+                  "frac_of_unpacked_fp32": round(flops_pair * pps / 1e12 / UNPACKED_FP32_TFLOPS, 4),
+                  "bytes_per_pair_alg": b1 + b2, "GBps_alg": round((b1 + b2) * pps / 1e9, 1),
+                  "frac_of_hbm_peak": round((b1 + b2) * pps / 1e9 / PEAK_HBM_GBS, 4),
+                  "frac_of_measured_copy": round((b1 + b2) * pps / 1e9 / copy_gbps, 4) if copy_gbps > 0 else None}
     out = {
         "metric": "corr windows/s at 32x32 (+% HBM peak); pairs/s learned vs plain at equal RMS err",
         "value": round(value, 1),
@@ -711,6 +727,7 @@ def main():
             "hbm_alg_frac": round(hbm_alg / PEAK_HBM_GBS, 4),
             "measured_copy_GBps": round(copy_gbps, 1),
         },
+        "whole_pair": whole_pair,
         "stages_ms": {
             "grid1_corr": round(corr1_ms, 4),
             "grid2_deform": round(def2_ms, 4),
