@@ -566,7 +566,14 @@ __global__ void __launch_bounds__(256) k_conv_bwd_data(ConvBwdDataArgs a) {
   const int gx = (int)(g % gpr);
   const long long r = g / gpr;
   const int iy = (int)(r % a.hin), b = (int)(r / a.hin);
-  const int ix0 = gx * 16 * PT;
+  // pixel of lane column c in tile pt: at stride 1 the run ix0 + pt * 16 + c;
+  // at stride 2 the wave takes the 16 pixels of one parity of a 32-pixel
+  // span, ix0 + 2 (pt * 16 + c) + par, so that a tap reaches either all or
+  // none of them and the taps that reach none are skipped (their K steps
+  // would add exact zeros)
+  const int par = S == 2 ? gx & 1 : 0;
+  const int ix0 = S == 2 ? (gx >> 1) * 32 * PT + par : gx * 16 * PT;
+  auto pix = [&](int pt, int c) { return S == 2 ? ix0 + 2 * (pt * 16 + c) : ix0 + pt * 16 + c; };
   const long long so = (long long)b * a.hout * a.wout * COUT;
   const uint16_t* gout = a.gout + so;
   const uint16_t* post = ACT ? a.post + so : nullptr;
@@ -576,13 +583,18 @@ __global__ void __launch_bounds__(256) k_conv_bwd_data(ConvBwdDataArgs a) {
 #pragma unroll
     for (int pt = 0; pt < PT; ++pt) acc[mt][pt] = f32x4{0.f, 0.f, 0.f, 0.f};
   for (int k0 = 0; k0 < KP; k0 += 32) {
+    if constexpr (S == 2 && COUT % 32 == 0) {
+      // the step's tap (COUT a multiple of 32: one tap per step) reaches this
+      // wave's row and column parity, or the step is all zeros
+      const int tap = k0 / COUT, ky = tap / 3, kx = tap - 3 * (tap / 3);
+      if (((iy + 1 - ky) & 1) || ((par + 1 - kx) & 1)) continue;
+    }
     const int k = k0 + 8 * kg;
     bf16x8 wa[MT], xb[PT];
 #pragma unroll
     for (int mt = 0; mt < MT; ++mt) wa[mt] = ld8(a.wt + (long long)(mt * 16 + col) * KP + k);
 #pragma unroll
-    for (int pt = 0; pt < PT; ++pt)
-      xb[pt] = gpatch8<COUT, S, ACT>(gout, post, a.hout, a.wout, iy, ix0 + pt * 16 + col, k);
+    for (int pt = 0; pt < PT; ++pt) xb[pt] = gpatch8<COUT, S, ACT>(gout, post, a.hout, a.wout, iy, pix(pt, col), k);
 #pragma unroll
     for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
@@ -590,7 +602,7 @@ __global__ void __launch_bounds__(256) k_conv_bwd_data(ConvBwdDataArgs a) {
   }
 #pragma unroll
   for (int pt = 0; pt < PT; ++pt) {
-    const int ix = ix0 + pt * 16 + col;
+    const int ix = pix(pt, col);
 #pragma unroll
     for (int mt = 0; mt < MT; ++mt) {
       const int ci0 = mt * 16 + 4 * kg;
@@ -976,13 +988,23 @@ hipError_t fwd(const ConvFwdArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
-template <int CIN, int COUT, int S, bool ACT>
-hipError_t bwd_data(const ConvBwdDataArgs& a, hipStream_t s) {
-  constexpr int PT = 1;
-  if (a.win % (16 * PT) != 0) return hipErrorInvalidValue;
+template <int CIN, int COUT, int S, bool ACT, int PT>
+hipError_t bwd_data_pt(const ConvBwdDataArgs& a, hipStream_t s) {
+  if (a.win % (16 * PT * S) != 0) return hipErrorInvalidValue;
   const long long groups = (long long)a.batch * a.hin * (a.win / (16 * PT));
   hipLaunchKernelGGL((k_conv_bwd_data<CIN, COUT, S, ACT, PT>), dim3(blocks(groups, 4)), dim3(256), 0, s, a);
   return hipGetLastError();
+}
+
+// Stride 2 skips the taps that reach none of a wave's pixels, which leaves
+// the waves short: up to four 16-pixel tiles per wave where the row allows.
+template <int CIN, int COUT, int S, bool ACT>
+hipError_t bwd_data(const ConvBwdDataArgs& a, hipStream_t s) {
+  if constexpr (S == 2) {
+    if (a.win % (16 * 4 * S) == 0) return bwd_data_pt<CIN, COUT, S, ACT, 4>(a, s);
+    if (a.win % (16 * 2 * S) == 0) return bwd_data_pt<CIN, COUT, S, ACT, 2>(a, s);
+  }
+  return bwd_data_pt<CIN, COUT, S, ACT, 1>(a, s);
 }
 
 template <int CIN, int COUT, int S, bool ACT>
