@@ -978,7 +978,7 @@ inline int blocks(long long n, int b) { return (int)((n + b - 1) / b); }
 template <int CIN, int COUT, int S, bool ACT, bool FINAL>
 hipError_t fwd(const ConvFwdArgs& a, hipStream_t s) {
   constexpr int PT = 1;
-  constexpr int MT = 1;  // one 16-channel tile per wave: the small layers need the waves
+  constexpr int MT = cnn_pad16(COUT) / 16;  // every channel block of a pixel group in one wave (one input operand per K step)
   constexpr int NCB = cnn_pad16(COUT) / 16 / MT;
   constexpr int KS = cnn_kpad(CIN) / 32 >= 8 ? 4 : 1;  // split the K steps of the deep layers over 4 waves
   if (a.wout % (16 * PT) != 0) return hipErrorInvalidValue;
