@@ -30,11 +30,15 @@ def _bf16_round(a):
     return u.astype(np.uint32).view(np.float32)
 
 
-def test_cnn_forward_and_gradient_match_oracle(pivx, orc):
+@pytest.mark.parametrize("w", [128, 256])
+def test_cnn_forward_and_gradient_match_oracle(pivx, orc, w):
+    """Forward pass and parameter gradient against the FP64 oracle; w = 256
+    gives the stride-2 data-gradient launches four 16-pixel tiles per wave
+    (layer 1) and two (layer 2), w = 128 two and one."""
     rng = np.random.default_rng(21)
     p = he_params(orc, rng).astype(np.float32)
-    x = rng.normal(0.0, 1.0, (2, 64, 128, 2)).astype(np.float32)
-    y = rng.normal(0.0, 3.0, (2, 2, 8 * 16)).astype(np.float32)
+    x = rng.normal(0.0, 1.0, (2, 64, w, 2)).astype(np.float32)
+    y = rng.normal(0.0, 3.0, (2, 2, 8 * (w // 8))).astype(np.float32)
     with pivx.Cnn(0) as net:
         net.set_params(p)
         np.testing.assert_array_equal(net.params(), p)
