@@ -849,6 +849,111 @@ __global__ void __launch_bounds__(320) k_conv_bwd_w0(ConvBwdWArgs a) {
   }
 }
 
+// The hidden layers (COUT = 64, CIN = 32 or 64) of k_conv_bwd_w from operands
+// staged in LDS, as k_conv_bwd_w0 stages layer 0: blocks of 64 output
+// pixels of one output row (the host checks wout and the chunk size), the
+// activation gradient as [channel][pixel] and this workgroup's 288 K columns
+// of the input patches (n = tap * CIN + ci, from n0 = 288 blockIdx.y) as
+// [n][pixel], both read with 16-byte loads.  Wave w of four computes the 18
+// output tiles of channel block w; wave 4 adds the staged gradients to the
+// bias sums (workgroups with blockIdx.y = 0).  Same products and sums, in the
+// same order, as k_conv_bwd_w.
+template <int CIN, int S, bool ACT>
+__global__ void __launch_bounds__(320) k_conv_bwd_wh(ConvBwdWArgs a) {
+  constexpr int COUT = 64, KPI = cnn_kpad(CIN), PB = 64, LP = PB + 8, NG = 288, NT = NG / 16;
+  static_assert(9 * CIN % NG == 0 && KPI == 9 * CIN, "K columns in groups of 288");
+  __shared__ __attribute__((aligned(16))) uint16_t gA[COUT][LP];  // activation gradient [co][pixel]
+  __shared__ __attribute__((aligned(16))) uint16_t gB[NG][LP];    // input patches [n - n0][pixel]
+  const long long npx = (long long)a.batch * a.hout * a.wout;
+  const long long ob = (long long)blockIdx.x * a.chunk_px;
+  const long long oe = ob + a.chunk_px < npx ? ob + a.chunk_px : npx;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int col = lane & 15, kg = lane >> 4;
+  const int n0 = blockIdx.y * NG;
+  const int hw = a.hout * a.wout;
+  f32x4 acc[NT];
+#pragma unroll
+  for (int nt = 0; nt < NT; ++nt) acc[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+  float bsum = 0.f;
+  for (long long p0 = ob; p0 < oe; p0 += PB) {
+    __syncthreads();  // the previous block's operands are read
+    if (t < 256) {
+      // the block's pixels p0 .. p0 + 63 lie in one output row
+      const int o0 = (int)p0;  // the host keeps batch x hout x wout below 2^31
+      const int b = o0 / hw, rem = o0 - b * hw;
+      const int oy = rem / a.wout, ox0 = rem - (rem / a.wout) * a.wout;
+      {
+        // gradient: pixel t / 4, channels 16 (t mod 4) .. + 15
+        const int p = t >> 2, cg = (t & 3) * 16;
+        const long long o = p0 + p;
+        const bool ov = o < oe;
+        const long long q = (ov ? o : ob) * COUT + cg;
+        u32x4 g[2], pp[2];
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          g[j] = *reinterpret_cast<const u32x4*>(a.gout + q + 8 * j);
+          if (ACT) pp[j] = *reinterpret_cast<const u32x4*>(a.post + q + 8 * j);
+        }
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+          for (int e = 0; e < 8; ++e) {
+            const uint16_t gv = (uint16_t)(g[j][e / 2] >> (16 * (e & 1)));
+            const uint16_t x = ACT ? act_grad(gv, (uint16_t)(pp[j][e / 2] >> (16 * (e & 1)))) : gv;
+            gA[cg + 8 * j + e][p] = ov ? x : (uint16_t)0;
+          }
+      }
+      // patches: 36 chunks of 8 K columns per pixel, consecutive threads
+      // taking consecutive chunks of one pixel
+      const uint16_t* in = a.in + (long long)(a.batch_index ? a.batch_index[b] : b) * a.in_bstride;
+      for (int i = t; i < PB * (NG / 8); i += 256) {
+        const int p = i / (NG / 8), ch = i - p * (NG / 8);
+        const int n = n0 + 8 * ch, tap = n / CIN, ci = n - tap * CIN;
+        const int ky = tap / 3, kx = tap - 3 * (tap / 3);
+        const int iy = oy * S + ky - 1, ix = (ox0 + p) * S + kx - 1;
+        const bool iv = p0 + p < oe && (unsigned)iy < (unsigned)a.hin && (unsigned)ix < (unsigned)a.win;
+        const u32x4 v = *reinterpret_cast<const u32x4*>(in + ((long long)(iv ? iy : 0) * a.win + (iv ? ix : 0)) * CIN +
+                                                         (iv ? ci : 0));
+#pragma unroll
+        for (int e = 0; e < 8; ++e) gB[8 * ch + e][p] = iv ? (uint16_t)(v[e / 2] >> (16 * (e & 1))) : (uint16_t)0;
+      }
+    }
+    __syncthreads();
+    if (wave < 4) {
+#pragma unroll
+      for (int st = 0; st < PB / 32; ++st) {
+        if (p0 + 32 * st >= oe) break;
+        const bf16x8 ga =
+            __builtin_bit_cast(bf16x8, *reinterpret_cast<const u32x4*>(&gA[wave * 16 + col][32 * st + 8 * kg]));
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) {
+          const bf16x8 xb =
+              __builtin_bit_cast(bf16x8, *reinterpret_cast<const u32x4*>(&gB[nt * 16 + col][32 * st + 8 * kg]));
+          acc[nt] = mfma(ga, xb, acc[nt]);
+        }
+      }
+    } else if (blockIdx.y == 0) {
+      // bias: channel `lane`, the block's pixels in order
+      const int n = oe - p0 < PB ? (int)(oe - p0) : PB;
+      for (int p8 = 0; p8 < n; p8 += 8) {
+        const u32x4 v = *reinterpret_cast<const u32x4*>(&gA[lane][p8]);
+#pragma unroll
+        for (int e = 0; e < 8; ++e)
+          if (p8 + e < n) bsum += bf2f((uint16_t)(v[e / 2] >> (16 * (e & 1))));
+      }
+    }
+  }
+  if (wave < 4) {
+    float* part = a.partial + (long long)blockIdx.x * COUT * KPI;
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) part[(long long)(wave * 16 + 4 * kg + j) * KPI + n0 + nt * 16 + col] = acc[nt][j];
+  } else if (blockIdx.y == 0) {
+    a.partial_b[(long long)blockIdx.x * COUT + lane] = bsum;
+  }
+}
+
 __global__ void k_wgrad_reduce(const float* partial, const float* partial_b, int nchunks, int cin, int cout, int kpi,
                                int mrows, float* grad) {
   const int nw = cout * 9 * cin;
@@ -1076,16 +1181,27 @@ hipError_t launch_conv_bwd_data(int layer, const ConvBwdDataArgs& a, hipStream_t
   }
 }
 
+// The staged hidden-layer kernel takes blocks of 64 pixels of one output row.
+bool staged_wh(const ConvBwdWArgs& a) {
+  return a.wout % 64 == 0 && a.chunk_px % 64 == 0 && (long long)a.batch * a.hout * a.wout < (1LL << 31) - 64;
+}
+
+template <int CIN, int S>
+hipError_t bwd_wh(const ConvBwdWArgs& a, int nchunks, hipStream_t s) {
+  hipLaunchKernelGGL((k_conv_bwd_wh<CIN, S, true>), dim3(nchunks, 9 * CIN / 288), dim3(320), 0, s, a);
+  return hipGetLastError();
+}
+
 hipError_t launch_conv_bwd_w(int layer, const ConvBwdWArgs& a, int nchunks, hipStream_t s) {
   switch (layer) {
     case 0:
       if (a.chunk_px % 32 != 0 || (long long)a.batch * a.hout * a.wout >= (1LL << 31) - 256) return hipErrorInvalidValue;
       hipLaunchKernelGGL(k_conv_bwd_w0<true>, dim3(nchunks), dim3(320), 0, s, a);
       return hipGetLastError();
-    case 1: return bwd_w<32, 64, 2, true>(a, nchunks, s);
-    case 2: return bwd_w<64, 64, 2, true>(a, nchunks, s);
-    case 3: return bwd_w<64, 64, 1, true>(a, nchunks, s);
-    case 4: return bwd_w<64, 64, 1, true>(a, nchunks, s);
+    case 1: return staged_wh(a) ? bwd_wh<32, 2>(a, nchunks, s) : bwd_w<32, 64, 2, true>(a, nchunks, s);
+    case 2: return staged_wh(a) ? bwd_wh<64, 2>(a, nchunks, s) : bwd_w<64, 64, 2, true>(a, nchunks, s);
+    case 3: return staged_wh(a) ? bwd_wh<64, 1>(a, nchunks, s) : bwd_w<64, 64, 1, true>(a, nchunks, s);
+    case 4: return staged_wh(a) ? bwd_wh<64, 1>(a, nchunks, s) : bwd_w<64, 64, 1, true>(a, nchunks, s);
     case 5: return bwd_w<64, 2, 1, false>(a, nchunks, s);
     default: return hipErrorInvalidValue;
   }

@@ -30,15 +30,17 @@ def _bf16_round(a):
     return u.astype(np.uint32).view(np.float32)
 
 
-@pytest.mark.parametrize("w", [128, 256])
-def test_cnn_forward_and_gradient_match_oracle(pivx, orc, w):
+@pytest.mark.parametrize("h,w", [(64, 128), (64, 256), (256, 512)])
+def test_cnn_forward_and_gradient_match_oracle(pivx, orc, h, w):
     """Forward pass and parameter gradient against the FP64 oracle; w = 256
     gives the stride-2 data-gradient launches four 16-pixel tiles per wave
-    (layer 1) and two (layer 2), w = 128 two and one."""
+    (layer 1) and two (layer 2), w = 128 two and one; 256 x 512 takes the
+    LDS-staged weight gradients of the hidden layers (whole 64-pixel blocks
+    of an output row)."""
     rng = np.random.default_rng(21)
     p = he_params(orc, rng).astype(np.float32)
-    x = rng.normal(0.0, 1.0, (2, 64, w, 2)).astype(np.float32)
-    y = rng.normal(0.0, 3.0, (2, 2, 8 * (w // 8))).astype(np.float32)
+    x = rng.normal(0.0, 1.0, (2, h, w, 2)).astype(np.float32)
+    y = rng.normal(0.0, 3.0, (2, 2, (h // 8) * (w // 8))).astype(np.float32)
     with pivx.Cnn(0) as net:
         net.set_params(p)
         np.testing.assert_array_equal(net.params(), p)
