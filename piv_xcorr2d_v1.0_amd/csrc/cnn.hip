@@ -363,8 +363,8 @@ constexpr int kNetThreads = 512;
 
 // Hidden layer CIN -> 64 from region src (side RS) into region dst (side
 // RD, stride S, origin (y0, x0) in its plane of h x w cells): wave w computes
-// output channels 16 (w mod 4) .. + 15, with their weights held in registers,
-// for every other block of 16 cells (w / 4 picks which).
+// output channels 16 (w mod 4) .. + 15, with their weights (lane-ordered, w)
+// held in registers, for every other block of 16 cells (w / 4 picks which).
 template <int CIN, int RS, int RD, int S>
 __device__ __forceinline__ void region_hidden64(const uint16_t* src, uint16_t* dst, const uint16_t* w,
                                                 const float* bias, int y0, int x0, int h, int wd, int wave, int col,
