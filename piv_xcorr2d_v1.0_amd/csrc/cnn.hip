@@ -135,10 +135,8 @@ __device__ __forceinline__ uint16_t down_f1(const uint8_t* f1, long long p1, int
   unsigned int s1 = 0;
   const uint8_t* q = f1 + (long long)y * kCnnDown * p1 + x * kCnnDown;
 #pragma unroll
-  for (int r = 0; r < kCnnDown; ++r, q += p1) {
-    const uchar4 a = *reinterpret_cast<const uchar4*>(q);
-    s1 += a.x + a.y + a.z + a.w;
-  }
+  for (int r = 0; r < kCnnDown; ++r, q += p1)  // the four bytes' sum, one v_sad_u8 against 0
+    s1 = __builtin_amdgcn_sad_u8(*reinterpret_cast<const unsigned int*>(q), 0u, s1);
   const float inv = 1.0f / (kCnnDown * kCnnDown);
   return f2bf(((float)s1 * inv - kCnnMu) * kCnnInvSigma);
 }
