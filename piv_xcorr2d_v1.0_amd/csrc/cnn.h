@@ -110,9 +110,18 @@ hipError_t launch_conv_head(const ConvHeadArgs& a, hipStream_t s);
 hipError_t launch_conv_tail(const ConvTailArgs& a, hipStream_t s);
 hipError_t launch_conv_bwd_data(int layer, const ConvBwdDataArgs& a, hipStream_t s);
 hipError_t launch_conv_bwd_w(int layer, const ConvBwdWArgs& a, int nchunks, hipStream_t s);
-// grad_w [cout][9][cin] and grad_b [cout] of one layer from the partial sums.
-hipError_t launch_wgrad_reduce(int layer, const float* partial, const float* partial_b, int nchunks, float* grad,
-                               hipStream_t s);
+// grad_w [cout][9][cin] and grad_b [cout] of every layer in one launch, from
+// per-layer partial buffers; grad is the whole parameter gradient (layer l at
+// cnn_weight_offset(l)).
+struct WgradReduceLayer {
+  const float* partial; const float* partial_b; float* grad;
+  int nchunks, cin, cout, kpi, mrows, first;
+};
+struct WgradReduceAll {
+  WgradReduceLayer layer[kCnnLayers];
+};
+hipError_t launch_wgrad_reduce_all(const float* const* partial, const float* const* partial_b, const int* nchunks,
+                                   float* grad, hipStream_t s);
 // Mean squared error of the FP32 output planes against labels [n][2][hw];
 // writes dL/d(out) as bf16 NHWC [batch][hw][2] and the per-block partial
 // sums of squared errors into loss_part (nblocks = cnn_loss_blocks(...)).

@@ -952,30 +952,37 @@ __global__ void __launch_bounds__(320) k_conv_bwd_wh(ConvBwdWArgs a) {
   }
 }
 
-__global__ void k_wgrad_reduce(const float* partial, const float* partial_b, int nchunks, int cin, int cout, int kpi,
-                               int mrows, float* grad) {
-  const int nw = cout * 9 * cin;
+// grad_w [cout][9][cin] and grad_b [cout] of every layer from the partial
+// sums, one launch for all layers (each layer's partial sums in a buffer of
+// their own): thread t takes element t - first of the layer whose range holds
+// it and sums it in chunk order, the loads of a batch of chunks issued first.
+__global__ void k_wgrad_reduce_all(WgradReduceAll r) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  // sums in chunk order; the loads of a batch of chunks are issued first
+  int l = 0;
+  while (l + 1 < kCnnLayers && t >= r.layer[l + 1].first) ++l;
+  const WgradReduceLayer& L = r.layer[l];
+  const int e = t - L.first;
+  const int nw = L.cout * 9 * L.cin;
+  if (e < 0 || e >= nw + L.cout) return;
   auto chunk_sum = [&](const float* p, long long stride) {
     constexpr int B = 16;
     float s = 0.f;
-    for (int c0 = 0; c0 < nchunks; c0 += B) {
+    for (int c0 = 0; c0 < L.nchunks; c0 += B) {
       float v[B];
 #pragma unroll
-      for (int j = 0; j < B; ++j) v[j] = c0 + j < nchunks ? p[(c0 + j) * stride] : 0.f;
+      for (int j = 0; j < B; ++j) v[j] = c0 + j < L.nchunks ? p[(c0 + j) * stride] : 0.f;
 #pragma unroll
       for (int j = 0; j < B; ++j)
-        if (c0 + j < nchunks) s += v[j];
+        if (c0 + j < L.nchunks) s += v[j];
     }
     return s;
   };
-  if (t < nw) {
-    const int co = t / (9 * cin), n = t - co * 9 * cin;  // n = tap * cin + ci
-    grad[t] = chunk_sum(partial + (long long)co * kpi + n, (long long)mrows * kpi);
-  } else if (t < nw + cout) {
-    const int co = t - nw;
-    grad[t] = chunk_sum(partial_b + co, cout);
+  if (e < nw) {
+    const int co = e / (9 * L.cin), n = e - co * 9 * L.cin;  // n = tap * cin + ci
+    L.grad[e] = chunk_sum(L.partial + (long long)co * L.kpi + n, (long long)L.mrows * L.kpi);
+  } else {
+    const int co = e - nw;
+    L.grad[e] = chunk_sum(L.partial_b + co, L.cout);
   }
 }
 
@@ -1205,12 +1212,25 @@ hipError_t launch_conv_bwd_w(int layer, const ConvBwdWArgs& a, int nchunks, hipS
   }
 }
 
-hipError_t launch_wgrad_reduce(int layer, const float* partial, const float* partial_b, int nchunks, float* grad,
-                               hipStream_t s) {
-  const CnnLayer& L = kCnnArch[layer];
-  const int n = L.cout * 9 * L.cin + L.cout;
-  hipLaunchKernelGGL(k_wgrad_reduce, dim3(blocks(n, 256)), dim3(256), 0, s, partial, partial_b, nchunks, L.cin,
-                     L.cout, cnn_kpad(L.cin), cnn_pad16(L.cout), grad);
+hipError_t launch_wgrad_reduce_all(const float* const* partial, const float* const* partial_b, const int* nchunks,
+                                   float* grad, hipStream_t s) {
+  WgradReduceAll r{};
+  int first = 0;
+  for (int l = 0; l < kCnnLayers; ++l) {
+    const CnnLayer& L = kCnnArch[l];
+    WgradReduceLayer& d = r.layer[l];
+    d.partial = partial[l];
+    d.partial_b = partial_b[l];
+    d.nchunks = nchunks[l];
+    d.cin = L.cin;
+    d.cout = L.cout;
+    d.kpi = cnn_kpad(L.cin);
+    d.mrows = cnn_pad16(L.cout);
+    d.grad = grad + cnn_weight_offset(l);
+    d.first = first;
+    first += L.cout * 9 * L.cin + L.cout;
+  }
+  hipLaunchKernelGGL(k_wgrad_reduce_all, dim3(blocks(first, 256)), dim3(256), 0, s, r);
   return hipGetLastError();
 }
 

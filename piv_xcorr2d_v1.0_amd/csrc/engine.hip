@@ -1835,6 +1835,7 @@ struct pivx_cnn {
   float* out = nullptr;
   uint16_t* g[2] = {};
   float *partial = nullptr, *partial_b = nullptr, *loss_part = nullptr, *loss = nullptr;
+  size_t pstride = 0;  // floats per layer in partial (one region per layer)
   int* dindex = nullptr;
   uint16_t* xin = nullptr;    // host-entry input batch
   float* lab = nullptr;       // host-entry labels
@@ -1979,6 +1980,11 @@ int cnn_forward_enqueue(const pivx_cnn* net, const uint16_t* x0, long long xstri
 int cnn_backward_enqueue(pivx_cnn* net, const uint16_t* x0, long long xstride, const int* bidx, int B, int H, int W,
                          hipStream_t s) {
   int cur = 0;
+  // every layer's partial sums in a buffer of its own, reduced in one launch
+  // at the end
+  const float* lpart[kCnnLayers];
+  const float* lpart_b[kCnnLayers];
+  int lchunks[kCnnLayers];
   for (int l = kCnnLayers - 1; l >= 0; --l) {
     const CnnLayer& L = kCnnArch[l];
     int hin, win, hout, wout;
@@ -1992,8 +1998,11 @@ int cnn_backward_enqueue(pivx_cnn* net, const uint16_t* x0, long long xstride, c
     w.batch_index = l == 0 ? bidx : nullptr;
     w.gout = net->g[cur];
     w.post = L.act ? net->act[l] : nullptr;
-    w.partial = net->partial;
-    w.partial_b = net->partial_b;
+    w.partial = net->partial + (size_t)l * net->pstride;
+    w.partial_b = net->partial_b + (size_t)l * 64 * kCnnChunks;
+    lpart[l] = w.partial;
+    lpart_b[l] = w.partial_b;
+    lchunks[l] = nchunks;
     w.batch = B;
     w.hin = hin;
     w.win = win;
@@ -2001,7 +2010,6 @@ int cnn_backward_enqueue(pivx_cnn* net, const uint16_t* x0, long long xstride, c
     w.wout = wout;
     w.chunk_px = chunk;
     hipError_t e = launch_conv_bwd_w(l, w, nchunks, s);
-    if (e == hipSuccess) e = launch_wgrad_reduce(l, net->partial, net->partial_b, nchunks, net->grad + cnn_weight_offset(l), s);
     if (e == hipSuccess && l > 0) {
       ConvBwdDataArgs d{};
       d.gout = net->g[cur];
@@ -2018,6 +2026,8 @@ int cnn_backward_enqueue(pivx_cnn* net, const uint16_t* x0, long long xstride, c
     }
     if (e != hipSuccess) return cnn_fail("network backward, layer %d: %s", l, hipGetErrorString(e));
   }
+  const hipError_t e = launch_wgrad_reduce_all(lpart, lpart_b, lchunks, net->grad, s);
+  if (e != hipSuccess) return cnn_fail("network gradient reduction: %s", hipGetErrorString(e));
   return PIVX_OK;
 }
 
@@ -2043,8 +2053,9 @@ int cnn_workspace(pivx_cnn* net, int B, int H, int W) {
   rc |= nalloc(net->ws, &net->out, (size_t)B * 2 * (H / 8) * (W / 8));
   rc |= nalloc(net->ws, &net->g[0], gmax);
   rc |= nalloc(net->ws, &net->g[1], gmax);
-  rc |= nalloc(net->ws, &net->partial, pmax * kCnnChunks);
-  rc |= nalloc(net->ws, &net->partial_b, (size_t)64 * kCnnChunks);
+  net->pstride = pmax * kCnnChunks;
+  rc |= nalloc(net->ws, &net->partial, net->pstride * kCnnLayers);
+  rc |= nalloc(net->ws, &net->partial_b, (size_t)64 * kCnnChunks * kCnnLayers);
   rc |= nalloc(net->ws, &net->loss_part, (size_t)cnn_loss_blocks(B, (H / 8) * (W / 8)));
   rc |= nalloc(net->ws, &net->loss, 1);
   rc |= nalloc(net->ws, &net->xin, (size_t)B * H * W * 2);
